@@ -1,0 +1,154 @@
+/*
+ * naga_amd.h -- C-ABI of the MI355X-native nonlocal lattice-Boltzmann
+ * acoustics step and its setup (one GPU, HIP/CDNA4).
+ *
+ * Plain pointers and sizes only.  Arrays are host memory, copied in and out;
+ * device state is owned by the handles.  Every function returns a status
+ * (NAGA_OK = 0) and sets a thread-local message readable with
+ * naga_last_error().  Reference interfaces are cited as path:line relative
+ * to the NAGA source tree; the C++ facade naga_amd/naga.hpp maps these
+ * entry points back to the reference's naga:: classes.
+ */
+#ifndef NAGA_AMD_H
+#define NAGA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NAGA_OK 0
+#define NAGA_INVALID_ARGUMENT 1
+#define NAGA_OUT_OF_MEMORY 2
+#define NAGA_DEVICE_ERROR 3
+#define NAGA_NUMERICAL_ERROR 4
+#define NAGA_NOT_SUPPORTED 5
+
+#define NAGA_RBF_MULTIQUADRIC 0
+#define NAGA_RBF_POLYHARMONIC3 1
+
+typedef struct naga_engine naga_engine;
+
+/* Scene configuration.  Physics parameters follow
+ * simulation_engine::set_problem_parameters(nu, rho0, dt, c)
+ * (detail/simulation_engine.cuh:94-124); the stencil constants that the
+ * reference hard-codes are fields here. */
+typedef struct naga_scene_config {
+    int32_t lattice;        /* 9 (D2Q9) or 27 (D3Q27) */
+    int32_t k;              /* divergence stencil width (reference: 32) */
+    int32_t k_boundary;     /* boundary RPIM stencil width (reference: 32) */
+    int32_t rbf;            /* RBF of the divergence weights */
+    int32_t rbf_boundary;   /* RBF of the boundary interpolator */
+    int32_t pml_ref_index;  /* 1: PML layer field reads Q at (j - num_bulk)
+                               as pml_absorption.inl:445-448 does; 0: at j */
+    double nu;              /* fluid viscosity */
+    double rho0;            /* nominal density */
+    double dt;              /* time step */
+    double c;               /* speed of sound */
+    double h;               /* nodal spacing (simulation_nodes::nodal_spacing) */
+} naga_scene_config;
+
+const char* naga_last_error(void);
+
+/* Lattice tables: c[Q*D], w[Q], bb[Q], K[Q*Q] (row alpha, column j).
+ * detail/d2q9_lattice.cuh:44-88,151-163; detail/d3q27_lattice.cuh:45-105,
+ * 380-387. */
+int naga_lattice_tables(int32_t q, double* c, double* w, int32_t* bb, double* K);
+
+/* Collision increment of one node on the host (FP64), for tests of the
+ * device formula: f full distribution, rho = sum f, u unitless velocity.
+ * lattice_interface<...>::compute_moment_projection + collision matrix,
+ * detail/simulation_engine.cuh:486-527. */
+int naga_collide(int32_t q, const double* f, double rho, const double* u,
+                 double nu_lat, double dt_lat, double* df);
+
+/* ---- engine: simulation_engine<Lattice>
+ *      (fluids/nonlocal_lattice_boltzmann/simulation_engine.cuh:39-121) ---- */
+int naga_engine_create(const naga_scene_config* cfg, naga_engine** out);
+int naga_engine_destroy(naga_engine* e);
+
+/* init_domain(simulation_nodes) (detail/simulation_engine.cuh:126-339):
+ * points[N*D] ordered [bulk | layer | boundary] (no ghost nodes),
+ * normals[n_boundary*D] inward, layer_absorption[n_layer]. */
+int naga_engine_init_domain(naga_engine* e, const float* points, int64_t n_bulk,
+                            int64_t n_layer, int64_t n_boundary,
+                            const float* normals, const float* layer_absorption);
+
+/* Initial condition f_alpha = w_alpha (1 + amp exp(-|x-c|^2 / (2 width^2))),
+ * the one-shot density source of examples/nonlocal_lattice_boltzmann.cu:
+ * 79-125 applied as an initial state; resets time to 0. */
+int naga_engine_set_pulse(naga_engine* e, const double* center, double amp,
+                          double width);
+/* f[Q*N] alpha-major full distributions (FP64 in, stored as FP32 deviations). */
+int naga_engine_set_distributions(naga_engine* e, const double* f);
+
+/* step_forward() x nsteps (detail/simulation_engine.cuh:1325-1375). */
+int naga_engine_step(naga_engine* e, int32_t nsteps);
+/* The same, timed on the engine's stream with HIP events (ms). */
+int naga_engine_step_timed(naga_engine* e, int32_t nsteps, double* ms);
+/* Per-kernel timing of one step: ms[11] for p1 r1 r2 p2 p3 p4 s1 s2 s3 s4 p5. */
+int naga_engine_profile_step(naga_engine* e, double* ms);
+
+/* solution().macroscopic_values: rho[N] (FP32), u[D*N] (d-major, FP32) and
+ * drho[N] = rho - rho0 (FP64, from the FP32 deviation state without
+ * cancellation).  fresh = 1 recomputes them from the current state;
+ * fresh = 0 returns the values held by the engine (computed at the start of
+ * the last step, boundary velocities zeroed by bounce-back), as the
+ * reference does. Any output may be NULL. */
+int naga_engine_get_macroscopic(naga_engine* e, float* rho, float* u,
+                                double* drho, int32_t fresh);
+/* f[Q*N] alpha-major (FP64 = w + deviation) and/or the deviation g[Q*N]. */
+int naga_engine_get_distributions(naga_engine* e, double* f, double* g);
+/* Setup products for parity tests: idx[N*k], W[N*k*D], filt[N*k],
+ * bidx[Nb*kb], bw[Nb*kb]. Any may be NULL. */
+int naga_engine_get_setup(naga_engine* e, uint32_t* idx, float* W, float* filt,
+                          uint32_t* bidx, float* bw);
+/* PML state: E = Q1 - w and Qp = Q2 (reference naming) as [Q*N] alpha-major;
+ * only rows of nodes whose PML values are used are maintained. */
+int naga_engine_get_pml(naga_engine* e, double* Q1_minus_w, double* Q2);
+int naga_engine_info(naga_engine* e, int64_t* n_nodes, double* time,
+                     int64_t* frame, double* setup_ms);
+
+/* ---- node provider: uniform_grid_provider<Lattice>::get()
+ *      (node_providers/uniform_grid_provider.cuh:65-161) with seeded
+ *      SplitMix64 jitter of interior nodes; layer by exact 1-NN distance to
+ *      the boundary nodes on the GPU.  Call naga_uniform_grid_size first. */
+int64_t naga_uniform_grid_size(int32_t dims, const double* lo, const double* hi,
+                               double h);
+int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h,
+                      double jitter_frac, uint64_t seed, double layer_thickness,
+                      double peak_absorption, float* points, float* normals,
+                      float* sigma, int64_t* counts);
+
+/* ---- setup primitives ---------------------------------------------------- */
+/* batched_nearest_neighbors (segmentation/nearest_neighbors.cuh:95-477),
+ * exact: the k smallest (d2, index), d2 FP32 = dz^2 + (dy^2 + dx^2). */
+int naga_knn(const float* ref, int64_t n_ref, const float* query, int64_t n_q,
+             int32_t dims, int32_t k, float* d2_out, uint32_t* idx_out);
+/* batched_matrix_inverse replacement (linalg/batched_matrix_inverse.cuh:
+ * 325-397): solves A X = B per item in FP64, A column-major n x n, B
+ * column-major n x nrhs; info[b] != 0 flags a zero pivot. */
+int naga_batched_lu_solve_f64(const double* A, int32_t n, int64_t batch,
+                              const double* B, int32_t nrhs, double* X,
+                              int32_t* info);
+/* radial_point_method::compute_weights, group size 1
+ * (interpolation/radial_point_method.cuh:118-139): w[n_q*k]. */
+int naga_rpim_weights(const float* src, int64_t n_src, const uint32_t* idx,
+                      int32_t k, const float* query, int64_t n_q, int32_t dims,
+                      int32_t rbf, double spacing, double* w);
+/* divergence_operator::create (nonlocal_calculus/divergence.cuh:53-166):
+ * kNN of the queries over src plus FP64 weights W[n_q*k*D] (FP64 out). */
+int naga_divergence_weights(const float* src, int64_t n_src, const float* query,
+                            int64_t n_q, int32_t dims, int32_t k, int32_t rbf,
+                            uint32_t* idx_out, double* W_out);
+/* divergence_operator::apply (nonlocal_calculus/divergence.cuh:183-212,
+ * 377-488) in FP32: out[i] = sum_s sum_d W(d,s,i) (F_d(idx(s,i)) - off). */
+int naga_divergence_apply(const uint32_t* idx, const float* W, int64_t n,
+                          int32_t k, int32_t dims, const float* F, int64_t n_f,
+                          float off, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

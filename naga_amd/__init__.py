@@ -1,0 +1,494 @@
+"""MI355X-native nonlocal lattice-Boltzmann acoustics (NAGA's nonlocal LBM
+step and setup) -- Python host mirror of the reference interface.
+
+The compute path is libnaga_amd.so (HIP kernels for gfx950 behind the C-ABI
+declared in include/naga_amd.h).  This module only marshals host arrays:
+it loads the library or raises, it never computes a fallback.
+
+Names follow the reference's namespaces:
+
+* ``naga::fluids::nonlocal_lbm``: :class:`SimulationEngine`,
+  :class:`SimulationNodes`, :class:`UniformGridProvider`, :func:`lattice_tables`;
+* ``naga::segmentation``: :func:`batched_nearest_neighbors`;
+* ``naga::linalg``: :func:`batched_lu_solve` (replaces batched_matrix_inverse);
+* ``naga::interpolation``: :func:`rpim_weights` (radial_point_method);
+* ``naga::nonlocal_calculus``: :class:`DivergenceOperator`.
+
+Errors raise ``ValueError`` where the reference throws std::invalid_argument
+and ``RuntimeError`` where it throws std::runtime_error, with the
+reference's "naga::..." prefixes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnaga_amd.so")
+
+CS_LAT = 0.57735026918962576450914878050196
+RBF_MULTIQUADRIC = 0
+RBF_POLYHARMONIC3 = 1
+
+_ENGINE_PREFIX = "naga::fluids::nonlocal_lbm::simulation_engine::"
+
+
+class SceneConfig(C.Structure):
+    _fields_ = [
+        ("lattice", C.c_int32), ("k", C.c_int32), ("k_boundary", C.c_int32),
+        ("rbf", C.c_int32), ("rbf_boundary", C.c_int32),
+        ("pml_ref_index", C.c_int32),
+        ("nu", C.c_double), ("rho0", C.c_double), ("dt", C.c_double),
+        ("c", C.c_double), ("h", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libnaga_amd.so; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m naga_amd.build` "
+            "(the HIP library is the only compute path)")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    i32, i64, f64, u64 = C.c_int32, C.c_int64, C.c_double, C.c_uint64
+    sig = {
+        "naga_last_error": (C.c_char_p, []),
+        "naga_lattice_tables": (C.c_int, [i32, vp, vp, vp, vp]),
+        "naga_collide": (C.c_int, [i32, vp, f64, vp, f64, f64, vp]),
+        "naga_engine_create": (C.c_int, [C.POINTER(SceneConfig), C.POINTER(vp)]),
+        "naga_engine_destroy": (C.c_int, [vp]),
+        "naga_engine_init_domain": (C.c_int, [vp, vp, i64, i64, i64, vp, vp]),
+        "naga_engine_set_pulse": (C.c_int, [vp, vp, f64, f64]),
+        "naga_engine_set_distributions": (C.c_int, [vp, vp]),
+        "naga_engine_step": (C.c_int, [vp, i32]),
+        "naga_engine_step_timed": (C.c_int, [vp, i32, vp]),
+        "naga_engine_profile_step": (C.c_int, [vp, vp]),
+        "naga_engine_get_macroscopic": (C.c_int, [vp, vp, vp, vp, i32]),
+        "naga_engine_get_distributions": (C.c_int, [vp, vp, vp]),
+        "naga_engine_get_setup": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "naga_engine_get_pml": (C.c_int, [vp, vp, vp]),
+        "naga_engine_info": (C.c_int, [vp, vp, vp, vp, vp]),
+        "naga_uniform_grid_size": (i64, [i32, vp, vp, f64]),
+        "naga_uniform_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, f64, f64, vp, vp, vp, vp]),
+        "naga_knn": (C.c_int, [vp, i64, vp, i64, i32, i32, vp, vp]),
+        "naga_batched_lu_solve_f64": (C.c_int, [vp, i32, i64, vp, i32, vp, vp]),
+        "naga_rpim_weights": (C.c_int, [vp, i64, vp, i32, vp, i64, i32, i32, f64, vp]),
+        "naga_divergence_weights": (C.c_int, [vp, i64, vp, i64, i32, i32, i32, vp, vp]),
+        "naga_divergence_apply": (C.c_int, [vp, vp, i64, i32, i32, vp, i64, C.c_float, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+EXPORTED = [
+    "naga_last_error", "naga_lattice_tables", "naga_collide",
+    "naga_engine_create", "naga_engine_destroy", "naga_engine_init_domain",
+    "naga_engine_set_pulse", "naga_engine_set_distributions", "naga_engine_step",
+    "naga_engine_step_timed", "naga_engine_profile_step",
+    "naga_engine_get_macroscopic", "naga_engine_get_distributions",
+    "naga_engine_get_setup", "naga_engine_get_pml", "naga_engine_info",
+    "naga_uniform_grid_size", "naga_uniform_grid", "naga_knn",
+    "naga_batched_lu_solve_f64", "naga_rpim_weights", "naga_divergence_weights",
+    "naga_divergence_apply",
+]
+
+STEP_KERNELS = ["p1_collide_pml_local", "r1_layer_sums", "r2_layer_means",
+                "p2_pml_divergence", "p3_boundary_interp", "p4_filter_bounce",
+                "s1_rk4_stage", "s2_rk4_stage", "s3_rk4_stage", "s4_rk4_stage",
+                "p5_boundary_interp"]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(status: int, prefix: str = "naga::") -> None:
+    if status == 0:
+        return
+    msg = prefix + load().naga_last_error().decode(errors="replace")
+    if status == 1:
+        raise ValueError(msg)
+    if status == 2:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- lattices
+def lattice_tables(Q: int):
+    """Velocities c[Q, D], weights w[Q], bounce-back bb[Q], collision
+    matrix K[Q, Q] of D2Q9 (Q = 9) or D3Q27 (Q = 27)."""
+    lib = load()
+    D = 2 if Q == 9 else 3
+    c = np.zeros(Q * D)
+    w = np.zeros(Q)
+    bb = np.zeros(Q, np.int32)
+    K = np.zeros(Q * Q)
+    _check(lib.naga_lattice_tables(Q, _ptr(c), _ptr(w), _ptr(bb), _ptr(K)))
+    return c.reshape(Q, D), w, bb, K.reshape(Q, Q)
+
+
+def collide(Q, f, rho, u, nu_lat, dt_lat):
+    """Host (FP64) evaluation of the device collision formula for one node."""
+    lib = load()
+    f = _f64(f)
+    u = _f64(u)
+    df = np.zeros(Q)
+    _check(lib.naga_collide(Q, _ptr(f), float(rho), _ptr(u), float(nu_lat),
+                            float(dt_lat), _ptr(df)))
+    return df
+
+
+# ------------------------------------------------------------------- nodes
+@dataclasses.dataclass
+class SimulationNodes:
+    """simulation_nodes<T> (fluids/nonlocal_lattice_boltzmann/
+    simulation_nodes.cuh:41-66), points stored [N, D] (one row per node)."""
+    points: np.ndarray
+    boundary_normals: np.ndarray
+    layer_absorption: np.ndarray
+    num_bulk_points: int
+    num_layer_points: int
+    num_boundary_points: int
+    nodal_spacing: float
+    num_ghost_nodes: int = 0
+
+    @property
+    def size(self) -> int:
+        return self.num_bulk_points + self.num_layer_points + self.num_boundary_points
+
+    def as_dict(self) -> dict:
+        return dict(points=self.points, normals=self.boundary_normals,
+                    sigma=self.layer_absorption, n_bulk=self.num_bulk_points,
+                    n_layer=self.num_layer_points, n_boundary=self.num_boundary_points)
+
+
+class UniformGridProvider:
+    """uniform_grid_provider<Lattice> (node_providers/uniform_grid_provider.cuh:
+    45-237) with seeded jitter of the interior nodes (SplitMix64)."""
+
+    def __init__(self, min_point, max_point, grid_spacing,
+                 absorption_layer_thickness=0.0, peak_absorption_rate=0.0,
+                 jitter=0.0, seed=0x4E414741):
+        self.lo = _f64(min_point)
+        self.hi = _f64(max_point)
+        self.h = float(grid_spacing)
+        self.layer = float(absorption_layer_thickness)
+        self.peak = float(peak_absorption_rate)
+        self.jitter = float(jitter)
+        self.seed = int(seed)
+
+    def get(self) -> SimulationNodes:
+        lib = load()
+        D = self.lo.size
+        N = lib.naga_uniform_grid_size(D, _ptr(self.lo), _ptr(self.hi), self.h)
+        if N <= 0:
+            raise ValueError("naga::ranges::uniform_grid:: invalid grid")
+        pts = np.zeros(N * D, np.float32)
+        nrm = np.zeros(N * D, np.float32)
+        sig = np.zeros(N, np.float32)
+        counts = np.zeros(3, np.int64)
+        _check(lib.naga_uniform_grid(D, _ptr(self.lo), _ptr(self.hi), self.h,
+                                     self.jitter, self.seed, self.layer, self.peak,
+                                     _ptr(pts), _ptr(nrm), _ptr(sig), _ptr(counts)),
+               "naga::fluids::nonlocal_lbm::uniform_grid_provider::")
+        nb, nl, nbd = (int(x) for x in counts)
+        return SimulationNodes(pts.reshape(N, D), nrm[: nbd * D].reshape(nbd, D).copy(),
+                               sig[:nl].copy(), nb, nl, nbd, self.h)
+
+
+# ------------------------------------------------------------------ engine
+def problem_parameters(lattice: int, tau: float, dt: float, c: float | None = None):
+    """(nu, dt, c) giving relaxation time tau: D3Q27 relaxes shear at
+    1/(3 nu_lat + 1/2)/dt_lat (detail/d3q27_lattice.cuh:177), D2Q9 at
+    1/(3 nu_lat + dt_lat/2) (detail/d2q9_lattice.cuh:136)."""
+    c = CS_LAT if c is None else c
+    ratio = CS_LAT / c
+    dt_lat = dt / ratio
+    nu_lat = (tau - 0.5) / 3.0 if lattice == 27 else (tau - 0.5) * dt_lat / 3.0
+    return dict(nu=nu_lat * ratio, dt=dt, c=c)
+
+
+class SimulationEngine:
+    """simulation_engine<Lattice> (fluids/nonlocal_lattice_boltzmann/
+    simulation_engine.cuh:39-121) backed by one GPU-resident scene."""
+
+    def __init__(self, lattice: int = 27, k: int | None = None,
+                 k_boundary: int | None = None, rbf: int = RBF_POLYHARMONIC3,
+                 rbf_boundary: int | None = None, pml_ref_index: bool = True):
+        if lattice not in (9, 27):
+            raise ValueError(_ENGINE_PREFIX + "lattice must be D2Q9 or D3Q27")
+        self.lattice = lattice
+        self.dimensions = 2 if lattice == 9 else 3
+        self.k = 32 if k is None else int(k)
+        self.k_boundary = self.k if k_boundary is None else int(k_boundary)
+        self.rbf = int(rbf)
+        self.rbf_boundary = self.rbf if rbf_boundary is None else int(rbf_boundary)
+        self.pml_ref_index = bool(pml_ref_index)
+        self._params = None
+        self._h = None
+        self._nodes = None
+
+    # set_problem_parameters (detail/simulation_engine.cuh:94-124)
+    def set_problem_parameters(self, fluid_viscosity, nominal_density, time_step,
+                               speed_of_sound, characteristic_frequency=0.0):
+        self._params = dict(nu=float(fluid_viscosity), rho0=float(nominal_density),
+                            dt=float(time_step), c=float(speed_of_sound))
+
+    def init_domain(self, nodes):
+        """init_domain(simulation_nodes | node_provider)."""
+        if self._params is None:
+            raise ValueError(_ENGINE_PREFIX + "set_problem_parameters must be called first")
+        if hasattr(nodes, "get") and not isinstance(nodes, SimulationNodes):
+            nodes = nodes.get()
+        D = self.dimensions
+        pts = _f32(nodes.points).reshape(-1)
+        if np.asarray(nodes.points).ndim != 2 or np.asarray(nodes.points).shape[1] != D:
+            raise ValueError(_ENGINE_PREFIX + "The number of dimensions in the simulation "
+                             "domain does not match the number of dimensions in the lattice.")
+        if nodes.num_ghost_nodes:
+            raise ValueError(_ENGINE_PREFIX + "ghost nodes are not supported")
+        if pts.size != nodes.size * D:
+            raise ValueError(_ENGINE_PREFIX + "The number of points in the simulation domain "
+                             "does not match the sum of the number of bulk, layer, and "
+                             "boundary points.")
+        nrm = _f32(nodes.boundary_normals).reshape(-1)
+        if nrm.size != nodes.num_boundary_points * D:
+            raise ValueError(_ENGINE_PREFIX + "The number of boundary normals does not match "
+                             "the number of boundary points.")
+        sig = _f32(nodes.layer_absorption).reshape(-1)
+        if sig.size != nodes.num_layer_points:
+            raise ValueError(_ENGINE_PREFIX + "The number of layer absorption coefficients "
+                             "does not match the number of layer points.")
+        lib = load()
+        self.close()
+        cfg = SceneConfig(lattice=self.lattice, k=self.k, k_boundary=self.k_boundary,
+                          rbf=self.rbf, rbf_boundary=self.rbf_boundary,
+                          pml_ref_index=int(self.pml_ref_index),
+                          nu=self._params["nu"], rho0=self._params["rho0"],
+                          dt=self._params["dt"], c=self._params["c"],
+                          h=float(nodes.nodal_spacing))
+        h = C.c_void_p()
+        _check(lib.naga_engine_create(C.byref(cfg), C.byref(h)), _ENGINE_PREFIX)
+        self._h = h
+        self._cfg = cfg
+        _check(lib.naga_engine_init_domain(h, _ptr(pts), nodes.num_bulk_points,
+                                           nodes.num_layer_points,
+                                           nodes.num_boundary_points,
+                                           _ptr(nrm) if nrm.size else None,
+                                           _ptr(sig) if sig.size else None), _ENGINE_PREFIX)
+        self._nodes = nodes
+
+    def close(self):
+        if self._h is not None:
+            load().naga_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _live(self):
+        if self._h is None:
+            raise ValueError(_ENGINE_PREFIX + "init_domain must be called first")
+        return self._h
+
+    @property
+    def num_nodes(self) -> int:
+        return self._nodes.size
+
+    def set_initial_pulse(self, center, amplitude, width):
+        _check(load().naga_engine_set_pulse(self._live(), _ptr(_f64(center)),
+                                            float(amplitude), float(width)), _ENGINE_PREFIX)
+
+    def reset(self):
+        """reset() (detail/simulation_engine.cuh:1377-1386): f = w, time 0."""
+        self.set_initial_pulse(np.zeros(self.dimensions), 0.0, 1.0)
+
+    def set_distributions(self, f):
+        _check(load().naga_engine_set_distributions(self._live(), _ptr(_f64(f))),
+               _ENGINE_PREFIX)
+
+    def step_forward(self, nsteps: int = 1):
+        _check(load().naga_engine_step(self._live(), int(nsteps)), _ENGINE_PREFIX)
+
+    def step_timed(self, nsteps: int) -> float:
+        ms = np.zeros(1)
+        _check(load().naga_engine_step_timed(self._live(), int(nsteps), _ptr(ms)),
+               _ENGINE_PREFIX)
+        return float(ms[0])
+
+    def profile_step(self) -> dict:
+        ms = np.zeros(11)
+        _check(load().naga_engine_profile_step(self._live(), _ptr(ms)), _ENGINE_PREFIX)
+        return dict(zip(STEP_KERNELS, ms.tolist()))
+
+    def info(self) -> dict:
+        n = np.zeros(1, np.int64)
+        t = np.zeros(1)
+        fr = np.zeros(1, np.int64)
+        sm = np.zeros(1)
+        _check(load().naga_engine_info(self._live(), _ptr(n), _ptr(t), _ptr(fr), _ptr(sm)))
+        return dict(nodes=int(n[0]), time=float(t[0]), frame=int(fr[0]), setup_ms=float(sm[0]))
+
+    def time(self) -> float:
+        return self.info()["time"]
+
+    def macroscopic(self, fresh: bool = True) -> dict:
+        N, D = self.num_nodes, self.dimensions
+        rho = np.zeros(N, np.float32)
+        u = np.zeros(D * N, np.float32)
+        drho = np.zeros(N)
+        _check(load().naga_engine_get_macroscopic(self._live(), _ptr(rho), _ptr(u),
+                                                  _ptr(drho), int(fresh)), _ENGINE_PREFIX)
+        return dict(rho=rho, u=u.reshape(D, N), drho=drho)
+
+    def distributions(self) -> dict:
+        N, Q = self.num_nodes, self.lattice
+        f = np.zeros(Q * N)
+        g = np.zeros(Q * N)
+        _check(load().naga_engine_get_distributions(self._live(), _ptr(f), _ptr(g)),
+               _ENGINE_PREFIX)
+        return dict(f=f.reshape(Q, N), g=g.reshape(Q, N))
+
+    def solution(self) -> dict:
+        """state_variables: distributions plus macroscopic values as held by
+        the engine (reference semantics, computed at the start of the last
+        step)."""
+        out = self.distributions()
+        out.update(self.macroscopic(fresh=False))
+        return out
+
+    def setup_products(self) -> dict:
+        N, k, kb, D = self.num_nodes, self.k, self.k_boundary, self.dimensions
+        nb = self._nodes.num_boundary_points
+        idx = np.zeros(N * k, np.uint32)
+        W = np.zeros(N * k * D, np.float32)
+        filt = np.zeros(N * k, np.float32)
+        bidx = np.zeros(max(nb, 1) * kb, np.uint32)
+        bw = np.zeros(max(nb, 1) * kb, np.float32)
+        _check(load().naga_engine_get_setup(self._live(), _ptr(idx), _ptr(W), _ptr(filt),
+                                            _ptr(bidx), _ptr(bw)), _ENGINE_PREFIX)
+        return dict(idx=idx.reshape(N, k), W=W.reshape(N, k, D), filt=filt.reshape(N, k),
+                    bidx=bidx[: nb * kb].reshape(nb, kb), bw=bw[: nb * kb].reshape(nb, kb))
+
+    def pml_state(self):
+        N, Q = self.num_nodes, self.lattice
+        e = np.zeros(Q * N)
+        q = np.zeros(Q * N)
+        _check(load().naga_engine_get_pml(self._live(), _ptr(e), _ptr(q)), _ENGINE_PREFIX)
+        return e.reshape(Q, N), q.reshape(Q, N)
+
+
+# ------------------------------------------------------- setup primitives
+def batched_nearest_neighbors(k: int, query, ref):
+    """segmentation::batched_nearest_neighbors (segmentation/nearest_neighbors.cuh:
+    454-477), exact: returns (distances_squared [nq, k], indices [nq, k])."""
+    ref = _f32(ref)
+    query = _f32(query)
+    D = ref.shape[1]
+    nq = query.shape[0]
+    d2 = np.zeros(nq * k, np.float32)
+    idx = np.zeros(nq * k, np.uint32)
+    _check(load().naga_knn(_ptr(ref), ref.shape[0], _ptr(query), nq, D, int(k),
+                           _ptr(d2), _ptr(idx)), "naga::segmentation::")
+    return d2.reshape(nq, k), idx.reshape(nq, k)
+
+
+def batched_lu_solve(A, B):
+    """Solve A_b X_b = B_b for a batch of n x n systems in FP64 (replaces
+    linalg::batched_matrix_inverse, linalg/batched_matrix_inverse.cuh:325-397).
+    A: [batch, n, n], B: [batch, n] or [batch, n, nrhs].  Returns (X, info)."""
+    A = np.asarray(A, np.float64)
+    B = np.asarray(B, np.float64)
+    batch, n, _ = A.shape
+    vec = B.ndim == 2
+    B3 = B[:, :, None] if vec else B
+    nrhs = B3.shape[2]
+    Ac = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))  # column-major per item
+    Bc = np.ascontiguousarray(np.transpose(B3, (0, 2, 1)))
+    X = np.zeros_like(Bc)
+    info = np.zeros(batch, np.int32)
+    _check(load().naga_batched_lu_solve_f64(_ptr(Ac), n, batch, _ptr(Bc), nrhs, _ptr(X),
+                                            _ptr(info)), "naga::linalg::")
+    X = np.transpose(X, (0, 2, 1))
+    return (X[:, :, 0] if vec else X), info
+
+
+def rpim_weights(src, idx, query, rbf=RBF_POLYHARMONIC3, spacing=1.0):
+    """interpolation::radial_point_method::compute_weights, group size 1:
+    weights [nq, k] for interpolating from src[idx[q]] to query[q]."""
+    src = _f32(src)
+    query = _f32(query)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    nq, k = idx.shape
+    w = np.zeros(nq * k)
+    _check(load().naga_rpim_weights(_ptr(src), src.shape[0], _ptr(idx), k, _ptr(query), nq,
+                                    src.shape[1], int(rbf), float(spacing), _ptr(w)),
+           "naga::interpolation::")
+    return w.reshape(nq, k)
+
+
+class DivergenceOperator:
+    """nonlocal_calculus::divergence_operator<T, D> (nonlocal_calculus/
+    divergence.cuh:44-612)."""
+
+    def __init__(self, weights, support_indices):
+        self._W = np.ascontiguousarray(weights, np.float64)
+        self._idx = np.ascontiguousarray(support_indices, np.uint32)
+
+    @classmethod
+    def create(cls, domain, query=None, k: int = 32, rbf: int = RBF_POLYHARMONIC3):
+        domain = _f32(domain)
+        query = domain if query is None else _f32(query)
+        D = domain.shape[1]
+        nq = query.shape[0]
+        idx = np.zeros(nq * k, np.uint32)
+        W = np.zeros(nq * k * D)
+        _check(load().naga_divergence_weights(_ptr(domain), domain.shape[0], _ptr(query), nq,
+                                              D, int(k), int(rbf), _ptr(idx), _ptr(W)),
+               "naga::nonlocal_calculus::divergence_operator::")
+        return cls(W.reshape(nq, k, D), idx.reshape(nq, k))
+
+    def weights(self):
+        return self._W
+
+    def support_indices(self):
+        return self._idx
+
+    def apply(self, field, centering_offset=0.0):
+        """out[i] = sum_s sum_d W(d, s, i) (F_d(idx(s, i)) - offset), FP32."""
+        F = _f32(field)
+        n, k, D = self._W.shape
+        if F.ndim != 2 or F.shape[1] != D:
+            raise ValueError("naga::nonlocal_calculus::divergence_operator::"
+                             "Field map has incorrect dimensions.")
+        out = np.zeros(n, np.float32)
+        W32 = _f32(self._W)
+        _check(load().naga_divergence_apply(_ptr(self._idx), _ptr(W32), n, k, D, _ptr(F),
+                                            F.shape[0], float(centering_offset), _ptr(out)),
+               "naga::nonlocal_calculus::divergence_operator::")
+        return out

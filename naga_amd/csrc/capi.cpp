@@ -1,0 +1,458 @@
+// extern "C" entry points (include/naga_amd.h): argument checks, exception
+// to status translation, thread-local error text.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/naga_amd.h"
+#include "engine.hpp"
+#include "setup_kernels.hpp"
+
+using namespace naga_amd;
+
+struct naga_engine {
+    std::unique_ptr<Engine> impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return NAGA_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return NAGA_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return NAGA_DEVICE_ERROR;
+    }
+}
+
+Engine& engine_of(naga_engine* e) {
+    require(e != nullptr && e->impl != nullptr, "null engine handle");
+    return *e->impl;
+}
+
+// ---- synthetic uniform grid (node_providers/uniform_grid_provider.cuh) ----
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint64_t morton(int D, const int64_t* g) {
+    uint64_t key = 0;
+    const int bits = D == 2 ? 32 : 21;
+    for (int b = 0; b < bits; ++b)
+        for (int d = 0; d < D; ++d) key |= ((uint64_t(g[d]) >> b) & 1ull) << (b * D + d);
+    return key;
+}
+
+// ranges/uniform_grid.cuh:55-65, size per dimension ceil(extent/h) + 1 in FP32
+int64_t grid_dims(int D, const double* lo, const double* hi, double h, int64_t* dims) {
+    int64_t n = 1;
+    for (int d = 0; d < D; ++d) {
+        const float ext = float(hi[d]) - float(lo[d]);
+        dims[d] = int64_t(std::ceil(ext / float(h))) + 1;
+        n *= dims[d];
+    }
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* naga_last_error(void) { return g_last_error.c_str(); }
+
+int naga_lattice_tables(int32_t q, double* c, double* w, int32_t* bb, double* K) {
+    return guarded([&] {
+        require(q == 9 || q == 27, "lattice must be 9 or 27");
+        const int D = q == 9 ? 2 : 3;
+        for (int a = 0; a < q; ++a) {
+            for (int d = 0; d < D; ++d)
+                if (c) c[a * D + d] = q == 9 ? Lattice<9>::c(a, d) : Lattice<27>::c(a, d);
+            if (w) w[a] = q == 9 ? Lattice<9>::w(a) : Lattice<27>::w(a);
+            if (bb) bb[a] = q == 9 ? Lattice<9>::bb(a) : Lattice<27>::bb(a);
+            for (int j = 0; j < q; ++j)
+                if (K) K[a * q + j] = collision_matrix_entry(q, a, j);
+        }
+    });
+}
+
+int naga_collide(int32_t q, const double* f, double rho, const double* u, double nu_lat,
+                 double dt_lat, double* df) {
+    return guarded([&] {
+        require(q == 9 || q == 27, "lattice must be 9 or 27");
+        require(f && u && df, "null argument");
+        const CollisionParams p = make_collision_params(q, nu_lat, dt_lat);
+        if (q == 27)
+            collide<27>(f, rho, u, p, df);
+        else
+            collide<9>(f, rho, u, p, df);
+    });
+}
+
+int naga_engine_create(const naga_scene_config* cfg, naga_engine** out) {
+    return guarded([&] {
+        require(cfg != nullptr && out != nullptr, "null argument");
+        SceneConfig c;
+        c.Q = cfg->lattice;
+        c.k = cfg->k;
+        c.kb = cfg->k_boundary;
+        c.rbf = cfg->rbf;
+        c.rbf_boundary = cfg->rbf_boundary;
+        c.pml_ref = cfg->pml_ref_index;
+        c.nu = cfg->nu;
+        c.rho0 = cfg->rho0;
+        c.dt = cfg->dt;
+        c.c = cfg->c;
+        c.h = cfg->h;
+        auto* e = new naga_engine{};
+        try {
+            e->impl = std::make_unique<Engine>(c);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+int naga_engine_destroy(naga_engine* e) {
+    return guarded([&] { delete e; });
+}
+
+int naga_engine_init_domain(naga_engine* e, const float* points, int64_t n_bulk,
+                            int64_t n_layer, int64_t n_boundary, const float* normals,
+                            const float* layer_absorption) {
+    return guarded([&] {
+        engine_of(e).init_domain(points, n_bulk, n_layer, n_boundary, normals,
+                                 layer_absorption);
+    });
+}
+
+int naga_engine_set_pulse(naga_engine* e, const double* center, double amp, double width) {
+    return guarded([&] {
+        require(center != nullptr, "null center");
+        engine_of(e).set_pulse(center, amp, width);
+    });
+}
+
+int naga_engine_set_distributions(naga_engine* e, const double* f) {
+    return guarded([&] {
+        require(f != nullptr, "null distributions");
+        engine_of(e).set_distributions(f);
+    });
+}
+
+int naga_engine_step(naga_engine* e, int32_t nsteps) {
+    return guarded([&] { engine_of(e).step(nsteps); });
+}
+
+int naga_engine_step_timed(naga_engine* e, int32_t nsteps, double* ms) {
+    return guarded([&] {
+        const double t = engine_of(e).step_timed(nsteps);
+        if (ms) *ms = t;
+    });
+}
+
+int naga_engine_profile_step(naga_engine* e, double* ms) {
+    return guarded([&] {
+        require(ms != nullptr, "null output");
+        engine_of(e).profile_step(ms);
+    });
+}
+
+int naga_engine_get_macroscopic(naga_engine* e, float* rho, float* u, double* drho,
+                                int32_t fresh) {
+    return guarded([&] { engine_of(e).get_macroscopic(rho, u, drho, fresh != 0); });
+}
+
+int naga_engine_get_distributions(naga_engine* e, double* f, double* g) {
+    return guarded([&] { engine_of(e).get_distributions(f, g); });
+}
+
+int naga_engine_get_setup(naga_engine* e, uint32_t* idx, float* W, float* filt,
+                          uint32_t* bidx, float* bw) {
+    return guarded([&] { engine_of(e).get_setup(idx, W, filt, bidx, bw); });
+}
+
+int naga_engine_get_pml(naga_engine* e, double* Q1_minus_w, double* Q2) {
+    return guarded([&] { engine_of(e).get_pml(Q1_minus_w, Q2); });
+}
+
+int naga_engine_info(naga_engine* e, int64_t* n_nodes, double* time, int64_t* frame,
+                     double* setup_ms) {
+    return guarded([&] {
+        Engine& en = engine_of(e);
+        if (n_nodes) *n_nodes = en.nodes();
+        if (time) *time = en.time();
+        if (frame) *frame = en.frame();
+        if (setup_ms) *setup_ms = en.setup_ms();
+    });
+}
+
+int64_t naga_uniform_grid_size(int32_t dims, const double* lo, const double* hi, double h) {
+    if ((dims != 2 && dims != 3) || !lo || !hi || !(h > 0)) return -1;
+    int64_t d[3];
+    return grid_dims(dims, lo, hi, h, d);
+}
+
+int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h,
+                      double jitter_frac, uint64_t seed, double layer_thickness,
+                      double peak_absorption, float* points, float* normals, float* sigma,
+                      int64_t* counts) {
+    return guarded([&] {
+        const int D = dims;
+        require(D == 2 || D == 3, "dims must be 2 or 3");
+        require(lo && hi && points && normals && sigma && counts, "null argument");
+        require(h > 0, "grid spacing must be positive");
+        int64_t gd[3];
+        const int64_t N = grid_dims(D, lo, hi, h, gd);
+        double sp[3];
+        for (int d = 0; d < D; ++d) {
+            require(gd[d] >= 2, "grid needs at least two nodes per dimension");
+            sp[d] = (hi[d] - lo[d]) / double(gd[d] - 1);
+        }
+        // grid points with jittered interior nodes (uniform_grid.cuh:89-97)
+        std::vector<float> xyz(N * D);
+        std::vector<uint8_t> isb(N);
+        uint64_t st = seed;
+        int64_t nb = 0;
+        for (int64_t g = 0; g < N; ++g) {
+            int64_t gi[3], rem = g;
+            bool bnd = false;
+            for (int d = 0; d < D; ++d) {
+                gi[d] = rem % gd[d];
+                rem /= gd[d];
+                bnd = bnd || gi[d] == 0 || gi[d] == gd[d] - 1;  // uniform_grid.cuh:242-253
+            }
+            isb[g] = bnd;
+            nb += bnd;
+            for (int d = 0; d < D; ++d) {
+                const double u01 = double(splitmix64(st) >> 11) * 0x1.0p-53;
+                double x = lo[d] + double(gi[d]) * sp[d];
+                if (!bnd) x += (2.0 * u01 - 1.0) * jitter_frac * h;
+                xyz[g * D + d] = float(x);
+            }
+        }
+        const int64_t ni = N - nb;
+        std::vector<float> bpts, ipts;
+        std::vector<int64_t> bg, ig;
+        bpts.reserve(nb * D);
+        ipts.reserve(ni * D);
+        for (int64_t g = 0; g < N; ++g) {
+            auto& dst = isb[g] ? bpts : ipts;
+            dst.insert(dst.end(), xyz.begin() + g * D, xyz.begin() + (g + 1) * D);
+            (isb[g] ? bg : ig).push_back(g);
+        }
+        // layer: exact 1-NN squared distance to the boundary nodes on the GPU
+        // (uniform_grid_provider.cuh:104-127); layer iff d2 <= L^2 in FP32
+        std::vector<float> d2(std::max<int64_t>(ni, 1), 0.0f);
+        std::vector<uint32_t> nn(std::max<int64_t>(ni, 1), 0u);
+        const bool layered = layer_thickness > 0 && ni > 0 && nb > 0;
+        if (layered) {
+            hipStream_t s;
+            NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            try {
+                knn_host(D, bpts.data(), nb, ipts.data(), ni, 1, d2.data(), nn.data(), s);
+            } catch (...) {
+                (void)hipStreamDestroy(s);
+                throw;
+            }
+            (void)hipStreamDestroy(s);
+        }
+        const float Lf = float(layer_thickness);
+        const float L2 = Lf * Lf;
+        auto gridkey = [&](int64_t g) {
+            int64_t gi[3], rem = g;
+            for (int d = 0; d < D; ++d) {
+                gi[d] = rem % gd[d];
+                rem /= gd[d];
+            }
+            return morton(D, gi);
+        };
+        std::vector<std::pair<uint64_t, int64_t>> kb, kl, kbd;
+        for (int64_t t = 0; t < ni; ++t) {
+            auto e = std::make_pair(gridkey(ig[t]), t);
+            (layered && d2[t] <= L2 ? kl : kb).push_back(e);
+        }
+        for (int64_t t = 0; t < nb; ++t) kbd.emplace_back(gridkey(bg[t]), t);
+        std::sort(kb.begin(), kb.end());
+        std::sort(kl.begin(), kl.end());
+        std::sort(kbd.begin(), kbd.end());
+        int64_t o = 0;
+        for (auto& e : kb) std::memcpy(points + D * o++, &ipts[e.second * D], sizeof(float) * D);
+        int64_t t = 0;
+        for (auto& e : kl) {
+            std::memcpy(points + D * o++, &ipts[e.second * D], sizeof(float) * D);
+            // detail/uniform_grid_provider.inl:148-159 in FP32
+            const float dist = std::sqrt(d2[e.second]);
+            const float r = (Lf - dist) / Lf;
+            sigma[t++] = float(peak_absorption) * (r * r);
+        }
+        t = 0;
+        for (auto& e : kbd) {
+            std::memcpy(points + D * o++, &bpts[e.second * D], sizeof(float) * D);
+            int64_t gi[3], rem = bg[e.second];
+            for (int d = 0; d < D; ++d) {
+                gi[d] = rem % gd[d];
+                rem /= gd[d];
+            }
+            // uniform_grid.cuh:255-293, negated to point inward
+            // (uniform_grid_provider.cuh:89-94)
+            float nv[3] = {0.f, 0.f, 0.f};
+            for (int d = 0; d < D; ++d) nv[d] = gi[d] == 0 ? -1.f : gi[d] == gd[d] - 1 ? 1.f : 0.f;
+            float n2 = nv[0] * nv[0];
+            for (int d = 1; d < D; ++d) n2 = nv[d] * nv[d] + n2;
+            const float nrm = std::sqrt(n2);
+            for (int d = 0; d < D; ++d) normals[t * D + d] = -(nv[d] / (nrm == 0.f ? 1.f : nrm));
+            ++t;
+        }
+        counts[0] = int64_t(kb.size());
+        counts[1] = int64_t(kl.size());
+        counts[2] = nb;
+    });
+}
+
+int naga_knn(const float* ref, int64_t n_ref, const float* query, int64_t n_q, int32_t dims,
+             int32_t k, float* d2_out, uint32_t* idx_out) {
+    return guarded([&] {
+        require(ref && query && d2_out && idx_out, "null argument");
+        hipStream_t s;
+        NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            knn_host(dims, ref, n_ref, query, n_q, k, d2_out, idx_out, s);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+    });
+}
+
+int naga_batched_lu_solve_f64(const double* A, int32_t n, int64_t batch, const double* B,
+                              int32_t nrhs, double* X, int32_t* info) {
+    return guarded([&] {
+        require(A && B && X, "null argument");
+        require(n >= 1 && n <= 64, "n must be in [1, 64]");
+        require(nrhs >= 1 && batch >= 0, "bad nrhs or batch");
+        hipStream_t s;
+        NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        DeviceBuffer<double> dA(size_t(batch) * n * n), dB(size_t(batch) * n * nrhs);
+        DeviceBuffer<int> dinfo(batch);
+        dA.upload(A, dA.size(), s);
+        dB.upload(B, dB.size(), s);
+        launch_batched_lu_solve(n, nrhs, batch, dA.get(), dB.get(), dinfo.get(), s);
+        dB.download(X, dB.size(), s);
+        std::vector<int> hi(batch);
+        dinfo.download(hi.data(), batch, s);
+        NAGA_HIP(hipStreamSynchronize(s));
+        (void)hipStreamDestroy(s);
+        for (int64_t b = 0; b < batch; ++b)
+            if (info) info[b] = hi[b];
+    });
+}
+
+int naga_rpim_weights(const float* src, int64_t n_src, const uint32_t* idx, int32_t k,
+                      const float* query, int64_t n_q, int32_t dims, int32_t rbf,
+                      double spacing, double* w) {
+    return guarded([&] {
+        require(src && idx && query && w, "null argument");
+        require(dims == 2 || dims == 3, "dims must be 2 or 3");
+        require(k >= 1 && k + dims + 1 <= 64, "k + dims + 1 must be <= 64");
+        for (int64_t t = 0; t < n_q * k; ++t) require(idx[t] < n_src, "index out of range");
+        hipStream_t s;
+        NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        DeviceBuffer<float> dsrc(n_src * dims), dq(n_q * dims), dw(n_q * k);
+        DeviceBuffer<uint32_t> didx(n_q * k);
+        DeviceBuffer<double> dw64(n_q * k);
+        DeviceBuffer<int> dinfo(n_q);
+        dsrc.upload(src, dsrc.size(), s);
+        dq.upload(query, dq.size(), s);
+        didx.upload(idx, didx.size(), s);
+        launch_rpim_weights(dims, dsrc.get(), dq.get(), n_q, didx.get(), k, rbf, spacing,
+                            dw.get(), dw64.get(), dinfo.get(), s);
+        dw64.download(w, n_q * k, s);
+        std::vector<int> hi(n_q);
+        dinfo.download(hi.data(), n_q, s);
+        NAGA_HIP(hipStreamSynchronize(s));
+        (void)hipStreamDestroy(s);
+        for (int64_t q = 0; q < n_q; ++q)
+            if (hi[q]) fail(kNumericalError, "singular RPIM system");
+    });
+}
+
+int naga_divergence_weights(const float* src, int64_t n_src, const float* query, int64_t n_q,
+                            int32_t dims, int32_t k, int32_t rbf, uint32_t* idx_out,
+                            double* W_out) {
+    return guarded([&] {
+        require(src && query && idx_out && W_out, "null argument");
+        require(dims == 2 || dims == 3, "dims must be 2 or 3");
+        require(k >= 2 && k + dims + 1 <= 64, "k must satisfy 2 <= k, k + dims + 1 <= 64");
+        hipStream_t s;
+        NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        std::vector<float> d2(n_q * k);
+        knn_host(dims, src, n_src, query, n_q, k, d2.data(), idx_out, s);
+        const int64_t nbatch = (n_q + 16383) / 16384;
+        std::vector<double> hbar(std::max<int64_t>(nbatch, 1), 1.0);
+        for (int64_t b = 0; b < nbatch; ++b) {
+            const int64_t e = std::min<int64_t>((b + 1) * 16384, n_q);
+            double acc = 0.0;
+            for (int64_t i = b * 16384; i < e; ++i) acc += std::sqrt(double(d2[i * k + 1]));
+            hbar[b] = acc / double(e - b * 16384);
+        }
+        DeviceBuffer<float> dsrc(n_src * dims), dq(n_q * dims), dd2(n_q * k), dW(n_q * k * dims);
+        DeviceBuffer<uint32_t> didx(n_q * k);
+        DeviceBuffer<double> dW64(n_q * k * dims);
+        DeviceBuffer<int> dinfo(n_q);
+        dsrc.upload(src, dsrc.size(), s);
+        dq.upload(query, dq.size(), s);
+        dd2.upload(d2.data(), d2.size(), s);
+        didx.upload(idx_out, didx.size(), s);
+        auto dh = to_device(hbar, s);
+        launch_divergence_weights(dims, dsrc.get(), dq.get(), n_q, didx.get(), dd2.get(), k,
+                                  rbf, dh.get(), dW.get(), dW64.get(), dinfo.get(), s);
+        dW64.download(W_out, dW64.size(), s);
+        std::vector<int> hi(n_q);
+        dinfo.download(hi.data(), n_q, s);
+        NAGA_HIP(hipStreamSynchronize(s));
+        (void)hipStreamDestroy(s);
+        for (int64_t q = 0; q < n_q; ++q)
+            if (hi[q]) fail(kNumericalError, "singular RBF system");
+    });
+}
+
+int naga_divergence_apply(const uint32_t* idx, const float* W, int64_t n, int32_t k,
+                          int32_t dims, const float* F, int64_t n_f, float off, float* out) {
+    return guarded([&] {
+        require(idx && W && F && out, "null argument");
+        require(dims == 2 || dims == 3, "dims must be 2 or 3");
+        for (int64_t t = 0; t < n * k; ++t) require(idx[t] < n_f, "index out of range");
+        hipStream_t s;
+        NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        DeviceBuffer<uint32_t> didx(n * k);
+        DeviceBuffer<float> dW(n * k * dims), dF(n_f * dims), dout(n);
+        didx.upload(idx, didx.size(), s);
+        dW.upload(W, dW.size(), s);
+        dF.upload(F, dF.size(), s);
+        launch_divergence_apply(dims, n, k, didx.get(), dW.get(), dF.get(), off, dout.get(), s);
+        dout.download(out, n, s);
+        NAGA_HIP(hipStreamSynchronize(s));
+        (void)hipStreamDestroy(s);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,431 @@
+// Engine: setup orchestration and the per-step schedule on one HIP stream.
+#include "engine.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "setup_kernels.hpp"
+
+namespace naga_amd {
+
+namespace {
+
+template <int Q>
+double lat_w(int a) {
+    return Lattice<Q>::w(a);
+}
+
+double weight(int Q, int a) { return Q == 27 ? lat_w<27>(a) : lat_w<9>(a); }
+
+struct Timer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+            .count();
+    }
+};
+
+}  // namespace
+
+void knn_host(int D, const float* ref, int64_t n_ref, const float* query, int64_t n_q,
+              int k, float* d2, uint32_t* idx, hipStream_t st) {
+    require(D == 2 || D == 3, "dimension must be 2 or 3");
+    require(k >= 1 && k <= 64, "k must be in [1, 64]");
+    require(n_ref >= k, "fewer reference points than neighbours requested");
+    CellGrid grid{};
+    std::vector<uint32_t> sorted;
+    std::vector<int64_t> start;
+    bin_points(D, ref, n_ref, grid, sorted, start);
+    DeviceBuffer<float> dref(n_ref * D), dq(n_q * D), dd2(n_q * k);
+    DeviceBuffer<uint32_t> didx(n_q * k);
+    dref.upload(ref, n_ref * D, st);
+    dq.upload(query, n_q * D, st);
+    auto dsorted = to_device(sorted, st);
+    auto dstart = to_device(start, st);
+    launch_knn(grid, dref.get(), dsorted.get(), dstart.get(), dq.get(), n_q, k, dd2.get(),
+               didx.get(), st);
+    dd2.download(d2, n_q * k, st);
+    didx.download(idx, n_q * k, st);
+    NAGA_HIP(hipStreamSynchronize(st));
+}
+
+Engine::Engine(const SceneConfig& cfg) : cfg_(cfg) {
+    require(cfg.Q == 9 || cfg.Q == 27, "lattice must be D2Q9 (9) or D3Q27 (27)");
+    D_ = cfg.Q == 9 ? 2 : 3;
+    RS_ = cfg.Q == 9 ? Lattice<9>::RS : Lattice<27>::RS;
+    require(cfg.k >= 2 && cfg.k + D_ + 1 <= 64, "k must satisfy 2 <= k and k + D + 1 <= 64");
+    require(cfg.kb >= 1 && cfg.kb + D_ + 1 <= 64, "k_boundary must satisfy k_boundary + D + 1 <= 64");
+    require(cfg.rbf == 0 || cfg.rbf == 1, "rbf must be 0 (multiquadric) or 1 (r^3)");
+    require(cfg.rbf_boundary == 0 || cfg.rbf_boundary == 1, "rbf_boundary must be 0 or 1");
+    require(cfg.dt > 0 && cfg.c > 0 && cfg.rho0 > 0 && cfg.h > 0,
+            "dt, c, rho0 and h must be positive");
+    NAGA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    // set_problem_parameters (detail/simulation_engine.cuh:94-124)
+    const double ratio = kCsLat / cfg.c;
+    const double dt_lat = cfg.dt / ratio;
+    const double nu_lat = cfg.nu / ratio;
+    cp_ = make_collision_params(cfg.Q, nu_lat, dt_lat);
+    geo_.dt = float(dt_lat);
+    geo_.rho0 = float(cfg.rho0);
+    geo_.vel_scale = float(cfg.c / kCsLat);
+    geo_.filter = 0.75f;
+    geo_.k = cfg.k;
+    geo_.kb = cfg.kb;
+    geo_.pml_ref = cfg.pml_ref ? 1 : 0;
+}
+
+Engine::~Engine() {
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+void Engine::require_domain() const {
+    require(ready_, "init_domain must be called first");
+}
+
+void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_t nbnd,
+                         const float* normals, const float* sigma) {
+    Timer timer;
+    const int D = D_, Q = cfg_.Q, k = cfg_.k, kb = cfg_.kb;
+    // shape checks of detail/simulation_engine.cuh:128-167
+    require(pts != nullptr, "The number of points in the simulation domain does not match "
+                            "the sum of the number of bulk, layer, and boundary points.");
+    require(nbulk >= 0 && nlayer >= 0 && nbnd >= 0, "negative node count");
+    require(nbnd == 0 || normals != nullptr,
+            "The number of boundary normals does not match the number of boundary points.");
+    require(nlayer == 0 || sigma != nullptr,
+            "The number of layer absorption coefficients does not match the number of layer "
+            "points.");
+    const int64_t N = nbulk + nlayer + nbnd;
+    const int64_t nfluid = nbulk + nlayer;
+    require(N >= k, "fewer nodes than the divergence stencil width");
+    require(nbnd == 0 || nfluid >= kb, "fewer fluid nodes than the boundary stencil width");
+    require(N < (int64_t(1) << 31), "at most 2^31 - 1 nodes per scene");
+    N_ = N;
+    nbulk_ = nbulk;
+    nlayer_ = nlayer;
+    nbnd_ = nbnd;
+    nfluid_ = nfluid;
+    ready_ = false;
+    hipStream_t st = stream_;
+
+    pts_.allocate(N * D);
+    pts_.upload(pts, N * D, st);
+
+    // divergence operator over all nodes (init_domain :169-181)
+    {
+        CellGrid grid{};
+        std::vector<uint32_t> sorted;
+        std::vector<int64_t> start;
+        bin_points(D, pts, N, grid, sorted, start);
+        auto dsorted = to_device(sorted, st);
+        auto dstart = to_device(start, st);
+        DeviceBuffer<float> d2(N * k);
+        idx_.allocate(N * k);
+        launch_knn(grid, pts_.get(), dsorted.get(), dstart.get(), pts_.get(), N, k, d2.get(),
+                   idx_.get(), st);
+        // multiquadric shape parameter: mean nearest distance per 16384-query
+        // batch (nonlocal_calculus/divergence.cuh:118-160, operators.cuh:96-117)
+        const int64_t nbatch = (N + 16383) / 16384;
+        std::vector<double> hbar(nbatch, 1.0);
+        if (cfg_.rbf == 0) {
+            std::vector<float> hd2(N * k);
+            d2.download(hd2.data(), N * k, st);
+            NAGA_HIP(hipStreamSynchronize(st));
+            for (int64_t b = 0; b < nbatch; ++b) {
+                const int64_t e = std::min<int64_t>((b + 1) * 16384, N);
+                double s = 0.0;
+                for (int64_t i = b * 16384; i < e; ++i) s += std::sqrt(double(hd2[i * k + 1]));
+                hbar[b] = s / double(e - b * 16384);
+            }
+        }
+        auto dhbar = to_device(hbar, st);
+        W_.allocate(N * k * D);
+        DeviceBuffer<int> info(N);
+        launch_divergence_weights(D, pts_.get(), pts_.get(), N, idx_.get(), d2.get(), k,
+                                  cfg_.rbf, dhbar.get(), W_.get(), nullptr, info.get(), st);
+        std::vector<int> hinfo(N);
+        info.download(hinfo.data(), N, st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < N; ++i)
+            if (hinfo[i])
+                fail(kNumericalError, "singular RBF system while building the divergence "
+                                      "operator at node " + std::to_string(i));
+    }
+    filt_.allocate(N * k);
+    launch_filter_weights(D, pts_.get(), N, idx_.get(), k, cfg_.h, filt_.get(), st);
+    slab_.allocate(nfluid * k * (1 + D));
+    launch_make_slab(nfluid, k, D, idx_.get(), W_.get(), slab_.get(), st);
+    fslab_.allocate(nfluid * k * 2);
+    launch_make_slab(nfluid, k, 1, idx_.get(), filt_.get(), fslab_.get(), st);
+
+    // boundary interpolator (init_domain :223-258): kNN of boundary nodes
+    // among the bulk and layer nodes, RPIM weights at spacing h
+    if (nbnd > 0) {
+        CellGrid grid{};
+        std::vector<uint32_t> sorted;
+        std::vector<int64_t> start;
+        bin_points(D, pts, nfluid, grid, sorted, start);
+        auto dsorted = to_device(sorted, st);
+        auto dstart = to_device(start, st);
+        DeviceBuffer<float> bd2(nbnd * kb);
+        bidx_.allocate(nbnd * kb);
+        launch_knn(grid, pts_.get(), dsorted.get(), dstart.get(), pts_.get() + nfluid * D, nbnd,
+                   kb, bd2.get(), bidx_.get(), st);
+        bw_.allocate(nbnd * kb);
+        DeviceBuffer<int> info(nbnd);
+        launch_rpim_weights(D, pts_.get(), pts_.get() + nfluid * D, nbnd, bidx_.get(), kb,
+                            cfg_.rbf_boundary, cfg_.h, bw_.get(), nullptr, info.get(), st);
+        std::vector<int> hinfo(nbnd);
+        info.download(hinfo.data(), nbnd, st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < nbnd; ++i)
+            if (hinfo[i])
+                fail(kNumericalError, "singular RBF system in the boundary interpolator");
+        // fluid nodes read by the interpolator keep their collision
+        // increment aside for it
+        std::vector<uint32_t> hb(nbnd * kb);
+        bidx_.download(hb.data(), hb.size(), st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        std::vector<uint8_t> src(nfluid, 0);
+        for (uint32_t j : hb) src[j] = 1;
+        bsrc_ = to_device(src, st);
+        bslab_.allocate(nbnd * kb * 2);
+        launch_make_slab(nbnd, kb, 1, bidx_.get(), bw_.get(), bslab_.get(), st);
+        normals_.allocate(nbnd * D);
+        normals_.upload(normals, nbnd * D, st);
+    }
+    // fluid nodes whose stencil reaches the absorbing layer
+    n_pml_ = 0;
+    if (nlayer > 0) {
+        DeviceBuffer<uint8_t> flag(nfluid);
+        launch_flag_neighbours(nfluid, k, idx_.get(), nbulk, nfluid, flag.get(), st);
+        std::vector<uint8_t> hflag(nfluid);
+        flag.download(hflag.data(), nfluid, st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        std::vector<int32_t> list;
+        for (int64_t i = 0; i < nfluid; ++i)
+            if (hflag[i]) list.push_back(int32_t(i));
+        n_pml_ = int64_t(list.size());
+        pml_list_ = to_device(list, st);
+        sigma_.allocate(nlayer);
+        sigma_.upload(sigma, nlayer, st);
+    }
+    // state
+    const size_t rows = size_t(N) * RS_;
+    if (nbnd == 0) {
+        std::vector<uint8_t> none(std::max<int64_t>(nfluid, 1), 0);
+        bsrc_ = to_device(none, st);
+    }
+    for (DeviceBuffer<float>* b : {&g_, &A_, &B_, &acc_, &E_, &Qp_, &Q2v_, &coll_}) {
+        b->allocate(rows);
+        b->zero(st);
+    }
+    rho_.allocate(N);
+    u_.allocate(N * D);
+    nparts_ = int(std::min<int64_t>(std::max<int64_t>((nlayer + 4095) / 4096, 1), 512));
+    partial_.allocate(size_t(nparts_) * 2 * Q);
+    partial_.zero(st);
+    means_.allocate(2 * Q);
+    means_.zero(st);
+
+    geo_.N = N;
+    geo_.nbulk = nbulk;
+    geo_.nlayer = nlayer;
+    geo_.nfluid = nfluid;
+    ready_ = true;
+    time_ = 0.0;
+    frame_ = 0;
+    refresh_macroscopic_state();
+    NAGA_HIP(hipStreamSynchronize(st));
+    setup_ms_ = timer.ms();
+}
+
+StepBuffers Engine::buffers() {
+    StepBuffers S{};
+    S.g = g_.get();
+    S.A = A_.get();
+    S.B = B_.get();
+    S.acc = acc_.get();
+    S.E = E_.get();
+    S.Qp = Qp_.get();
+    S.Q2v = Q2v_.get();
+    S.slab = slab_.get();
+    S.fslab = fslab_.get();
+    S.bslab = bslab_.get();
+    S.sigma = sigma_.get();
+    S.normals = normals_.get();
+    S.coll = coll_.get();
+    S.bsrc = bsrc_.get();
+    S.pml_list = pml_list_.get();
+    S.n_pml = n_pml_;
+    S.rho = rho_.get();
+    S.u = u_.get();
+    S.partial = partial_.get();
+    S.means = means_.get();
+    S.nparts = nparts_;
+    return S;
+}
+
+void Engine::refresh_macroscopic_state() {
+    if (cfg_.Q == 27)
+        launch_macroscopic<27>(geo_, g_.get(), rho_.get(), nullptr, u_.get(), stream_);
+    else
+        launch_macroscopic<9>(geo_, g_.get(), rho_.get(), nullptr, u_.get(), stream_);
+}
+
+void Engine::launch_one_step(hipEvent_t* ev) {
+    const StepBuffers S = buffers();
+    if (cfg_.Q == 27)
+        launch_step<27>(geo_, S, cp_, stream_, ev);
+    else
+        launch_step<9>(geo_, S, cp_, stream_, ev);
+    time_ += cfg_.dt;
+    ++frame_;
+}
+
+void Engine::step(int nsteps) {
+    require_domain();
+    require(nsteps >= 0, "nsteps must be >= 0");
+    for (int n = 0; n < nsteps; ++n) launch_one_step(nullptr);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+}
+
+double Engine::step_timed(int nsteps) {
+    require_domain();
+    hipEvent_t a, b;
+    NAGA_HIP(hipEventCreate(&a));
+    NAGA_HIP(hipEventCreate(&b));
+    NAGA_HIP(hipEventRecord(a, stream_));
+    for (int n = 0; n < nsteps; ++n) launch_one_step(nullptr);
+    NAGA_HIP(hipEventRecord(b, stream_));
+    NAGA_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    NAGA_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return double(ms);
+}
+
+void Engine::profile_step(double* ms) {
+    require_domain();
+    hipEvent_t ev[kStepKernels + 1];
+    for (auto& e : ev) NAGA_HIP(hipEventCreate(&e));
+    launch_one_step(ev);
+    NAGA_HIP(hipEventSynchronize(ev[kStepKernels]));
+    for (int t = 0; t < kStepKernels; ++t) {
+        float v = 0.0f;
+        NAGA_HIP(hipEventElapsedTime(&v, ev[t], ev[t + 1]));
+        ms[t] = double(v);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+}
+
+void Engine::get_macroscopic(float* rho, float* u, double* drho, bool fresh) {
+    require_domain();
+    const int64_t N = N_;
+    hipStream_t st = stream_;
+    if (fresh) {
+        DeviceBuffer<float> r(N), uu(N * D_);
+        DeviceBuffer<double> dr(N);
+        if (cfg_.Q == 27)
+            launch_macroscopic<27>(geo_, g_.get(), r.get(), dr.get(), uu.get(), st);
+        else
+            launch_macroscopic<9>(geo_, g_.get(), r.get(), dr.get(), uu.get(), st);
+        if (rho) r.download(rho, N, st);
+        if (u) uu.download(u, N * D_, st);
+        if (drho) dr.download(drho, N, st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    std::vector<float> hr(N);
+    rho_.download(hr.data(), N, st);
+    if (u) u_.download(u, N * D_, st);
+    NAGA_HIP(hipStreamSynchronize(st));
+    if (rho) std::memcpy(rho, hr.data(), sizeof(float) * N);
+    if (drho)
+        for (int64_t i = 0; i < N; ++i) drho[i] = double(hr[i]) - cfg_.rho0;
+}
+
+void Engine::get_distributions(double* f, double* g) {
+    require_domain();
+    const int Q = cfg_.Q;
+    std::vector<float> rows(size_t(N_) * RS_);
+    g_.download(rows.data(), rows.size(), stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    for (int a = 0; a < Q; ++a) {
+        const double w = weight(Q, a);
+        for (int64_t i = 0; i < N_; ++i) {
+            const double v = double(rows[size_t(i) * RS_ + a]);
+            if (g) g[a * N_ + i] = v;
+            if (f) f[a * N_ + i] = w + v;
+        }
+    }
+}
+
+void Engine::set_distributions(const double* f) {
+    require_domain();
+    const int Q = cfg_.Q;
+    std::vector<float> rows(size_t(N_) * RS_, 0.0f);
+    for (int a = 0; a < Q; ++a) {
+        const double w = weight(Q, a);
+        for (int64_t i = 0; i < N_; ++i) rows[size_t(i) * RS_ + a] = float(f[a * N_ + i] - w);
+    }
+    g_.upload(rows.data(), rows.size(), stream_);
+    refresh_macroscopic_state();
+    NAGA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::set_pulse(const double* center, double amp, double width) {
+    require_domain();
+    require(width > 0, "pulse width must be positive");
+    const int Q = cfg_.Q, D = D_;
+    std::vector<float> hp(size_t(N_) * D);
+    pts_.download(hp.data(), hp.size(), stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    std::vector<float> rows(size_t(N_) * RS_, 0.0f);
+    for (int64_t i = 0; i < N_; ++i) {
+        double r2 = 0.0;
+        for (int d = 0; d < D; ++d) {
+            const double t = double(hp[i * D + d]) - center[d];
+            r2 += t * t;
+        }
+        const double p = amp * std::exp(-r2 / (2.0 * width * width));
+        for (int a = 0; a < Q; ++a) rows[size_t(i) * RS_ + a] = float(weight(Q, a) * p);
+    }
+    g_.upload(rows.data(), rows.size(), stream_);
+    refresh_macroscopic_state();
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    time_ = 0.0;
+    frame_ = 0;
+}
+
+void Engine::get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, float* bw) {
+    require_domain();
+    const int k = cfg_.k, kb = cfg_.kb;
+    if (idx) idx_.download(idx, N_ * k, stream_);
+    if (W) W_.download(W, N_ * k * D_, stream_);
+    if (filt) filt_.download(filt, N_ * k, stream_);
+    if (bidx && nbnd_) bidx_.download(bidx, nbnd_ * kb, stream_);
+    if (bw && nbnd_) bw_.download(bw, nbnd_ * kb, stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::get_pml(double* E, double* Qp) {
+    require_domain();
+    const int Q = cfg_.Q;
+    std::vector<float> e(size_t(N_) * RS_), q(size_t(N_) * RS_);
+    E_.download(e.data(), e.size(), stream_);
+    Qp_.download(q.data(), q.size(), stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    for (int a = 0; a < Q; ++a)
+        for (int64_t i = 0; i < N_; ++i) {
+            if (E) E[a * N_ + i] = double(e[size_t(i) * RS_ + a]);
+            if (Qp) Qp[a * N_ + i] = double(q[size_t(i) * RS_ + a]);
+        }
+}
+
+}  // namespace naga_amd

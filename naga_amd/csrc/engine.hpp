@@ -1,0 +1,86 @@
+// One scene of the nonlocal LBM resident on the GPU: setup products, state
+// and the step schedule.  Mirrors detail::simulation_engine<Lattice>
+// (detail/simulation_engine.cuh:85-1486) behind the C-ABI.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "device_buffer.hpp"
+#include "step_kernels.hpp"
+
+namespace naga_amd {
+
+struct SceneConfig {
+    int Q = 27;
+    int k = 32;
+    int kb = 32;
+    int rbf = 1;
+    int rbf_boundary = 1;
+    int pml_ref = 1;
+    double nu = 0.0, rho0 = 1.0, dt = 0.0, c = kCsLat, h = 0.01;
+};
+
+class Engine {
+  public:
+    explicit Engine(const SceneConfig& cfg);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_t nbnd,
+                     const float* normals, const float* sigma);
+    void set_pulse(const double* center, double amp, double width);
+    void set_distributions(const double* f);
+    void step(int nsteps);
+    double step_timed(int nsteps);
+    void profile_step(double* ms);
+    void get_macroscopic(float* rho, float* u, double* drho, bool fresh);
+    void get_distributions(double* f, double* g);
+    void get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, float* bw);
+    void get_pml(double* E, double* Qp);
+
+    int64_t nodes() const { return N_; }
+    double time() const { return time_; }
+    int64_t frame() const { return frame_; }
+    double setup_ms() const { return setup_ms_; }
+    int lattice() const { return cfg_.Q; }
+    int dims() const { return D_; }
+    hipStream_t stream() const { return stream_; }
+
+  private:
+    void require_domain() const;
+    void launch_one_step(hipEvent_t* ev);
+    void refresh_macroscopic_state();
+    StepBuffers buffers();
+
+    SceneConfig cfg_;
+    int D_ = 3, RS_ = 32;
+    int64_t N_ = 0, nbulk_ = 0, nlayer_ = 0, nbnd_ = 0, nfluid_ = 0;
+    hipStream_t stream_ = nullptr;
+    StepGeom geo_{};
+    CollisionParams cp_{};
+    double time_ = 0.0, setup_ms_ = 0.0;
+    int64_t frame_ = 0;
+    bool ready_ = false;
+
+    DeviceBuffer<float> pts_, normals_, sigma_;
+    DeviceBuffer<float> slab_, fslab_, bslab_;
+    DeviceBuffer<uint32_t> idx_, bidx_;
+    DeviceBuffer<float> W_, filt_, bw_;
+    DeviceBuffer<int32_t> pml_list_;
+    DeviceBuffer<uint8_t> bsrc_;
+    DeviceBuffer<float> coll_;
+    DeviceBuffer<float> g_, A_, B_, acc_, E_, Qp_, Q2v_;
+    DeviceBuffer<float> rho_, u_;
+    DeviceBuffer<double> partial_;
+    DeviceBuffer<float> means_;
+    int nparts_ = 1;
+    int64_t n_pml_ = 0;
+};
+
+// Exact kNN of queries among ref (all host arrays), on the GPU.
+void knn_host(int D, const float* ref, int64_t n_ref, const float* query,
+              int64_t n_q, int k, float* d2, uint32_t* idx, hipStream_t stream);
+
+}  // namespace naga_amd

@@ -1,0 +1,57 @@
+// Common definitions for the MI355X nonlocal-LBM library: status codes,
+// error text, HIP call checking, host/device qualifiers.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define NAGA_HD __host__ __device__
+#define NAGA_DEV __device__
+#else
+#define NAGA_HD
+#define NAGA_DEV
+#endif
+
+namespace naga_amd {
+
+// Status codes of the C-ABI (include/naga_amd.h).
+enum Status : int {
+    kOk = 0,
+    kInvalidArgument = 1,
+    kOutOfMemory = 2,
+    kDeviceError = 3,
+    kNumericalError = 4,
+    kNotSupported = 5,
+};
+
+// Exceptions carry a status; the C-ABI layer turns them into codes plus
+// thread-local error text.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+[[noreturn]] inline void fail(int status, const std::string& msg) {
+    throw Error(status, msg);
+}
+
+inline void require(bool ok, const std::string& msg) {
+    if (!ok) fail(kInvalidArgument, msg);
+}
+
+#if defined(__HIPCC__)
+inline void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    int status = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+                     ? kOutOfMemory
+                     : kDeviceError;
+    fail(status, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define NAGA_HIP(call) ::naga_amd::hip_check((call), #call)
+#endif
+
+}  // namespace naga_amd

@@ -1,0 +1,698 @@
+// Setup kernels (see setup_kernels.hpp).
+#include <cmath>
+#include <vector>
+
+#include "setup_kernels.hpp"
+
+namespace naga_amd {
+
+namespace {
+
+constexpr uint64_t kMaxKey = ~0ull;
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const int lo = __shfl(int(uint32_t(v)), src);
+    const int hi = __shfl(int(uint32_t(v >> 32)), src);
+    return (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const int lo = __shfl_xor(int(uint32_t(v)), m);
+    const int hi = __shfl_xor(int(uint32_t(v >> 32)), m);
+    return (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo);
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) {
+    return __longlong_as_double(
+        (long long)shfl_xor64(uint64_t(__double_as_longlong(v)), m));
+}
+
+// value of lane `src` (wave-uniform) for every lane
+__device__ __forceinline__ double bcast_d(double v, int src) {
+    const int s = __builtin_amdgcn_readfirstlane(src);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(int(uint32_t(b)), s);
+    const int hi = __builtin_amdgcn_readlane(int(uint32_t(uint64_t(b) >> 32)), s);
+    return __longlong_as_double((long long)((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
+}
+
+// FP32 squared distance, fixed order, no contraction (the kNN key).
+__device__ __forceinline__ float dist2_f32(int D, const float* a, const float* b) {
+#pragma clang fp contract(off)
+    float s = (a[0] - b[0]) * (a[0] - b[0]);
+    for (int d = 1; d < D; ++d) {
+        const float t = (a[d] - b[d]) * (a[d] - b[d]);
+        s = t + s;
+    }
+    return s;
+}
+
+// bitonic sort of one key per lane across the wavefront, ascending
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int kk = 2; kk <= 64; kk <<= 1)
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            const uint64_t p = shfl_xor64(v, j);
+            const bool up = (lane & kk) == 0, lower = (lane & j) == 0;
+            v = (lower == up) ? (p < v ? p : v) : (p > v ? p : v);
+        }
+    return v;
+}
+
+// sort a bitonic sequence ascending
+__device__ __forceinline__ uint64_t wave_merge64(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) {
+        const uint64_t p = shfl_xor64(v, j);
+        const bool lower = (lane & j) == 0;
+        v = lower ? (p < v ? p : v) : (p > v ? p : v);
+    }
+    return v;
+}
+
+// Merge the candidates of one 64-wide batch into the running top list
+// (lane t holds the t-th smallest key).
+__device__ __forceinline__ void knn_merge(uint64_t key, uint64_t& top, uint64_t& kth, int k) {
+    if (!__any(key < kth)) return;
+    const int lane = threadIdx.x & 63;
+    const uint64_t sorted = wave_sort64(key);
+    const uint64_t rev = shfl64(sorted, 63 - lane);
+    top = wave_merge64(top < rev ? top : rev);
+    kth = shfl64(top, k - 1);
+}
+
+__global__ __launch_bounds__(256) void k_knn(CellGrid G, const float* ref,
+                                             const uint32_t* sorted,
+                                             const int64_t* start,
+                                             const float* query, int64_t n_q,
+                                             int k, float* d2o, uint32_t* idxo) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    if (q >= n_q) return;
+    const int D = G.D;
+    float x[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < D; ++d) x[d] = query[q * D + d];
+    int64_t qc[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+        int64_t c = int64_t(floor((double(x[d]) - G.lo[d]) / G.w));
+        qc[d] = c < 0 ? 0 : c >= G.dims[d] ? G.dims[d] - 1 : c;
+    }
+    uint64_t top = kMaxKey, kth = kMaxKey;
+    for (int64_t r = 0;; ++r) {
+        const int64_t zr = D == 3 ? r : 0;
+        for (int64_t dz = -zr; dz <= zr; ++dz) {
+            const int64_t cz = qc[2] + dz;
+            if (cz < 0 || cz >= G.dims[2]) continue;
+            for (int64_t dy = -r; dy <= r; ++dy) {
+                const int64_t cy = qc[1] + dy;
+                if (cy < 0 || cy >= G.dims[1]) continue;
+                const bool face = (dy == r || dy == -r) || (D == 3 && (dz == r || dz == -r));
+                const int64_t row = G.dims[0] * (cy + G.dims[1] * cz);
+                int64_t seg[2][2];
+                int nseg = 0;
+                if (face) {
+                    int64_t a = qc[0] - r, b = qc[0] + r;
+                    if (a < 0) a = 0;
+                    if (b >= G.dims[0]) b = G.dims[0] - 1;
+                    seg[nseg][0] = a;
+                    seg[nseg++][1] = b;
+                } else {
+                    if (qc[0] - r >= 0) {
+                        seg[nseg][0] = qc[0] - r;
+                        seg[nseg++][1] = qc[0] - r;
+                    }
+                    if (qc[0] + r < G.dims[0]) {
+                        seg[nseg][0] = qc[0] + r;
+                        seg[nseg++][1] = qc[0] + r;
+                    }
+                }
+                for (int t = 0; t < nseg; ++t) {
+                    const int64_t p0 = start[row + seg[t][0]];
+                    const int64_t p1 = start[row + seg[t][1] + 1];
+                    for (int64_t base = p0; base < p1; base += 64) {
+                        const int64_t p = base + lane;
+                        uint64_t key = kMaxKey;
+                        if (p < p1) {
+                            const uint32_t j = sorted[p];
+                            const float dd = dist2_f32(D, x, ref + int64_t(j) * D);
+                            key = (uint64_t(__float_as_uint(dd)) << 32) | j;
+                        }
+                        knn_merge(key, top, kth, k);
+                    }
+                }
+            }
+        }
+        double bound = 1e300;
+        bool all = true;
+        for (int d = 0; d < D; ++d) {
+            if (qc[d] - r > 0) {
+                const double t = double(x[d]) - (G.lo[d] + double(qc[d] - r) * G.w);
+                bound = t < bound ? t : bound;
+                all = false;
+            }
+            if (qc[d] + r < G.dims[d] - 1) {
+                const double t = (G.lo[d] + double(qc[d] + r + 1) * G.w) - double(x[d]);
+                bound = t < bound ? t : bound;
+                all = false;
+            }
+        }
+        if (all) break;
+        if (kth != kMaxKey && bound > 0.0) {
+            const double kd = double(__uint_as_float(uint32_t(kth >> 32)));
+            if (kd < bound * bound * (1.0 - 1e-5)) break;
+        }
+    }
+    if (lane < k) {
+        d2o[q * k + lane] = __uint_as_float(uint32_t(top >> 32));
+        idxo[q * k + lane] = uint32_t(top);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront LU with partial pivoting (Golub & Van Loan, Alg. 3.4.1).  Lane r
+// holds row r of A (row[c] = A[r][c]) and its right-hand sides.  Row swaps
+// are tracked as positions instead of moving data; ties in |pivot| go to the
+// smallest current position, as a row-swapping LU scanning downwards picks.
+// On return the lane at position p holds unknown p in rhs[].
+// Returns the lane's position, or -1 for a zero pivot (wave-uniform).
+// ---------------------------------------------------------------------------
+template <int NMAX, int NR>
+__device__ int wave_lu_solve(double (&row)[NMAX], double (&rhs)[NR], int n) {
+    const int lane = threadIdx.x & 63;
+    int pos = lane;
+    bool done = lane >= n;
+    bool singular = false;
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        if (j >= n) break;
+        double v = done ? -1.0 : fabs(row[j]);
+        int bp = done ? 1 << 20 : pos, bl = lane;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const double ov = shfl_xor_d(v, o);
+            const int op = __shfl_xor(bp, o), ol = __shfl_xor(bl, o);
+            if (ov > v || (ov == v && op < bp)) {
+                v = ov;
+                bp = op;
+                bl = ol;
+            }
+        }
+        const int p = __builtin_amdgcn_readfirstlane(bl);
+        if (!(v > 0.0)) singular = true;
+        const int pos_p = __builtin_amdgcn_readlane(pos, p);
+        if (pos == j) pos = pos_p;
+        if (lane == p) pos = j;
+        const double piv = bcast_d(row[j], p);
+        double prhs[NR];
+#pragma unroll
+        for (int m = 0; m < NR; ++m) prhs[m] = bcast_d(rhs[m], p);
+        const bool upd = !done && lane != p;
+        const double l = upd ? row[j] / piv : 0.0;
+#pragma unroll
+        for (int c = j + 1; c < NMAX; ++c) {
+            const double pc = bcast_d(row[c], p);
+            if (upd) row[c] -= l * pc;
+        }
+        if (upd) {
+            row[j] = l;
+#pragma unroll
+            for (int m = 0; m < NR; ++m) rhs[m] -= l * prhs[m];
+        }
+        if (lane == p) done = true;
+    }
+    if (singular) return -1;
+    // back substitution, unknown j lives in the lane at position j
+    double s[NR];
+#pragma unroll
+    for (int m = 0; m < NR; ++m) s[m] = rhs[m];
+#pragma unroll
+    for (int j = NMAX - 1; j >= 0; --j) {
+        if (j >= n) continue;
+        const unsigned long long mask = __ballot(pos == j && lane < n);
+        const int lj = __builtin_amdgcn_readfirstlane(int(__ffsll((long long)mask) - 1));
+        const double ujj = bcast_d(row[j], lj);
+#pragma unroll
+        for (int m = 0; m < NR; ++m) {
+            const double xj = bcast_d(s[m], lj) / ujj;
+            if (pos == j) rhs[m] = xj;
+            if (pos < j && lane < n) s[m] -= row[j] * xj;
+        }
+    }
+    return lane < n ? pos : 64;
+}
+
+__constant__ double c_quad_unit[64][3];  // unscaled quadrature points
+__constant__ double c_quad_rw[4];        // radial Gauss weights
+
+__device__ __forceinline__ double dist2_f64(int D, const double* a, const double* b) {
+    double s = (a[0] - b[0]) * (a[0] - b[0]);
+    for (int d = 1; d < D; ++d) s = (a[d] - b[d]) * (a[d] - b[d]) + s;
+    return s;
+}
+
+__device__ __forceinline__ double rbf_eval(int rbf, double r2, double spacing) {
+    if (rbf == 1) return r2 * sqrt(r2);
+    const double c = spacing * 0.1;
+    return sqrt(r2 + c * c);
+}
+
+// Row r of the augmented RBF matrix [[Phi, P], [P^T, 0]] in raw coordinates
+// (interpolation/detail/radial_point_method.cuh:349-398).
+template <int NMAX>
+__device__ void g0_row(int D, const double (*xs)[3], int k, int rbf,
+                       double spacing, int r, double (&row)[NMAX]) {
+    const int n = k + D + 1;
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) {
+        double v = 0.0;
+        if (r < n && c < n) {
+            if (r < k && c < k)
+                v = rbf_eval(rbf, dist2_f64(D, xs[r], xs[c]), spacing);
+            else if (r < k)
+                v = (c == k) ? 1.0 : xs[r][c - k - 1];
+            else if (c < k)
+                v = (r == k) ? 1.0 : xs[c][r - k - 1];
+        }
+        row[c] = v;
+    }
+}
+
+// entry r of G_x (:400-433)
+__device__ __forceinline__ double gx_entry(int D, const double (*xs)[3], int k,
+                                           int rbf, double spacing, int r,
+                                           const double* xq) {
+    if (r < k) return rbf_eval(rbf, dist2_f64(D, xs[r], xq), spacing);
+    if (r == k) return 1.0;
+    return xq[r - k - 1];
+}
+
+// antisymmetric w4 B-spline kernel times the quadrature weight
+// (nonlocal_calculus/detail/divergence.inl:55-133, :167-201)
+__device__ void quad_term(int D, double delta, const double* xi, const double* xq,
+                          int q, double* aw) {
+    double dx[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < D; ++d) dx[d] = xq[d] - xi[d];
+    const double rxy = sqrt(dx[1] * dx[1] + dx[0] * dx[0]);
+    const double r = D == 3 ? sqrt(dx[2] * dx[2] + (dx[1] * dx[1] + dx[0] * dx[0])) : rxy;
+    double g;
+    if (2.0 * r / delta < 1.0)
+        g = r * (2.0 * delta - 3.0 * r);
+    else if (2.0 * r / delta < 2.0)
+        g = (delta - r) * (delta - r);
+    else
+        g = 0.0;
+    double qw = c_quad_rw[q % 4] * delta / 2.0;
+    qw *= 2.0 * M_PI / 4.0;
+    if (D == 3) qw *= M_PI / 4.0;
+    if (D == 2) {
+        const double C = (240.0 / 7.0) / (M_PI * pow(delta, 5.0));
+        qw *= r;
+        aw[0] = C * g * (dx[0] / r) * qw;
+        aw[1] = C * g * (dx[1] / r) * qw;
+    } else {
+        const double C = 48.0 / (M_PI * pow(delta, 6.0));
+        const double sp = rxy / r, cp = dx[2] / r;
+        const double ct = rxy != 0.0 ? dx[0] / rxy : 0.0;
+        const double st = rxy != 0.0 ? dx[1] / rxy : 0.0;
+        qw *= r * rxy;
+        aw[0] = C * g * sp * ct * qw;
+        aw[1] = C * g * sp * st * qw;
+        aw[2] = C * g * cp * qw;
+    }
+}
+
+// One wavefront per query node: assemble G0, the D quadrature right-hand
+// sides b_d = sum_q alpha_d(q) w(q) G_x(q), solve G0 y_d = b_d in FP64 and
+// write W(d, s) = y_d[s] (+ sum_q alpha_d w for s = 0).  By linearity this
+// equals the reference's per-quadrature-point RPIM weights summed against
+// the kernel (detail/divergence.inl:203-215) with D solves instead of Nq.
+template <int NMAX, int D>
+__global__ __launch_bounds__(64) void k_divw(const float* src, const float* query,
+                                             int64_t n_q, const uint32_t* idx,
+                                             const float* d2, int k, int rbf,
+                                             const double* hbar, float* W,
+                                             double* W64, int* info) {
+    __shared__ double xs[64][3];
+    const int lane = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    if (i >= n_q) return;
+    const int n = k + D + 1;
+    if (lane < k) {
+        const int64_t j = idx[i * k + lane];
+        for (int d = 0; d < D; ++d) xs[lane][d] = double(src[j * D + d]);
+    }
+    __syncthreads();
+    double xi[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < D; ++d) xi[d] = double(query[i * D + d]);
+    const double delta = 0.05 * sqrt(double(d2[i * k + 1]));
+    const double spacing = hbar[i / 16384];
+    double row[NMAX];
+    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row);
+    double b[D], S[D];
+    for (int d = 0; d < D; ++d) b[d] = S[d] = 0.0;
+    const int Nq = D == 2 ? 16 : 64;
+    for (int q = 0; q < Nq; ++q) {
+        double xq[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < D; ++d) xq[d] = c_quad_unit[q][d] * delta + xi[d];
+        double aw[3];
+        quad_term(D, delta, xi, xq, q, aw);
+        const double gx = lane < n ? gx_entry(D, xs, k, rbf, spacing, lane, xq) : 0.0;
+        for (int d = 0; d < D; ++d) {
+            b[d] += aw[d] * gx;
+            S[d] += aw[d];
+        }
+    }
+    const int pos = wave_lu_solve<NMAX, D>(row, b, n);
+    if (pos < 0) {
+        if (lane == 0) info[i] = 1;
+        return;
+    }
+    if (lane == 0) info[i] = 0;
+    if (pos < k) {
+        for (int d = 0; d < D; ++d) {
+            const double v = b[d] + (pos == 0 ? S[d] : 0.0);
+            W[(i * k + pos) * D + d] = float(v);
+            if (W64) W64[(i * k + pos) * D + d] = v;
+        }
+    }
+}
+
+template <int NMAX, int D>
+__global__ __launch_bounds__(64) void k_rpim(const float* src, const float* query,
+                                             int64_t n_q, const uint32_t* idx,
+                                             int k, int rbf, double spacing,
+                                             float* w, double* w64, int* info) {
+    __shared__ double xs[64][3];
+    const int lane = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    if (i >= n_q) return;
+    const int n = k + D + 1;
+    if (lane < k) {
+        const int64_t j = idx[i * k + lane];
+        for (int d = 0; d < D; ++d) xs[lane][d] = double(src[j * D + d]);
+    }
+    __syncthreads();
+    double xq[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < D; ++d) xq[d] = double(query[i * D + d]);
+    double row[NMAX];
+    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row);
+    double b[1] = {lane < n ? gx_entry(D, xs, k, rbf, spacing, lane, xq) : 0.0};
+    const int pos = wave_lu_solve<NMAX, 1>(row, b, n);
+    if (pos < 0) {
+        if (lane == 0) info[i] = 1;
+        return;
+    }
+    if (lane == 0) info[i] = 0;
+    if (pos < k) {
+        w[i * k + pos] = float(b[0]);
+        if (w64) w64[i * k + pos] = b[0];
+    }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(64) void k_batched_lu(int n, int nrhs, int64_t batch,
+                                                   double* A, double* B, int* info) {
+    const int lane = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    if (b >= batch) return;
+    const double* Ab = A + b * int64_t(n) * n;
+    double* Bb = B + b * int64_t(n) * nrhs;
+    double row[NMAX];
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) row[c] = (lane < n && c < n) ? Ab[lane + int64_t(c) * n] : 0.0;
+    // right-hand sides are processed four at a time
+    for (int m0 = 0; m0 < nrhs; m0 += 4) {
+        double r2[NMAX];
+#pragma unroll
+        for (int c = 0; c < NMAX; ++c) r2[c] = row[c];
+        double rhs[4];
+        for (int m = 0; m < 4; ++m)
+            rhs[m] = (lane < n && m0 + m < nrhs) ? Bb[lane + int64_t(m0 + m) * n] : 0.0;
+        const int pos = wave_lu_solve<NMAX, 4>(r2, rhs, n);
+        if (pos < 0) {
+            if (lane == 0) info[b] = 1;
+            return;
+        }
+        if (pos < n)
+            for (int m = 0; m < 4 && m0 + m < nrhs; ++m) Bb[pos + int64_t(m0 + m) * n] = rhs[m];
+    }
+    if (lane == 0) info[b] = 0;
+}
+
+__global__ void k_filter(int D, const float* pts, int64_t n, const uint32_t* idx,
+                         int k, double sg, float* wf) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xi[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < D; ++d) xi[d] = double(pts[i * D + d]);
+    double sum = 0.0;
+    for (int s = 0; s < k; ++s) {
+        double xj[3] = {0.0, 0.0, 0.0};
+        const int64_t j = idx[i * k + s];
+        for (int d = 0; d < D; ++d) xj[d] = double(pts[j * D + d]);
+        const double r = sqrt(dist2_f64(D, xi, xj)) / sg;
+        sum += exp(-0.5 * r * r);
+    }
+    for (int s = 0; s < k; ++s) {
+        double xj[3] = {0.0, 0.0, 0.0};
+        const int64_t j = idx[i * k + s];
+        for (int d = 0; d < D; ++d) xj[d] = double(pts[j * D + d]);
+        const double r = sqrt(dist2_f64(D, xi, xj)) / sg;
+        wf[i * k + s] = float(exp(-0.5 * r * r) / sum);
+    }
+}
+
+__global__ void k_make_slab(int64_t n, int k, int nc, const uint32_t* idx,
+                            const float* w, float* slab) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    float* r = slab + t * (1 + nc);
+    r[0] = __uint_as_float(idx[t]);
+    for (int c = 0; c < nc; ++c) r[1 + c] = w[t * nc + c];
+}
+
+__global__ void k_flag(int64_t n, int k, const uint32_t* idx, int64_t lo,
+                       int64_t hi, uint8_t* flag) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint8_t f = 0;
+    for (int s = 0; s < k; ++s) {
+        const int64_t j = idx[i * k + s];
+        f |= (j >= lo && j < hi) ? 1 : 0;
+    }
+    flag[i] = f;
+}
+
+__global__ void k_div_apply(int D, int64_t n, int k, const uint32_t* idx,
+                            const float* W, const float* F, float off, float* out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int s = 0; s < k; ++s) {
+        const int64_t j = idx[i * k + s];
+        float v = 0.0f;
+        for (int d = 0; d < D; ++d) v += W[(i * k + s) * D + d] * (F[j * D + d] - off);
+        acc += v;
+    }
+    out[i] = acc;
+}
+
+int pick_nmax(int n) {
+    const int opts[] = {20, 28, 36, 44, 52, 64};
+    for (int o : opts)
+        if (n <= o) return o;
+    return -1;
+}
+
+void upload_quadrature(int D) {
+    static const double xr[4] = {0.339981, -0.339981, 0.861136, -0.861136};
+    static const double wr[4] = {0.652145, 0.652145, 0.347855, 0.347855};
+    double tab[64][3] = {};
+    const int Nq = D == 2 ? 16 : 64;
+    for (int q = 0; q < Nq; ++q) {
+        // nonlocal_calculus/detail/quadrature_points.cuh:74-91, :127-152
+        const double r = (xr[q % 4] + 1.0) / 2.0;
+        const double ts = 3.0 * M_PI / 2.0 / 3.0;
+        if (D == 2) {
+            const double th = (q / 4) * ts;
+            tab[q][0] = r * cos(th);
+            tab[q][1] = r * sin(th);
+        } else {
+            const double th = ((q / 4) % 4) * ts;
+            const double ph = (q / 16) * (ts / 2.0) + M_PI / 4.0 / 2.0;
+            tab[q][0] = r * sin(ph) * cos(th);
+            tab[q][1] = r * sin(ph) * sin(th);
+            tab[q][2] = r * cos(ph);
+        }
+    }
+    NAGA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_quad_unit), tab, sizeof(tab)));
+    NAGA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_quad_rw), wr, sizeof(wr)));
+}
+
+template <int D>
+void launch_divw_d(int nmax, const float* src, const float* query, int64_t n_q,
+                   const uint32_t* idx, const float* d2, int k, int rbf,
+                   const double* hbar, float* W, double* W64, int* info,
+                   hipStream_t st) {
+#define NAGA_DIVW(NM)                                                              \
+    case NM:                                                                       \
+        k_divw<NM, D><<<n_q, 64, 0, st>>>(src, query, n_q, idx, d2, k, rbf, hbar, \
+                                           W, W64, info);                          \
+        break;
+    switch (nmax) {
+        NAGA_DIVW(20) NAGA_DIVW(28) NAGA_DIVW(36) NAGA_DIVW(44) NAGA_DIVW(52) NAGA_DIVW(64)
+        default: fail(kInvalidArgument, "stencil too wide");
+    }
+#undef NAGA_DIVW
+}
+
+template <int D>
+void launch_rpim_d(int nmax, const float* src, const float* query, int64_t n_q,
+                   const uint32_t* idx, int k, int rbf, double spacing, float* w,
+                   double* w64, int* info, hipStream_t st) {
+#define NAGA_RPIM(NM)                                                               \
+    case NM:                                                                        \
+        k_rpim<NM, D><<<n_q, 64, 0, st>>>(src, query, n_q, idx, k, rbf, spacing, w, \
+                                           w64, info);                              \
+        break;
+    switch (nmax) {
+        NAGA_RPIM(20) NAGA_RPIM(28) NAGA_RPIM(36) NAGA_RPIM(44) NAGA_RPIM(52) NAGA_RPIM(64)
+        default: fail(kInvalidArgument, "stencil too wide");
+    }
+#undef NAGA_RPIM
+}
+
+}  // namespace
+
+void bin_points(int D, const float* pts, int64_t n, CellGrid& g,
+                std::vector<uint32_t>& sorted, std::vector<int64_t>& start) {
+    require(n > 0, "no reference points");
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, vol = 1.0;
+    for (int d = 0; d < D; ++d) {
+        lo[d] = 1e300;
+        hi[d] = -1e300;
+    }
+    for (int64_t j = 0; j < n; ++j)
+        for (int d = 0; d < D; ++d) {
+            const double x = pts[j * D + d];
+            lo[d] = x < lo[d] ? x : lo[d];
+            hi[d] = x > hi[d] ? x : hi[d];
+        }
+    for (int d = 0; d < D; ++d) {
+        if (hi[d] - lo[d] < 1e-6) hi[d] = lo[d] + 1e-6;
+        vol *= hi[d] - lo[d];
+    }
+    g.D = D;
+    g.w = std::pow(2.0 * vol / double(n), 1.0 / D);
+    int64_t ncell = 1;
+    for (int d = 0; d < 3; ++d) {
+        g.lo[d] = d < D ? lo[d] : 0.0;
+        int64_t m = d < D ? int64_t(std::ceil((hi[d] - lo[d]) / g.w)) : 1;
+        m = m < 1 ? 1 : m > 2048 ? 2048 : m;
+        g.dims[d] = m;
+        ncell *= m;
+    }
+    std::vector<int64_t> cell(n);
+    start.assign(ncell + 1, 0);
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t c = 0, mul = 1;
+        for (int d = 0; d < D; ++d) {
+            int64_t ci = int64_t(std::floor((double(pts[j * D + d]) - g.lo[d]) / g.w));
+            ci = ci < 0 ? 0 : ci >= g.dims[d] ? g.dims[d] - 1 : ci;
+            c += ci * mul;
+            mul *= g.dims[d];
+        }
+        cell[j] = c;
+        start[c + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
+    sorted.resize(n);
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int64_t j = 0; j < n; ++j) sorted[fill[cell[j]]++] = uint32_t(j);
+}
+
+void launch_knn(const CellGrid& grid, const float* ref, const uint32_t* sorted,
+                const int64_t* start, const float* query, int64_t n_q, int k,
+                float* d2_out, uint32_t* idx_out, hipStream_t st) {
+    require(k >= 1 && k <= 64, "k must be in [1, 64]");
+    if (n_q == 0) return;
+    const int64_t blocks = (n_q * 64 + 255) / 256;
+    k_knn<<<blocks, 256, 0, st>>>(grid, ref, sorted, start, query, n_q, k, d2_out, idx_out);
+    hip_check(hipGetLastError(), "knn");
+}
+
+void launch_batched_lu_solve(int n, int nrhs, int64_t batch, double* A, double* B,
+                             int* info, hipStream_t st) {
+    const int nmax = pick_nmax(n);
+    require(nmax > 0 && n >= 1, "n must be in [1, 64]");
+    require(nrhs >= 1, "nrhs must be >= 1");
+    if (batch == 0) return;
+#define NAGA_LU(NM) \
+    case NM: k_batched_lu<NM><<<batch, 64, 0, st>>>(n, nrhs, batch, A, B, info); break;
+    switch (nmax) { NAGA_LU(20) NAGA_LU(28) NAGA_LU(36) NAGA_LU(44) NAGA_LU(52) NAGA_LU(64) }
+#undef NAGA_LU
+    hip_check(hipGetLastError(), "batched lu");
+}
+
+void launch_divergence_weights(int D, const float* src, const float* query,
+                               int64_t n_q, const uint32_t* idx, const float* d2,
+                               int k, int rbf, const double* hbar, float* W,
+                               double* W64, int* info, hipStream_t st) {
+    const int nmax = pick_nmax(k + D + 1);
+    require(nmax > 0, "stencil too wide (k + D + 1 must be <= 64)");
+    require(k >= 2, "k must be >= 2");
+    if (n_q == 0) return;
+    upload_quadrature(D);
+    if (D == 2)
+        launch_divw_d<2>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, W, W64, info, st);
+    else
+        launch_divw_d<3>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, W, W64, info, st);
+    hip_check(hipGetLastError(), "divergence weights");
+}
+
+void launch_rpim_weights(int D, const float* src, const float* query, int64_t n_q,
+                         const uint32_t* idx, int k, int rbf, double spacing,
+                         float* w, double* w64, int* info, hipStream_t st) {
+    const int nmax = pick_nmax(k + D + 1);
+    require(nmax > 0, "stencil too wide (k + D + 1 must be <= 64)");
+    if (n_q == 0) return;
+    if (D == 2)
+        launch_rpim_d<2>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+    else
+        launch_rpim_d<3>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+    hip_check(hipGetLastError(), "rpim weights");
+}
+
+void launch_filter_weights(int D, const float* pts, int64_t n, const uint32_t* idx,
+                           int k, double h, float* wf, hipStream_t st) {
+    if (n == 0) return;
+    k_filter<<<(n + 255) / 256, 256, 0, st>>>(D, pts, n, idx, k, h / 2.5, wf);
+    hip_check(hipGetLastError(), "filter weights");
+}
+
+void launch_make_slab(int64_t n, int k, int nc, const uint32_t* idx, const float* w,
+                      float* slab, hipStream_t st) {
+    if (n == 0) return;
+    k_make_slab<<<(n * k + 255) / 256, 256, 0, st>>>(n, k, nc, idx, w, slab);
+    hip_check(hipGetLastError(), "slab");
+}
+
+void launch_flag_neighbours(int64_t n, int k, const uint32_t* idx, int64_t lo,
+                            int64_t hi, uint8_t* flag, hipStream_t st) {
+    if (n == 0) return;
+    k_flag<<<(n + 255) / 256, 256, 0, st>>>(n, k, idx, lo, hi, flag);
+    hip_check(hipGetLastError(), "flag");
+}
+
+void launch_divergence_apply(int D, int64_t n, int k, const uint32_t* idx,
+                             const float* W, const float* F, float off, float* out,
+                             hipStream_t st) {
+    if (n == 0) return;
+    k_div_apply<<<(n + 255) / 256, 256, 0, st>>>(D, n, k, idx, W, F, off, out);
+    hip_check(hipGetLastError(), "divergence apply");
+}
+
+}  // namespace naga_amd

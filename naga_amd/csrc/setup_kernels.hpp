@@ -1,0 +1,73 @@
+// Setup on the GPU: exact kNN, FP64 batched LU solves, RPIM weights,
+// nonlocal divergence weights, filter weights and stencil slabs.
+#pragma once
+
+#include <vector>
+
+#include "naga_common.hpp"
+
+namespace naga_amd {
+
+// Uniform cell binning of the reference points (built on the host):
+// points sorted by cell, cell c holds sorted[start[c] .. start[c+1]).
+struct CellGrid {
+    double lo[3];
+    double w;          // cell width
+    int64_t dims[3];
+    int D;
+};
+
+// Exact k-nearest neighbours of each query among ref, key (d2, index)
+// ascending, d2 in FP32 as dz^2 + (dy^2 + dx^2) with no contraction.
+// All pointers are device pointers; start has ncell+1 entries.
+void launch_knn(const CellGrid& grid, const float* ref, const uint32_t* sorted,
+                const int64_t* start, const float* query, int64_t n_q, int k,
+                float* d2_out, uint32_t* idx_out, hipStream_t stream);
+
+// Host-side binning helper: returns sorted point indices and cell starts.
+void bin_points(int D, const float* pts, int64_t n, CellGrid& grid,
+                std::vector<uint32_t>& sorted, std::vector<int64_t>& start);
+
+// Batched dense LU with partial pivoting + solve, FP64, one wavefront per
+// system (n <= 64).  A: batch x n x n column-major; B: batch x n x nrhs
+// column-major, overwritten by X.  info[b] = 0 or first zero pivot (1-based).
+void launch_batched_lu_solve(int n, int nrhs, int64_t batch, double* A,
+                             double* B, int* info, hipStream_t stream);
+
+// Nonlocal divergence weights of every query (one wavefront each):
+// W[(i*k + s)*D + d] (FP32 and, if W64 != nullptr, FP64).
+// rbf: 0 multiquadric (shape 0.1*hbar), 1 polyharmonic r^3.
+// hbar[b] is the mean nearest-neighbour distance of query batch b (16384).
+void launch_divergence_weights(int D, const float* src, const float* query,
+                               int64_t n_q, const uint32_t* idx,
+                               const float* d2, int k, int rbf,
+                               const double* hbar, float* W, double* W64,
+                               int* info, hipStream_t stream);
+
+// RPIM interpolation weights, group size 1: w[q*k + s].
+void launch_rpim_weights(int D, const float* src, const float* query,
+                         int64_t n_q, const uint32_t* idx, int k, int rbf,
+                         double spacing, float* w, double* w64, int* info,
+                         hipStream_t stream);
+
+// Normalised Gaussian filter weights over the stencil (sigma_g = h/2.5).
+void launch_filter_weights(int D, const float* pts, int64_t n,
+                           const uint32_t* idx, int k, double h, float* wf,
+                           hipStream_t stream);
+
+// Interleave indices and weights into slab records:
+// rec[(i*k + s)*(1+ncomp) + 0] = idx bits, then ncomp weights.
+void launch_make_slab(int64_t n, int k, int ncomp, const uint32_t* idx,
+                      const float* w, float* slab, hipStream_t stream);
+
+// flag[i] = 1 if any stencil neighbour of fluid node i is in [lo, hi).
+void launch_flag_neighbours(int64_t n, int k, const uint32_t* idx, int64_t lo,
+                            int64_t hi, uint8_t* flag, hipStream_t stream);
+
+// Divergence apply (divergence_operator::apply with the advection field
+// map): out[i] = sum_s sum_d W(d,s,i) F_d(idx(s,i)), F given per node.
+void launch_divergence_apply(int D, int64_t n, int k, const uint32_t* idx,
+                             const float* W, const float* F, float off,
+                             float* out, hipStream_t stream);
+
+}  // namespace naga_amd

@@ -1,0 +1,449 @@
+// Stepping kernels (see step_kernels.hpp for the layout and schedule).
+#include "step_kernels.hpp"
+
+namespace naga_amd {
+
+namespace {
+
+constexpr int kBlock = 256;  // 4 wavefronts
+
+// Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD works
+// on one contiguous range of tiles (neighbouring nodes share its L2).
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+    const int64_t q = nb / 8, r = nb % 8;
+    const int64_t x = b % 8, l = b / 8;
+    return x * q + (x < r ? x : r) + l;
+}
+
+template <int Q>
+struct LaneMap {
+    using L = Lattice<Q>;
+    int64_t node;  // node slot (tile-relative index into the node range)
+    int a;         // alpha served by this lane (clamped to Q-1 when idle)
+    bool active;   // lane serves a real (node, alpha)
+    __device__ LaneMap(int64_t tile) {
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const int sub = lane / L::LPN;
+        const int al = lane - sub * L::LPN;
+        node = (tile * (kBlock / 64) + wave) * L::NPW + (sub < L::NPW ? sub : 0);
+        active = sub < L::NPW && al < Q;
+        a = al < Q ? al : Q - 1;
+    }
+};
+
+template <int Q>
+__device__ __forceinline__ float lane_c(int a, int d) {
+    return float(Lattice<Q>::c(a, d));
+}
+
+// ---------------------------------------------------------------------------
+// p1: macroscopic values, FP64 collision, PML local terms.  One thread per
+// node.  Writes s1 (into A): g - PML_local + collision for fluid nodes, the
+// collision increment alone for boundary nodes (the boundary interpolation
+// adds the interpolated state).
+// Reference: simulation_engine.cuh:366-433 (macroscopic), :469-531
+// (collision), pml_absorption.inl:551-631 (Q values and layer term).
+// ---------------------------------------------------------------------------
+template <int Q>
+__global__ __launch_bounds__(kBlock) void k_p1(StepGeom G, StepBuffers S,
+                                               CollisionParams cp) {
+    using L = Lattice<Q>;
+    constexpr int D = L::D, RS = L::RS;
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= G.N) return;
+    const float* gi = S.g + i * RS;
+    double gd[Q], f[Q], df[Q];
+    double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        gd[a] = double(gi[a]);
+        rt += gd[a];
+#pragma unroll
+        for (int d = 0; d < D; ++d) mom[d] += gd[a] * double(L::c(a, d));
+    }
+    const double rho = 1.0 + rt;
+    double u[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = mom[d] / rho;
+    const bool bnd = i >= G.nfluid;
+    S.rho[i] = float(rho * double(G.rho0));
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        S.u[d * G.N + i] = bnd ? 0.0f : float(u[d] * double(G.vel_scale));
+#pragma unroll
+    for (int a = 0; a < Q; ++a) f[a] = L::w(a) + gd[a];
+    collide<Q>(f, rho, u, cp, df);
+
+    double s[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) s[a] = bnd ? df[a] : gd[a] + df[a];
+
+    const bool in_layer = i >= G.nbulk && i < G.nfluid;
+    const bool in_R = in_layer || (G.pml_ref && i < G.nlayer);
+    if (in_R) {
+        const double cs2 = kCsLat * kCsLat;
+        const double uu = D == 3 ? u[2] * u[2] + (u[1] * u[1] + u[0] * u[0])
+                                 : u[1] * u[1] + u[0] * u[0];
+        const double hdt = double(G.dt) / 2.0;
+        const double sig = in_layer ? double(S.sigma[i - G.nbulk]) : 0.0;
+        float* Ei = S.E + i * RS;
+        float* Qpi = S.Qp + i * RS;
+        float* Q2i = S.Q2v + i * RS;
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+            double uc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) uc += u[d] * double(L::c(a, d));
+            // feq - w (equilibrium_distribution.cuh:52-63), cancellation-free
+            const double poly = uc / cs2 + (uc * uc) / (2.0 * cs2 * cs2) - uu;
+            const double fte = L::w(a) * (rt + rho * poly);
+            const double q1 = (fte + double(Ei[a])) * hdt;
+            const double q2 = (q1 + double(Qpi[a])) * hdt;
+            Ei[a] = float(fte);
+            Qpi[a] = float(q1);
+            Q2i[a] = float(q2);
+            if (in_layer) s[a] -= sig * (3.0 * fte + 3.0 * sig * q1 + sig * sig * q2);
+        }
+    }
+    float* si = S.A + i * RS;
+#pragma unroll
+    for (int a = 0; a < Q; ++a) si[a] = float(s[a]);
+    if (!bnd && S.bsrc[i]) {
+        float* ci = S.coll + i * RS;
+#pragma unroll
+        for (int a = 0; a < Q; ++a) ci[a] = float(df[a]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// r1/r2: layer means of Q1 and Q2 (pml_absorption.inl:775-785, :816-826).
+// Deterministic: fixed partition of the layer rows into blocks, fixed
+// in-block order, FP64 accumulation.
+// ---------------------------------------------------------------------------
+template <int Q>
+__global__ __launch_bounds__(kBlock) void k_r1(StepGeom G, StepBuffers S,
+                                               int64_t rows_per_part) {
+    using L = Lattice<Q>;
+    constexpr int RS = L::RS;
+    constexpr int SLOTS = kBlock / 64 * L::NPW;  // rows in flight per block
+    __shared__ double red[SLOTS][2][Q];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int sub = lane / L::LPN, a = lane - sub * L::LPN;
+    const bool ok = sub < L::NPW && a < Q;
+    const int slot = wave * L::NPW + (sub < L::NPW ? sub : 0);
+    const int64_t r0 = G.nbulk + int64_t(blockIdx.x) * rows_per_part;
+    int64_t r1 = r0 + rows_per_part;
+    if (r1 > G.nfluid) r1 = G.nfluid;
+    double s1 = 0.0, s2 = 0.0;
+    if (ok)
+        for (int64_t r = r0 + slot; r < r1; r += SLOTS) {
+            s1 += double(S.Qp[r * RS + a]);
+            s2 += double(S.Q2v[r * RS + a]);
+        }
+    if (ok) {
+        red[slot][0][a] = s1;
+        red[slot][1][a] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * Q) {
+        const int which = threadIdx.x / Q, al = threadIdx.x % Q;
+        double t = 0.0;
+        for (int sl = 0; sl < SLOTS; ++sl) t += red[sl][which][al];
+        S.partial[int64_t(blockIdx.x) * 2 * Q + threadIdx.x] = t;
+    }
+}
+
+template <int Q>
+__global__ void k_r2(StepGeom G, StepBuffers S) {
+    const int t = threadIdx.x;
+    if (t >= 2 * Q) return;
+    double s = 0.0;
+    for (int p = 0; p < S.nparts; ++p) s += S.partial[int64_t(p) * 2 * Q + t];
+    S.means[t] = G.nlayer > 0 ? float(s / double(G.nlayer)) : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// p2: divergence of the two PML auxiliary fields, added to s1
+// (pml_absorption.inl:787-851; field maps :440-453, :500-513).  Only fluid
+// nodes whose stencil reaches the layer contribute.
+// ---------------------------------------------------------------------------
+template <int Q>
+__global__ __launch_bounds__(kBlock) void k_p2(StepGeom G, StepBuffers S) {
+    using L = Lattice<Q>;
+    constexpr int D = L::D, RS = L::RS, REC = 1 + D;
+    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
+    if (m.node >= S.n_pml) return;
+    const int64_t i = S.pml_list[m.node];
+    const int a = m.a;
+    const float cx = lane_c<Q>(a, 0), cy = lane_c<Q>(a, 1),
+                cz = D == 3 ? lane_c<Q>(a, 2) : 0.0f;
+    const float m1 = S.means[a], m2 = S.means[Q + a];
+    const float* rec = S.slab + i * int64_t(G.k) * REC;
+    float acc = 0.0f;
+    for (int s = 0; s < G.k; ++s) {
+        const float* r = rec + s * REC;
+        const int64_t j = int64_t(__float_as_uint(r[0]));
+        if (j < G.nbulk || j >= G.nfluid) continue;
+        const int64_t p = G.pml_ref ? j - G.nbulk : j;
+        const float sig = S.sigma[j - G.nbulk];
+        const float q1 = S.Qp[p * RS + a], q2 = S.Q2v[p * RS + a];
+        const float gf = -2.0f * sig * (q1 - m1) + sig * sig * (q2 - m2);
+        float v = cx * r[1] + cy * r[2];
+        if (D == 3) v += cz * r[3];
+        acc += v * gf;
+    }
+    if (m.active) S.A[i * RS + a] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// boundary interpolation (simulation_engine.cuh:434-467 ->
+// radial_point_method.cuh:235-250) in deviation form: g_b = sum_s w_s g_src.
+// accumulate = true (p3): sources are s1 minus their collision increment
+// (the reference interpolates before colliding) and the result is added to
+// the row, which holds the boundary node's own collision increment;
+// false (p5) overwrites.
+// ---------------------------------------------------------------------------
+template <int Q, bool ACCUMULATE>
+__global__ __launch_bounds__(kBlock) void k_binterp(StepGeom G, float* field,
+                                                    const float* bslab,
+                                                    const float* coll) {
+    using L = Lattice<Q>;
+    constexpr int RS = L::RS;
+    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
+    const int64_t nb = G.N - G.nfluid;
+    if (m.node >= nb) return;
+    const int a = m.a;
+    const float* rec = bslab + m.node * int64_t(G.kb) * 2;
+    float acc = 0.0f;
+    for (int s = 0; s < G.kb; ++s) {
+        const int64_t j = int64_t(__float_as_uint(rec[2 * s]));
+        const float v = ACCUMULATE ? field[j * RS + a] - coll[j * RS + a] : field[j * RS + a];
+        acc += rec[2 * s + 1] * v;
+    }
+    const int64_t b = G.nfluid + m.node;
+    if (m.active) field[b * RS + a] = ACCUMULATE ? field[b * RS + a] + acc : acc;
+}
+
+// ---------------------------------------------------------------------------
+// p4: selective filter on fluid nodes (simulation_engine.cuh:1233-1323, with
+// the weights normalised once at setup) and bounce-back on boundary nodes
+// (:789-944).  Reads s1 (A), writes the new state g.
+// ---------------------------------------------------------------------------
+template <int Q>
+__global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S) {
+    using L = Lattice<Q>;
+    constexpr int D = L::D, RS = L::RS;
+    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
+    const int64_t i = m.node;
+    if (i >= G.N) return;
+    const int a = m.a;
+    const float* s1 = S.A;
+    float out;
+    if (i < G.nfluid) {
+        const float si = s1[i * RS + a];
+        const float* rec = S.fslab + i * int64_t(G.k) * 2;
+        float sum = 0.0f;
+        for (int s = 0; s < G.k; ++s) {
+            const int64_t j = int64_t(__float_as_uint(rec[2 * s]));
+            sum += rec[2 * s + 1] * (s1[j * RS + a] - si);
+        }
+        out = si + G.filter * sum;
+    } else {
+        const int64_t b = i - G.nfluid;
+        float cn = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) cn += lane_c<Q>(a, d) * S.normals[b * D + d];
+        const int src = cn > 0.0f ? L::bb(a) : a;
+        out = s1[i * RS + src];
+    }
+    if (m.active) S.g[i * RS + a] = out;
+}
+
+// ---------------------------------------------------------------------------
+// RK4 stage (rk4_solver.cuh:48-171 with pde_right_hand_side.cuh:82-176):
+// rate_alpha(i) = -sum_s (c_alpha . W(s,i)) g_in(idx(s,i))_alpha on fluid
+// nodes (rates are zero on boundary nodes, so their state is constant
+// through the stages).
+//   stage 1: A = g + dt/2 r, acc = r/6      (and A = B = g on boundary rows)
+//   stage 2: B = g + dt/2 r, acc += r/3
+//   stage 3: A = g + dt r,   acc += r/3
+//   stage 4: g = g + dt (acc + r/6)   (in place: each node reads only its
+//            own g row here; neighbours are gathered from A)
+// ---------------------------------------------------------------------------
+template <int Q, int STAGE>
+__global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S) {
+    using L = Lattice<Q>;
+    constexpr int D = L::D, RS = L::RS, REC = 1 + D;
+    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
+    const int64_t i = m.node;
+    const int a = m.a;
+    const float* gin = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
+    float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
+    if (STAGE == 1 && i >= G.nfluid && i < G.N) {
+        if (m.active) {
+            const float v = S.g[i * RS + a];
+            S.A[i * RS + a] = v;
+            S.B[i * RS + a] = v;
+        }
+        return;
+    }
+    if (i >= G.nfluid) return;
+    const float cx = lane_c<Q>(a, 0), cy = lane_c<Q>(a, 1),
+                cz = D == 3 ? lane_c<Q>(a, 2) : 0.0f;
+    const float* rec = S.slab + i * int64_t(G.k) * REC;
+    float r0 = 0.0f, r1 = 0.0f;
+    int s = 0;
+    for (; s + 1 < G.k; s += 2) {
+        const float* p = rec + s * REC;
+        const int64_t j0 = int64_t(__float_as_uint(p[0]));
+        const int64_t j1 = int64_t(__float_as_uint(p[REC]));
+        float v0 = cx * p[1] + cy * p[2];
+        float v1 = cx * p[REC + 1] + cy * p[REC + 2];
+        if (D == 3) {
+            v0 += cz * p[3];
+            v1 += cz * p[REC + 3];
+        }
+        r0 += v0 * gin[j0 * RS + a];
+        r1 += v1 * gin[j1 * RS + a];
+    }
+    if (s < G.k) {
+        const float* p = rec + s * REC;
+        const int64_t j0 = int64_t(__float_as_uint(p[0]));
+        float v0 = cx * p[1] + cy * p[2];
+        if (D == 3) v0 += cz * p[3];
+        r0 += v0 * gin[j0 * RS + a];
+    }
+    const float rate = -(r0 + r1);
+    if (!m.active) return;
+    const int64_t o = i * RS + a;
+    const float g0 = S.g[o];
+    const float dt = G.dt;
+    if (STAGE == 1) {
+        gout[o] = g0 + (dt * 0.5f) * rate;
+        S.acc[o] = rate * (1.0f / 6.0f);
+    } else if (STAGE == 2) {
+        gout[o] = g0 + (dt * 0.5f) * rate;
+        S.acc[o] += rate * (2.0f / 6.0f);
+    } else if (STAGE == 3) {
+        gout[o] = g0 + dt * rate;
+        S.acc[o] += rate * (2.0f / 6.0f);
+    } else {
+        gout[o] = g0 + dt * (S.acc[o] + rate * (1.0f / 6.0f));
+    }
+}
+
+template <int Q>
+__global__ __launch_bounds__(kBlock) void k_macro(StepGeom G, const float* g,
+                                                  float* rho, double* drho,
+                                                  float* u) {
+    using L = Lattice<Q>;
+    constexpr int D = L::D, RS = L::RS;
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= G.N) return;
+    double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        const double v = double(g[i * RS + a]);
+        rt += v;
+#pragma unroll
+        for (int d = 0; d < D; ++d) mom[d] += v * double(L::c(a, d));
+    }
+    const double r = 1.0 + rt;
+    if (rho) rho[i] = float(r * double(G.rho0));
+    if (drho) drho[i] = rt * double(G.rho0);
+    if (u)
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d * G.N + i] = float(mom[d] / r * double(G.vel_scale));
+}
+
+template <int Q>
+int64_t lane_blocks(int64_t nodes) {
+    constexpr int npb = kBlock / 64 * Lattice<Q>::NPW;
+    return (nodes + npb - 1) / npb;
+}
+
+inline void check_launch(const char* what) {
+    hip_check(hipGetLastError(), what);
+}
+
+}  // namespace
+
+template <int Q>
+void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S,
+                      hipStream_t st) {
+    const int64_t nb = lane_blocks<Q>(stage == 1 ? G.N : G.nfluid);
+    if (nb == 0) return;
+    switch (stage) {
+        case 1: k_stage<Q, 1><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 2: k_stage<Q, 2><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 3: k_stage<Q, 3><<<nb, kBlock, 0, st>>>(G, S); break;
+        default: k_stage<Q, 4><<<nb, kBlock, 0, st>>>(G, S); break;
+    }
+    check_launch("rk4 stage");
+}
+
+template <int Q>
+void launch_step(const StepGeom& G, const StepBuffers& S,
+                 const CollisionParams& cp, hipStream_t st, hipEvent_t* ev) {
+    const int64_t nbnd = G.N - G.nfluid;
+    int mark = 0;
+    auto tick = [&]() {
+        if (ev) hip_check(hipEventRecord(ev[mark], st), "event");
+        ++mark;
+    };
+    tick();
+    k_p1<Q><<<(G.N + kBlock - 1) / kBlock, kBlock, 0, st>>>(G, S, cp);
+    check_launch("p1");
+    const bool pml = G.nlayer > 0;
+    tick();
+    if (pml) {
+        const int64_t rows = (G.nlayer + S.nparts - 1) / S.nparts;
+        k_r1<Q><<<S.nparts, kBlock, 0, st>>>(G, S, rows);
+        check_launch("r1");
+    }
+    tick();
+    if (pml) {
+        k_r2<Q><<<1, 64, 0, st>>>(G, S);
+        check_launch("r2");
+    }
+    tick();
+    if (pml && S.n_pml > 0) {
+        k_p2<Q><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
+        check_launch("p2");
+    }
+    tick();
+    if (nbnd > 0) {
+        k_binterp<Q, true><<<lane_blocks<Q>(nbnd), kBlock, 0, st>>>(G, S.A, S.bslab, S.coll);
+        check_launch("p3");
+    }
+    tick();
+    k_p4<Q><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
+    check_launch("p4");
+    for (int stage = 1; stage <= 4; ++stage) {
+        tick();
+        launch_rk4_stage<Q>(stage, G, S, st);
+    }
+    tick();
+    if (nbnd > 0) {
+        k_binterp<Q, false><<<lane_blocks<Q>(nbnd), kBlock, 0, st>>>(G, S.g, S.bslab, nullptr);
+        check_launch("p5");
+    }
+    tick();
+}
+
+template <int Q>
+void launch_macroscopic(const StepGeom& G, const float* g, float* rho,
+                        double* drho, float* u, hipStream_t st) {
+    k_macro<Q><<<(G.N + kBlock - 1) / kBlock, kBlock, 0, st>>>(G, g, rho, drho, u);
+    check_launch("macroscopic");
+}
+
+template void launch_step<9>(const StepGeom&, const StepBuffers&, const CollisionParams&, hipStream_t, hipEvent_t*);
+template void launch_step<27>(const StepGeom&, const StepBuffers&, const CollisionParams&, hipStream_t, hipEvent_t*);
+template void launch_rk4_stage<9>(int, const StepGeom&, const StepBuffers&, hipStream_t);
+template void launch_rk4_stage<27>(int, const StepGeom&, const StepBuffers&, hipStream_t);
+template void launch_macroscopic<9>(const StepGeom&, const float*, float*, double*, float*, hipStream_t);
+template void launch_macroscopic<27>(const StepGeom&, const float*, float*, double*, float*, hipStream_t);
+
+}  // namespace naga_amd

@@ -1,0 +1,90 @@
+// Stepping kernels of the nonlocal LBM on CDNA4 (gfx950).
+//
+// State layout in HBM: each node owns one row of RS floats holding the
+// deviation g_alpha = f_alpha - w_alpha of its Q distributions (AoS).  Rows
+// are ordered [bulk | layer | boundary].  The divergence stencil of fluid
+// node i is a slab of k records {neighbour index, W_x, W_y(, W_z)}.
+//
+// Kernels that gather neighbours map one node to LPN consecutive lanes with
+// lane = alpha ("lane over alpha"): the Q lanes of a node read one
+// contiguous row of each neighbour, so every gather is one cache line, and
+// the stencil sum over s runs sequentially inside the lane (no cross-lane
+// reduction).  Kernels doing per-node algebra (macroscopic values, the FP64
+// collision, the PML recurrences) map one node to one thread.
+//
+// One time step is eleven launches:
+//   p1      macroscopic, FP64 collision, PML local terms     (all nodes)
+//   r1, r2  layer means of the two PML auxiliary fields      (layer rows)
+//   p2      PML divergence of the auxiliary fields            (nodes with layer neighbours)
+//   p3      boundary interpolation of the post-PML state      (boundary;
+//           the sources are read before their collision increment, which
+//           p1 keeps aside for them, as the reference interpolates before
+//           colliding)
+//   p4      selective filter + bounce-back                    (all nodes)
+//   s1..s4  the four RK4 stages of the streaming equation     (fluid nodes)
+//   p5      boundary interpolation of the new state           (boundary)
+#pragma once
+
+#include "collision.hpp"
+
+namespace naga_amd {
+
+struct StepGeom {
+    int64_t N;        // all nodes
+    int64_t nbulk;    // [0, nbulk) bulk
+    int64_t nlayer;   // [nbulk, nfluid) absorbing layer
+    int64_t nfluid;   // nbulk + nlayer; [nfluid, N) boundary
+    int k;            // divergence stencil width
+    int kb;           // boundary interpolation stencil width
+    int pml_ref;      // layer field reads Q at (j - nbulk) as the reference
+    float dt;         // lattice time step
+    float rho0;       // nominal density
+    float vel_scale;  // c / c_s
+    float filter;     // selective filter strength (0.75)
+};
+
+struct StepBuffers {
+    float* g;        // [N][RS] current deviation state
+    float* A;        // [N][RS] stage ping
+    float* B;        // [N][RS] stage pong
+    float* acc;      // [N][RS] RK4 accumulator
+    float* E;        // [N][RS] previous equilibrium deviation (PML Q1 - w)
+    float* Qp;       // [N][RS] previous Q1 value (PML Q2 source)
+    float* Q2v;      // [N][RS] current Q2 value
+    const float* slab;    // [nfluid][k][1+D] stencil records
+    const float* fslab;   // [nfluid][k][2] filter records {idx, weight}
+    const float* bslab;   // [nbnd][kb][2] boundary interpolation records
+    const float* sigma;   // [nlayer] absorption
+    const float* normals; // [nbnd][D] inward normals
+    float* coll;     // [N][RS] collision increment of boundary-source nodes
+    const uint8_t* bsrc;  // [nfluid] 1 if the node feeds the boundary interpolator
+    const int32_t* pml_list;  // fluid nodes with a layer neighbour
+    int64_t n_pml;
+    float* rho;      // [N] density (reference semantics, FP32)
+    float* u;        // [D][N] velocity
+    double* partial; // [nparts][2Q] layer column sums
+    float* means;    // [2Q] layer means of Q1, Q2
+    int nparts;
+};
+
+// Launch the eleven kernels of one step on `stream`.  If `events` is not
+// null, events[t] is recorded before kernel t and events[11] after the last
+// (kernels with no work still get their pair of marks).
+constexpr int kStepKernels = 11;
+template <int Q>
+void launch_step(const StepGeom& geo, const StepBuffers& buf,
+                 const CollisionParams& cp, hipStream_t stream,
+                 hipEvent_t* events = nullptr);
+
+// Per-kernel entry points (used by the profiler and the standalone C-ABI).
+template <int Q>
+void launch_rk4_stage(int stage, const StepGeom& geo, const StepBuffers& buf,
+                      hipStream_t stream);
+
+// Fresh macroscopic values from the current state: rho (FP32, reference
+// units), drho = rho - rho0 in FP64 without cancellation, u (FP32).
+template <int Q>
+void launch_macroscopic(const StepGeom& geo, const float* g, float* rho,
+                        double* drho, float* u, hipStream_t stream);
+
+}  // namespace naga_amd
