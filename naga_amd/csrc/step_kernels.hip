@@ -164,25 +164,123 @@ __global__ void k_r2(StepGeom G, StepBuffers S) {
 }
 
 // ---------------------------------------------------------------------------
+// Stencil records staged in LDS.  A block serves NPB node slots; the records
+// of its nodes are copied once into LDS (coalesced), then every lane of a
+// node reads them with broadcast LDS reads while its gathers go to global
+// memory in unrolled batches, so many independent loads are in flight per
+// wavefront.
+// ---------------------------------------------------------------------------
+template <int Q>
+struct Tile {
+    static constexpr int NPB = kBlock / 64 * Lattice<Q>::NPW;  // node slots
+};
+
+// Copy the records of nodes [node0, node0 + cnt) (contiguous slab rows of
+// K*REC floats) into lds.
+template <int REC>
+__device__ __forceinline__ void stage_contiguous(const float* slab, int64_t node0, int cnt,
+                                                 int K, float* lds) {
+    const float* src = slab + node0 * int64_t(K) * REC;
+    const int tot = cnt * K * REC;
+    if (REC == 4) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        float4* d4 = reinterpret_cast<float4*>(lds);
+        for (int t = threadIdx.x; t < tot / 4; t += kBlock) d4[t] = s4[t];
+    } else if (REC == 2) {
+        const float2* s2 = reinterpret_cast<const float2*>(src);
+        float2* d2 = reinterpret_cast<float2*>(lds);
+        for (int t = threadIdx.x; t < tot / 2; t += kBlock) d2[t] = s2[t];
+    } else {
+        for (int t = threadIdx.x; t < tot; t += kBlock) lds[t] = src[t];
+    }
+}
+
+// Copy the records of one node (lanes of the node share the work).
+template <int Q, int REC>
+__device__ __forceinline__ void stage_node(const float* slab, int64_t node, bool valid, int K,
+                                           int a_lane, float* lds) {
+    if (!valid) return;
+    const float* src = slab + node * int64_t(K) * REC;
+    for (int r = a_lane; r < K; r += Lattice<Q>::LPN)
+#pragma unroll
+        for (int c = 0; c < REC; ++c) lds[r * REC + c] = src[r * REC + c];
+}
+
+template <int Q>
+struct Lanes {
+    using L = Lattice<Q>;
+    int slot, a, al;
+    bool active, in_wave;
+    __device__ Lanes() {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int sub = lane / L::LPN;
+        al = lane - sub * L::LPN;
+        in_wave = sub < L::NPW;
+        active = in_wave && al < Q;
+        a = al < Q ? al : Q - 1;
+        slot = wave * L::NPW + (sub < L::NPW ? sub : 0);
+    }
+};
+
+// sum_s (c_a . W_s) fin(idx_s)_a over the staged records rp (REC = 1 + D),
+// batches of 8 neighbours.
+template <int Q, int KT>
+__device__ __forceinline__ float stencil_sum(const float* rp, int K, const float* fin, int a) {
+    using L = Lattice<Q>;
+    constexpr int D = L::D, RS = L::RS, REC = 1 + D, KMAX = KT ? KT : 64;
+    const float cx = lane_c<Q>(a, 0), cy = lane_c<Q>(a, 1),
+                cz = D == 3 ? lane_c<Q>(a, 2) : 0.0f;
+    float acc = 0.0f;
+#pragma unroll
+    for (int s0 = 0; s0 < KMAX; s0 += 8) {
+        if (!KT && s0 >= K) break;
+        float v[8], gv[8];
+        uint32_t j[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int s = s0 + u;
+            const bool in = KT ? (s < KT) : (s < K);
+            const float* r = rp + (in ? s : 0) * REC;
+            j[u] = __float_as_uint(r[0]);
+            float w = cx * r[1] + cy * r[2];
+            if (D == 3) w += cz * r[3];
+            v[u] = in ? w : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) gv[u] = fin[int64_t(j[u]) * RS + a];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u] * gv[u];
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
 // p2: divergence of the two PML auxiliary fields, added to s1
 // (pml_absorption.inl:787-851; field maps :440-453, :500-513).  Only fluid
 // nodes whose stencil reaches the layer contribute.
 // ---------------------------------------------------------------------------
-template <int Q>
+template <int Q, int KT>
 __global__ __launch_bounds__(kBlock) void k_p2(StepGeom G, StepBuffers S) {
     using L = Lattice<Q>;
-    constexpr int D = L::D, RS = L::RS, REC = 1 + D;
-    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
-    if (m.node >= S.n_pml) return;
-    const int64_t i = S.pml_list[m.node];
-    const int a = m.a;
+    constexpr int D = L::D, RS = L::RS, REC = 1 + D, KMAX = KT ? KT : 64;
+    constexpr int NPB = Tile<Q>::NPB;
+    __shared__ float srec[NPB * KMAX * REC];
+    const int K = KT ? KT : G.k;
+    const Lanes<Q> ln;
+    const int64_t item = xcd_tile(blockIdx.x, gridDim.x) * NPB + ln.slot;
+    const bool valid = item < S.n_pml;
+    const int64_t i = valid ? S.pml_list[item] : 0;
+    float* rp = srec + ln.slot * K * REC;
+    stage_node<Q, REC>(S.slab, i, valid && ln.in_wave, K, ln.al, rp);
+    __syncthreads();
+    if (!valid) return;
+    const int a = ln.a;
     const float cx = lane_c<Q>(a, 0), cy = lane_c<Q>(a, 1),
                 cz = D == 3 ? lane_c<Q>(a, 2) : 0.0f;
     const float m1 = S.means[a], m2 = S.means[Q + a];
-    const float* rec = S.slab + i * int64_t(G.k) * REC;
     float acc = 0.0f;
-    for (int s = 0; s < G.k; ++s) {
-        const float* r = rec + s * REC;
+    for (int s = 0; s < K; ++s) {
+        const float* r = rp + s * REC;
         const int64_t j = int64_t(__float_as_uint(r[0]));
         if (j < G.nbulk || j >= G.nfluid) continue;
         const int64_t p = G.pml_ref ? j - G.nbulk : j;
@@ -193,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_p2(StepGeom G, StepBuffers S) {
         if (D == 3) v += cz * r[3];
         acc += v * gf;
     }
-    if (m.active) S.A[i * RS + a] += acc;
+    if (ln.active) S.A[i * RS + a] += acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -230,23 +328,43 @@ __global__ __launch_bounds__(kBlock) void k_binterp(StepGeom G, float* field,
 // the weights normalised once at setup) and bounce-back on boundary nodes
 // (:789-944).  Reads s1 (A), writes the new state g.
 // ---------------------------------------------------------------------------
-template <int Q>
+template <int Q, int KT>
 __global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S) {
     using L = Lattice<Q>;
-    constexpr int D = L::D, RS = L::RS;
-    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
-    const int64_t i = m.node;
+    constexpr int D = L::D, RS = L::RS, KMAX = KT ? KT : 64;
+    constexpr int NPB = Tile<Q>::NPB;
+    __shared__ float srec[NPB * KMAX * 2];
+    const int K = KT ? KT : G.k;
+    const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * NPB;
+    const int64_t nf = G.nfluid - node0;
+    stage_contiguous<2>(S.fslab, node0, nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB), K, srec);
+    __syncthreads();
+    const Lanes<Q> ln;
+    const int64_t i = node0 + ln.slot;
     if (i >= G.N) return;
-    const int a = m.a;
+    const int a = ln.a;
     const float* s1 = S.A;
     float out;
     if (i < G.nfluid) {
         const float si = s1[i * RS + a];
-        const float* rec = S.fslab + i * int64_t(G.k) * 2;
+        const float* rp = srec + ln.slot * K * 2;
         float sum = 0.0f;
-        for (int s = 0; s < G.k; ++s) {
-            const int64_t j = int64_t(__float_as_uint(rec[2 * s]));
-            sum += rec[2 * s + 1] * (s1[j * RS + a] - si);
+#pragma unroll
+        for (int s0 = 0; s0 < KMAX; s0 += 8) {
+            if (!KT && s0 >= K) break;
+            float w[8], gv[8];
+            uint32_t j[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int s = s0 + u;
+                const bool in = KT ? (s < KT) : (s < K);
+                j[u] = __float_as_uint(rp[(in ? s : 0) * 2]);
+                w[u] = in ? rp[s * 2 + 1] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) gv[u] = s1[int64_t(j[u]) * RS + a];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sum += w[u] * (gv[u] - si);
         }
         out = si + G.filter * sum;
     } else {
@@ -257,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S) {
         const int src = cn > 0.0f ? L::bb(a) : a;
         out = s1[i * RS + src];
     }
-    if (m.active) S.g[i * RS + a] = out;
+    if (ln.active) S.g[i * RS + a] = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -271,17 +389,24 @@ __global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S) {
 //   stage 4: g = g + dt (acc + r/6)   (in place: each node reads only its
 //            own g row here; neighbours are gathered from A)
 // ---------------------------------------------------------------------------
-template <int Q, int STAGE>
+template <int Q, int STAGE, int KT>
 __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S) {
     using L = Lattice<Q>;
-    constexpr int D = L::D, RS = L::RS, REC = 1 + D;
-    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
-    const int64_t i = m.node;
-    const int a = m.a;
+    constexpr int D = L::D, RS = L::RS, REC = 1 + D, KMAX = KT ? KT : 64;
+    constexpr int NPB = Tile<Q>::NPB;
+    __shared__ float srec[NPB * KMAX * REC];
+    const int K = KT ? KT : G.k;
+    const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * NPB;
+    const int64_t nf = G.nfluid - node0;
+    stage_contiguous<REC>(S.slab, node0, nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB), K, srec);
+    __syncthreads();
+    const Lanes<Q> ln;
+    const int64_t i = node0 + ln.slot;
+    const int a = ln.a;
     const float* gin = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
     float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
     if (STAGE == 1 && i >= G.nfluid && i < G.N) {
-        if (m.active) {
+        if (ln.active) {
             const float v = S.g[i * RS + a];
             S.A[i * RS + a] = v;
             S.B[i * RS + a] = v;
@@ -289,33 +414,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S) {
         return;
     }
     if (i >= G.nfluid) return;
-    const float cx = lane_c<Q>(a, 0), cy = lane_c<Q>(a, 1),
-                cz = D == 3 ? lane_c<Q>(a, 2) : 0.0f;
-    const float* rec = S.slab + i * int64_t(G.k) * REC;
-    float r0 = 0.0f, r1 = 0.0f;
-    int s = 0;
-    for (; s + 1 < G.k; s += 2) {
-        const float* p = rec + s * REC;
-        const int64_t j0 = int64_t(__float_as_uint(p[0]));
-        const int64_t j1 = int64_t(__float_as_uint(p[REC]));
-        float v0 = cx * p[1] + cy * p[2];
-        float v1 = cx * p[REC + 1] + cy * p[REC + 2];
-        if (D == 3) {
-            v0 += cz * p[3];
-            v1 += cz * p[REC + 3];
-        }
-        r0 += v0 * gin[j0 * RS + a];
-        r1 += v1 * gin[j1 * RS + a];
-    }
-    if (s < G.k) {
-        const float* p = rec + s * REC;
-        const int64_t j0 = int64_t(__float_as_uint(p[0]));
-        float v0 = cx * p[1] + cy * p[2];
-        if (D == 3) v0 += cz * p[3];
-        r0 += v0 * gin[j0 * RS + a];
-    }
-    const float rate = -(r0 + r1);
-    if (!m.active) return;
+    const float rate = -stencil_sum<Q, KT>(srec + ln.slot * K * REC, K, gin, a);
+    if (!ln.active) return;
     const int64_t o = i * RS + a;
     const float g0 = S.g[o];
     const float dt = G.dt;
@@ -369,17 +469,28 @@ inline void check_launch(const char* what) {
 
 }  // namespace
 
+template <int Q, int KT>
+void launch_stage_k(int stage, int64_t nb, const StepGeom& G, const StepBuffers& S,
+                    hipStream_t st) {
+    switch (stage) {
+        case 1: k_stage<Q, 1, KT><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 2: k_stage<Q, 2, KT><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 3: k_stage<Q, 3, KT><<<nb, kBlock, 0, st>>>(G, S); break;
+        default: k_stage<Q, 4, KT><<<nb, kBlock, 0, st>>>(G, S); break;
+    }
+}
+
 template <int Q>
 void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S,
                       hipStream_t st) {
     const int64_t nb = lane_blocks<Q>(stage == 1 ? G.N : G.nfluid);
     if (nb == 0) return;
-    switch (stage) {
-        case 1: k_stage<Q, 1><<<nb, kBlock, 0, st>>>(G, S); break;
-        case 2: k_stage<Q, 2><<<nb, kBlock, 0, st>>>(G, S); break;
-        case 3: k_stage<Q, 3><<<nb, kBlock, 0, st>>>(G, S); break;
-        default: k_stage<Q, 4><<<nb, kBlock, 0, st>>>(G, S); break;
-    }
+    if (G.k == 32)
+        launch_stage_k<Q, 32>(stage, nb, G, S, st);
+    else if (G.k == 16)
+        launch_stage_k<Q, 16>(stage, nb, G, S, st);
+    else
+        launch_stage_k<Q, 0>(stage, nb, G, S, st);
     check_launch("rk4 stage");
 }
 
@@ -409,7 +520,12 @@ void launch_step(const StepGeom& G, const StepBuffers& S,
     }
     tick();
     if (pml && S.n_pml > 0) {
-        k_p2<Q><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
+        if (G.k == 32)
+            k_p2<Q, 32><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
+        else if (G.k == 16)
+            k_p2<Q, 16><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
+        else
+            k_p2<Q, 0><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
         check_launch("p2");
     }
     tick();
@@ -418,7 +534,12 @@ void launch_step(const StepGeom& G, const StepBuffers& S,
         check_launch("p3");
     }
     tick();
-    k_p4<Q><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
+    if (G.k == 32)
+        k_p4<Q, 32><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
+    else if (G.k == 16)
+        k_p4<Q, 16><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
+    else
+        k_p4<Q, 0><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
     check_launch("p4");
     for (int stage = 1; stage <= 4; ++stage) {
         tick();
