@@ -87,7 +87,10 @@ int naga_engine_set_distributions(naga_engine* e, const double* f);
 int naga_engine_step(naga_engine* e, int32_t nsteps);
 /* The same, timed on the engine's stream with HIP events (ms). */
 int naga_engine_step_timed(naga_engine* e, int32_t nsteps, double* ms);
-/* Per-kernel timing of one step: ms[11] for p1 r1 r2 p2 p3 p4 s1 s2 s3 s4 p5. */
+/* Per-kernel timing of one step: ms[12] for p1 (collision), pl (PML local),
+ * pm (layer means), pf (PML field), p2 (PML divergence), p3 (boundary
+ * interpolation), p4 (filter + bounce-back), s1..s4 (RK4 stages), p5
+ * (boundary interpolation). */
 int naga_engine_profile_step(naga_engine* e, double* ms);
 
 /* solution().macroscopic_values: rho[N] (FP32), u[D*N] (d-major, FP32) and

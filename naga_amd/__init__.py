@@ -106,7 +106,7 @@ EXPORTED = [
     "naga_divergence_apply",
 ]
 
-STEP_KERNELS = ["p1_collide_pml_local", "r1_layer_sums", "r2_layer_means",
+STEP_KERNELS = ["p1_collide", "pl_pml_local", "pm_layer_means", "pf_pml_field",
                 "p2_pml_divergence", "p3_boundary_interp", "p4_filter_bounce",
                 "s1_rk4_stage", "s2_rk4_stage", "s3_rk4_stage", "s4_rk4_stage",
                 "p5_boundary_interp"]
@@ -342,7 +342,7 @@ class SimulationEngine:
         return float(ms[0])
 
     def profile_step(self) -> dict:
-        ms = np.zeros(11)
+        ms = np.zeros(len(STEP_KERNELS))
         _check(load().naga_engine_profile_step(self._live(), _ptr(ms)), _ENGINE_PREFIX)
         return dict(zip(STEP_KERNELS, ms.tolist()))
 

@@ -214,19 +214,35 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         sigma_.allocate(nlayer);
         sigma_.upload(sigma, nlayer, st);
     }
+    // nodes whose PML Q values are maintained: the layer, plus [0, nlayer)
+    // when the layer field reads Q at j - num_bulk
+    n_rq_ = 0;
+    if (nlayer > 0) {
+        std::vector<uint8_t> inq(nfluid, 0);
+        for (int64_t i = nbulk; i < nfluid; ++i) inq[i] = 1;
+        if (cfg_.pml_ref)
+            for (int64_t i = 0; i < nlayer; ++i) inq[i] = 1;
+        std::vector<int32_t> rq;
+        for (int64_t i = 0; i < nfluid; ++i)
+            if (inq[i]) rq.push_back(int32_t(i));
+        n_rq_ = int64_t(rq.size());
+        rq_list_ = to_device(rq, st);
+    }
     // state
     const size_t rows = size_t(N) * RS_;
     if (nbnd == 0) {
         std::vector<uint8_t> none(std::max<int64_t>(nfluid, 1), 0);
         bsrc_ = to_device(none, st);
     }
-    for (DeviceBuffer<float>* b : {&g_, &A_, &B_, &acc_, &E_, &Qp_, &Q2v_, &coll_}) {
+    for (DeviceBuffer<float>* b : {&g_, &A_, &B_, &acc_, &E_, &Qp_, &Q2v_, &coll_, &Gf_}) {
         b->allocate(rows);
         b->zero(st);
     }
     rho_.allocate(N);
     u_.allocate(N * D);
-    nparts_ = int(std::min<int64_t>(std::max<int64_t>((nlayer + 4095) / 4096, 1), 512));
+    ru_.allocate(N);
+    const int64_t per = kPmlItemsPerBlock(Q);
+    nparts_ = int(std::max<int64_t>((n_rq_ + per - 1) / per, 1));
     partial_.allocate(size_t(nparts_) * 2 * Q);
     partial_.zero(st);
     means_.allocate(2 * Q);
@@ -259,6 +275,10 @@ StepBuffers Engine::buffers() {
     S.sigma = sigma_.get();
     S.normals = normals_.get();
     S.coll = coll_.get();
+    S.ru = ru_.get();
+    S.Gf = Gf_.get();
+    S.rq_list = rq_list_.get();
+    S.n_rq = n_rq_;
     S.bsrc = bsrc_.get();
     S.pml_list = pml_list_.get();
     S.n_pml = n_pml_;

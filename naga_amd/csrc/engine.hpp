@@ -70,7 +70,10 @@ class Engine {
     DeviceBuffer<float> W_, filt_, bw_;
     DeviceBuffer<int32_t> pml_list_;
     DeviceBuffer<uint8_t> bsrc_;
-    DeviceBuffer<float> coll_;
+    DeviceBuffer<float> coll_, Gf_;
+    DeviceBuffer<float4> ru_;
+    DeviceBuffer<int32_t> rq_list_;
+    int64_t n_rq_ = 0;
     DeviceBuffer<float> g_, A_, B_, acc_, E_, Qp_, Q2v_;
     DeviceBuffer<float> rho_, u_;
     DeviceBuffer<double> partial_;

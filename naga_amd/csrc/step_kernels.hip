@@ -1,4 +1,6 @@
 // Stepping kernels (see step_kernels.hpp for the layout and schedule).
+#include <cstdlib>
+
 #include "step_kernels.hpp"
 
 namespace naga_amd {
@@ -38,129 +40,163 @@ __device__ __forceinline__ float lane_c(int a, int d) {
 }
 
 // ---------------------------------------------------------------------------
-// p1: macroscopic values, FP64 collision, PML local terms.  One thread per
-// node.  Writes s1 (into A): g - PML_local + collision for fluid nodes, the
-// collision increment alone for boundary nodes (the boundary interpolation
-// adds the interpolated state).
+// p1: macroscopic values and the FP64 collision.  One thread per node; the
+// block's rows are moved through LDS so global loads and stores are
+// coalesced (rows padded by one float in LDS against bank conflicts).
+// Writes s1 (into A): g + collision for fluid nodes, the collision
+// increment alone for boundary nodes (p3 adds the interpolated state); the
+// unitless (rho - 1, u) per node for the PML pass; the reference's
+// macroscopic state (rho, u with boundary velocities zeroed).
 // Reference: simulation_engine.cuh:366-433 (macroscopic), :469-531
-// (collision), pml_absorption.inl:551-631 (Q values and layer term).
+// (collision), :876-877 (boundary velocity reset).
 // ---------------------------------------------------------------------------
 template <int Q>
 __global__ __launch_bounds__(kBlock) void k_p1(StepGeom G, StepBuffers S,
                                                CollisionParams cp) {
     using L = Lattice<Q>;
-    constexpr int D = L::D, RS = L::RS;
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= G.N) return;
-    const float* gi = S.g + i * RS;
-    double gd[Q], f[Q], df[Q];
-    double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int a = 0; a < Q; ++a) {
-        gd[a] = double(gi[a]);
-        rt += gd[a];
-#pragma unroll
-        for (int d = 0; d < D; ++d) mom[d] += gd[a] * double(L::c(a, d));
-    }
-    const double rho = 1.0 + rt;
-    double u[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int d = 0; d < D; ++d) u[d] = mom[d] / rho;
-    const bool bnd = i >= G.nfluid;
-    S.rho[i] = float(rho * double(G.rho0));
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-        S.u[d * G.N + i] = bnd ? 0.0f : float(u[d] * double(G.vel_scale));
-#pragma unroll
-    for (int a = 0; a < Q; ++a) f[a] = L::w(a) + gd[a];
-    collide<Q>(f, rho, u, cp, df);
-
-    double s[Q];
-#pragma unroll
-    for (int a = 0; a < Q; ++a) s[a] = bnd ? df[a] : gd[a] + df[a];
-
-    const bool in_layer = i >= G.nbulk && i < G.nfluid;
-    const bool in_R = in_layer || (G.pml_ref && i < G.nlayer);
-    if (in_R) {
-        const double cs2 = kCsLat * kCsLat;
-        const double uu = D == 3 ? u[2] * u[2] + (u[1] * u[1] + u[0] * u[0])
-                                 : u[1] * u[1] + u[0] * u[0];
-        const double hdt = double(G.dt) / 2.0;
-        const double sig = in_layer ? double(S.sigma[i - G.nbulk]) : 0.0;
-        float* Ei = S.E + i * RS;
-        float* Qpi = S.Qp + i * RS;
-        float* Q2i = S.Q2v + i * RS;
+    constexpr int D = L::D, RS = L::RS, LS = RS + 1;
+    __shared__ float rows[kBlock * LS];
+    const int64_t node0 = int64_t(blockIdx.x) * kBlock;
+    const int64_t left = G.N - node0;
+    const int cnt = left < kBlock ? int(left) : kBlock;
+    const float* src = S.g + node0 * RS;
+    for (int e = threadIdx.x; e < cnt * RS; e += kBlock) rows[(e / RS) * LS + e % RS] = src[e];
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < cnt) {
+        const int64_t i = node0 + t;
+        float* r = rows + t * LS;
+        double f[Q];
+        double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int a = 0; a < Q; ++a) {
-            double uc = 0.0;
+            const double gv = double(r[a]);
+            rt += gv;
 #pragma unroll
-            for (int d = 0; d < D; ++d) uc += u[d] * double(L::c(a, d));
-            // feq - w (equilibrium_distribution.cuh:52-63), cancellation-free
-            const double poly = uc / cs2 + (uc * uc) / (2.0 * cs2 * cs2) - uu;
-            const double fte = L::w(a) * (rt + rho * poly);
-            const double q1 = (fte + double(Ei[a])) * hdt;
-            const double q2 = (q1 + double(Qpi[a])) * hdt;
-            Ei[a] = float(fte);
-            Qpi[a] = float(q1);
-            Q2i[a] = float(q2);
-            if (in_layer) s[a] -= sig * (3.0 * fte + 3.0 * sig * q1 + sig * sig * q2);
+            for (int d = 0; d < D; ++d) mom[d] += gv * double(L::c(a, d));
+            f[a] = L::w(a) + gv;
+        }
+        const double rho = 1.0 + rt;
+        double u[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d] = mom[d] / rho;
+        const bool bnd = i >= G.nfluid;
+        S.rho[i] = float(rho * double(G.rho0));
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            S.u[d * G.N + i] = bnd ? 0.0f : float(u[d] * double(G.vel_scale));
+        S.ru[i] = make_float4(float(rt), float(u[0]), float(u[1]), float(u[2]));
+        double df[Q];
+        collide<Q>(f, rho, u, cp, df);
+        const bool src_node = !bnd && S.bsrc[i];
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+            const double v = bnd ? df[a] : double(r[a]) + df[a];
+            r[a] = float(v);
+            if (src_node) S.coll[i * RS + a] = float(df[a]);
         }
     }
-    float* si = S.A + i * RS;
-#pragma unroll
-    for (int a = 0; a < Q; ++a) si[a] = float(s[a]);
-    if (!bnd && S.bsrc[i]) {
-        float* ci = S.coll + i * RS;
-#pragma unroll
-        for (int a = 0; a < Q; ++a) ci[a] = float(df[a]);
-    }
+    __syncthreads();
+    float* dst = S.A + node0 * RS;
+    for (int e = threadIdx.x; e < cnt * RS; e += kBlock) dst[e] = rows[(e / RS) * LS + e % RS];
 }
 
 // ---------------------------------------------------------------------------
-// r1/r2: layer means of Q1 and Q2 (pml_absorption.inl:775-785, :816-826).
-// Deterministic: fixed partition of the layer rows into blocks, fixed
-// in-block order, FP64 accumulation.
+// PML local terms (pml_absorption.inl:551-631) over the nodes whose Q values
+// are used (the layer, plus nodes [0, num_layer) when the layer field reads
+// Q at j - num_bulk): Q1 = (feq - w + E) dt/2, Q2 = (Q1 + Q1_prev) dt/2,
+// E <- feq - w, and on layer nodes s1 -= sigma (3 (feq - w) + 3 sigma Q1 +
+// sigma^2 Q2).  Lane over alpha; each block also emits its partial column
+// sums of Q1 and Q2 over layer nodes for the layer means (fixed partition,
+// fixed order: deterministic).
 // ---------------------------------------------------------------------------
+constexpr int kPmlRounds = 8;
+
 template <int Q>
-__global__ __launch_bounds__(kBlock) void k_r1(StepGeom G, StepBuffers S,
-                                               int64_t rows_per_part) {
+__global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S) {
     using L = Lattice<Q>;
-    constexpr int RS = L::RS;
-    constexpr int SLOTS = kBlock / 64 * L::NPW;  // rows in flight per block
-    __shared__ double red[SLOTS][2][Q];
+    constexpr int D = L::D, RS = L::RS;
+    constexpr int NPB = kBlock / 64 * L::NPW;
+    __shared__ double red[NPB][2][Q];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int sub = lane / L::LPN, a = lane - sub * L::LPN;
-    const bool ok = sub < L::NPW && a < Q;
+    const int sub = lane / L::LPN, al = lane - sub * L::LPN;
+    const bool active = sub < L::NPW && al < Q;
     const int slot = wave * L::NPW + (sub < L::NPW ? sub : 0);
-    const int64_t r0 = G.nbulk + int64_t(blockIdx.x) * rows_per_part;
-    int64_t r1 = r0 + rows_per_part;
-    if (r1 > G.nfluid) r1 = G.nfluid;
+    const int a = al < Q ? al : Q - 1;
+    const double cs2 = kCsLat * kCsLat;
+    const double hdt = double(G.dt) / 2.0;
+    const double ca[3] = {double(L::c(a, 0)), double(L::c(a, 1)), D == 3 ? double(L::c(a, 2)) : 0.0};
     double s1 = 0.0, s2 = 0.0;
-    if (ok)
-        for (int64_t r = r0 + slot; r < r1; r += SLOTS) {
-            s1 += double(S.Qp[r * RS + a]);
-            s2 += double(S.Q2v[r * RS + a]);
+    const int64_t base = int64_t(blockIdx.x) * (NPB * kPmlRounds);
+    for (int rd = 0; rd < kPmlRounds; ++rd) {
+        const int64_t item = base + rd * NPB + slot;
+        if (!active || item >= S.n_rq) continue;
+        const int64_t i = S.rq_list[item];
+        const float4 ru = S.ru[i];
+        const double rt = double(ru.x), rho = 1.0 + rt;
+        const double u0 = ru.y, u1 = ru.z, u2 = ru.w;
+        const double uc = D == 3 ? u0 * ca[0] + u1 * ca[1] + u2 * ca[2] : u0 * ca[0] + u1 * ca[1];
+        const double uu = D == 3 ? u2 * u2 + (u1 * u1 + u0 * u0) : u1 * u1 + u0 * u0;
+        // feq - w (equilibrium_distribution.cuh:52-63), cancellation-free
+        const double fte = L::w(a) * (rt + rho * (uc / cs2 + (uc * uc) / (2.0 * cs2 * cs2) - uu));
+        const int64_t o = i * RS + a;
+        const double q1 = (fte + double(S.E[o])) * hdt;
+        const double q2 = (q1 + double(S.Qp[o])) * hdt;
+        S.E[o] = float(fte);
+        S.Qp[o] = float(q1);
+        S.Q2v[o] = float(q2);
+        if (i >= G.nbulk && i < G.nfluid) {
+            const double sig = double(S.sigma[i - G.nbulk]);
+            S.A[o] = float(double(S.A[o]) - sig * (3.0 * fte + 3.0 * sig * q1 + sig * sig * q2));
+            s1 += q1;
+            s2 += q2;
         }
-    if (ok) {
+    }
+    if (active) {  // every (slot, alpha) pair has exactly one active lane
         red[slot][0][a] = s1;
         red[slot][1][a] = s2;
     }
     __syncthreads();
     if (threadIdx.x < 2 * Q) {
-        const int which = threadIdx.x / Q, al = threadIdx.x % Q;
+        const int which = threadIdx.x / Q, aa = threadIdx.x % Q;
         double t = 0.0;
-        for (int sl = 0; sl < SLOTS; ++sl) t += red[sl][which][al];
+        for (int sl = 0; sl < NPB; ++sl) t += red[sl][which][aa];
         S.partial[int64_t(blockIdx.x) * 2 * Q + threadIdx.x] = t;
     }
 }
 
+// layer means (pml_absorption.inl:775-785, :816-826): one block per column
 template <int Q>
-__global__ void k_r2(StepGeom G, StepBuffers S) {
-    const int t = threadIdx.x;
-    if (t >= 2 * Q) return;
+__global__ __launch_bounds__(256) void k_means(StepGeom G, StepBuffers S) {
+    __shared__ double red[256];
+    const int col = blockIdx.x;  // 0..2Q-1
     double s = 0.0;
-    for (int p = 0; p < S.nparts; ++p) s += S.partial[int64_t(p) * 2 * Q + t];
-    S.means[t] = G.nlayer > 0 ? float(s / double(G.nlayer)) : 0.0f;
+    for (int p = threadIdx.x; p < S.nparts; p += 256) s += S.partial[int64_t(p) * 2 * Q + col];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S.means[col] = G.nlayer > 0 ? float(red[0] / double(G.nlayer)) : 0.0f;
+}
+
+// PML auxiliary fields on layer rows (pml_absorption.inl:440-453, :500-513):
+// Gf_a(j) = -2 sigma_j (Q1_a(p) - m1_a) + sigma_j^2 (Q2_a(p) - m2_a),
+// p = j - num_bulk (reference indexing) or j.  Rows outside the layer stay
+// zero, so the divergence pass needs no branch.
+template <int Q>
+__global__ __launch_bounds__(kBlock) void k_pml_field(StepGeom G, StepBuffers S) {
+    using L = Lattice<Q>;
+    constexpr int RS = L::RS;
+    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
+    if (m.node >= G.nlayer || !m.active) return;
+    const int64_t j = G.nbulk + m.node;
+    const int64_t p = G.pml_ref ? j - G.nbulk : j;
+    const int a = m.a;
+    const float sig = S.sigma[m.node];
+    const float q1 = S.Qp[p * RS + a], q2 = S.Q2v[p * RS + a];
+    S.Gf[j * RS + a] = -2.0f * sig * (q1 - S.means[a]) + sig * sig * (q2 - S.means[Q + a]);
 }
 
 // ---------------------------------------------------------------------------
@@ -224,7 +260,7 @@ struct Lanes {
 
 // sum_s (c_a . W_s) fin(idx_s)_a over the staged records rp (REC = 1 + D),
 // batches of 8 neighbours.
-template <int Q, int KT>
+template <int Q, int KT, int BATCH = 8>
 __device__ __forceinline__ float stencil_sum(const float* rp, int K, const float* fin, int a) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS, REC = 1 + D, KMAX = KT ? KT : 64;
@@ -232,12 +268,12 @@ __device__ __forceinline__ float stencil_sum(const float* rp, int K, const float
                 cz = D == 3 ? lane_c<Q>(a, 2) : 0.0f;
     float acc = 0.0f;
 #pragma unroll
-    for (int s0 = 0; s0 < KMAX; s0 += 8) {
+    for (int s0 = 0; s0 < KMAX; s0 += BATCH) {
         if (!KT && s0 >= K) break;
-        float v[8], gv[8];
-        uint32_t j[8];
+        float v[BATCH], gv[BATCH];
+        uint32_t j[BATCH];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < BATCH; ++u) {
             const int s = s0 + u;
             const bool in = KT ? (s < KT) : (s < K);
             const float* r = rp + (in ? s : 0) * REC;
@@ -247,17 +283,17 @@ __device__ __forceinline__ float stencil_sum(const float* rp, int K, const float
             v[u] = in ? w : 0.0f;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) gv[u] = fin[int64_t(j[u]) * RS + a];
+        for (int u = 0; u < BATCH; ++u) gv[u] = fin[int64_t(j[u]) * RS + a];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += v[u] * gv[u];
+        for (int u = 0; u < BATCH; ++u) acc += v[u] * gv[u];
     }
     return acc;
 }
 
 // ---------------------------------------------------------------------------
-// p2: divergence of the two PML auxiliary fields, added to s1
-// (pml_absorption.inl:787-851; field maps :440-453, :500-513).  Only fluid
-// nodes whose stencil reaches the layer contribute.
+// p2: divergence of the PML auxiliary fields, added to s1
+// (pml_absorption.inl:787-851), over the fluid nodes whose stencil reaches
+// the layer; the field rows are zero outside the layer.
 // ---------------------------------------------------------------------------
 template <int Q, int KT>
 __global__ __launch_bounds__(kBlock) void k_p2(StepGeom G, StepBuffers S) {
@@ -274,24 +310,8 @@ __global__ __launch_bounds__(kBlock) void k_p2(StepGeom G, StepBuffers S) {
     stage_node<Q, REC>(S.slab, i, valid && ln.in_wave, K, ln.al, rp);
     __syncthreads();
     if (!valid) return;
-    const int a = ln.a;
-    const float cx = lane_c<Q>(a, 0), cy = lane_c<Q>(a, 1),
-                cz = D == 3 ? lane_c<Q>(a, 2) : 0.0f;
-    const float m1 = S.means[a], m2 = S.means[Q + a];
-    float acc = 0.0f;
-    for (int s = 0; s < K; ++s) {
-        const float* r = rp + s * REC;
-        const int64_t j = int64_t(__float_as_uint(r[0]));
-        if (j < G.nbulk || j >= G.nfluid) continue;
-        const int64_t p = G.pml_ref ? j - G.nbulk : j;
-        const float sig = S.sigma[j - G.nbulk];
-        const float q1 = S.Qp[p * RS + a], q2 = S.Q2v[p * RS + a];
-        const float gf = -2.0f * sig * (q1 - m1) + sig * sig * (q2 - m2);
-        float v = cx * r[1] + cy * r[2];
-        if (D == 3) v += cz * r[3];
-        acc += v * gf;
-    }
-    if (ln.active) S.A[i * RS + a] += acc;
+    const float acc = stencil_sum<Q, KT>(rp, K, S.Gf, ln.a);
+    if (ln.active) S.A[i * RS + ln.a] += acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -389,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S) {
 //   stage 4: g = g + dt (acc + r/6)   (in place: each node reads only its
 //            own g row here; neighbours are gathered from A)
 // ---------------------------------------------------------------------------
-template <int Q, int STAGE, int KT>
+template <int Q, int STAGE, int KT, int BATCH>
 __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS, REC = 1 + D, KMAX = KT ? KT : 64;
@@ -397,39 +417,41 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S) {
     __shared__ float srec[NPB * KMAX * REC];
     const int K = KT ? KT : G.k;
     const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * NPB;
-    const int64_t nf = G.nfluid - node0;
-    stage_contiguous<REC>(S.slab, node0, nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB), K, srec);
-    __syncthreads();
     const Lanes<Q> ln;
     const int64_t i = node0 + ln.slot;
     const int a = ln.a;
+    const int64_t o = i * RS + a;
     const float* gin = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
     float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
-    if (STAGE == 1 && i >= G.nfluid && i < G.N) {
+    // own rows first: their HBM latency overlaps the record staging
+    const bool fluid = i < G.nfluid;
+    const float g0 = (i < G.N) ? S.g[o] : 0.0f;
+    const float acc0 = (STAGE != 1 && fluid) ? S.acc[o] : 0.0f;
+    const int64_t nf = G.nfluid - node0;
+    stage_contiguous<REC>(S.slab, node0, nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB), K, srec);
+    __syncthreads();
+    if (STAGE == 1 && !fluid && i < G.N) {
         if (ln.active) {
-            const float v = S.g[i * RS + a];
-            S.A[i * RS + a] = v;
-            S.B[i * RS + a] = v;
+            S.A[o] = g0;
+            S.B[o] = g0;
         }
         return;
     }
-    if (i >= G.nfluid) return;
-    const float rate = -stencil_sum<Q, KT>(srec + ln.slot * K * REC, K, gin, a);
+    if (!fluid) return;
+    const float rate = -stencil_sum<Q, KT, BATCH>(srec + ln.slot * K * REC, K, gin, a);
     if (!ln.active) return;
-    const int64_t o = i * RS + a;
-    const float g0 = S.g[o];
     const float dt = G.dt;
     if (STAGE == 1) {
         gout[o] = g0 + (dt * 0.5f) * rate;
         S.acc[o] = rate * (1.0f / 6.0f);
     } else if (STAGE == 2) {
         gout[o] = g0 + (dt * 0.5f) * rate;
-        S.acc[o] += rate * (2.0f / 6.0f);
+        S.acc[o] = acc0 + rate * (2.0f / 6.0f);
     } else if (STAGE == 3) {
         gout[o] = g0 + dt * rate;
-        S.acc[o] += rate * (2.0f / 6.0f);
+        S.acc[o] = acc0 + rate * (2.0f / 6.0f);
     } else {
-        gout[o] = g0 + dt * (S.acc[o] + rate * (1.0f / 6.0f));
+        gout[o] = g0 + dt * (acc0 + rate * (1.0f / 6.0f));
     }
 }
 
@@ -469,21 +491,43 @@ inline void check_launch(const char* what) {
 
 }  // namespace
 
+int stage_batch() {
+    static const int b = [] {
+        const char* e = std::getenv("NAGA_STAGE_BATCH");
+        const int v = e ? std::atoi(e) : 16;
+        return (v == 8 || v == 16 || v == 32) ? v : 16;
+    }();
+    return b;
+}
+
+template <int Q, int KT, int BATCH>
+void launch_stage_kb(int stage, int64_t nb, const StepGeom& G, const StepBuffers& S,
+                     hipStream_t st) {
+    switch (stage) {
+        case 1: k_stage<Q, 1, KT, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 2: k_stage<Q, 2, KT, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 3: k_stage<Q, 3, KT, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+        default: k_stage<Q, 4, KT, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+    }
+}
+
 template <int Q, int KT>
 void launch_stage_k(int stage, int64_t nb, const StepGeom& G, const StepBuffers& S,
                     hipStream_t st) {
-    switch (stage) {
-        case 1: k_stage<Q, 1, KT><<<nb, kBlock, 0, st>>>(G, S); break;
-        case 2: k_stage<Q, 2, KT><<<nb, kBlock, 0, st>>>(G, S); break;
-        case 3: k_stage<Q, 3, KT><<<nb, kBlock, 0, st>>>(G, S); break;
-        default: k_stage<Q, 4, KT><<<nb, kBlock, 0, st>>>(G, S); break;
-    }
+    const int b = KT ? stage_batch() : 8;
+    if (b == 32)
+        launch_stage_kb<Q, KT, 32>(stage, nb, G, S, st);
+    else if (b == 16)
+        launch_stage_kb<Q, KT, 16>(stage, nb, G, S, st);
+    else
+        launch_stage_kb<Q, KT, 8>(stage, nb, G, S, st);
 }
 
 template <int Q>
 void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S,
                       hipStream_t st) {
-    const int64_t nb = lane_blocks<Q>(stage == 1 ? G.N : G.nfluid);
+    const int64_t nrange = stage == 1 ? G.N : G.nfluid;
+    const int64_t nb = lane_blocks<Q>(nrange);
     if (nb == 0) return;
     if (G.k == 32)
         launch_stage_k<Q, 32>(stage, nb, G, S, st);
@@ -506,17 +550,21 @@ void launch_step(const StepGeom& G, const StepBuffers& S,
     tick();
     k_p1<Q><<<(G.N + kBlock - 1) / kBlock, kBlock, 0, st>>>(G, S, cp);
     check_launch("p1");
-    const bool pml = G.nlayer > 0;
+    const bool pml = G.nlayer > 0 && S.n_rq > 0;
     tick();
     if (pml) {
-        const int64_t rows = (G.nlayer + S.nparts - 1) / S.nparts;
-        k_r1<Q><<<S.nparts, kBlock, 0, st>>>(G, S, rows);
-        check_launch("r1");
+        k_pml_local<Q><<<S.nparts, kBlock, 0, st>>>(G, S);
+        check_launch("pml local");
     }
     tick();
     if (pml) {
-        k_r2<Q><<<1, 64, 0, st>>>(G, S);
-        check_launch("r2");
+        k_means<Q><<<2 * Q, 256, 0, st>>>(G, S);
+        check_launch("layer means");
+    }
+    tick();
+    if (pml) {
+        k_pml_field<Q><<<lane_blocks<Q>(G.nlayer), kBlock, 0, st>>>(G, S);
+        check_launch("pml field");
     }
     tick();
     if (pml && S.n_pml > 0) {

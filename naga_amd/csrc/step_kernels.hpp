@@ -12,9 +12,11 @@
 // reduction).  Kernels doing per-node algebra (macroscopic values, the FP64
 // collision, the PML recurrences) map one node to one thread.
 //
-// One time step is eleven launches:
-//   p1      macroscopic, FP64 collision, PML local terms     (all nodes)
-//   r1, r2  layer means of the two PML auxiliary fields      (layer rows)
+// One time step is twelve launches:
+//   p1      macroscopic values, FP64 collision                (all nodes)
+//   pl      PML local terms + partial layer sums              (nodes whose Q is used)
+//   pm      layer means of the two PML auxiliary values       (2Q columns)
+//   pf      PML auxiliary field rows                          (layer rows)
 //   p2      PML divergence of the auxiliary fields            (nodes with layer neighbours)
 //   p3      boundary interpolation of the post-PML state      (boundary;
 //           the sources are read before their collision increment, which
@@ -48,6 +50,10 @@ struct StepBuffers {
     float* A;        // [N][RS] stage ping
     float* B;        // [N][RS] stage pong
     float* acc;      // [N][RS] RK4 accumulator
+    float4* ru;      // [N] unitless (rho - 1, u) from p1
+    float* Gf;       // [N][RS] PML auxiliary field (zero outside the layer)
+    const int32_t* rq_list;  // nodes whose PML Q values are maintained
+    int64_t n_rq;
     float* E;        // [N][RS] previous equilibrium deviation (PML Q1 - w)
     float* Qp;       // [N][RS] previous Q1 value (PML Q2 source)
     float* Q2v;      // [N][RS] current Q2 value
@@ -62,15 +68,16 @@ struct StepBuffers {
     int64_t n_pml;
     float* rho;      // [N] density (reference semantics, FP32)
     float* u;        // [D][N] velocity
-    double* partial; // [nparts][2Q] layer column sums
+    double* partial; // [nparts][2Q] layer column sums (one row per pl block)
     float* means;    // [2Q] layer means of Q1, Q2
     int nparts;
 };
 
-// Launch the eleven kernels of one step on `stream`.  If `events` is not
-// null, events[t] is recorded before kernel t and events[11] after the last
-// (kernels with no work still get their pair of marks).
-constexpr int kStepKernels = 11;
+// Launch the kernels of one step on `stream`.  If `events` is not null,
+// events[t] is recorded before kernel t and events[kStepKernels] after the
+// last (kernels with no work still get their pair of marks).
+constexpr int kStepKernels = 12;
+constexpr int kPmlItemsPerBlock(int Q) { return (256 / 64) * (64 / (Q == 27 ? 32 : 9)) * 8; }
 template <int Q>
 void launch_step(const StepGeom& geo, const StepBuffers& buf,
                  const CollisionParams& cp, hipStream_t stream,

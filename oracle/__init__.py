@@ -27,7 +27,7 @@ class Config(C.Structure):
         ("rbf", C.c_int), ("rbf_boundary", C.c_int),
         ("pml_ref_index", C.c_int), ("threads", C.c_int),
         ("nu", C.c_double), ("rho0", C.c_double), ("dt", C.c_double),
-        ("c", C.c_double), ("h", C.c_double),
+        ("c", C.c_double), ("h", C.c_double), ("filter", C.c_double),
     ]
 
 
@@ -229,7 +229,8 @@ class OracleEngine:
                  k_boundary: int | None = None, rbf: int = 1,
                  rbf_boundary: int | None = None, tau: float = 0.6,
                  dt: float | None = None, h: float = 0.01, rho0: float = 1.0,
-                 pml_ref_index: int = 1, threads: int = 0, c=None):
+                 pml_ref_index: int = 1, threads: int = 0, c=None,
+                 filter_strength: float = 0.75):
         self.o = oracle
         self.Q = Q
         self.D = 2 if Q == 9 else 3
@@ -242,7 +243,8 @@ class OracleEngine:
         cfg = Config(Q=Q, k=k, k_boundary=self.kb, rbf=rbf,
                      rbf_boundary=rbf if rbf_boundary is None else rbf_boundary,
                      pml_ref_index=pml_ref_index, threads=threads,
-                     nu=ph["nu"], rho0=rho0, dt=ph["dt"], c=ph["c"], h=h)
+                     nu=ph["nu"], rho0=rho0, dt=ph["dt"], c=ph["c"], h=h,
+                     filter=filter_strength)
         self.cfg = cfg
         pts = np.ascontiguousarray(nodes["points"], np.float32).reshape(-1)
         nrm = np.ascontiguousarray(nodes["normals"], np.float32).reshape(-1)

@@ -1235,7 +1235,7 @@ static void eng_filter(no_engine* e) {
             no_real fi = s1[i], sum = 0;
             for (int s = 0; s < k; ++s)
                 sum += e->filt[i * k + s] * (s1[e->idx[i * k + s]] - fi);
-            e->s2[a * N + i] = fi + (no_real)0.75f * sum;
+            e->s2[a * N + i] = fi + (no_real)e->cfg.filter * sum;
         }
 }
 

@@ -135,6 +135,9 @@ typedef struct no_config {
     double dt;          /* physical time step */
     double c;           /* physical speed of sound */
     double h;           /* nodal spacing */
+    double filter;      /* selective filter strength: 0.75 in the reference
+                           (simulation_engine.cuh:1242); other values exist
+                           only for tests of the scheme without it */
 } no_config;
 
 no_engine* NO_FN(engine_create)(const no_config* cfg, const float* pts,

@@ -39,3 +39,44 @@ def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def mirror_nodes(D, h=0.05, m=8, n=17, layer=0.15, sigma_max=0.5, seed=3):
+    """A node set that is exactly symmetric under x -> -x.
+
+    A jittered half grid (x = (i + 1/2) h, i < m; other axes j h, j < n) is
+    reflected by negating x, which is exact in floating point, so every
+    FP32 distance, stencil and weight of a node equals that of its mirror
+    image with x flipped.  Boundary nodes close the box (i = m-1 or any other
+    axis at 0 or n-1); layer and absorption follow the provider's rules.
+    Returns the node dict and, per node, the index of its mirror image."""
+    rng = np.random.default_rng(seed)
+    grids = np.stack(np.meshgrid(*([np.arange(m)] + [np.arange(n)] * (D - 1)),
+                                 indexing="ij"), -1).reshape(-1, D)
+    bnd = (grids[:, 0] == m - 1) | np.any((grids[:, 1:] == 0) | (grids[:, 1:] == n - 1), axis=1)
+    x = grids.astype(np.float64) * h
+    x[:, 0] += 0.5 * h
+    x[~bnd] += (2 * rng.random((int((~bnd).sum()), D)) - 1) * 0.25 * h
+    nrm = np.zeros((len(x), D))
+    nrm[:, 0] = np.where(grids[:, 0] == m - 1, -1.0, 0.0)
+    nrm[:, 1:] = np.where(grids[:, 1:] == 0, 1.0, np.where(grids[:, 1:] == n - 1, -1.0, 0.0))
+    half = x.astype(np.float32)
+    flip = np.ones(D, np.float32)
+    flip[0] = -1
+    P = np.concatenate([half, half * flip])
+    bnd = np.concatenate([bnd, bnd])
+    nrm = np.concatenate([nrm, nrm * flip])
+    bp = P[bnd]
+    d2 = np.min([((P - b) ** 2).sum(-1) for b in bp], axis=0)
+    inl = ~bnd & (d2 <= np.float32(layer) ** 2)
+    blk = ~bnd & ~inl
+    order = np.concatenate([np.where(blk)[0], np.where(inl)[0], np.where(bnd)[0]])
+    r = (np.float32(layer) - np.sqrt(d2[order][blk.sum():blk.sum() + inl.sum()])) / np.float32(layer)
+    nb = nrm[order][-int(bnd.sum()):]
+    nodes = dict(points=np.ascontiguousarray(P[order]), n_bulk=int(blk.sum()),
+                 n_layer=int(inl.sum()), n_boundary=int(bnd.sum()),
+                 normals=(nb / np.linalg.norm(nb, axis=1, keepdims=True)).astype(np.float32),
+                 sigma=(sigma_max * r * r).astype(np.float32))
+    pos = {tuple(p): i for i, p in enumerate(nodes["points"].tolist())}
+    mirror = np.array([pos[(-p[0],) + tuple(p[1:])] for p in nodes["points"].tolist()])
+    return nodes, mirror
