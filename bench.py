@@ -29,6 +29,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+ROOFLINE_KERNEL = "k_stage_v4<2, 32, 8>"
+# HBM bytes per launch of ROOFLINE_KERNEL on this workload, from two
+# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) summarised by
+# naga_amd.profsum; the file records the commands.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_stage2_d3q27_h0.01_traffic.md")
+
+
+def measured_traffic(h: float):
+    if h != 0.01 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    text = open(TRAFFIC_PROFILE).read()
+    if f"`{ROOFLINE_KERNEL}`" not in text.splitlines()[0]:
+        return None  # the profile is of another kernel build
+    for line in text.splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def copy_calibration(nbytes: int) -> int:
+    """One device-to-device copy of nbytes (the runtime's copy kernel): the
+    known-size reference for scaling HBM byte counters."""
+    import torch
+    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copied_bytes": nbytes}))
+    return 0
 
 
 def stage_bytes_per_node(Q: int, D: int, k: int, stage: int) -> int:
@@ -61,7 +91,7 @@ def cpu_baseline(threads: int) -> dict:
     eng = o.engine(nodes, Q=27, k=32, h=h, tau=0.6, threads=threads)
     eng.set_pulse([0.3, 0.5, 0.5], 1e-3, 0.05)
     eng.step(1)  # warm-up
-    n = 3
+    n = 80  # about 10 s of CPU work on 16 threads
     t0 = time.perf_counter()
     eng.step(n)
     dt = time.perf_counter() - t0
@@ -80,7 +110,11 @@ def main() -> int:
     ap.add_argument("--h", type=float, default=0.01)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; multi-GPU sharding is "
               "designed (halo exchange per stencil pass) but not built", file=sys.stderr)
@@ -118,8 +152,9 @@ def main() -> int:
     sbytes = nfluid * stage_bytes_per_node(Q, D, k, 2)
     achieved = sbytes / (stage_ms * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                "kernel": "k_stage<27,2> (RK4 stage 2)",
+                "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": measured_traffic(h),
+                "kernel": ROOFLINE_KERNEL + " (RK4 stage 2)",
                 "bytes_per_launch": sbytes, "avg_ms": stage_ms}
     model = step_model_bytes(nodes, Q, D, k)
     out = {
@@ -127,7 +162,8 @@ def main() -> int:
         "value": value, "unit": "node-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (state), f64 (collision, setup)",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "dtype_note": "FP32 state and stencils; FP64 collision and setup solves",
         "data": "synthetic",
         "config": {"workload": f"D3Q27 unit cube h={h} jittered 0.25h, k=32 r^3, "
                                f"PML 8h sigma_max 0.5, tau 0.6, dt 0.4h",

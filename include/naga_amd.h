@@ -80,6 +80,18 @@ int naga_engine_init_domain(naga_engine* e, const float* points, int64_t n_bulk,
  * 79-125 applied as an initial state; resets time to 0. */
 int naga_engine_set_pulse(naga_engine* e, const double* center, double amp,
                           double width);
+/* Density source, the declarative replacement of a density_source plugin
+ * (density_source.cuh:41-82; applied as detail/simulation_engine.cuh:
+ * 347-364,1057-1107 do, with the density update made once per node).  At
+ * the start of each step at time t, every node within `radius` of `center`
+ * gets s = amplitude * cos(2 pi frequency t + phase) * P(r / radius),
+ * P(x) = (1 - (2x)^2 / 4) exp(-(2x)^2) (the profile of the reference
+ * example's source), while t_start <= t < t_end; sources add.  Where s != 0:
+ * f_alpha += w_alpha s / rho0 and rho <- rho + s / 2.  At most 16. */
+int naga_engine_add_density_source(naga_engine* e, const double* center, double radius,
+                                   double amplitude, double frequency, double phase,
+                                   double t_start, double t_end);
+int naga_engine_clear_density_sources(naga_engine* e);
 /* f[Q*N] alpha-major full distributions (FP64 in, stored as FP32 deviations). */
 int naga_engine_set_distributions(naga_engine* e, const double* f);
 

@@ -85,6 +85,8 @@ def load() -> C.CDLL:
         "naga_rpim_weights": (C.c_int, [vp, i64, vp, i32, vp, i64, i32, i32, f64, vp]),
         "naga_divergence_weights": (C.c_int, [vp, i64, vp, i64, i32, i32, i32, vp, vp]),
         "naga_divergence_apply": (C.c_int, [vp, vp, i64, i32, i32, vp, i64, C.c_float, vp]),
+        "naga_engine_add_density_source": (C.c_int, [vp, vp, f64, f64, f64, f64, f64, f64]),
+        "naga_engine_clear_density_sources": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -103,7 +105,8 @@ EXPORTED = [
     "naga_engine_get_setup", "naga_engine_get_pml", "naga_engine_info",
     "naga_uniform_grid_size", "naga_uniform_grid", "naga_knn",
     "naga_batched_lu_solve_f64", "naga_rpim_weights", "naga_divergence_weights",
-    "naga_divergence_apply",
+    "naga_divergence_apply", "naga_engine_add_density_source",
+    "naga_engine_clear_density_sources",
 ]
 
 STEP_KERNELS = ["p1_collide", "pl_pml_local", "pm_layer_means", "pf_pml_field",
@@ -230,6 +233,24 @@ def problem_parameters(lattice: int, tau: float, dt: float, c: float | None = No
     return dict(nu=nu_lat * ratio, dt=dt, c=c)
 
 
+class DensitySource:
+    """A density source (the declarative form of density_source<Lattice>,
+    density_source.cuh:41-82): at the start of each step at time t, nodes
+    within `radius` of `center` receive
+    s = amplitude * cos(2 pi frequency t + phase) * P(r / radius),
+    P(x) = (1 - (2x)^2 / 4) exp(-(2x)^2), while t_start <= t < t_end."""
+
+    def __init__(self, center, radius, amplitude, frequency=0.0, phase=0.0,
+                 t_start=0.0, t_end=float("inf")):
+        self.center = list(center)
+        self.radius = float(radius)
+        self.amplitude = float(amplitude)
+        self.frequency = float(frequency)
+        self.phase = float(phase)
+        self.t_start = float(t_start)
+        self.t_end = float(t_end)
+
+
 class SimulationEngine:
     """simulation_engine<Lattice> (fluids/nonlocal_lattice_boltzmann/
     simulation_engine.cuh:39-121) backed by one GPU-resident scene."""
@@ -327,6 +348,19 @@ class SimulationEngine:
     def reset(self):
         """reset() (detail/simulation_engine.cuh:1377-1386): f = w, time 0."""
         self.set_initial_pulse(np.zeros(self.dimensions), 0.0, 1.0)
+
+    def register_density_source(self, source: "DensitySource"):
+        """register_density_source (simulation_engine.cuh:79-81) for a
+        declarative source; see DensitySource."""
+        c = np.zeros(3)
+        c[: self.dimensions] = np.asarray(source.center, np.float64)[: self.dimensions]
+        _check(load().naga_engine_add_density_source(
+            self._live(), _ptr(c), float(source.radius), float(source.amplitude),
+            float(source.frequency), float(source.phase), float(source.t_start),
+            float(source.t_end)), _ENGINE_PREFIX)
+
+    def clear_density_sources(self):
+        _check(load().naga_engine_clear_density_sources(self._live()), _ENGINE_PREFIX)
 
     def set_distributions(self, f):
         _check(load().naga_engine_set_distributions(self._live(), _ptr(_f64(f))),

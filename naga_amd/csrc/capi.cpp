@@ -150,6 +150,28 @@ int naga_engine_set_pulse(naga_engine* e, const double* center, double amp, doub
     });
 }
 
+int naga_engine_add_density_source(naga_engine* e, const double* center, double radius,
+                                   double amplitude, double frequency, double phase,
+                                   double t_start, double t_end) {
+    return guarded([&] {
+        require(center != nullptr, "null center");
+        DensitySource s;
+        Engine& eng = engine_of(e);
+        for (int d = 0; d < eng.dims(); ++d) s.center[d] = center[d];
+        s.radius = radius;
+        s.amplitude = amplitude;
+        s.frequency = frequency;
+        s.phase = phase;
+        s.t_start = t_start;
+        s.t_end = t_end;
+        eng.add_density_source(s);
+    });
+}
+
+int naga_engine_clear_density_sources(naga_engine* e) {
+    return guarded([&] { engine_of(e).clear_density_sources(); });
+}
+
 int naga_engine_set_distributions(naga_engine* e, const double* f) {
     return guarded([&] {
         require(f != nullptr, "null distributions");

@@ -114,6 +114,9 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
 
     pts_.allocate(N * D);
     pts_.upload(pts, N * D, st);
+    hpts_.assign(pts, pts + N * D);
+    sources_.clear();
+    n_snodes_ = 0;
 
     // divergence operator over all nodes (init_domain :169-181)
     {
@@ -287,7 +290,58 @@ StepBuffers Engine::buffers() {
     S.partial = partial_.get();
     S.means = means_.get();
     S.nparts = nparts_;
+    S.src = sources_.empty() ? nullptr : src_field_.get();
     return S;
+}
+
+// Density sources (detail/simulation_engine.cuh:347-364, 1057-1107): the
+// nodes inside any source region, with their (source, profile) entries in
+// source order, are listed once; each step a small kernel writes the
+// step's source value of every listed node into src_field_, read by p1.
+void Engine::add_density_source(const DensitySource& src) {
+    require_domain();
+    require(int(sources_.size()) < kMaxSources, "at most 16 density sources per engine");
+    require(src.radius > 0.0, "density source radius must be positive");
+    sources_.push_back(src);
+    const int D = D_;
+    std::vector<int32_t> nodes, off{0}, ids;
+    std::vector<float> prof;
+    for (int64_t i = 0; i < N_; ++i) {
+        bool any = false;
+        for (size_t j = 0; j < sources_.size(); ++j) {
+            const DensitySource& S = sources_[j];
+            double r2 = 0.0;
+            for (int d = 0; d < D; ++d) {
+                const double dx = double(hpts_[i * D + d]) - S.center[d];
+                r2 += dx * dx;
+            }
+            const double r = std::sqrt(r2);
+            if (r < S.radius) {
+                const double x = 2.0 * r / S.radius;
+                ids.push_back(int32_t(j));
+                prof.push_back(float((1.0 - x * x / 4.0) * std::exp(-x * x)));
+                any = true;
+            }
+        }
+        if (any) {
+            nodes.push_back(int32_t(i));
+            off.push_back(int32_t(ids.size()));
+        }
+    }
+    n_snodes_ = int64_t(nodes.size());
+    snodes_ = to_device(nodes, stream_);
+    soff_ = to_device(off, stream_);
+    sids_ = to_device(ids.empty() ? std::vector<int32_t>{0} : ids, stream_);
+    sprof_ = to_device(prof.empty() ? std::vector<float>{0.0f} : prof, stream_);
+    if (src_field_.size() != size_t(N_)) src_field_.allocate(N_);
+    src_field_.zero(stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::clear_density_sources() {
+    require_domain();
+    sources_.clear();
+    n_snodes_ = 0;
 }
 
 void Engine::refresh_macroscopic_state() {
@@ -299,6 +353,18 @@ void Engine::refresh_macroscopic_state() {
 
 void Engine::launch_one_step(hipEvent_t* ev) {
     const StepBuffers S = buffers();
+    if (!sources_.empty()) {
+        SourceAmps amps{};
+        for (size_t j = 0; j < sources_.size(); ++j) {
+            const DensitySource& D = sources_[j];
+            const bool on = time_ >= D.t_start && time_ < D.t_end;
+            amps.a[j] = on ? float(D.amplitude * std::cos(2.0 * M_PI * D.frequency * time_ +
+                                                          D.phase))
+                           : 0.0f;
+        }
+        launch_source_field(snodes_.get(), soff_.get(), sids_.get(), sprof_.get(), n_snodes_,
+                            amps, src_field_.get(), stream_);
+    }
     if (cfg_.Q == 27)
         launch_step<27>(geo_, S, cp_, stream_, ev);
     else

@@ -21,6 +21,15 @@ struct SceneConfig {
     double nu = 0.0, rho0 = 1.0, dt = 0.0, c = kCsLat, h = 0.01;
 };
 
+// A declarative density source: amplitude * cos(2 pi frequency t + phase)
+// times the radial profile (1 - (2r/R)^2 / 4) exp(-(2r/R)^2) within radius
+// R of the centre, for t_start <= t < t_end.
+struct DensitySource {
+    double center[3] = {0.0, 0.0, 0.0};
+    double radius = 0.0, amplitude = 0.0, frequency = 0.0, phase = 0.0;
+    double t_start = 0.0, t_end = 0.0;
+};
+
 class Engine {
   public:
     explicit Engine(const SceneConfig& cfg);
@@ -39,6 +48,8 @@ class Engine {
     void get_distributions(double* f, double* g);
     void get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, float* bw);
     void get_pml(double* E, double* Qp);
+    void add_density_source(const DensitySource& src);
+    void clear_density_sources();
 
     int64_t nodes() const { return N_; }
     double time() const { return time_; }
@@ -80,6 +91,12 @@ class Engine {
     DeviceBuffer<float> means_;
     int nparts_ = 1;
     int64_t n_pml_ = 0;
+
+    std::vector<float> hpts_;  // host copy of the node coordinates
+    std::vector<DensitySource> sources_;
+    DeviceBuffer<int32_t> snodes_, soff_, sids_;
+    DeviceBuffer<float> sprof_, src_field_;
+    int64_t n_snodes_ = 0;
 };
 
 // Exact kNN of queries among ref (all host arrays), on the GPU.

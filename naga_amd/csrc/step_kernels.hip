@@ -76,10 +76,18 @@ __global__ __launch_bounds__(kBlock) void k_p1(StepGeom G, StepBuffers S,
             for (int d = 0; d < D; ++d) mom[d] += gv * double(L::c(a, d));
             f[a] = L::w(a) + gv;
         }
-        const double rho = 1.0 + rt;
         double u[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int d = 0; d < D; ++d) u[d] = mom[d] / rho;
+        for (int d = 0; d < D; ++d) u[d] = mom[d] / (1.0 + rt);
+        // density source (simulation_engine.cuh:1057-1107, once per node):
+        // f += w s / rho0, rho <- (rho + (rho + s)) / 2, u unchanged
+        const double ws = S.src ? double(S.src[i]) / double(G.rho0) : 0.0;
+        if (ws != 0.0) {
+#pragma unroll
+            for (int a = 0; a < Q; ++a) f[a] += L::w(a) * ws;
+            rt += 0.5 * ws;
+        }
+        const double rho = 1.0 + rt;
         const bool bnd = i >= G.nfluid;
         S.rho[i] = float(rho * double(G.rho0));
 #pragma unroll
@@ -91,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_p1(StepGeom G, StepBuffers S,
         const bool src_node = !bnd && S.bsrc[i];
 #pragma unroll
         for (int a = 0; a < Q; ++a) {
-            const double v = bnd ? df[a] : double(r[a]) + df[a];
+            const double v = bnd ? df[a] : double(r[a]) + L::w(a) * ws + df[a];
             r[a] = float(v);
             if (src_node) S.coll[i * RS + a] = float(df[a]);
         }
@@ -455,6 +463,225 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// D3Q27 kernels with four distributions per lane.  A node's 128-byte row is
+// served by 8 lanes, each holding alpha = 4q .. 4q+3 (q = lane % 8; the last
+// five slots of the row are padding with zero lattice velocity), so a
+// wavefront covers 8 nodes and every neighbour gather is one 16-byte load
+// per lane.  Rows are addressed through buffer resources with 32-bit byte
+// offsets (row j at j * 128), which bounds the node count by 2^25; loads
+// past the end of a buffer return zero instead of faulting.
+// ---------------------------------------------------------------------------
+constexpr int kV4Lanes = 8;                   // lanes per node
+constexpr int kV4Nodes = kBlock / kV4Lanes;  // nodes per block
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint64_t bytes) {
+    const uint32_t n = bytes > 0xffffffffull ? 0xffffffffu : uint32_t(bytes);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), int(n), 0x00020000);
+}
+
+__device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
+                       __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
+__device__ __forceinline__ void st4(float* base, uint32_t off, float4 v) {
+    *reinterpret_cast<float4*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+__device__ __forceinline__ float get(const float4& v, int e) {
+    return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+}
+
+__device__ __forceinline__ void set(float4& v, int e, float x) {
+    if (e == 0) v.x = x;
+    else if (e == 1) v.y = x;
+    else if (e == 2) v.z = x;
+    else v.w = x;
+}
+
+// lattice velocity component of slot alpha (zero on padding slots)
+__device__ __forceinline__ float c27(int a, int d) {
+    return a < 27 ? float(Lattice<27>::c(a, d)) : 0.0f;
+}
+
+// T_d[e] = sum_s W_d(s) gin(idx_s)[4q + e] over the staged records rp
+// ({idx, W_x, W_y, W_z} per neighbour); returns -sum_d c_d T_d per slot.
+template <int K, int BATCH>
+__device__ __forceinline__ float4 rate_v4(const float4* rp, __amdgpu_buffer_rsrc_t gin,
+                                          int q) {
+    float tx[4] = {0.f, 0.f, 0.f, 0.f}, ty[4] = {0.f, 0.f, 0.f, 0.f},
+          tz[4] = {0.f, 0.f, 0.f, 0.f};
+    const uint32_t lane_off = uint32_t(q) * 16u;
+#pragma unroll
+    for (int s0 = 0; s0 < K; s0 += BATCH) {
+        float4 rec[BATCH], gv[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) rec[u] = rp[s0 + u];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u)
+            gv[u] = ld4(gin, __float_as_uint(rec[u].x) * 128u + lane_off);
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float v = get(gv[u], e);
+                tx[e] += rec[u].y * v;
+                ty[e] += rec[u].z * v;
+                tz[e] += rec[u].w * v;
+            }
+    }
+    float4 r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int a = 4 * q + e;
+        set(r, e, -(c27(a, 0) * tx[e] + c27(a, 1) * ty[e] + c27(a, 2) * tz[e]));
+    }
+    return r;
+}
+
+template <int STAGE, int K, int BATCH>
+__global__ __launch_bounds__(kBlock) void k_stage_v4(StepGeom G, StepBuffers S) {
+    __shared__ float4 srec[kV4Nodes * K];
+    const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * kV4Nodes;
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const int64_t i = node0 + slot;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
+    const float* gin_p = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
+    float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
+    const bool fluid = i < G.nfluid;
+    // own rows first: their latency overlaps the record staging
+    const float4 g0 = ld4(rsrc(S.g, rows), own);
+    const float4 acc0 = STAGE != 1 ? ld4(rsrc(S.acc, rows), own) : make_float4(0.f, 0.f, 0.f, 0.f);
+    {
+        const int64_t nf = G.nfluid - node0;
+        const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+        const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
+        for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
+    }
+    __syncthreads();
+    if (STAGE == 1 && !fluid && i < G.N) {  // boundary rows: constant through the stages
+        st4(S.A, own, g0);
+        st4(S.B, own, g0);
+        return;
+    }
+    if (!fluid) return;
+    const float4 r = rate_v4<K, BATCH>(srec + slot * K, rsrc(gin_p, rows), q);
+    const float dt = G.dt;
+    float4 o, a;
+    if (STAGE == 1) {
+        o = make_float4(g0.x + (dt * 0.5f) * r.x, g0.y + (dt * 0.5f) * r.y,
+                        g0.z + (dt * 0.5f) * r.z, g0.w + (dt * 0.5f) * r.w);
+        a = make_float4(r.x * (1.0f / 6.0f), r.y * (1.0f / 6.0f), r.z * (1.0f / 6.0f),
+                        r.w * (1.0f / 6.0f));
+    } else if (STAGE == 2 || STAGE == 3) {
+        const float h = STAGE == 2 ? dt * 0.5f : dt;
+        o = make_float4(g0.x + h * r.x, g0.y + h * r.y, g0.z + h * r.z, g0.w + h * r.w);
+        a = make_float4(acc0.x + r.x * (2.0f / 6.0f), acc0.y + r.y * (2.0f / 6.0f),
+                        acc0.z + r.z * (2.0f / 6.0f), acc0.w + r.w * (2.0f / 6.0f));
+    } else {
+        o = make_float4(g0.x + dt * (acc0.x + r.x * (1.0f / 6.0f)),
+                        g0.y + dt * (acc0.y + r.y * (1.0f / 6.0f)),
+                        g0.z + dt * (acc0.z + r.z * (1.0f / 6.0f)),
+                        g0.w + dt * (acc0.w + r.w * (1.0f / 6.0f)));
+    }
+    st4(gout, own, o);
+    if (STAGE != 4) st4(S.acc, own, a);
+}
+
+// p4 for D3Q27 with four distributions per lane: selective filter on fluid
+// nodes, bounce-back on boundary nodes.
+template <int K, int BATCH>
+__global__ __launch_bounds__(kBlock) void k_p4_v4(StepGeom G, StepBuffers S) {
+    __shared__ float2 srec[kV4Nodes * K];
+    const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * kV4Nodes;
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const int64_t i = node0 + slot;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
+    const __amdgpu_buffer_rsrc_t s1 = rsrc(S.A, rows);
+    const float4 si = ld4(s1, own);
+    {
+        const int64_t nf = G.nfluid - node0;
+        const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+        const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
+        for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
+    }
+    __syncthreads();
+    if (i >= G.N) return;
+    float4 out;
+    if (i < G.nfluid) {
+        const float2* rp = srec + slot * K;
+        float sum[4] = {0.f, 0.f, 0.f, 0.f};
+        const uint32_t lane_off = uint32_t(q) * 16u;
+#pragma unroll
+        for (int s0 = 0; s0 < K; s0 += BATCH) {
+            float2 rec[BATCH];
+            float4 gv[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) rec[u] = rp[s0 + u];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u)
+                gv[u] = ld4(s1, __float_as_uint(rec[u].x) * 128u + lane_off);
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) sum[e] += rec[u].y * (get(gv[u], e) - get(si, e));
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) set(out, e, get(si, e) + G.filter * sum[e]);
+    } else {
+        const int64_t b = i - G.nfluid;
+        const float n0 = S.normals[b * 3], n1 = S.normals[b * 3 + 1], n2 = S.normals[b * 3 + 2];
+        const float* row = S.A + i * 32;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int a = 4 * q + e;
+            const float cn = c27(a, 0) * n0 + c27(a, 1) * n1 + c27(a, 2) * n2;
+            const int src = (a < 27 && cn > 0.0f) ? Lattice<27>::bb(a) : a;
+            set(out, e, src == a ? get(si, e) : row[src]);
+        }
+    }
+    st4(S.g, own, out);
+}
+
+// p2 for D3Q27 with four distributions per lane (PML divergence over the
+// listed nodes, added to s1).
+template <int K, int BATCH>
+__global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S) {
+    __shared__ float4 srec[kV4Nodes * K];
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const int64_t item = xcd_tile(blockIdx.x, gridDim.x) * kV4Nodes + slot;
+    const bool valid = item < S.n_pml;
+    const int64_t i = valid ? S.pml_list[item] : 0;
+    float4* rp = srec + slot * K;
+    if (valid) {
+        const float4* src = reinterpret_cast<const float4*>(S.slab) + i * K;
+        for (int r = q; r < K; r += kV4Lanes) rp[r] = src[r];
+    }
+    __syncthreads();
+    if (!valid) return;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const float4 r = rate_v4<K, BATCH>(rp, rsrc(S.Gf, rows), q);
+    // the divergence of F = c Gf is the negated rate of Gf
+    const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
+    const float4 s = ld4(rsrc(S.A, rows), own);
+    st4(S.A, own, make_float4(s.x - r.x, s.y - r.y, s.z - r.z, s.w - r.w));
+}
+
+__global__ __launch_bounds__(kBlock) void k_source_field(const int32_t* nodes,
+                                                       const int32_t* off, const int32_t* ids,
+                                                       const float* prof, int64_t n,
+                                                       SourceAmps amps, float* field) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int e = off[i]; e < off[i + 1]; ++e) s += prof[e] * amps.a[ids[e]];
+    field[nodes[i]] = s;
+}
+
 template <int Q>
 __global__ __launch_bounds__(kBlock) void k_macro(StepGeom G, const float* g,
                                                   float* rho, double* drho,
@@ -523,9 +750,60 @@ void launch_stage_k(int stage, int64_t nb, const StepGeom& G, const StepBuffers&
         launch_stage_kb<Q, KT, 8>(stage, nb, G, S, st);
 }
 
+// The four-per-lane D3Q27 kernels need k in {16, 32} and 32-bit row offsets.
+// NAGA_V4=0 selects the one-per-lane kernels; NAGA_V4_BATCH (4 or 8) sets
+// their gather batch.
+template <int Q>
+bool use_v4(const StepGeom& G) {
+    static const bool on = [] {
+        const char* e = std::getenv("NAGA_V4");
+        return !(e && e[0] == '0');
+    }();
+    return Q == 27 && on && (G.k == 32 || G.k == 16) && G.N < (int64_t(1) << 25);
+}
+
+int v4_batch() {
+    static const int b = [] {
+        const char* e = std::getenv("NAGA_V4_BATCH");
+        const int v = e ? std::atoi(e) : 8;
+        return v == 4 ? 4 : 8;
+    }();
+    return b;
+}
+
+template <int K, int BATCH>
+void launch_stage_v4_b(int stage, int64_t nb, const StepGeom& G, const StepBuffers& S,
+                       hipStream_t st) {
+    switch (stage) {
+        case 1: k_stage_v4<1, K, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 2: k_stage_v4<2, K, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 3: k_stage_v4<3, K, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+        default: k_stage_v4<4, K, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+    }
+}
+
+template <int K>
+void launch_stage_v4(int stage, const StepGeom& G, const StepBuffers& S, hipStream_t st) {
+    const int64_t nrange = stage == 1 ? G.N : G.nfluid;
+    const int64_t nb = (nrange + kV4Nodes - 1) / kV4Nodes;
+    if (nb == 0) return;
+    if (v4_batch() == 4)
+        launch_stage_v4_b<K, 4>(stage, nb, G, S, st);
+    else
+        launch_stage_v4_b<K, 8>(stage, nb, G, S, st);
+}
+
 template <int Q>
 void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S,
                       hipStream_t st) {
+    if (use_v4<Q>(G)) {
+        if (G.k == 32)
+            launch_stage_v4<32>(stage, G, S, st);
+        else
+            launch_stage_v4<16>(stage, G, S, st);
+        check_launch("rk4 stage");
+        return;
+    }
     const int64_t nrange = stage == 1 ? G.N : G.nfluid;
     const int64_t nb = lane_blocks<Q>(nrange);
     if (nb == 0) return;
@@ -567,7 +845,15 @@ void launch_step(const StepGeom& G, const StepBuffers& S,
         check_launch("pml field");
     }
     tick();
-    if (pml && S.n_pml > 0) {
+    const bool v4 = use_v4<Q>(G);
+    if (pml && S.n_pml > 0 && v4) {
+        const int64_t nb = (S.n_pml + kV4Nodes - 1) / kV4Nodes;
+        if (G.k == 32)
+            k_p2_v4<32, 8><<<nb, kBlock, 0, st>>>(G, S);
+        else
+            k_p2_v4<16, 8><<<nb, kBlock, 0, st>>>(G, S);
+        check_launch("p2");
+    } else if (pml && S.n_pml > 0) {
         if (G.k == 32)
             k_p2<Q, 32><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
         else if (G.k == 16)
@@ -582,7 +868,11 @@ void launch_step(const StepGeom& G, const StepBuffers& S,
         check_launch("p3");
     }
     tick();
-    if (G.k == 32)
+    if (v4 && G.k == 32)
+        k_p4_v4<32, 8><<<(G.N + kV4Nodes - 1) / kV4Nodes, kBlock, 0, st>>>(G, S);
+    else if (v4)
+        k_p4_v4<16, 8><<<(G.N + kV4Nodes - 1) / kV4Nodes, kBlock, 0, st>>>(G, S);
+    else if (G.k == 32)
         k_p4<Q, 32><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
     else if (G.k == 16)
         k_p4<Q, 16><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
@@ -599,6 +889,15 @@ void launch_step(const StepGeom& G, const StepBuffers& S,
         check_launch("p5");
     }
     tick();
+}
+
+void launch_source_field(const int32_t* nodes, const int32_t* off, const int32_t* ids,
+                         const float* prof, int64_t n, const SourceAmps& amps, float* field,
+                         hipStream_t st) {
+    if (n == 0) return;
+    k_source_field<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nodes, off, ids, prof, n, amps,
+                                                                 field);
+    check_launch("density source");
 }
 
 template <int Q>

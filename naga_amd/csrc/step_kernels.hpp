@@ -71,7 +71,18 @@ struct StepBuffers {
     double* partial; // [nparts][2Q] layer column sums (one row per pl block)
     float* means;    // [2Q] layer means of Q1, Q2
     int nparts;
+    const float* src;  // [N] density source of this step (physical units), or null
 };
+
+// Density sources: per step, field[nodes[i]] = sum over entries e in
+// [off[i], off[i+1]) of prof[e] * amp[ids[e]] (sources in a fixed order).
+constexpr int kMaxSources = 16;
+struct SourceAmps {
+    float a[kMaxSources];
+};
+void launch_source_field(const int32_t* nodes, const int32_t* off, const int32_t* ids,
+                         const float* prof, int64_t n, const SourceAmps& amps, float* field,
+                         hipStream_t stream);
 
 // Launch the kernels of one step on `stream`.  If `events` is not null,
 // events[t] is recorded before kernel t and events[kStepKernels] after the

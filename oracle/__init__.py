@@ -93,6 +93,9 @@ class Oracle:
                          [C.c_void_p, _f64p, C.c_double, C.c_double])
         self._setf = fn("engine_set_distributions", C.c_int, [C.c_void_p, _f64p])
         self._step = fn("engine_step", C.c_int, [C.c_void_p, C.c_int])
+        self._add_src = fn("engine_add_density_source", C.c_int,
+                           [C.c_void_p, _f64p] + [C.c_double] * 6)
+        self._clear_src = fn("engine_clear_density_sources", C.c_int, [C.c_void_p])
         opt64 = C.c_void_p
         self._get = fn("engine_get", C.c_int,
                        [C.c_void_p, opt64, opt64, opt64, opt64, C.c_int])
@@ -268,6 +271,16 @@ class OracleEngine:
 
     def set_distributions(self, f):
         self.o._setf(self.h, np.ascontiguousarray(f, np.float64).reshape(-1))
+
+    def add_density_source(self, center, radius, amplitude, frequency=0.0, phase=0.0,
+                           t_start=0.0, t_end=float("inf")):
+        c = np.zeros(3)
+        c[: self.D] = np.asarray(center, np.float64)[: self.D]
+        if self.o._add_src(self.h, c, radius, amplitude, frequency, phase, t_start, t_end):
+            raise ValueError("density source rejected")
+
+    def clear_density_sources(self):
+        self.o._clear_src(self.h)
 
     def step(self, n=1):
         self.o._step(self.h, n)

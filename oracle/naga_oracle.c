@@ -1078,6 +1078,9 @@ struct no_engine {
     int bb[27];
     no_real rho0, vel_scale, dt_lat, nu_lat, dt;
     double time;
+    int nsrc;
+    double src[NO_MAX_SOURCES][9]; /* centre[3], radius, amplitude, frequency,
+                                      phase, t_start, t_end */
 };
 
 static void eng_macroscopic(no_engine* e, const no_real* f) {
@@ -1097,6 +1100,74 @@ static void eng_macroscopic(no_engine* e, const no_real* f) {
         for (int d = 0; d < D; ++d)
             e->u[d * N + i] = u[d] * e->vel_scale;
     }
+}
+
+/* Density sources: the deterministic restatement of the density_source
+ * plugin path (simulation_engine.cuh:347-364 computes the per-node source
+ * terms at the step's start time, :1057-1107 applies them).  Source j adds
+ *   s_j(x, t) = A_j cos(2 pi f_j t + phi_j) P(|x - c_j| / R_j)
+ * for |x - c_j| < R_j and t_start <= t < t_end, with the radial profile of
+ * the reference example's source, P(r) = (1 - (2r)^2 / 4) exp(-(2r)^2)
+ * (examples/nonlocal_lattice_boltzmann.cu:97-106).  Where the summed source
+ * s is non-zero: f_a += w_a s / rho0 and rho <- (rho + (rho + s)) / 2,
+ * once per node (the reference repeats the density update once per
+ * distribution without synchronisation); u is unchanged. */
+static double src_profile(double r, double R) {
+    const double x = 2.0 * r / R;
+    return (1.0 - x * x / 4.0) * exp(-x * x);
+}
+
+static void eng_sources(no_engine* e) {
+    if (!e->nsrc) return;
+    const int Q = e->Q, D = e->D;
+    const int64_t N = e->N;
+    const double t = e->time;
+    double amp[NO_MAX_SOURCES];
+    for (int j = 0; j < e->nsrc; ++j) {
+        const double* S = e->src[j];
+        amp[j] = (t >= S[7] && t < S[8]) ? S[4] * cos(2.0 * M_PI * S[5] * t + S[6]) : 0.0;
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        double s = 0.0;
+        int hit = 0;
+        for (int j = 0; j < e->nsrc; ++j) {
+            const double* S = e->src[j];
+            double r2 = 0.0;
+            for (int d = 0; d < D; ++d) {
+                const double dx = (double)e->pts[i * D + d] - S[d];
+                r2 += dx * dx;
+            }
+            const double r = sqrt(r2);
+            if (r < S[3]) {
+                s += src_profile(r, S[3]) * amp[j];
+                hit = 1;
+            }
+        }
+        if (!hit || s == 0.0) continue;
+        for (int a = 0; a < Q; ++a)
+            e->f[a * N + i] += (no_real)(lat_w(Q, a) * s / (double)e->rho0);
+        e->rho[i] = (no_real)(((double)e->rho[i] + ((double)e->rho[i] + s)) / 2.0);
+    }
+}
+
+int NO_FN(engine_add_density_source)(no_engine* e, const double* center, double radius,
+                                     double amplitude, double frequency, double phase,
+                                     double t_start, double t_end) {
+    if (e->nsrc >= NO_MAX_SOURCES || !(radius > 0)) return 1;
+    double* S = e->src[e->nsrc++];
+    for (int d = 0; d < 3; ++d) S[d] = d < e->D ? center[d] : 0.0;
+    S[3] = radius;
+    S[4] = amplitude;
+    S[5] = frequency;
+    S[6] = phase;
+    S[7] = t_start;
+    S[8] = t_end;
+    return 0;
+}
+
+int NO_FN(engine_clear_density_sources)(no_engine* e) {
+    e->nsrc = 0;
+    return 0;
 }
 
 /* divergence of F_d(j) = c_d * (-(f(j) - w)) at node i (pde_right_hand_side
@@ -1422,6 +1493,7 @@ int NO_FN(engine_set_pulse)(no_engine* e, const double* center, double amp,
             e->f[a * N + i] = (no_real)(lat_w(e->Q, a) * (1.0 + p));
     }
     eng_macroscopic(e, e->f);
+    e->time = 0.0;
     return 0;
 }
 
@@ -1436,6 +1508,7 @@ int NO_FN(engine_step)(no_engine* e, int nsteps) {
     const int64_t QN = (int64_t)e->Q * e->N;
     for (int n = 0; n < nsteps; ++n) {
         eng_macroscopic(e, e->f);
+        eng_sources(e);
         memcpy(e->s1, e->f, sizeof(no_real) * QN);
         eng_pml(e);
         eng_interp_boundary(e, e->s1);

@@ -161,6 +161,13 @@ int NO_FN(engine_get_setup)(no_engine* e, uint32_t* idx, double* W,
                             double* filt, uint32_t* bidx, double* bw);
 int NO_FN(engine_get_pml)(no_engine* e, double* Q1, double* Q2);
 double NO_FN(engine_time)(no_engine* e);
+/* density sources (see eng_sources in naga_oracle.c); at most
+ * NO_MAX_SOURCES; returns non-zero when full or radius <= 0 */
+#define NO_MAX_SOURCES 16
+int NO_FN(engine_add_density_source)(no_engine* e, const double* center, double radius,
+                                     double amplitude, double frequency, double phase,
+                                     double t_start, double t_end);
+int NO_FN(engine_clear_density_sources)(no_engine* e);
 
 #ifdef __cplusplus
 }

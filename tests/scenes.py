@@ -66,12 +66,14 @@ def mirror_nodes(D, h=0.05, m=8, n=17, layer=0.15, sigma_max=0.5, seed=3):
     P = np.concatenate([half, half * flip])
     bnd = np.concatenate([bnd, bnd])
     nrm = np.concatenate([nrm, nrm * flip])
-    bp = P[bnd]
-    d2 = np.min([((P - b) ** 2).sum(-1) for b in bp], axis=0)
-    inl = ~bnd & (d2 <= np.float32(layer) ** 2)
+    from scipy.spatial import cKDTree
+    # distance to the nearest boundary node (negating x leaves it unchanged)
+    dist = cKDTree(P[bnd].astype(np.float64)).query(P.astype(np.float64))[0]
+    inl = ~bnd & (dist <= layer)
     blk = ~bnd & ~inl
     order = np.concatenate([np.where(blk)[0], np.where(inl)[0], np.where(bnd)[0]])
-    r = (np.float32(layer) - np.sqrt(d2[order][blk.sum():blk.sum() + inl.sum()])) / np.float32(layer)
+    dl = dist[order][blk.sum():blk.sum() + inl.sum()].astype(np.float32)
+    r = (np.float32(layer) - dl) / np.float32(layer)
     nb = nrm[order][-int(bnd.sum()):]
     nodes = dict(points=np.ascontiguousarray(P[order]), n_bulk=int(blk.sum()),
                  n_layer=int(inl.sum()), n_boundary=int(bnd.sum()),

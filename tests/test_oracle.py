@@ -483,3 +483,35 @@ def test_engine_pulse_speed(oracle64):
     t1 = _peak_times(oracle64, 0.75, radii, width)
     speed1 = np.polyfit(t1, radii, 1)[0] / cs
     assert 0.94 < speed1 < 0.98, speed1
+
+
+def test_density_source(oracle64):
+    """Declarative density source (detail/simulation_engine.cuh:347-364,
+    1057-1107): from rest, one step with a source active during [0, dt)
+    sets the step's density output to rho0 + s/2 at every node (the
+    reference's rho <- (rho + (rho + s)) / 2, once per node) with
+    s = A P(r / R), P(x) = (1 - (2x)^2/4) exp(-(2x)^2), and adds
+    sum(s) / rho0 of mass (to the scheme's non-conservation, < 0.5%); a
+    source that starts later leaves the rest state untouched until then."""
+    h, rho0, A, R = 0.02, 1.2, 1e-3, 0.1
+    nd = oracle64.uniform_grid(2, [0, 0], [1, 1], h, jitter=0.25, layer=0.1, sigma_max=0.5)
+    dt = 0.4 * h
+    e = oracle64.engine(nd, Q=9, k=16, h=h, rho0=rho0)
+    e.set_pulse([0.5, 0.5], 0.0, 1.0)
+    e.add_density_source([0.5, 0.5], R, A, t_end=dt)
+    m0 = e.get()["f"].sum()
+    e.step(1)
+    p = nd["points"].astype(np.float64)
+    x = 2 * np.hypot(p[:, 0] - 0.5, p[:, 1] - 0.5) / R
+    s = np.where(x < 2, A * (1 - x * x / 4) * np.exp(-x * x), 0.0)
+    assert s.max() == pytest.approx(A, rel=2e-2)
+    assert np.abs(e.get(fresh=False)["rho"] - (rho0 + s / 2)).max() < 1e-14
+    gain = e.get()["f"].sum() - m0
+    assert abs(gain / (s.sum() / rho0) - 1) < 5e-3
+    late = oracle64.engine(nd, Q=9, k=16, h=h, rho0=rho0)
+    late.set_pulse([0.5, 0.5], 0.0, 1.0)
+    late.add_density_source([0.5, 0.5], R, A, frequency=3.0, t_start=5.5 * dt)
+    late.step(6)
+    assert np.abs(late.get()["drho"]).max() < 1e-13
+    late.step(1)
+    assert np.abs(late.get()["drho"]).max() > 1e-6
