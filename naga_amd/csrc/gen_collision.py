@@ -7,7 +7,9 @@ first principles instead of being transcribed:
   polynomial P_j evaluated at the 27 velocities (listed in ``BASIS``; the
   reference's integer table is exactly these values);
 * the cascaded moments of f about the local velocity u are
-  C_j = sum_a f_a P_j(c_a - u);
+  C_j = sum_a f_a P_j(c_a - u), with the zeroth central moment taken as rho
+  and the first central moments as zero (the reference's formulas contain
+  neither);
 * k_0..k_3 = 0 (mass and momentum are conserved) and the remaining k solve
   B(u) k' = rho e - C with k_j = omega_j k'_j, where
   B(u)_{jl} = sum_a P_l(c_a) P_j(c_a - u) is lower triangular with a
@@ -168,8 +170,13 @@ def emit(path):
         name = f"b{j}_{l}"
         offv[(j, l)] = sp.Symbol(name)
         w(f"    const double {name} = {ccode(ex)};")
+    # The reference's closed forms contain no zeroth or first central moment:
+    # they are the cascade with m000 = rho and m100 = m010 = m001 = 0, which
+    # differs from the general solve when rho != sum f or u != j / sum f
+    # (a density source changes rho without changing u).
+    subs = {m[(0, 0, 0)]: rho, m[(1, 0, 0)]: 0, m[(0, 1, 0)]: 0, m[(0, 0, 1)]: 0}
     for j in R:
-        rhs = rho * e[j] - sum(v * m[mon] for mon, v in coeffs[j].items())
+        rhs = (rho * e[j] - sum(v * m[mon] for mon, v in coeffs[j].items())).subs(subs)
         acc = sp.expand(rhs) - sum(offv[(j, l)] * kp[l]
                                    for l in R if (j, l) in offv)
         w(f"    const double kp{j} = ({ccode(acc)}) / {ccode(B[j][j])}.0;")

@@ -49,6 +49,11 @@ def test_host_collision_matches_oracle(oracle64, Q):
             # increments are O(1e-5) differences of O(1) moments: compare at
             # FP64 resolution of the populations
             assert np.abs(a - b).max() <= 1e-13 + 1e-10 * np.abs(b).max()
+            # a density source makes rho differ from sum f (u is kept)
+            rs = rho * (1 + 3e-4)
+            a = naga_amd.collide(Q, f, rs, u, nu, dt)
+            b = oracle64.collide(Q, f, rs, u, nu, dt)
+            assert np.abs(a - b).max() <= 1e-13 + 1e-10 * np.abs(b).max()
 
 
 def test_engine_rejects_bad_config(lib):

@@ -78,20 +78,32 @@ def test_feq_moments(oracle64, Q):
         assert np.allclose(np.array([oracle64.feq(Q, rho, 0 * u, a) for a in range(Q)]), w * rho)
 
 
-def _cascade_k(Q, c, K, f, rho, u, rates):
+def _cascade_k(Q, c, K, f, rho, u, rates, reference_form=False):
     """First-principles cascade: k_0..k_D = 0 and k = Omega B(u)^-1 (rho e - C)
     with C_j = sum_a f_a P_j(c_a - u), B_jl = sum_a P_l(c_a) P_j(c_a - u), e_j
-    the unit-density Maxwellian (c_s^2 = 1/3) central moments of P_j."""
+    the unit-density Maxwellian (c_s^2 = 1/3) central moments of P_j.
+    reference_form: C_j expanded over the central moments with the zeroth
+    taken as rho and the first as zero (the reference's closed forms contain
+    neither; it matters when rho != sum f or u != j / sum f)."""
     D = c.shape[1]
     basis = BASIS27 if Q == 27 else BASIS9
     x = c - u
     Pshift = np.array([[P(*row) for P in basis] for row in x])  # [a, j]
-    C = f @ Pshift
     B = Pshift.T @ K
-    # Maxwellian central moments <x^a y^b z^c> via exact Gaussian moments
     mons = list(itertools.product(range(3), repeat=D))
     V = np.array([[np.prod([row[d] ** m[d] for d in range(D)]) for m in mons] for row in c])
-    coef = np.linalg.solve(V, K)
+    coef = np.linalg.solve(V, K)  # P_j = sum_mon coef[mon, j] mon
+    if reference_form:
+        M = np.array([f @ np.prod([x[:, d] ** m[d] for d in range(D)], axis=0) for m in mons])
+        for i, m in enumerate(mons):
+            if sum(m) == 0:
+                M[i] = rho
+            elif sum(m) == 1:
+                M[i] = 0.0
+        C = M @ coef
+    else:
+        C = f @ Pshift
+    # Maxwellian central moments <x^a y^b z^c> via exact Gaussian moments
     gm = {0: 1.0, 1: 0.0, 2: CS2}
     e = np.array([sum(coef[i, j] * np.prod([gm[m[d]] for d in range(D)])
                       for i, m in enumerate(mons)) for j in range(Q)])
@@ -124,6 +136,11 @@ def test_collision_d3q27_is_the_cascade(oracle64):
         assert np.abs(k - kfp).max() <= 1e-9 * np.abs(kfp).max()
         # conservation: mass and momentum increments vanish
         assert abs(df.sum()) < 1e-15 and np.abs(df @ c).max() < 1e-15
+        # rho != sum f (a density source) and u != j / sum f
+        rs, us = rho * (1 + 1e-3), u * 0.9 + 1e-4
+        k = np.linalg.solve(K, oracle64.collide(27, f, rs, us, nu, dt) / dt)
+        kfp = _cascade_k(27, c, K, f, rs, us, rates, reference_form=True)
+        assert np.abs(k - kfp).max() <= 1e-9 * np.abs(kfp).max()
 
 
 def test_collision_d2q9(oracle64):
