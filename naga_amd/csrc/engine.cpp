@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -164,6 +165,30 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     launch_make_slab(nfluid, k, D, idx_.get(), W_.get(), slab_.get(), st);
     fslab_.allocate(nfluid * k * 2);
     launch_make_slab(nfluid, k, 1, idx_.get(), filt_.get(), fslab_.get(), st);
+    ucap_ = 0;
+    if (step_uses_unions(Q, k, N) && nfluid > 0) {
+        // tile unions; slab index fields become union positions
+        const int64_t ntiles = (nfluid + kTileNodes - 1) / kTileNodes;
+        DeviceBuffer<int32_t> counts(ntiles);
+        launch_tile_unions(nfluid, k, idx_.get(), counts.get(), nullptr, nullptr, nullptr, 0,
+                           nullptr, 0, st);
+        std::vector<int32_t> hc(ntiles);
+        counts.download(hc.data(), ntiles, st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        std::vector<int64_t> off(ntiles + 1, 0);
+        int maxu = 0;
+        for (int64_t t = 0; t < ntiles; ++t) {
+            off[t + 1] = off[t] + hc[t];
+            maxu = std::max(maxu, hc[t]);
+        }
+        uoff_ = to_device(off, st);
+        ulist_.allocate(std::max<int64_t>(off[ntiles], 1));
+        launch_tile_unions(nfluid, k, idx_.get(), nullptr, uoff_.get(), ulist_.get(),
+                           slab_.get(), D, fslab_.get(), 1, st);
+        const char* env = std::getenv("NAGA_UCAP");
+        const int cap = env ? std::max(0, std::min(std::atoi(env), 384)) : kUnionCapDefault;
+        ucap_ = std::min(maxu, cap);
+    }
 
     // boundary interpolator (init_domain :223-258): kNN of boundary nodes
     // among the bulk and layer nodes, RPIM weights at spacing h
@@ -291,6 +316,9 @@ StepBuffers Engine::buffers() {
     S.means = means_.get();
     S.nparts = nparts_;
     S.src = sources_.empty() ? nullptr : src_field_.get();
+    S.ulist = ulist_.get();
+    S.uoff = uoff_.get();
+    S.ucap = ucap_;
     return S;
 }
 

@@ -92,6 +92,10 @@ class Engine {
     int nparts_ = 1;
     int64_t n_pml_ = 0;
 
+    DeviceBuffer<uint32_t> ulist_;  // tile unions (four-per-lane D3Q27 kernels)
+    DeviceBuffer<int64_t> uoff_;
+    int ucap_ = 0;
+
     std::vector<float> hpts_;  // host copy of the node coordinates
     std::vector<DensitySource> sources_;
     DeviceBuffer<int32_t> snodes_, soff_, sids_;

@@ -474,6 +474,95 @@ __global__ void k_make_slab(int64_t n, int k, int nc, const uint32_t* idx,
     for (int c = 0; c < nc; ++c) r[1 + c] = w[t * nc + c];
 }
 
+// One block per tile: the tile's k * kTileNodes (<= 2048) neighbour indices
+// are sorted in LDS (bitonic), flagged where they differ from their
+// predecessor and compacted by a block-wide scan of the flags.
+constexpr int kUnionMax = 2048;
+
+__device__ __forceinline__ int block_exclusive_scan(int* v, int n, int* tmp) {
+    // v[0..n) in LDS, n <= kUnionMax, blockDim.x == 256: per-thread chunks of
+    // n / 256 (n is a power of two >= 256 here), then a scan of the chunk sums.
+    const int per = n / 256, t = threadIdx.x;
+    int sum = 0;
+    for (int e = 0; e < per; ++e) sum += v[t * per + e];
+    tmp[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int add = t >= off ? tmp[t - off] : 0;
+        __syncthreads();
+        tmp[t] += add;
+        __syncthreads();
+    }
+    int run = tmp[t] - sum;  // exclusive prefix of this chunk
+    for (int e = 0; e < per; ++e) {
+        const int f = v[t * per + e];
+        v[t * per + e] = run;
+        run += f;
+    }
+    __syncthreads();
+    const int total = tmp[255];
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, const uint32_t* idx,
+                                                    int32_t* counts, const int64_t* uoff,
+                                                    uint32_t* ulist, float* slab_a, int nca,
+                                                    float* slab_b, int ncb) {
+    __shared__ uint32_t keys[kUnionMax];
+    __shared__ uint32_t uniq[kUnionMax];
+    __shared__ int rank[kUnionMax];
+    __shared__ int tmp[256];
+    const int64_t node0 = int64_t(blockIdx.x) * kTileNodes;
+    const int64_t left = nfluid - node0;
+    const int cnt = left < kTileNodes ? int(left) : kTileNodes;
+    const int M = cnt * k;
+    int P = 256;
+    while (P < M) P <<= 1;
+    const uint32_t* src = idx + node0 * k;
+    for (int e = threadIdx.x; e < P; e += 256) keys[e] = e < M ? src[e] : 0xffffffffu;
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int e = threadIdx.x; e < P / 2; e += 256) {
+                const int i = 2 * stride * (e / stride) + (e % stride), j = i + stride;
+                const bool up = (i & size) == 0;
+                const uint32_t a = keys[i], b = keys[j];
+                if ((a > b) == up) {
+                    keys[i] = b;
+                    keys[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    for (int e = threadIdx.x; e < P; e += 256)
+        rank[e] = (e < M && (e == 0 || keys[e] != keys[e - 1])) ? 1 : 0;
+    __syncthreads();
+    const int U = block_exclusive_scan(rank, P, tmp);
+    if (!ulist) {
+        if (threadIdx.x == 0) counts[blockIdx.x] = U;
+        return;
+    }
+    // compact the distinct keys (rank is the exclusive scan of the flags)
+    for (int e = threadIdx.x; e < M; e += 256)
+        if (e == 0 || keys[e] != keys[e - 1]) uniq[rank[e]] = keys[e];
+    __syncthreads();
+    uint32_t* out = ulist + uoff[blockIdx.x];
+    for (int e = threadIdx.x; e < U; e += 256) out[e] = uniq[e];
+    for (int r = threadIdx.x; r < M; r += 256) {
+        const uint32_t j = src[r];
+        int lo = 0, hi = U;  // first position with uniq[pos] >= j
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (uniq[mid] < j) lo = mid + 1;
+            else hi = mid;
+        }
+        const float slot = __uint_as_float(uint32_t(lo));
+        slab_a[(node0 * k + r) * (1 + nca)] = slot;
+        if (slab_b) slab_b[(node0 * k + r) * (1 + ncb)] = slot;
+    }
+}
+
 __global__ void k_flag(int64_t n, int k, const uint32_t* idx, int64_t lo,
                        int64_t hi, uint8_t* flag) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -678,6 +767,17 @@ void launch_make_slab(int64_t n, int k, int nc, const uint32_t* idx, const float
     if (n == 0) return;
     k_make_slab<<<(n * k + 255) / 256, 256, 0, st>>>(n, k, nc, idx, w, slab);
     hip_check(hipGetLastError(), "slab");
+}
+
+void launch_tile_unions(int64_t nfluid, int k, const uint32_t* idx, int32_t* counts,
+                        const int64_t* uoff, uint32_t* ulist, float* slab_a, int ncomp_a,
+                        float* slab_b, int ncomp_b, hipStream_t st) {
+    require(k * kTileNodes <= kUnionMax, "stencil too wide for tile unions");
+    const int64_t ntiles = (nfluid + kTileNodes - 1) / kTileNodes;
+    if (ntiles == 0) return;
+    k_tile_union<<<ntiles, 256, 0, st>>>(nfluid, k, idx, counts, uoff, ulist, slab_a, ncomp_a,
+                                          slab_b, ncomp_b);
+    hip_check(hipGetLastError(), "tile unions");
 }
 
 void launch_flag_neighbours(int64_t n, int k, const uint32_t* idx, int64_t lo,

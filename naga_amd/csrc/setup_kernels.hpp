@@ -60,6 +60,18 @@ void launch_filter_weights(int D, const float* pts, int64_t n,
 void launch_make_slab(int64_t n, int k, int ncomp, const uint32_t* idx,
                       const float* w, float* slab, hipStream_t stream);
 
+// Tile unions: fluid nodes are grouped in tiles of kTileNodes consecutive
+// nodes; a tile's union is the sorted list of the distinct neighbour
+// indices of its nodes' k-stencils.  Counting pass (ulist == nullptr):
+// counts[t] = union size.  Filling pass (uoff = exclusive scan of counts,
+// ntiles + 1 entries): ulist[uoff[t] ..] = the union and, for each slab
+// given (records of 1 + ncomp floats, field 0 the neighbour index), the
+// index field is replaced by the neighbour's position in its tile's union.
+constexpr int kTileNodes = 32;
+void launch_tile_unions(int64_t nfluid, int k, const uint32_t* idx, int32_t* counts,
+                        const int64_t* uoff, uint32_t* ulist, float* slab_a, int ncomp_a,
+                        float* slab_b, int ncomp_b, hipStream_t stream);
+
 // flag[i] = 1 if any stencil neighbour of fluid node i is in [lo, hi).
 void launch_flag_neighbours(int64_t n, int k, const uint32_t* idx, int64_t lo,
                             int64_t hi, uint8_t* flag, hipStream_t stream);

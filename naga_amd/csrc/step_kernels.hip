@@ -467,10 +467,13 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S) {
 // D3Q27 kernels with four distributions per lane.  A node's 128-byte row is
 // served by 8 lanes, each holding alpha = 4q .. 4q+3 (q = lane % 8; the last
 // five slots of the row are padding with zero lattice velocity), so a
-// wavefront covers 8 nodes and every neighbour gather is one 16-byte load
-// per lane.  Rows are addressed through buffer resources with 32-bit byte
-// offsets (row j at j * 128), which bounds the node count by 2^25; loads
-// past the end of a buffer return zero instead of faulting.
+// wavefront covers 8 nodes and a block one tile of 32 consecutive nodes.
+// The distinct stencil neighbours of a tile (about 220 rows for 1024
+// stencil entries on jittered grids) are loaded once into LDS, and the
+// stencil sums read them from there.  Rows in HBM are addressed through
+// buffer resources with 32-bit byte offsets (row j at j * 128), which
+// bounds the node count by 2^25; loads past the end of a buffer return
+// zero instead of faulting.
 // ---------------------------------------------------------------------------
 constexpr int kV4Lanes = 8;                   // lanes per node
 constexpr int kV4Nodes = kBlock / kV4Lanes;  // nodes per block
@@ -506,22 +509,20 @@ __device__ __forceinline__ float c27(int a, int d) {
     return a < 27 ? float(Lattice<27>::c(a, d)) : 0.0f;
 }
 
-// T_d[e] = sum_s W_d(s) gin(idx_s)[4q + e] over the staged records rp
-// ({idx, W_x, W_y, W_z} per neighbour); returns -sum_d c_d T_d per slot.
-template <int K, int BATCH>
-__device__ __forceinline__ float4 rate_v4(const float4* rp, __amdgpu_buffer_rsrc_t gin,
-                                          int q) {
+// T_d[e] = sum_s W_d(s) g_s[4q + e] over the staged records rp
+// ({index field, W_x, W_y, W_z} per neighbour), the neighbour rows fetched
+// by fetch(index field); returns -sum_d c_d T_d per slot.
+template <int K, int BATCH, class Fetch>
+__device__ __forceinline__ float4 rate_fetch(const float4* rp, Fetch fetch, int q) {
     float tx[4] = {0.f, 0.f, 0.f, 0.f}, ty[4] = {0.f, 0.f, 0.f, 0.f},
           tz[4] = {0.f, 0.f, 0.f, 0.f};
-    const uint32_t lane_off = uint32_t(q) * 16u;
 #pragma unroll
     for (int s0 = 0; s0 < K; s0 += BATCH) {
         float4 rec[BATCH], gv[BATCH];
 #pragma unroll
         for (int u = 0; u < BATCH; ++u) rec[u] = rp[s0 + u];
 #pragma unroll
-        for (int u = 0; u < BATCH; ++u)
-            gv[u] = ld4(gin, __float_as_uint(rec[u].x) * 128u + lane_off);
+        for (int u = 0; u < BATCH; ++u) gv[u] = fetch(__float_as_uint(rec[u].x));
 #pragma unroll
         for (int u = 0; u < BATCH; ++u)
 #pragma unroll
@@ -541,10 +542,21 @@ __device__ __forceinline__ float4 rate_v4(const float4* rp, __amdgpu_buffer_rsrc
     return r;
 }
 
+// Stage the union rows of a tile into LDS (8 lanes per 128-byte row).
+__device__ __forceinline__ void stage_union(float4* urows, const uint32_t* ul, int U,
+                                            __amdgpu_buffer_rsrc_t src, int q) {
+    for (int r = threadIdx.x / kV4Lanes; r < U; r += kBlock / kV4Lanes)
+        urows[r * kV4Lanes + q] = ld4(src, ul[r] * 128u + uint32_t(q) * 16u);
+}
+
+// RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
+// union staged in LDS (unions larger than ucap rows read through the list).
 template <int STAGE, int K, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_stage_v4(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S) {
     __shared__ float4 srec[kV4Nodes * K];
-    const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * kV4Nodes;
+    extern __shared__ float4 urows[];
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t node0 = tile * kV4Nodes;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
@@ -552,23 +564,38 @@ __global__ __launch_bounds__(kBlock) void k_stage_v4(StepGeom G, StepBuffers S) 
     const float* gin_p = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
     float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
     const bool fluid = i < G.nfluid;
-    // own rows first: their latency overlaps the record staging
     const float4 g0 = ld4(rsrc(S.g, rows), own);
     const float4 acc0 = STAGE != 1 ? ld4(rsrc(S.acc, rows), own) : make_float4(0.f, 0.f, 0.f, 0.f);
-    {
-        const int64_t nf = G.nfluid - node0;
-        const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+    const int64_t nf = G.nfluid - node0;
+    const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+    const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
+    int U = 0;
+    const uint32_t* ul = S.ulist;
+    if (cnt > 0) {
+        const int64_t u0 = S.uoff[tile];
+        U = int(S.uoff[tile + 1] - u0);
+        ul += u0;
         const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
         for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
+        if (U <= S.ucap) stage_union(urows, ul, U, gin, q);
     }
     __syncthreads();
-    if (STAGE == 1 && !fluid && i < G.N) {  // boundary rows: constant through the stages
+    if (STAGE == 1 && !fluid && i < G.N) {
         st4(S.A, own, g0);
         st4(S.B, own, g0);
         return;
     }
     if (!fluid) return;
-    const float4 r = rate_v4<K, BATCH>(srec + slot * K, rsrc(gin_p, rows), q);
+    float4 r;
+    if (U <= S.ucap)
+        r = rate_fetch<K, BATCH>(srec + slot * K,
+                                 [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
+    else
+        r = rate_fetch<K, BATCH>(srec + slot * K,
+                                 [&](uint32_t j) {
+                                     return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
+                                 },
+                                 q);
     const float dt = G.dt;
     float4 o, a;
     if (STAGE == 1) {
@@ -591,31 +618,39 @@ __global__ __launch_bounds__(kBlock) void k_stage_v4(StepGeom G, StepBuffers S) 
     if (STAGE != 4) st4(S.acc, own, a);
 }
 
-// p4 for D3Q27 with four distributions per lane: selective filter on fluid
-// nodes, bounce-back on boundary nodes.
+// p4 for D3Q27 with the tile unions: selective filter on fluid nodes,
+// bounce-back on boundary nodes.
 template <int K, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_p4_v4(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S) {
     __shared__ float2 srec[kV4Nodes * K];
-    const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * kV4Nodes;
+    extern __shared__ float4 urows[];
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t node0 = tile * kV4Nodes;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const __amdgpu_buffer_rsrc_t s1 = rsrc(S.A, rows);
     const float4 si = ld4(s1, own);
-    {
-        const int64_t nf = G.nfluid - node0;
-        const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+    const int64_t nf = G.nfluid - node0;
+    const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+    int U = 0;
+    const uint32_t* ul = S.ulist;
+    if (cnt > 0) {
+        const int64_t u0 = S.uoff[tile];
+        U = int(S.uoff[tile + 1] - u0);
+        ul += u0;
         const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
         for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
+        if (U <= S.ucap) stage_union(urows, ul, U, s1, q);
     }
     __syncthreads();
     if (i >= G.N) return;
     float4 out;
     if (i < G.nfluid) {
         const float2* rp = srec + slot * K;
+        const bool lds = U <= S.ucap;
         float sum[4] = {0.f, 0.f, 0.f, 0.f};
-        const uint32_t lane_off = uint32_t(q) * 16u;
 #pragma unroll
         for (int s0 = 0; s0 < K; s0 += BATCH) {
             float2 rec[BATCH];
@@ -623,8 +658,10 @@ __global__ __launch_bounds__(kBlock) void k_p4_v4(StepGeom G, StepBuffers S) {
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) rec[u] = rp[s0 + u];
 #pragma unroll
-            for (int u = 0; u < BATCH; ++u)
-                gv[u] = ld4(s1, __float_as_uint(rec[u].x) * 128u + lane_off);
+            for (int u = 0; u < BATCH; ++u) {
+                const uint32_t j = __float_as_uint(rec[u].x);
+                gv[u] = lds ? urows[j * kV4Lanes + q] : ld4(s1, ul[j] * 128u + uint32_t(q) * 16u);
+            }
 #pragma unroll
             for (int u = 0; u < BATCH; ++u)
 #pragma unroll
@@ -664,7 +701,10 @@ __global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S) {
     __syncthreads();
     if (!valid) return;
     const uint64_t rows = uint64_t(G.N) * 128u;
-    const float4 r = rate_v4<K, BATCH>(rp, rsrc(S.Gf, rows), q);
+    const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
+    const uint32_t* ul = S.ulist + S.uoff[i / kV4Nodes];  // records hold union positions
+    const float4 r = rate_fetch<K, BATCH>(
+        rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + uint32_t(q) * 16u); }, q);
     // the divergence of F = c Gf is the negated rate of Gf
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const float4 s = ld4(rsrc(S.A, rows), own);
@@ -753,13 +793,17 @@ void launch_stage_k(int stage, int64_t nb, const StepGeom& G, const StepBuffers&
 // The four-per-lane D3Q27 kernels need k in {16, 32} and 32-bit row offsets.
 // NAGA_V4=0 selects the one-per-lane kernels; NAGA_V4_BATCH (4 or 8) sets
 // their gather batch.
-template <int Q>
-bool use_v4(const StepGeom& G) {
+bool step_uses_unions(int Q, int k, int64_t N) {
     static const bool on = [] {
         const char* e = std::getenv("NAGA_V4");
         return !(e && e[0] == '0');
     }();
-    return Q == 27 && on && (G.k == 32 || G.k == 16) && G.N < (int64_t(1) << 25);
+    return Q == 27 && on && (k == 32 || k == 16) && N < (int64_t(1) << 25);
+}
+
+template <int Q>
+bool use_v4(const StepGeom& G) {
+    return step_uses_unions(Q, G.k, G.N);
 }
 
 int v4_batch() {
@@ -774,11 +818,12 @@ int v4_batch() {
 template <int K, int BATCH>
 void launch_stage_v4_b(int stage, int64_t nb, const StepGeom& G, const StepBuffers& S,
                        hipStream_t st) {
+    const size_t lds = size_t(S.ucap) * 128u;
     switch (stage) {
-        case 1: k_stage_v4<1, K, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
-        case 2: k_stage_v4<2, K, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
-        case 3: k_stage_v4<3, K, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
-        default: k_stage_v4<4, K, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
+        case 1: k_stage_u<1, K, BATCH><<<nb, kBlock, lds, st>>>(G, S); break;
+        case 2: k_stage_u<2, K, BATCH><<<nb, kBlock, lds, st>>>(G, S); break;
+        case 3: k_stage_u<3, K, BATCH><<<nb, kBlock, lds, st>>>(G, S); break;
+        default: k_stage_u<4, K, BATCH><<<nb, kBlock, lds, st>>>(G, S); break;
     }
 }
 
@@ -869,9 +914,9 @@ void launch_step(const StepGeom& G, const StepBuffers& S,
     }
     tick();
     if (v4 && G.k == 32)
-        k_p4_v4<32, 8><<<(G.N + kV4Nodes - 1) / kV4Nodes, kBlock, 0, st>>>(G, S);
+        k_p4_u<32, 8><<<(G.N + kV4Nodes - 1) / kV4Nodes, kBlock, size_t(S.ucap) * 128u, st>>>(G, S);
     else if (v4)
-        k_p4_v4<16, 8><<<(G.N + kV4Nodes - 1) / kV4Nodes, kBlock, 0, st>>>(G, S);
+        k_p4_u<16, 8><<<(G.N + kV4Nodes - 1) / kV4Nodes, kBlock, size_t(S.ucap) * 128u, st>>>(G, S);
     else if (G.k == 32)
         k_p4<Q, 32><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
     else if (G.k == 16)

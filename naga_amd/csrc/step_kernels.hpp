@@ -72,6 +72,14 @@ struct StepBuffers {
     float* means;    // [2Q] layer means of Q1, Q2
     int nparts;
     const float* src;  // [N] density source of this step (physical units), or null
+    // Tile unions (D3Q27 four-per-lane kernels): the fluid nodes form tiles
+    // of 32; ulist[uoff[t] .. uoff[t+1]) are the distinct stencil neighbours
+    // of tile t, and the index field of slab / fslab records holds the
+    // neighbour's position in that list.  Unions of at most ucap rows are
+    // staged in LDS by the gather kernels.
+    const uint32_t* ulist;
+    const int64_t* uoff;
+    int ucap;
 };
 
 // Density sources: per step, field[nodes[i]] = sum over entries e in
@@ -93,6 +101,12 @@ template <int Q>
 void launch_step(const StepGeom& geo, const StepBuffers& buf,
                  const CollisionParams& cp, hipStream_t stream,
                  hipEvent_t* events = nullptr);
+
+// Whether the step runs the D3Q27 four-per-lane kernels, which need the
+// tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
+// positions: D3Q27, k in {16, 32}, fewer than 2^25 nodes.
+bool step_uses_unions(int Q, int k, int64_t N);
+constexpr int kUnionCapDefault = 296;  // LDS rows per block (3 blocks per CU)
 
 // Per-kernel entry points (used by the profiler and the standalone C-ABI).
 template <int Q>
