@@ -29,6 +29,7 @@ extern "C" {
 #define NAGA_RBF_POLYHARMONIC3 1
 
 typedef struct naga_engine naga_engine;
+typedef struct naga_batch naga_batch;
 
 /* Scene configuration.  Physics parameters follow
  * simulation_engine::set_problem_parameters(nu, rho0, dt, c)
@@ -125,6 +126,20 @@ int naga_engine_get_pml(naga_engine* e, double* Q1_minus_w, double* Q2);
 int naga_engine_info(naga_engine* e, int64_t* n_nodes, double* time,
                      int64_t* frame, double* setup_ms);
 
+/* ---- batch: many scenes stepped together.  No counterpart in the
+ *      reference (which steps one engine at a time): every pass of a step
+ *      is one launch over all scenes of a lattice/kernel group, replayed
+ *      from a captured hipGraph.  Engines keep ownership of their scenes;
+ *      an engine is in at most one batch, and destroying it removes it. */
+int naga_batch_create(naga_batch** out);
+int naga_batch_destroy(naga_batch* b);
+int naga_batch_add(naga_batch* b, naga_engine* e);     /* e needs a domain */
+int naga_batch_remove(naga_batch* b, naga_engine* e);
+int naga_batch_set_graph(naga_batch* b, int32_t on);  /* default on */
+/* nsteps steps of every scene (each engine's time advances by its dt) */
+int naga_batch_step(naga_batch* b, int32_t nsteps);
+int naga_batch_step_timed(naga_batch* b, int32_t nsteps, double* ms);
+
 /* ---- node provider: uniform_grid_provider<Lattice>::get()
  *      (node_providers/uniform_grid_provider.cuh:65-161) with seeded
  *      SplitMix64 jitter of interior nodes; layer by exact 1-NN distance to
@@ -135,6 +150,24 @@ int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h
                       double jitter_frac, uint64_t seed, double layer_thickness,
                       double peak_absorption, float* points, float* normals,
                       float* sigma, int64_t* counts);
+
+/* ---- procedural geometry: the box [lo, hi] minus n_obstacles balls
+ *      (obstacles[j*(dims+1) ..] = centre, radius).  No single reference
+ *      counterpart; it plays the role of the conforming point-cloud provider
+ *      (node_providers/experimental/conforming_point_cloud_provider.cuh:
+ *      147-525) for circles/spheres.  Grid nodes follow naga_uniform_grid
+ *      (same SplitMix64 stream and jitter) and are dropped within half a
+ *      spacing of an obstacle; obstacle surfaces get nodes at about the grid
+ *      spacing (equal angles on circles, a Fibonacci lattice on spheres)
+ *      with normals pointing into the fluid.  The absorbing layer is the
+ *      fluid within layer_thickness of the box faces (exact distance d),
+ *      sigma = peak_absorption (1 - d/L)^2.  With points == NULL only
+ *      counts[3] = {bulk, layer, boundary} are returned (size query). */
+int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double h,
+                       double jitter_frac, uint64_t seed, int32_t n_obstacles,
+                       const double* obstacles, double layer_thickness, double peak_absorption,
+                       int64_t capacity, float* points, float* normals, float* sigma,
+                       int64_t* counts);
 
 /* ---- setup primitives ---------------------------------------------------- */
 /* batched_nearest_neighbors (segmentation/nearest_neighbors.cuh:95-477),

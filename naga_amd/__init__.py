@@ -47,6 +47,7 @@ class SceneConfig(C.Structure):
 
 
 _lib = None
+_BATCH_PREFIX = "naga::batch: "
 
 
 def load() -> C.CDLL:
@@ -87,6 +88,15 @@ def load() -> C.CDLL:
         "naga_divergence_apply": (C.c_int, [vp, vp, i64, i32, i32, vp, i64, C.c_float, vp]),
         "naga_engine_add_density_source": (C.c_int, [vp, vp, f64, f64, f64, f64, f64, f64]),
         "naga_engine_clear_density_sources": (C.c_int, [vp]),
+        "naga_batch_create": (C.c_int, [C.POINTER(vp)]),
+        "naga_batch_destroy": (C.c_int, [vp]),
+        "naga_batch_add": (C.c_int, [vp, vp]),
+        "naga_batch_remove": (C.c_int, [vp, vp]),
+        "naga_batch_set_graph": (C.c_int, [vp, i32]),
+        "naga_batch_step": (C.c_int, [vp, i32]),
+        "naga_batch_step_timed": (C.c_int, [vp, i32, vp]),
+        "naga_obstacle_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, i32, vp, f64, f64, i64,
+                                         vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -106,7 +116,9 @@ EXPORTED = [
     "naga_uniform_grid_size", "naga_uniform_grid", "naga_knn",
     "naga_batched_lu_solve_f64", "naga_rpim_weights", "naga_divergence_weights",
     "naga_divergence_apply", "naga_engine_add_density_source",
-    "naga_engine_clear_density_sources",
+    "naga_engine_clear_density_sources", "naga_batch_create", "naga_batch_destroy",
+    "naga_batch_add", "naga_batch_remove", "naga_batch_set_graph", "naga_batch_step",
+    "naga_batch_step_timed", "naga_obstacle_grid",
 ]
 
 STEP_KERNELS = ["p1_collide", "pl_pml_local", "pm_layer_means", "pf_pml_field",
@@ -249,6 +261,41 @@ class DensitySource:
         self.phase = float(phase)
         self.t_start = float(t_start)
         self.t_end = float(t_end)
+
+
+class ObstacleGridProvider:
+    """Box minus circles/spheres (naga_obstacle_grid): obstacles is a list of
+    (centre, radius); layer absorption at the box faces."""
+
+    def __init__(self, min_point, max_point, grid_spacing, obstacles=(),
+                 absorption_layer_thickness=0.0, peak_absorption_rate=0.0,
+                 jitter=0.0, seed=0x4E414741):
+        self.lo = _f64(min_point)
+        self.hi = _f64(max_point)
+        self.D = self.lo.size
+        self.h = float(grid_spacing)
+        self.obst = np.array([list(c) + [r] for c, r in obstacles], np.float64).reshape(-1, self.D + 1)
+        self.layer = float(absorption_layer_thickness)
+        self.peak = float(peak_absorption_rate)
+        self.jitter = float(jitter)
+        self.seed = int(seed)
+
+    def get(self) -> "SimulationNodes":
+        lib = load()
+        counts = np.zeros(3, np.int64)
+        ob = np.ascontiguousarray(self.obst.reshape(-1))
+        args = (self.D, _ptr(self.lo), _ptr(self.hi), self.h, self.jitter, C.c_uint64(self.seed),
+                len(self.obst), _ptr(ob) if ob.size else None, self.layer, self.peak)
+        _check(lib.naga_obstacle_grid(*args, 0, None, None, None, _ptr(counts)),
+               "naga::nonlocal_lbm::obstacle_grid_provider::get: ")
+        n = int(counts.sum())
+        pts = np.zeros((n, self.D), np.float32)
+        nrm = np.zeros((max(int(counts[2]), 1), self.D), np.float32)
+        sig = np.zeros(max(int(counts[1]), 1), np.float32)
+        _check(lib.naga_obstacle_grid(*args, n, _ptr(pts), _ptr(nrm), _ptr(sig), _ptr(counts)),
+               "naga::nonlocal_lbm::obstacle_grid_provider::get: ")
+        return SimulationNodes(pts, nrm[: counts[2]], sig[: counts[1]], int(counts[0]),
+                               int(counts[1]), int(counts[2]), self.h)
 
 
 class SimulationEngine:
@@ -526,3 +573,52 @@ class DivergenceOperator:
                                             F.shape[0], float(centering_offset), _ptr(out)),
                "naga::nonlocal_calculus::divergence_operator::")
         return out
+
+
+class SceneBatch:
+    """Many engines stepped together (one launch per pass over all scenes,
+    replayed from a captured hipGraph).  The engines stay usable on their
+    own for reading results; destroying an engine removes it."""
+
+    def __init__(self, engines=(), graph: bool = True):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.naga_batch_create(C.byref(h)), _BATCH_PREFIX)
+        self._h = h
+        self._engines = []
+        self.set_graph(graph)
+        for e in engines:
+            self.add(e)
+
+    def add(self, engine):
+        _check(load().naga_batch_add(self._h, engine._live()), _BATCH_PREFIX)
+        self._engines.append(engine)
+
+    def remove(self, engine):
+        _check(load().naga_batch_remove(self._h, engine._live()), _BATCH_PREFIX)
+        self._engines.remove(engine)
+
+    def set_graph(self, on: bool):
+        _check(load().naga_batch_set_graph(self._h, int(bool(on))), _BATCH_PREFIX)
+
+    def step_forward(self, nsteps: int = 1):
+        _check(load().naga_batch_step(self._h, int(nsteps)), _BATCH_PREFIX)
+
+    def step_timed(self, nsteps: int) -> float:
+        ms = C.c_double()
+        _check(load().naga_batch_step_timed(self._h, int(nsteps), C.byref(ms)), _BATCH_PREFIX)
+        return ms.value
+
+    def __len__(self):
+        return len(self._engines)
+
+    def close(self):
+        if self._h is not None:
+            load().naga_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

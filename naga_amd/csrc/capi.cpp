@@ -1,6 +1,7 @@
 // extern "C" entry points (include/naga_amd.h): argument checks, exception
 // to status translation, thread-local error text.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -13,6 +14,10 @@ using namespace naga_amd;
 
 struct naga_engine {
     std::unique_ptr<Engine> impl;
+};
+
+struct naga_batch {
+    std::unique_ptr<Batch> impl;
 };
 
 namespace {
@@ -40,6 +45,11 @@ int guarded(F&& f) {
 Engine& engine_of(naga_engine* e) {
     require(e != nullptr && e->impl != nullptr, "null engine handle");
     return *e->impl;
+}
+
+Batch& batch_of(naga_batch* b) {
+    require(b != nullptr && b->impl != nullptr, "null batch handle");
+    return *b->impl;
 }
 
 // ---- synthetic uniform grid (node_providers/uniform_grid_provider.cuh) ----
@@ -170,6 +180,47 @@ int naga_engine_add_density_source(naga_engine* e, const double* center, double 
 
 int naga_engine_clear_density_sources(naga_engine* e) {
     return guarded([&] { engine_of(e).clear_density_sources(); });
+}
+
+int naga_batch_create(naga_batch** out) {
+    return guarded([&] {
+        require(out != nullptr, "null output handle");
+        auto* b = new naga_batch;
+        try {
+            b->impl = std::make_unique<Batch>();
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int naga_batch_destroy(naga_batch* b) {
+    return guarded([&] { delete b; });
+}
+
+int naga_batch_add(naga_batch* b, naga_engine* e) {
+    return guarded([&] { batch_of(b).add(&engine_of(e)); });
+}
+
+int naga_batch_remove(naga_batch* b, naga_engine* e) {
+    return guarded([&] { batch_of(b).remove(&engine_of(e)); });
+}
+
+int naga_batch_set_graph(naga_batch* b, int32_t on) {
+    return guarded([&] { batch_of(b).set_graph(on != 0); });
+}
+
+int naga_batch_step(naga_batch* b, int32_t nsteps) {
+    return guarded([&] { batch_of(b).step(nsteps); });
+}
+
+int naga_batch_step_timed(naga_batch* b, int32_t nsteps, double* ms) {
+    return guarded([&] {
+        const double v = batch_of(b).step_timed(nsteps);
+        if (ms) *ms = v;
+    });
 }
 
 int naga_engine_set_distributions(naga_engine* e, const double* f) {
@@ -346,6 +397,166 @@ int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h
         counts[0] = int64_t(kb.size());
         counts[1] = int64_t(kl.size());
         counts[2] = nb;
+    });
+}
+
+int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double h,
+                       double jitter_frac, uint64_t seed, int32_t n_obstacles,
+                       const double* obstacles, double layer_thickness, double peak_absorption,
+                       int64_t capacity, float* points, float* normals, float* sigma,
+                       int64_t* counts) {
+    return guarded([&] {
+        const int D = dims;
+        require(D == 2 || D == 3, "dims must be 2 or 3");
+        require(lo && hi && counts, "null argument");
+        require(h > 0, "grid spacing must be positive");
+        require(n_obstacles >= 0 && (n_obstacles == 0 || obstacles), "bad obstacle list");
+        for (int j = 0; j < n_obstacles; ++j)
+            require(obstacles[j * (D + 1) + D] > 0, "obstacle radius must be positive");
+        int64_t gd[3];
+        const int64_t N = grid_dims(D, lo, hi, h, gd);
+        double sp[3];
+        for (int d = 0; d < D; ++d) {
+            require(gd[d] >= 2, "grid needs at least two nodes per dimension");
+            sp[d] = (hi[d] - lo[d]) / double(gd[d] - 1);
+        }
+        auto inside = [&](const double* x, double pad) {
+            for (int j = 0; j < n_obstacles; ++j) {
+                const double* o = obstacles + j * (D + 1);
+                double r2 = 0.0;
+                for (int d = 0; d < D; ++d) r2 += (x[d] - o[d]) * (x[d] - o[d]);
+                if (std::sqrt(r2) < o[D] + pad) return true;
+            }
+            return false;
+        };
+        auto qkey = [&](const double* x) {
+            int64_t gi[3];
+            for (int d = 0; d < D; ++d) {
+                const double t = std::round((x[d] - lo[d]) / sp[d]);
+                gi[d] = int64_t(std::min(std::max(t, 0.0), double(gd[d] - 1)));
+            }
+            return morton(D, gi);
+        };
+        struct Node {
+            uint64_t key;
+            int64_t order;
+            float x[3];
+            float n[3];
+        };
+        std::vector<Node> bulk, layer, bnd;
+        std::vector<float> sig;
+        const double L = layer_thickness;
+        // grid nodes: the uniform grid's stream and jitter
+        uint64_t st = seed;
+        for (int64_t g = 0; g < N; ++g) {
+            int64_t gi[3], rem = g;
+            bool onface = false;
+            for (int d = 0; d < D; ++d) {
+                gi[d] = rem % gd[d];
+                rem /= gd[d];
+                onface = onface || gi[d] == 0 || gi[d] == gd[d] - 1;
+            }
+            double x[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < D; ++d) {
+                const double u01 = double(splitmix64(st) >> 11) * 0x1.0p-53;
+                x[d] = lo[d] + double(gi[d]) * sp[d];
+                if (!onface) x[d] += (2.0 * u01 - 1.0) * jitter_frac * h;
+            }
+            if (inside(x, 0.5 * h)) continue;
+            Node nd{morton(D, gi), g, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+            for (int d = 0; d < D; ++d) nd.x[d] = float(x[d]);
+            if (onface) {
+                float nv[3] = {0.f, 0.f, 0.f}, n2 = 0.f;
+                for (int d = 0; d < D; ++d) {
+                    nv[d] = gi[d] == 0 ? 1.f : gi[d] == gd[d] - 1 ? -1.f : 0.f;
+                    n2 += nv[d] * nv[d];
+                }
+                for (int d = 0; d < D; ++d) nd.n[d] = nv[d] / std::sqrt(n2);
+                bnd.push_back(nd);
+                continue;
+            }
+            double dmin = 1e300;
+            for (int d = 0; d < D; ++d) dmin = std::min({dmin, x[d] - lo[d], hi[d] - x[d]});
+            if (L > 0 && dmin <= L) {
+                layer.push_back(nd);
+                const double r = (L - std::max(dmin, 0.0)) / L;
+                sig.push_back(float(peak_absorption * r * r));
+            } else {
+                bulk.push_back(nd);
+            }
+        }
+        // obstacle surfaces, normals into the fluid
+        int64_t order = N;
+        for (int j = 0; j < n_obstacles; ++j) {
+            const double* o = obstacles + j * (D + 1);
+            const double R = o[D];
+            std::vector<std::array<double, 3>> dirs;
+            if (D == 2) {
+                const int M = std::max(8, int(std::ceil(2.0 * M_PI * R / h)));
+                for (int m = 0; m < M; ++m) {
+                    const double t = 2.0 * M_PI * (m + 0.5) / M;
+                    dirs.push_back({std::cos(t), std::sin(t), 0.0});
+                }
+            } else {
+                const int M = std::max(16, int(std::ceil(4.0 * M_PI * R * R / (h * h))));
+                const double ga = M_PI * (3.0 - std::sqrt(5.0));
+                for (int m = 0; m < M; ++m) {
+                    const double z = 1.0 - 2.0 * (m + 0.5) / M, rr = std::sqrt(1.0 - z * z);
+                    dirs.push_back({rr * std::cos(ga * m), rr * std::sin(ga * m), z});
+                }
+            }
+            for (const auto& v : dirs) {
+                double x[3] = {0.0, 0.0, 0.0};
+                bool ok = true;
+                for (int d = 0; d < D; ++d) {
+                    x[d] = o[d] + R * v[d];
+                    ok = ok && x[d] > lo[d] && x[d] < hi[d];
+                }
+                // points inside another obstacle are not on the fluid boundary
+                for (int i = 0; ok && i < n_obstacles; ++i) {
+                    if (i == j) continue;
+                    const double* p = obstacles + i * (D + 1);
+                    double r2 = 0.0;
+                    for (int d = 0; d < D; ++d) r2 += (x[d] - p[d]) * (x[d] - p[d]);
+                    ok = std::sqrt(r2) >= p[D];
+                }
+                if (!ok) continue;
+                Node nd{qkey(x), order++, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+                for (int d = 0; d < D; ++d) {
+                    nd.x[d] = float(x[d]);
+                    nd.n[d] = float(v[d]);
+                }
+                bnd.push_back(nd);
+            }
+        }
+        const int64_t total = int64_t(bulk.size() + layer.size() + bnd.size());
+        counts[0] = int64_t(bulk.size());
+        counts[1] = int64_t(layer.size());
+        counts[2] = int64_t(bnd.size());
+        if (!points) return;  // size query
+        require(capacity >= total, "capacity below the node count (query the size first)");
+        require(normals && sigma, "null argument");
+        auto by_key = [](const Node& a, const Node& b) {
+            return a.key != b.key ? a.key < b.key : a.order < b.order;
+        };
+        // sort the layer together with its absorption values
+        std::vector<int64_t> lperm(layer.size());
+        for (size_t i = 0; i < lperm.size(); ++i) lperm[i] = int64_t(i);
+        std::sort(lperm.begin(), lperm.end(),
+                  [&](int64_t a, int64_t b) { return by_key(layer[a], layer[b]); });
+        std::sort(bulk.begin(), bulk.end(), by_key);
+        std::sort(bnd.begin(), bnd.end(), by_key);
+        int64_t o = 0;
+        for (const Node& nd : bulk) std::memcpy(points + D * o++, nd.x, sizeof(float) * D);
+        for (size_t i = 0; i < lperm.size(); ++i) {
+            std::memcpy(points + D * o++, layer[lperm[i]].x, sizeof(float) * D);
+            sigma[i] = sig[lperm[i]];
+        }
+        int64_t b = 0;
+        for (const Node& nd : bnd) {
+            std::memcpy(points + D * o++, nd.x, sizeof(float) * D);
+            std::memcpy(normals + D * b++, nd.n, sizeof(float) * D);
+        }
     });
 }
 

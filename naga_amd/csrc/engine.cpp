@@ -77,6 +77,7 @@ Engine::Engine(const SceneConfig& cfg) : cfg_(cfg) {
 }
 
 Engine::~Engine() {
+    if (batch) batch->remove(this);
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
@@ -89,6 +90,7 @@ void Engine::require_domain() const {
 
 void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_t nbnd,
                          const float* normals, const float* sigma) {
+    require(batch == nullptr, "remove the engine from its batch before init_domain");
     Timer timer;
     const int D = D_, Q = cfg_.Q, k = cfg_.k, kb = cfg_.kb;
     // shape checks of detail/simulation_engine.cuh:128-167
@@ -328,6 +330,7 @@ StepBuffers Engine::buffers() {
 // step's source value of every listed node into src_field_, read by p1.
 void Engine::add_density_source(const DensitySource& src) {
     require_domain();
+    require(batch == nullptr, "remove the engine from its batch before adding sources");
     require(int(sources_.size()) < kMaxSources, "at most 16 density sources per engine");
     require(src.radius > 0.0, "density source radius must be positive");
     sources_.push_back(src);
@@ -377,6 +380,33 @@ void Engine::refresh_macroscopic_state() {
         launch_macroscopic<27>(geo_, g_.get(), rho_.get(), nullptr, u_.get(), stream_);
     else
         launch_macroscopic<9>(geo_, g_.get(), rho_.get(), nullptr, u_.get(), stream_);
+}
+
+SceneDesc Engine::scene_desc() {
+    require_domain();
+    SceneDesc d{};
+    d.G = geo_;
+    d.S = buffers();
+    d.cp = cp_;
+    return d;
+}
+
+void Engine::prepare_batched_step(hipStream_t stream) {
+    if (sources_.empty()) return;
+    SourceAmps amps{};
+    for (size_t j = 0; j < sources_.size(); ++j) {
+        const DensitySource& D = sources_[j];
+        const bool on = time_ >= D.t_start && time_ < D.t_end;
+        amps.a[j] = on ? float(D.amplitude * std::cos(2.0 * M_PI * D.frequency * time_ + D.phase))
+                       : 0.0f;
+    }
+    launch_source_field(snodes_.get(), soff_.get(), sids_.get(), sprof_.get(), n_snodes_, amps,
+                        src_field_.get(), stream);
+}
+
+void Engine::finish_batched_step() {
+    time_ += cfg_.dt;
+    ++frame_;
 }
 
 void Engine::launch_one_step(hipEvent_t* ev) {
@@ -436,6 +466,162 @@ void Engine::profile_step(double* ms) {
         ms[t] = double(v);
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
+}
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+Batch::Batch() { NAGA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking)); }
+
+Batch::~Batch() {
+    for (Engine* e : scenes_) e->batch = nullptr;
+    drop_graph();
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+void Batch::drop_graph() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+}
+
+void Batch::add(Engine* e) {
+    require(e != nullptr && e->has_domain(), "a batch takes engines with a domain");
+    require(e->batch == nullptr, "engine already in a batch");
+    scenes_.push_back(e);
+    e->batch = this;
+    dirty_ = true;
+}
+
+void Batch::remove(Engine* e) {
+    for (size_t i = 0; i < scenes_.size(); ++i)
+        if (scenes_[i] == e) {
+            scenes_.erase(scenes_.begin() + i);
+            e->batch = nullptr;
+            dirty_ = true;
+            return;
+        }
+    fail(kInvalidArgument, "engine is not in the batch");
+}
+
+void Batch::set_graph(bool on) {
+    use_graph_ = on;
+    dirty_ = true;
+}
+
+// Scenes are grouped by lattice and kernel variant (the four-per-lane D3Q27
+// kernels by stencil width; D2Q9 scenes share the runtime-k kernels unless
+// all have the same specialised width).
+void Batch::rebuild() {
+    drop_graph();
+    groups_.clear();
+    any_sources_ = false;
+    auto key_of = [](Engine* e) {
+        const int Q = e->lattice();
+        const bool un = step_uses_unions(Q, e->stencil_k(), e->nodes());
+        return Q * 1000 + (un ? e->stencil_k() : 0);
+    };
+    std::vector<int> keys;
+    for (Engine* e : scenes_) {
+        const int key = key_of(e);
+        size_t g = 0;
+        while (g < keys.size() && keys[g] != key) ++g;
+        if (g == keys.size()) {
+            keys.push_back(key);
+            groups_.push_back(std::make_unique<Group>());
+            groups_.back()->Q = e->lattice();
+        }
+        groups_[g]->members.push_back(e);
+    }
+    for (auto& gp : groups_) {
+        Group& g = *gp;
+        const int n = int(g.members.size());
+        std::vector<SceneDesc> descs(n);
+        std::vector<int32_t> bstart(size_t(kStepKernels) * (n + 1), 0);
+        BatchPlan& P = g.plan;
+        P = BatchPlan{};
+        P.Q = g.Q;
+        P.nscenes = n;
+        int kcommon = g.members[0]->stencil_k();
+        for (int s = 0; s < n; ++s) {
+            Engine* e = g.members[s];
+            descs[s] = e->scene_desc();
+            if (e->stencil_k() != kcommon) kcommon = -1;
+            P.ucap = std::max(P.ucap, descs[s].S.ucap);
+            any_sources_ = any_sources_ || descs[s].S.src != nullptr;
+            int64_t nb[kStepKernels];
+            step_block_counts(g.Q, descs[s].G, descs[s].S, nb);
+            for (int p = 0; p < kStepKernels; ++p) {
+                int32_t* row = bstart.data() + size_t(p) * (n + 1);
+                const int64_t next = int64_t(row[s]) + nb[p];
+                require(next < (int64_t(1) << 31), "batch too large for one launch");
+                row[s + 1] = int32_t(next);
+            }
+        }
+        for (int p = 0; p < kStepKernels; ++p) P.blocks[p] = bstart[size_t(p) * (n + 1) + n];
+        P.unions = step_uses_unions(g.Q, g.members[0]->stencil_k(), g.members[0]->nodes());
+        P.k_template = (kcommon == 32 || kcommon == 16) ? kcommon : 0;
+        require(!P.unions || P.k_template != 0, "four-per-lane scenes need a common k");
+        g.table = to_device(descs, stream_);
+        g.bstart = to_device(bstart, stream_);
+        P.dev_scenes = g.table.get();
+        P.dev_bstart = g.bstart.get();
+    }
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    dirty_ = false;
+}
+
+void Batch::launch_one_step() {
+    for (auto& gp : groups_) {
+        if (gp->Q == 27)
+            launch_step_batch<27>(gp->plan, stream_);
+        else
+            launch_step_batch<9>(gp->plan, stream_);
+    }
+}
+
+void Batch::step(int nsteps) {
+    require(nsteps >= 0, "nsteps must be >= 0");
+    if (scenes_.empty() || nsteps == 0) return;
+    if (dirty_) rebuild();
+    // steps with density sources change kernel arguments per step: no graph
+    const bool graph = use_graph_ && !any_sources_;
+    if (graph && !exec_) {
+        NAGA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        launch_one_step();
+        NAGA_HIP(hipStreamEndCapture(stream_, &graph_));
+        NAGA_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        if (graph) {
+            NAGA_HIP(hipGraphLaunch(exec_, stream_));
+        } else {
+            for (Engine* e : scenes_) e->prepare_batched_step(stream_);
+            launch_one_step();
+        }
+        for (Engine* e : scenes_) e->finish_batched_step();
+    }
+    NAGA_HIP(hipStreamSynchronize(stream_));
+}
+
+double Batch::step_timed(int nsteps) {
+    hipEvent_t a, b;
+    NAGA_HIP(hipEventCreate(&a));
+    NAGA_HIP(hipEventCreate(&b));
+    if (dirty_ && !scenes_.empty()) rebuild();
+    NAGA_HIP(hipEventRecord(a, stream_));
+    step(nsteps);
+    NAGA_HIP(hipEventRecord(b, stream_));
+    NAGA_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    NAGA_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return double(ms);
 }
 
 void Engine::get_macroscopic(float* rho, float* u, double* drho, bool fresh) {

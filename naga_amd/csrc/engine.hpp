@@ -24,6 +24,8 @@ struct SceneConfig {
 // A declarative density source: amplitude * cos(2 pi frequency t + phase)
 // times the radial profile (1 - (2r/R)^2 / 4) exp(-(2r/R)^2) within radius
 // R of the centre, for t_start <= t < t_end.
+class Batch;
+
 struct DensitySource {
     double center[3] = {0.0, 0.0, 0.0};
     double radius = 0.0, amplitude = 0.0, frequency = 0.0, phase = 0.0;
@@ -50,6 +52,17 @@ class Engine {
     void get_pml(double* E, double* Qp);
     void add_density_source(const DensitySource& src);
     void clear_density_sources();
+
+    // Descriptor of this scene for batched launches.
+    SceneDesc scene_desc();
+    // Per-step host work of a batched step: the density-source field of
+    // this step (on `stream`), and the clock advance after it.
+    void prepare_batched_step(hipStream_t stream);
+    void finish_batched_step();
+    bool has_domain() const { return ready_; }
+    Batch* batch = nullptr;  // the batch holding this scene, if any
+    int stencil_k() const { return cfg_.k; }
+    int64_t fluid_nodes() const { return nfluid_; }
 
     int64_t nodes() const { return N_; }
     double time() const { return time_; }
@@ -101,6 +114,46 @@ class Engine {
     DeviceBuffer<int32_t> snodes_, soff_, sids_;
     DeviceBuffer<float> sprof_, src_field_;
     int64_t n_snodes_ = 0;
+};
+
+// Many scenes stepped together: every pass of a step is one launch over all
+// scenes of a group (scenes sharing lattice and kernel variant), optionally
+// replayed from a captured hipGraph.  Scenes stay owned by their engines.
+class Batch {
+  public:
+    Batch();
+    ~Batch();
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    void add(Engine* e);
+    void remove(Engine* e);
+    void step(int nsteps);
+    double step_timed(int nsteps);
+    void set_graph(bool on);
+    int size() const { return int(scenes_.size()); }
+    hipStream_t stream() const { return stream_; }
+
+  private:
+    struct Group {
+        int Q = 9;
+        std::vector<Engine*> members;
+        BatchPlan plan;
+        DeviceBuffer<SceneDesc> table;
+        DeviceBuffer<int32_t> bstart;
+    };
+    void rebuild();
+    void launch_one_step();
+    void drop_graph();
+
+    std::vector<Engine*> scenes_;
+    std::vector<std::unique_ptr<Group>> groups_;
+    hipStream_t stream_ = nullptr;
+    bool dirty_ = true;
+    bool use_graph_ = true;
+    bool any_sources_ = false;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
 };
 
 // Exact kNN of queries among ref (all host arrays), on the GPU.

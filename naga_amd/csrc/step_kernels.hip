@@ -17,6 +17,27 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
     return x * q + (x < r ? x : r) + l;
 }
 
+// Block position within its scene: the launch's own block for a single
+// scene, else the scene owning the block (last s with bstart[s] <= block)
+// whose descriptor replaces G and S.
+struct BlockCtx {
+    int64_t b, nb;
+    int scene;
+};
+
+__device__ __forceinline__ BlockCtx locate(const BatchArgs& B, StepGeom& G, StepBuffers& S) {
+    if (!B.scenes) return {int64_t(blockIdx.x), int64_t(gridDim.x), 0};
+    int lo = 0, hi = B.nscenes;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (B.bstart[mid] <= int32_t(blockIdx.x)) lo = mid;
+        else hi = mid;
+    }
+    G = B.scenes[lo].G;
+    S = B.scenes[lo].S;
+    return {int64_t(blockIdx.x) - B.bstart[lo], int64_t(B.bstart[lo + 1] - B.bstart[lo]), lo};
+}
+
 template <int Q>
 struct LaneMap {
     using L = Lattice<Q>;
@@ -52,11 +73,13 @@ __device__ __forceinline__ float lane_c(int a, int d) {
 // ---------------------------------------------------------------------------
 template <int Q>
 __global__ __launch_bounds__(kBlock) void k_p1(StepGeom G, StepBuffers S,
-                                               CollisionParams cp) {
+                                               CollisionParams cp, BatchArgs B) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS, LS = RS + 1;
     __shared__ float rows[kBlock * LS];
-    const int64_t node0 = int64_t(blockIdx.x) * kBlock;
+    const BlockCtx bc = locate(B, G, S);
+    if (B.scenes) cp = B.scenes[bc.scene].cp;
+    const int64_t node0 = bc.b * kBlock;
     const int64_t left = G.N - node0;
     const int cnt = left < kBlock ? int(left) : kBlock;
     const float* src = S.g + node0 * RS;
@@ -121,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_p1(StepGeom G, StepBuffers S,
 constexpr int kPmlRounds = 8;
 
 template <int Q>
-__global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S, BatchArgs B) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS;
     constexpr int NPB = kBlock / 64 * L::NPW;
@@ -131,11 +154,12 @@ __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S)
     const bool active = sub < L::NPW && al < Q;
     const int slot = wave * L::NPW + (sub < L::NPW ? sub : 0);
     const int a = al < Q ? al : Q - 1;
+    const BlockCtx bc = locate(B, G, S);
     const double cs2 = kCsLat * kCsLat;
     const double hdt = double(G.dt) / 2.0;
     const double ca[3] = {double(L::c(a, 0)), double(L::c(a, 1)), D == 3 ? double(L::c(a, 2)) : 0.0};
     double s1 = 0.0, s2 = 0.0;
-    const int64_t base = int64_t(blockIdx.x) * (NPB * kPmlRounds);
+    const int64_t base = bc.b * (NPB * kPmlRounds);
     for (int rd = 0; rd < kPmlRounds; ++rd) {
         const int64_t item = base + rd * NPB + slot;
         if (!active || item >= S.n_rq) continue;
@@ -169,15 +193,16 @@ __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S)
         const int which = threadIdx.x / Q, aa = threadIdx.x % Q;
         double t = 0.0;
         for (int sl = 0; sl < NPB; ++sl) t += red[sl][which][aa];
-        S.partial[int64_t(blockIdx.x) * 2 * Q + threadIdx.x] = t;
+        S.partial[bc.b * 2 * Q + threadIdx.x] = t;
     }
 }
 
 // layer means (pml_absorption.inl:775-785, :816-826): one block per column
 template <int Q>
-__global__ __launch_bounds__(256) void k_means(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(256) void k_means(StepGeom G, StepBuffers S, BatchArgs B) {
     __shared__ double red[256];
-    const int col = blockIdx.x;  // 0..2Q-1
+    const BlockCtx bc = locate(B, G, S);
+    const int col = int(bc.b);  // 0..2Q-1
     double s = 0.0;
     for (int p = threadIdx.x; p < S.nparts; p += 256) s += S.partial[int64_t(p) * 2 * Q + col];
     red[threadIdx.x] = s;
@@ -194,10 +219,11 @@ __global__ __launch_bounds__(256) void k_means(StepGeom G, StepBuffers S) {
 // p = j - num_bulk (reference indexing) or j.  Rows outside the layer stay
 // zero, so the divergence pass needs no branch.
 template <int Q>
-__global__ __launch_bounds__(kBlock) void k_pml_field(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_pml_field(StepGeom G, StepBuffers S, BatchArgs B) {
     using L = Lattice<Q>;
     constexpr int RS = L::RS;
-    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
+    const BlockCtx bc = locate(B, G, S);
+    const LaneMap<Q> m(xcd_tile(bc.b, bc.nb));
     if (m.node >= G.nlayer || !m.active) return;
     const int64_t j = G.nbulk + m.node;
     const int64_t p = G.pml_ref ? j - G.nbulk : j;
@@ -304,14 +330,15 @@ __device__ __forceinline__ float stencil_sum(const float* rp, int K, const float
 // the layer; the field rows are zero outside the layer.
 // ---------------------------------------------------------------------------
 template <int Q, int KT>
-__global__ __launch_bounds__(kBlock) void k_p2(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_p2(StepGeom G, StepBuffers S, BatchArgs B) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS, REC = 1 + D, KMAX = KT ? KT : 64;
     constexpr int NPB = Tile<Q>::NPB;
     __shared__ float srec[NPB * KMAX * REC];
+    const BlockCtx bc = locate(B, G, S);
     const int K = KT ? KT : G.k;
     const Lanes<Q> ln;
-    const int64_t item = xcd_tile(blockIdx.x, gridDim.x) * NPB + ln.slot;
+    const int64_t item = xcd_tile(bc.b, bc.nb) * NPB + ln.slot;
     const bool valid = item < S.n_pml;
     const int64_t i = valid ? S.pml_list[item] : 0;
     float* rp = srec + ln.slot * K * REC;
@@ -331,12 +358,14 @@ __global__ __launch_bounds__(kBlock) void k_p2(StepGeom G, StepBuffers S) {
 // false (p5) overwrites.
 // ---------------------------------------------------------------------------
 template <int Q, bool ACCUMULATE>
-__global__ __launch_bounds__(kBlock) void k_binterp(StepGeom G, float* field,
-                                                    const float* bslab,
-                                                    const float* coll) {
+__global__ __launch_bounds__(kBlock) void k_binterp(StepGeom G, StepBuffers S, BatchArgs B) {
     using L = Lattice<Q>;
     constexpr int RS = L::RS;
-    const LaneMap<Q> m(xcd_tile(blockIdx.x, gridDim.x));
+    const BlockCtx bc = locate(B, G, S);
+    float* field = ACCUMULATE ? S.A : S.g;
+    const float* bslab = S.bslab;
+    const float* coll = S.coll;
+    const LaneMap<Q> m(xcd_tile(bc.b, bc.nb));
     const int64_t nb = G.N - G.nfluid;
     if (m.node >= nb) return;
     const int a = m.a;
@@ -357,13 +386,14 @@ __global__ __launch_bounds__(kBlock) void k_binterp(StepGeom G, float* field,
 // (:789-944).  Reads s1 (A), writes the new state g.
 // ---------------------------------------------------------------------------
 template <int Q, int KT>
-__global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S, BatchArgs B) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS, KMAX = KT ? KT : 64;
     constexpr int NPB = Tile<Q>::NPB;
     __shared__ float srec[NPB * KMAX * 2];
+    const BlockCtx bc = locate(B, G, S);
     const int K = KT ? KT : G.k;
-    const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * NPB;
+    const int64_t node0 = xcd_tile(bc.b, bc.nb) * NPB;
     const int64_t nf = G.nfluid - node0;
     stage_contiguous<2>(S.fslab, node0, nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB), K, srec);
     __syncthreads();
@@ -418,13 +448,14 @@ __global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S) {
 //            own g row here; neighbours are gathered from A)
 // ---------------------------------------------------------------------------
 template <int Q, int STAGE, int KT, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S, BatchArgs B) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS, REC = 1 + D, KMAX = KT ? KT : 64;
     constexpr int NPB = Tile<Q>::NPB;
     __shared__ float srec[NPB * KMAX * REC];
+    const BlockCtx bc = locate(B, G, S);
     const int K = KT ? KT : G.k;
-    const int64_t node0 = xcd_tile(blockIdx.x, gridDim.x) * NPB;
+    const int64_t node0 = xcd_tile(bc.b, bc.nb) * NPB;
     const Lanes<Q> ln;
     const int64_t i = node0 + ln.slot;
     const int a = ln.a;
@@ -552,10 +583,11 @@ __device__ __forceinline__ void stage_union(float4* urows, const uint32_t* ul, i
 // RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
 // union staged in LDS (unions larger than ucap rows read through the list).
 template <int STAGE, int K, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, BatchArgs B) {
     __shared__ float4 srec[kV4Nodes * K];
     extern __shared__ float4 urows[];
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const BlockCtx bc = locate(B, G, S);
+    const int64_t tile = xcd_tile(bc.b, bc.nb);
     const int64_t node0 = tile * kV4Nodes;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
@@ -621,10 +653,11 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S) {
 // p4 for D3Q27 with the tile unions: selective filter on fluid nodes,
 // bounce-back on boundary nodes.
 template <int K, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, BatchArgs B) {
     __shared__ float2 srec[kV4Nodes * K];
     extern __shared__ float4 urows[];
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const BlockCtx bc = locate(B, G, S);
+    const int64_t tile = xcd_tile(bc.b, bc.nb);
     const int64_t node0 = tile * kV4Nodes;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
@@ -687,10 +720,11 @@ __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S) {
 // p2 for D3Q27 with four distributions per lane (PML divergence over the
 // listed nodes, added to s1).
 template <int K, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S) {
+__global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S, BatchArgs B) {
     __shared__ float4 srec[kV4Nodes * K];
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
-    const int64_t item = xcd_tile(blockIdx.x, gridDim.x) * kV4Nodes + slot;
+    const BlockCtx bc = locate(B, G, S);
+    const int64_t item = xcd_tile(bc.b, bc.nb) * kV4Nodes + slot;
     const bool valid = item < S.n_pml;
     const int64_t i = valid ? S.pml_list[item] : 0;
     float4* rp = srec + slot * K;
@@ -767,32 +801,8 @@ int stage_batch() {
     return b;
 }
 
-template <int Q, int KT, int BATCH>
-void launch_stage_kb(int stage, int64_t nb, const StepGeom& G, const StepBuffers& S,
-                     hipStream_t st) {
-    switch (stage) {
-        case 1: k_stage<Q, 1, KT, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
-        case 2: k_stage<Q, 2, KT, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
-        case 3: k_stage<Q, 3, KT, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
-        default: k_stage<Q, 4, KT, BATCH><<<nb, kBlock, 0, st>>>(G, S); break;
-    }
-}
-
-template <int Q, int KT>
-void launch_stage_k(int stage, int64_t nb, const StepGeom& G, const StepBuffers& S,
-                    hipStream_t st) {
-    const int b = KT ? stage_batch() : 8;
-    if (b == 32)
-        launch_stage_kb<Q, KT, 32>(stage, nb, G, S, st);
-    else if (b == 16)
-        launch_stage_kb<Q, KT, 16>(stage, nb, G, S, st);
-    else
-        launch_stage_kb<Q, KT, 8>(stage, nb, G, S, st);
-}
-
 // The four-per-lane D3Q27 kernels need k in {16, 32} and 32-bit row offsets.
-// NAGA_V4=0 selects the one-per-lane kernels; NAGA_V4_BATCH (4 or 8) sets
-// their gather batch.
+// NAGA_V4=0 selects the one-per-lane kernels.
 bool step_uses_unions(int Q, int k, int64_t N) {
     static const bool on = [] {
         const char* e = std::getenv("NAGA_V4");
@@ -801,139 +811,148 @@ bool step_uses_unions(int Q, int k, int64_t N) {
     return Q == 27 && on && (k == 32 || k == 16) && N < (int64_t(1) << 25);
 }
 
-template <int Q>
-bool use_v4(const StepGeom& G) {
-    return step_uses_unions(Q, G.k, G.N);
+namespace {
+
+// Passes of one step, in launch order.
+enum Pass { kP1, kPl, kPm, kPf, kP2, kP3, kP4, kS1, kS2, kS3, kS4, kP5 };
+
+// How a pass is launched: kernel template choices shared by every scene of
+// the launch, and the batch table (or none).
+struct PassCfg {
+    int ktmpl;     // K template: 32, 16 or 0 (runtime k)
+    bool unions;   // D3Q27 four-per-lane kernels with tile unions
+    int ucap;      // LDS union rows (dynamic shared memory / 128 B)
+    BatchArgs B;
+};
+
+template <int Q, int STAGE, int KT>
+void launch_stage_scalar(int64_t nb, const StepGeom& G, const StepBuffers& S, BatchArgs B,
+                         hipStream_t st) {
+    const int b = KT ? stage_batch() : 8;
+    if (b == 32)
+        k_stage<Q, STAGE, KT, 32><<<nb, kBlock, 0, st>>>(G, S, B);
+    else if (b == 16)
+        k_stage<Q, STAGE, KT, 16><<<nb, kBlock, 0, st>>>(G, S, B);
+    else
+        k_stage<Q, STAGE, KT, 8><<<nb, kBlock, 0, st>>>(G, S, B);
 }
 
-int v4_batch() {
-    static const int b = [] {
-        const char* e = std::getenv("NAGA_V4_BATCH");
-        const int v = e ? std::atoi(e) : 8;
-        return v == 4 ? 4 : 8;
-    }();
-    return b;
-}
-
-template <int K, int BATCH>
-void launch_stage_v4_b(int stage, int64_t nb, const StepGeom& G, const StepBuffers& S,
+template <int Q, int STAGE>
+void launch_stage_pass(int64_t nb, const StepGeom& G, const StepBuffers& S, const PassCfg& c,
                        hipStream_t st) {
-    const size_t lds = size_t(S.ucap) * 128u;
-    switch (stage) {
-        case 1: k_stage_u<1, K, BATCH><<<nb, kBlock, lds, st>>>(G, S); break;
-        case 2: k_stage_u<2, K, BATCH><<<nb, kBlock, lds, st>>>(G, S); break;
-        case 3: k_stage_u<3, K, BATCH><<<nb, kBlock, lds, st>>>(G, S); break;
-        default: k_stage_u<4, K, BATCH><<<nb, kBlock, lds, st>>>(G, S); break;
-    }
-}
-
-template <int K>
-void launch_stage_v4(int stage, const StepGeom& G, const StepBuffers& S, hipStream_t st) {
-    const int64_t nrange = stage == 1 ? G.N : G.nfluid;
-    const int64_t nb = (nrange + kV4Nodes - 1) / kV4Nodes;
-    if (nb == 0) return;
-    if (v4_batch() == 4)
-        launch_stage_v4_b<K, 4>(stage, nb, G, S, st);
-    else
-        launch_stage_v4_b<K, 8>(stage, nb, G, S, st);
-}
-
-template <int Q>
-void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S,
-                      hipStream_t st) {
-    if (use_v4<Q>(G)) {
-        if (G.k == 32)
-            launch_stage_v4<32>(stage, G, S, st);
+    if (Q == 27 && c.unions) {
+        const size_t lds = size_t(c.ucap) * 128u;
+        if (c.ktmpl == 32)
+            k_stage_u<STAGE, 32, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
         else
-            launch_stage_v4<16>(stage, G, S, st);
-        check_launch("rk4 stage");
-        return;
+            k_stage_u<STAGE, 16, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
+    } else if (c.ktmpl == 32) {
+        launch_stage_scalar<Q, STAGE, 32>(nb, G, S, c.B, st);
+    } else if (c.ktmpl == 16) {
+        launch_stage_scalar<Q, STAGE, 16>(nb, G, S, c.B, st);
+    } else {
+        launch_stage_scalar<Q, STAGE, 0>(nb, G, S, c.B, st);
     }
-    const int64_t nrange = stage == 1 ? G.N : G.nfluid;
-    const int64_t nb = lane_blocks<Q>(nrange);
-    if (nb == 0) return;
-    if (G.k == 32)
-        launch_stage_k<Q, 32>(stage, nb, G, S, st);
-    else if (G.k == 16)
-        launch_stage_k<Q, 16>(stage, nb, G, S, st);
-    else
-        launch_stage_k<Q, 0>(stage, nb, G, S, st);
-    check_launch("rk4 stage");
 }
 
 template <int Q>
-void launch_step(const StepGeom& G, const StepBuffers& S,
-                 const CollisionParams& cp, hipStream_t st, hipEvent_t* ev) {
-    const int64_t nbnd = G.N - G.nfluid;
-    int mark = 0;
-    auto tick = [&]() {
-        if (ev) hip_check(hipEventRecord(ev[mark], st), "event");
-        ++mark;
-    };
-    tick();
-    k_p1<Q><<<(G.N + kBlock - 1) / kBlock, kBlock, 0, st>>>(G, S, cp);
-    check_launch("p1");
+void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
+                 const CollisionParams& cp, const PassCfg& c, hipStream_t st) {
+    if (nb <= 0) return;
+    switch (pass) {
+        case kP1: k_p1<Q><<<nb, kBlock, 0, st>>>(G, S, cp, c.B); break;
+        case kPl: k_pml_local<Q><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+        case kPm: k_means<Q><<<nb, 256, 0, st>>>(G, S, c.B); break;
+        case kPf: k_pml_field<Q><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+        case kP2:
+            if (Q == 27 && c.unions) {
+                if (c.ktmpl == 32) k_p2_v4<32, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);
+                else k_p2_v4<16, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            } else if (c.ktmpl == 32) {
+                k_p2<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            } else if (c.ktmpl == 16) {
+                k_p2<Q, 16><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            } else {
+                k_p2<Q, 0><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            }
+            break;
+        case kP3: k_binterp<Q, true><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+        case kP4:
+            if (Q == 27 && c.unions) {
+                const size_t lds = size_t(c.ucap) * 128u;
+                if (c.ktmpl == 32) k_p4_u<32, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
+                else k_p4_u<16, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
+            } else if (c.ktmpl == 32) {
+                k_p4<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            } else if (c.ktmpl == 16) {
+                k_p4<Q, 16><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            } else {
+                k_p4<Q, 0><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            }
+            break;
+        case kS1: launch_stage_pass<Q, 1>(nb, G, S, c, st); break;
+        case kS2: launch_stage_pass<Q, 2>(nb, G, S, c, st); break;
+        case kS3: launch_stage_pass<Q, 3>(nb, G, S, c, st); break;
+        case kS4: launch_stage_pass<Q, 4>(nb, G, S, c, st); break;
+        default: k_binterp<Q, false><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+    }
+    check_launch("step pass");
+}
+
+int ktemplate(int k) { return k == 32 ? 32 : k == 16 ? 16 : 0; }
+
+}  // namespace
+
+void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* nb) {
+    const bool un = step_uses_unions(Q, G.k, G.N);
+    const int npb = kBlock / 64 * (Q == 27 ? Lattice<27>::NPW : Lattice<9>::NPW);
+    auto lanes = [&](int64_t n) { return (n + npb - 1) / npb; };
+    auto tiles = [&](int64_t n) { return (n + kV4Nodes - 1) / kV4Nodes; };
     const bool pml = G.nlayer > 0 && S.n_rq > 0;
-    tick();
-    if (pml) {
-        k_pml_local<Q><<<S.nparts, kBlock, 0, st>>>(G, S);
-        check_launch("pml local");
+    const int64_t nbnd = G.N - G.nfluid;
+    nb[kP1] = (G.N + kBlock - 1) / kBlock;
+    nb[kPl] = pml ? S.nparts : 0;
+    nb[kPm] = pml ? 2 * Q : 0;
+    nb[kPf] = pml ? lanes(G.nlayer) : 0;
+    nb[kP2] = (pml && S.n_pml > 0) ? (un ? tiles(S.n_pml) : lanes(S.n_pml)) : 0;
+    nb[kP3] = nbnd > 0 ? lanes(nbnd) : 0;
+    nb[kP4] = un ? tiles(G.N) : lanes(G.N);
+    nb[kS1] = un ? tiles(G.N) : lanes(G.N);
+    nb[kS2] = nb[kS3] = nb[kS4] = un ? tiles(G.nfluid) : lanes(G.nfluid);
+    nb[kP5] = nb[kP3];
+}
+
+template <int Q>
+void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S, hipStream_t st) {
+    int64_t nb[kStepKernels];
+    step_block_counts(Q, G, S, nb);
+    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}};
+    launch_pass<Q>(kS1 + stage - 1, nb[kS1 + stage - 1], G, S, CollisionParams{}, c, st);
+}
+
+template <int Q>
+void launch_step(const StepGeom& G, const StepBuffers& S, const CollisionParams& cp,
+                 hipStream_t st, hipEvent_t* ev) {
+    int64_t nb[kStepKernels];
+    step_block_counts(Q, G, S, nb);
+    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}};
+    for (int p = 0; p < kStepKernels; ++p) {
+        if (ev) hip_check(hipEventRecord(ev[p], st), "event");
+        launch_pass<Q>(p, nb[p], G, S, cp, c, st);
     }
-    tick();
-    if (pml) {
-        k_means<Q><<<2 * Q, 256, 0, st>>>(G, S);
-        check_launch("layer means");
+    if (ev) hip_check(hipEventRecord(ev[kStepKernels], st), "event");
+}
+
+template <int Q>
+void launch_step_batch(const BatchPlan& plan, hipStream_t st) {
+    const StepGeom G{};
+    const StepBuffers S{};
+    for (int p = 0; p < kStepKernels; ++p) {
+        const PassCfg c{plan.k_template, plan.unions, plan.ucap,
+                        BatchArgs{plan.dev_scenes, plan.dev_bstart + p * (plan.nscenes + 1),
+                                  plan.nscenes}};
+        launch_pass<Q>(p, plan.blocks[p], G, S, CollisionParams{}, c, st);
     }
-    tick();
-    if (pml) {
-        k_pml_field<Q><<<lane_blocks<Q>(G.nlayer), kBlock, 0, st>>>(G, S);
-        check_launch("pml field");
-    }
-    tick();
-    const bool v4 = use_v4<Q>(G);
-    if (pml && S.n_pml > 0 && v4) {
-        const int64_t nb = (S.n_pml + kV4Nodes - 1) / kV4Nodes;
-        if (G.k == 32)
-            k_p2_v4<32, 8><<<nb, kBlock, 0, st>>>(G, S);
-        else
-            k_p2_v4<16, 8><<<nb, kBlock, 0, st>>>(G, S);
-        check_launch("p2");
-    } else if (pml && S.n_pml > 0) {
-        if (G.k == 32)
-            k_p2<Q, 32><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
-        else if (G.k == 16)
-            k_p2<Q, 16><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
-        else
-            k_p2<Q, 0><<<lane_blocks<Q>(S.n_pml), kBlock, 0, st>>>(G, S);
-        check_launch("p2");
-    }
-    tick();
-    if (nbnd > 0) {
-        k_binterp<Q, true><<<lane_blocks<Q>(nbnd), kBlock, 0, st>>>(G, S.A, S.bslab, S.coll);
-        check_launch("p3");
-    }
-    tick();
-    if (v4 && G.k == 32)
-        k_p4_u<32, 8><<<(G.N + kV4Nodes - 1) / kV4Nodes, kBlock, size_t(S.ucap) * 128u, st>>>(G, S);
-    else if (v4)
-        k_p4_u<16, 8><<<(G.N + kV4Nodes - 1) / kV4Nodes, kBlock, size_t(S.ucap) * 128u, st>>>(G, S);
-    else if (G.k == 32)
-        k_p4<Q, 32><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
-    else if (G.k == 16)
-        k_p4<Q, 16><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
-    else
-        k_p4<Q, 0><<<lane_blocks<Q>(G.N), kBlock, 0, st>>>(G, S);
-    check_launch("p4");
-    for (int stage = 1; stage <= 4; ++stage) {
-        tick();
-        launch_rk4_stage<Q>(stage, G, S, st);
-    }
-    tick();
-    if (nbnd > 0) {
-        k_binterp<Q, false><<<lane_blocks<Q>(nbnd), kBlock, 0, st>>>(G, S.g, S.bslab, nullptr);
-        check_launch("p5");
-    }
-    tick();
 }
 
 void launch_source_field(const int32_t* nodes, const int32_t* off, const int32_t* ids,
@@ -955,6 +974,8 @@ void launch_macroscopic(const StepGeom& G, const float* g, float* rho,
 template void launch_step<9>(const StepGeom&, const StepBuffers&, const CollisionParams&, hipStream_t, hipEvent_t*);
 template void launch_step<27>(const StepGeom&, const StepBuffers&, const CollisionParams&, hipStream_t, hipEvent_t*);
 template void launch_rk4_stage<9>(int, const StepGeom&, const StepBuffers&, hipStream_t);
+template void launch_step_batch<9>(const BatchPlan&, hipStream_t);
+template void launch_step_batch<27>(const BatchPlan&, hipStream_t);
 template void launch_rk4_stage<27>(int, const StepGeom&, const StepBuffers&, hipStream_t);
 template void launch_macroscopic<9>(const StepGeom&, const float*, float*, double*, float*, hipStream_t);
 template void launch_macroscopic<27>(const StepGeom&, const float*, float*, double*, float*, hipStream_t);

@@ -82,6 +82,20 @@ struct StepBuffers {
     int ucap;
 };
 
+// Batched launches: one launch of a pass covers the scenes of a batch; the
+// blocks [bstart[s], bstart[s+1]) of the launch belong to scene s, which
+// reads its geometry, buffers and collision parameters from scenes[s].
+struct SceneDesc {
+    StepGeom G;
+    StepBuffers S;
+    CollisionParams cp;
+};
+struct BatchArgs {
+    const SceneDesc* scenes;  // null for a single-scene launch
+    const int32_t* bstart;    // nscenes + 1 block offsets of this pass
+    int nscenes;
+};
+
 // Density sources: per step, field[nodes[i]] = sum over entries e in
 // [off[i], off[i+1]) of prof[e] * amp[ids[e]] (sources in a fixed order).
 constexpr int kMaxSources = 16;
@@ -101,6 +115,25 @@ template <int Q>
 void launch_step(const StepGeom& geo, const StepBuffers& buf,
                  const CollisionParams& cp, hipStream_t stream,
                  hipEvent_t* events = nullptr);
+
+// One step of every scene of a batch: kStepKernels batched launches.  The
+// host-side table describes the same scenes as the device table (block
+// counts and kernel choices are derived from it); bstart holds
+// kStepKernels consecutive arrays of nscenes + 1 block offsets (device).
+struct BatchPlan {
+    int Q = 9;
+    int nscenes = 0;
+    const SceneDesc* dev_scenes = nullptr;
+    const int32_t* dev_bstart = nullptr;
+    int64_t blocks[kStepKernels] = {};  // total blocks per pass
+    int k_template = 0;                 // K template of the gather kernels (0: dynamic)
+    bool unions = false;                // D3Q27 four-per-lane kernels
+    int ucap = 0;                       // max LDS union rows over the scenes
+};
+// Block counts of each pass for one scene (the same counts launch_step uses).
+void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* blocks);
+template <int Q>
+void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 
 // Whether the step runs the D3Q27 four-per-lane kernels, which need the
 // tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union

@@ -1,0 +1,72 @@
+"""Batched stepping: every scene of a batch advances exactly as it does
+alone (same kernels per scene, one launch per pass over all scenes)."""
+import numpy as np
+import pytest
+
+import naga_amd
+from tests.scenes import gpu_engine, grid_nodes, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(oracle64, D, h, k, layer_cells, centre):
+    nodes = grid_nodes(oracle64, D, h, layer=layer_cells * h)
+    e = gpu_engine(nodes, 9 if D == 2 else 27, k, h)
+    e.set_initial_pulse(centre, 1e-3, 0.1)
+    return e
+
+
+def _specs():
+    return [(2, 0.02, 16, 5, [0.5, 0.5]), (2, 0.025, 12, 4, [0.3, 0.6]),
+            (2, 0.03125, 20, 3, [0.6, 0.4]), (2, 0.05, 16, 0, [0.5, 0.5]),
+            (3, 0.1, 32, 2, [0.5, 0.4, 0.5]), (3, 0.125, 32, 2, [0.4, 0.5, 0.5])]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_alone(oracle64, graph):
+    alone = [_scene(oracle64, *s) for s in _specs()]
+    mixed = [_scene(oracle64, *s) for s in _specs()]
+    batch = naga_amd.SceneBatch(mixed, graph=graph)
+    batch.step_forward(25)
+    for e in alone:
+        e.step_forward(25)
+    for a, b in zip(alone, mixed):
+        ma, mb = a.macroscopic(fresh=True), b.macroscopic(fresh=True)
+        assert abs(a.time() - b.time()) < 1e-12
+        assert rel_l2(mb["drho"], ma["drho"]) < 1e-6
+        assert rel_l2(mb["u"], ma["u"]) < 1e-6
+
+
+def test_batch_membership_changes(oracle64):
+    """Scenes join and leave between steps (as a serving loop does)."""
+    specs = _specs()
+    ref = [_scene(oracle64, *s) for s in specs[:3]]
+    for e in ref:
+        e.step_forward(30)
+    run = [_scene(oracle64, *s) for s in specs[:3]]
+    batch = naga_amd.SceneBatch([run[0]])
+    batch.step_forward(10)           # scene 0: 10
+    batch.add(run[1])
+    batch.step_forward(10)           # scene 0: 20, scene 1: 10
+    batch.add(run[2])
+    batch.remove(run[0])
+    run[0].step_forward(10)          # scene 0 alone: 30
+    batch.step_forward(20)           # scene 1: 30, scene 2: 20
+    batch.remove(run[2])
+    run[2].step_forward(10)          # scene 2: 30
+    for a, b in zip(ref, run):
+        assert rel_l2(b.macroscopic(fresh=True)["drho"], a.macroscopic(fresh=True)["drho"]) < 1e-6
+
+
+def test_batch_with_density_sources(oracle64):
+    specs = _specs()[:2]
+    alone = [_scene(oracle64, *s) for s in specs]
+    mixed = [_scene(oracle64, *s) for s in specs]
+    for e in alone + mixed:
+        e.register_density_source(naga_amd.DensitySource([0.5, 0.5], 0.1, 1e-3, 5.0))
+    batch = naga_amd.SceneBatch(mixed)
+    batch.step_forward(15)
+    for e in alone:
+        e.step_forward(15)
+    for a, b in zip(alone, mixed):
+        assert rel_l2(b.macroscopic(fresh=True)["drho"], a.macroscopic(fresh=True)["drho"]) < 1e-6

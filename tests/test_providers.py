@@ -1,0 +1,89 @@
+"""Procedural geometry (box minus circles/spheres): host-side node
+generation, checked for its defining properties."""
+import numpy as np
+import pytest
+
+import naga_amd
+
+
+def _grid_stream_points(D, h, jitter, seed):
+    """The uniform grid's node positions (SplitMix64, D draws per node)."""
+    n1 = int(np.ceil(np.float32(1.0) / np.float32(h))) + 1
+    sp = 1.0 / (n1 - 1)
+    s = seed
+    out = []
+    M = (1 << 64) - 1
+    for g in range(n1 ** D):
+        gi = [(g // n1 ** d) % n1 for d in range(D)]
+        face = any(v in (0, n1 - 1) for v in gi)
+        x = []
+        for d in range(D):
+            s = (s + 0x9E3779B97F4A7C15) & M
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+            z ^= z >> 31
+            u = (z >> 11) * 2.0 ** -53
+            v = gi[d] * sp + (0.0 if face else (2 * u - 1) * jitter * h)
+            x.append(v)
+        out.append((face, x))
+    return out
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_obstacle_grid_properties(D):
+    h, L, peak, jit, seed = (0.04, 0.12, 0.5, 0.25, 7) if D == 2 else (0.1, 0.2, 0.5, 0.25, 9)
+    obst = ([([0.3, 0.6], 0.1), ([0.7, 0.35], 0.08), ([0.72, 0.42], 0.05)] if D == 2
+            else [([0.5, 0.5, 0.5], 0.22)])
+    nodes = naga_amd.ObstacleGridProvider([0] * D, [1] * D, h, obst, L, peak, jitter=jit,
+                                          seed=seed).get()
+    P = nodes.points.astype(np.float64)
+    nb, nl, nbd = nodes.num_bulk_points, nodes.num_layer_points, nodes.num_boundary_points
+    assert P.shape == (nb + nl + nbd, D)
+    fluid = P[: nb + nl]
+    for c, r in obst:
+        dist = np.linalg.norm(fluid - np.asarray(c), axis=1)
+        assert dist.min() >= r + 0.5 * h - 1e-6
+    # every grid node outside the padded obstacles is kept, at the uniform
+    # grid's jittered position
+    expect = []
+    for face, x in _grid_stream_points(D, h, jit, seed):
+        if all(np.linalg.norm(np.asarray(x) - np.asarray(c)) >= r + 0.5 * h for c, r in obst):
+            expect.append((face, x))
+    got_fluid = {tuple(np.round(p, 5)) for p in fluid}
+    exp_fluid = {tuple(np.round(np.float32(x).astype(np.float64), 5)) for f, x in expect if not f}
+    assert got_fluid == exp_fluid
+    # layer: fluid within L of the faces, sigma = peak (1 - d/L)^2
+    dface = np.minimum(fluid, 1 - fluid).min(1)
+    assert (dface[:nb] > L).all() and (dface[nb:] <= L + 1e-6).all()
+    assert np.allclose(nodes.layer_absorption, peak * (1 - dface[nb:] / L) ** 2, atol=1e-5)
+    # boundary: outer faces with inward normals, obstacle surfaces with
+    # normals pointing into the fluid (radially out of the obstacle)
+    B, Nrm = P[nb + nl:], nodes.boundary_normals.astype(np.float64)
+    assert np.allclose(np.linalg.norm(Nrm, axis=1), 1, atol=1e-6)
+    on_face = np.isclose(B, 0).any(1) | np.isclose(B, 1).any(1)
+    inner = ~on_face
+    assert inner.sum() > 0
+    for p, n in zip(B[inner], Nrm[inner]):
+        rs = [abs(np.linalg.norm(p - c) - r) for c, r in obst]
+        j = int(np.argmin(rs))
+        assert rs[j] < 1e-5
+        radial = (p - np.asarray(obst[j][0])) / obst[j][1]
+        assert np.allclose(n, radial, atol=1e-5)
+    fn = Nrm[on_face]
+    fp = B[on_face]
+    assert np.all((fn * (0.5 - fp)).sum(1) > 0)  # inward
+    # about one surface node per spacing
+    per = [np.sum(np.abs(np.linalg.norm(B - c, axis=1) - r) < 1e-5) for c, r in obst]
+    if D == 2:
+        assert per[0] == int(np.ceil(2 * np.pi * 0.1 / h))
+    else:
+        assert per[0] == int(np.ceil(4 * np.pi * 0.22 ** 2 / h ** 2))
+
+
+def test_obstacle_grid_without_obstacles_matches_uniform_grid_nodes():
+    h = 0.05
+    a = naga_amd.ObstacleGridProvider([0, 0], [1, 1], h, (), 0.0, 0.0, jitter=0.25).get()
+    pts = {tuple(p) for p in a.points.tolist()}
+    exp = {tuple(np.float32(x).tolist()) for f, x in _grid_stream_points(2, h, 0.25, 0x4E414741)}
+    assert pts == exp
