@@ -168,7 +168,13 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     fslab_.allocate(nfluid * k * 2);
     launch_make_slab(nfluid, k, 1, idx_.get(), filt_.get(), fslab_.get(), st);
     ucap_ = 0;
-    if (step_uses_unions(Q, k, N) && nfluid > 0) {
+    ulist_ = DeviceBuffer<uint32_t>();
+    uoff_ = DeviceBuffer<int64_t>();
+    // Tile unions staged in LDS measured slower than direct gathers on the
+    // 1M-node D3Q27 scene (0.61 vs 0.31 ms per RK4 stage: three blocks per
+    // CU and a load-then-compute phase per tile); NAGA_UNIONS=1 enables them.
+    const char* un_env = std::getenv("NAGA_UNIONS");
+    if (un_env && un_env[0] == '1' && step_uses_unions(Q, k, N) && nfluid > 0) {
         // tile unions; slab index fields become union positions
         const int64_t ntiles = (nfluid + kTileNodes - 1) / kTileNodes;
         DeviceBuffer<int32_t> counts(ntiles);

@@ -499,9 +499,10 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S, Bat
 // served by 8 lanes, each holding alpha = 4q .. 4q+3 (q = lane % 8; the last
 // five slots of the row are padding with zero lattice velocity), so a
 // wavefront covers 8 nodes and a block one tile of 32 consecutive nodes.
-// The distinct stencil neighbours of a tile (about 220 rows for 1024
-// stencil entries on jittered grids) are loaded once into LDS, and the
-// stencil sums read them from there.  Rows in HBM are addressed through
+// Neighbour rows are gathered from HBM/L2 with one 16-byte load per lane,
+// or, with tile unions enabled, the distinct stencil neighbours of a tile
+// (about 220 rows for 1024 stencil entries on jittered grids) are loaded
+// once into LDS and read from there.  Rows in HBM are addressed through
 // buffer resources with 32-bit byte offsets (row j at j * 128), which
 // bounds the node count by 2^25; loads past the end of a buffer return
 // zero instead of faulting.
@@ -604,12 +605,14 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     int U = 0;
     const uint32_t* ul = S.ulist;
     if (cnt > 0) {
-        const int64_t u0 = S.uoff[tile];
-        U = int(S.uoff[tile + 1] - u0);
-        ul += u0;
         const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
         for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
-        if (U <= S.ucap) stage_union(urows, ul, U, gin, q);
+        if (ul) {
+            const int64_t u0 = S.uoff[tile];
+            U = int(S.uoff[tile + 1] - u0);
+            ul += u0;
+            if (U <= S.ucap) stage_union(urows, ul, U, gin, q);
+        }
     }
     __syncthreads();
     if (STAGE == 1 && !fluid && i < G.N) {
@@ -619,7 +622,11 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     }
     if (!fluid) return;
     float4 r;
-    if (U <= S.ucap)
+    if (!ul)  // records hold node indices
+        r = rate_fetch<K, BATCH>(
+            srec + slot * K,
+            [&](uint32_t j) { return ld4(gin, j * 128u + uint32_t(q) * 16u); }, q);
+    else if (U <= S.ucap)
         r = rate_fetch<K, BATCH>(srec + slot * K,
                                  [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
     else
@@ -670,19 +677,21 @@ __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, Batc
     int U = 0;
     const uint32_t* ul = S.ulist;
     if (cnt > 0) {
-        const int64_t u0 = S.uoff[tile];
-        U = int(S.uoff[tile + 1] - u0);
-        ul += u0;
         const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
         for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
-        if (U <= S.ucap) stage_union(urows, ul, U, s1, q);
+        if (ul) {
+            const int64_t u0 = S.uoff[tile];
+            U = int(S.uoff[tile + 1] - u0);
+            ul += u0;
+            if (U <= S.ucap) stage_union(urows, ul, U, s1, q);
+        }
     }
     __syncthreads();
     if (i >= G.N) return;
     float4 out;
     if (i < G.nfluid) {
         const float2* rp = srec + slot * K;
-        const bool lds = U <= S.ucap;
+        const bool lds = ul && U <= S.ucap;
         float sum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s0 = 0; s0 < K; s0 += BATCH) {
@@ -693,7 +702,8 @@ __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, Batc
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
                 const uint32_t j = __float_as_uint(rec[u].x);
-                gv[u] = lds ? urows[j * kV4Lanes + q] : ld4(s1, ul[j] * 128u + uint32_t(q) * 16u);
+                gv[u] = lds ? urows[j * kV4Lanes + q]
+                            : ld4(s1, (ul ? ul[j] : j) * 128u + uint32_t(q) * 16u);
             }
 #pragma unroll
             for (int u = 0; u < BATCH; ++u)
@@ -736,9 +746,11 @@ __global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S, Bat
     if (!valid) return;
     const uint64_t rows = uint64_t(G.N) * 128u;
     const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
-    const uint32_t* ul = S.ulist + S.uoff[i / kV4Nodes];  // records hold union positions
+    // records hold node indices, or positions in the node's tile union
+    const uint32_t* ul = S.ulist ? S.ulist + S.uoff[i / kV4Nodes] : nullptr;
     const float4 r = rate_fetch<K, BATCH>(
-        rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + uint32_t(q) * 16u); }, q);
+        rp, [&](uint32_t j) { return ld4(gf, (ul ? ul[j] : j) * 128u + uint32_t(q) * 16u); },
+        q);
     // the divergence of F = c Gf is the negated rate of Gf
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const float4 s = ld4(rsrc(S.A, rows), own);
