@@ -102,6 +102,105 @@ def cpu_baseline(threads: int) -> dict:
                       f"after 1 warm-up step, FP32 oracle port, OpenMP {threads} threads"}
 
 
+def batched_workload(steps: int, n_scenes: int = 256, seed: int = 0x4E414741) -> dict:
+    """Many small D2Q9 scenes: node counts log-uniform in [2k, 50k], 0-5
+    circular obstacles of radius 0.03-0.1, k in [12, 24], random pulse.
+    The same scenes are stepped one scene at a time (each engine's own
+    launches, scene after scene) and together in one batch (one launch per
+    pass over all scenes, graph replay, autotuned gather batch)."""
+    import naga_amd
+    rng = np.random.default_rng(seed)
+    engines, total = [], 0
+    t0 = time.perf_counter()
+    for _ in range(n_scenes):
+        n = float(np.exp(rng.uniform(np.log(2e3), np.log(5e4))))
+        h = 1.0 / (np.sqrt(n) - 1.0)
+        obst = [([float(v) for v in rng.uniform(0.15, 0.85, 2)], float(rng.uniform(0.03, 0.1)))
+                for _ in range(int(rng.integers(0, 6)))]
+        k = int(rng.integers(12, 25))
+        nodes = naga_amd.ObstacleGridProvider([0, 0], [1, 1], h, obst, 8 * h, 0.5, jitter=0.25,
+                                              seed=int(rng.integers(1, 1 << 62))).get()
+        eng = naga_amd.SimulationEngine(lattice=9, k=k, k_boundary=k,
+                                        rbf=naga_amd.RBF_POLYHARMONIC3,
+                                        rbf_boundary=naga_amd.RBF_POLYHARMONIC3)
+        p = naga_amd.problem_parameters(9, 0.6, 0.4 * h)
+        eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+        eng.init_domain(nodes)
+        eng.set_initial_pulse([float(v) for v in rng.uniform(0.2, 0.8, 2)], 1e-3, 0.05)
+        engines.append(eng)
+        total += eng.num_nodes
+    setup_s = time.perf_counter() - t0
+    print(f"batched: {n_scenes} scenes, {total} nodes, setup {setup_s:.1f} s", file=sys.stderr,
+          flush=True)
+    for e in engines:
+        e.step_forward(2)
+    t0 = time.perf_counter()
+    for e in engines:
+        e.step_forward(steps)
+    plain_s = time.perf_counter() - t0
+    batch = naga_amd.SceneBatch(engines)
+    gb = batch.autotune(3)
+    batch.step_forward(2)
+    t0 = time.perf_counter()
+    batch.step_forward(steps)
+    batched_s = time.perf_counter() - t0
+    plain, batched = total * steps / plain_s, total * steps / batched_s
+    batch.close()
+    return {"scenes": n_scenes, "nodes": total, "steps": steps, "setup_s": setup_s,
+            "plain_node_updates_per_s": plain, "batched_node_updates_per_s": batched,
+            "speedup": batched / plain, "gather_batch": gb,
+            "workload": "D2Q9, 256 scenes, N log-uniform 2k-50k, 0-5 obstacles r 0.03-0.1, "
+                        "k 12-24, PML 8h, tau 0.6, dt 0.4h"}
+
+
+def served_workload(n_requests: int = 48, rate: float = 50.0, seed: int = 0x4E414742) -> dict:
+    """Poisson arrivals at `rate` requests/s, D2Q9/D3Q27 50/50, node counts
+    log-uniform in [5k, 200k], steps uniform in [100, 500], random pulse and
+    obstacles; served with continuous batching, then the same requests
+    solved one at a time with their own launches (setup included in both)."""
+    import naga_amd
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_requests):
+        lat = 9 if rng.random() < 0.5 else 27
+        D = 2 if lat == 9 else 3
+        n = float(np.exp(rng.uniform(np.log(5e3), np.log(2e5))))
+        h = 1.0 / (n ** (1.0 / D) - 1.0)
+        if D == 2:
+            obst = [([float(v) for v in rng.uniform(0.15, 0.85, 2)], float(rng.uniform(0.03, 0.1)))
+                    for _ in range(int(rng.integers(0, 6)))]
+        else:
+            obst = [([0.5, 0.5, 0.5], 0.15)]
+        reqs.append(naga_amd.make_request(
+            lat, h, int(rng.integers(100, 501)), obst, seed=int(rng.integers(1, 1 << 62)),
+            pulse_center=[float(v) for v in rng.uniform(0.2, 0.35, D)], pulse_width=0.05))
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    srv = naga_amd.SolveServer(hbm_budget_bytes=96 << 30, max_active=64, chunk_steps=50)
+    t0 = time.perf_counter()
+    for r, g in zip(reqs, gaps):
+        time.sleep(float(g))
+        srv.submit(r)
+    srv.drain()
+    st = srv.stats()
+    srv.close()
+    print(f"served: {n_requests} requests in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    work = 0.0
+    t1 = time.perf_counter()
+    for r in reqs:
+        out = naga_amd.solve_alone(r)
+        work += out["nodes"] * r.steps
+    plain_s = time.perf_counter() - t1
+    return {"requests": n_requests, "rate_per_s": rate, "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "served_node_updates_per_s": st["node_updates_per_s"],
+            "plain_node_updates_per_s": work / plain_s,
+            "speedup": st["node_updates_per_s"] / (work / plain_s),
+            "trace_s": time.perf_counter() - t0 - plain_s,
+            "workload": "Poisson 50 req/s, D2Q9/D3Q27 50/50, N log-uniform 5k-200k, "
+                        "steps 100-500, obstacles (2D: 0-5 circles; 3D: sphere r 0.15)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +211,9 @@ def main() -> int:
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched and served workloads")
+    ap.add_argument("--batched-steps", type=int, default=100)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -176,6 +278,14 @@ def main() -> int:
         "kernel_ms": prof,
         "roofline": roofline,
     }
+    if not args.no_batched:
+        eng.close()
+        for name, fn in (("batched", lambda: batched_workload(args.batched_steps)),
+                         ("served", served_workload)):
+            try:
+                out[name] = fn()
+            except Exception as exc:  # reported, not fatal to the headline line
+                out[name] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(threads)

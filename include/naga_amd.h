@@ -93,6 +93,16 @@ int naga_engine_add_density_source(naga_engine* e, const double* center, double 
                                    double amplitude, double frequency, double phase,
                                    double t_start, double t_end);
 int naga_engine_clear_density_sources(naga_engine* e);
+/* Snapshot/restore of the scene state that carries between steps
+ * (distributions, PML history, held macroscopic values, clock) to a host
+ * buffer; replaces simulation_engine::save_state/load_state
+ * (fluids/nonlocal_lattice_boltzmann/simulation_engine.cuh:108-116) with a
+ * binary format: a header (magic "NAGAST01", lattice, sizes, time, frame)
+ * then FP32 arrays.  Setup products are rebuilt by init_domain and are not
+ * part of the state. */
+int naga_engine_state_size(naga_engine* e, int64_t* bytes);
+int naga_engine_save_state(naga_engine* e, void* buf, int64_t capacity);
+int naga_engine_load_state(naga_engine* e, const void* buf, int64_t bytes);
 /* f[Q*N] alpha-major full distributions (FP64 in, stored as FP32 deviations). */
 int naga_engine_set_distributions(naga_engine* e, const double* f);
 
@@ -136,9 +146,63 @@ int naga_batch_destroy(naga_batch* b);
 int naga_batch_add(naga_batch* b, naga_engine* e);     /* e needs a domain */
 int naga_batch_remove(naga_batch* b, naga_engine* e);
 int naga_batch_set_graph(naga_batch* b, int32_t on);  /* default on */
+/* Autotuner: times trial_steps steps of the batch for each gather batch
+ * (neighbours gathered per unrolled step: 8, 16, 32) with and without graph
+ * replay, keeps the fastest, and restores every scene's state afterwards. */
+int naga_batch_autotune(naga_batch* b, int32_t trial_steps, int32_t* gather_batch);
 /* nsteps steps of every scene (each engine's time advances by its dt) */
 int naga_batch_step(naga_batch* b, int32_t nsteps);
 int naga_batch_step_timed(naga_batch* b, int32_t nsteps, double* ms);
+
+/* ---- serving loop: acoustic-pulse solves with continuous batching.  No
+ *      counterpart in the reference.  Requests are admitted while their
+ *      estimated device footprint fits hbm_budget_bytes and fewer than
+ *      max_active are resident; a setup thread builds admitted scenes (host
+ *      node generation, GPU setup on the scene's own stream) while a
+ *      stepping thread advances all resident scenes together in chunks of
+ *      at most chunk_steps steps; finished scenes leave at once and free
+ *      their memory.  Scenes are the unit box minus obstacles
+ *      (naga_obstacle_grid), dt = 0.4 h, c = c_s, rho0 = 1, r^3 RBFs. */
+typedef struct naga_server naga_server;
+#define NAGA_MAX_OBSTACLES 8
+typedef struct naga_request {
+    int32_t lattice;           /* 9 (D2Q9) or 27 (D3Q27) */
+    int32_t k;                 /* stencil width; 0: 16 (D2Q9) or 32 (D3Q27) */
+    int32_t steps;
+    int32_t n_obstacles;       /* <= NAGA_MAX_OBSTACLES */
+    double h;                  /* grid spacing */
+    double jitter;             /* interior jitter, fraction of h */
+    uint64_t seed;
+    double obstacles[NAGA_MAX_OBSTACLES * 4];  /* per obstacle: centre[D], radius */
+    double layer_thickness;    /* absorbing layer at the box faces */
+    double peak_absorption;
+    double tau;                /* relaxation time (0: 0.6) */
+    double pulse_center[3];
+    double pulse_amplitude;
+    double pulse_width;
+} naga_request;
+
+typedef struct naga_server_stats {
+    int64_t submitted, completed, failed, resident, queued;
+    int64_t bytes_in_use;
+    double p50_ms, p99_ms, mean_ms;  /* latency, submit to result ready */
+    double node_updates_per_s;       /* completed work / (last completion - first submit) */
+} naga_server_stats;
+
+int naga_server_create(int64_t hbm_budget_bytes, int32_t max_active, int32_t chunk_steps,
+                       int32_t use_graph, naga_server** out);
+int naga_server_destroy(naga_server* s);  /* waits for resident work */
+int naga_server_submit(naga_server* s, const naga_request* req, int64_t* id);
+/* *state: 0 queued or running, 1 done (see naga_server_result) */
+int naga_server_status(naga_server* s, int64_t id, int32_t* state);
+/* Result of a done request: *n_nodes always; rho[N], u[D*N], drho[N] when
+ * not NULL (fresh values of the final state).  Fails with the request's
+ * error if its setup failed. */
+int naga_server_result(naga_server* s, int64_t id, int64_t* n_nodes, float* rho, float* u,
+                       double* drho, double* latency_ms);
+int naga_server_release(naga_server* s, int64_t id);
+int naga_server_drain(naga_server* s);   /* block until all submitted are done */
+int naga_server_get_stats(naga_server* s, naga_server_stats* out);
 
 /* ---- node provider: uniform_grid_provider<Lattice>::get()
  *      (node_providers/uniform_grid_provider.cuh:65-161) with seeded

@@ -46,6 +46,30 @@ class SceneConfig(C.Structure):
     ]
 
 
+MAX_OBSTACLES = 8
+
+
+class Request(C.Structure):
+    """naga_request: one acoustic-pulse solve for the serving loop."""
+    _fields_ = [
+        ("lattice", C.c_int32), ("k", C.c_int32), ("steps", C.c_int32),
+        ("n_obstacles", C.c_int32), ("h", C.c_double), ("jitter", C.c_double),
+        ("seed", C.c_uint64), ("obstacles", C.c_double * (MAX_OBSTACLES * 4)),
+        ("layer_thickness", C.c_double), ("peak_absorption", C.c_double),
+        ("tau", C.c_double), ("pulse_center", C.c_double * 3),
+        ("pulse_amplitude", C.c_double), ("pulse_width", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+        ("resident", C.c_int64), ("queued", C.c_int64), ("bytes_in_use", C.c_int64),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+        ("node_updates_per_s", C.c_double),
+    ]
+
+
 _lib = None
 _BATCH_PREFIX = "naga::batch: "
 
@@ -97,6 +121,18 @@ def load() -> C.CDLL:
         "naga_batch_step_timed": (C.c_int, [vp, i32, vp]),
         "naga_obstacle_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, i32, vp, f64, f64, i64,
                                          vp, vp, vp, vp]),
+        "naga_server_create": (C.c_int, [i64, i32, i32, i32, C.POINTER(vp)]),
+        "naga_server_destroy": (C.c_int, [vp]),
+        "naga_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(i64)]),
+        "naga_server_status": (C.c_int, [vp, i64, C.POINTER(i32)]),
+        "naga_server_result": (C.c_int, [vp, i64, C.POINTER(i64), vp, vp, vp, C.POINTER(f64)]),
+        "naga_server_release": (C.c_int, [vp, i64]),
+        "naga_server_drain": (C.c_int, [vp]),
+        "naga_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "naga_batch_autotune": (C.c_int, [vp, i32, C.POINTER(i32)]),
+        "naga_engine_state_size": (C.c_int, [vp, C.POINTER(i64)]),
+        "naga_engine_save_state": (C.c_int, [vp, vp, i64]),
+        "naga_engine_load_state": (C.c_int, [vp, vp, i64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -118,7 +154,11 @@ EXPORTED = [
     "naga_divergence_apply", "naga_engine_add_density_source",
     "naga_engine_clear_density_sources", "naga_batch_create", "naga_batch_destroy",
     "naga_batch_add", "naga_batch_remove", "naga_batch_set_graph", "naga_batch_step",
-    "naga_batch_step_timed", "naga_obstacle_grid",
+    "naga_batch_step_timed", "naga_obstacle_grid", "naga_server_create",
+    "naga_server_destroy", "naga_server_submit", "naga_server_status", "naga_server_result",
+    "naga_server_release", "naga_server_drain", "naga_server_get_stats",
+    "naga_batch_autotune", "naga_engine_state_size", "naga_engine_save_state",
+    "naga_engine_load_state",
 ]
 
 STEP_KERNELS = ["p1_collide", "pl_pml_local", "pm_layer_means", "pf_pml_field",
@@ -396,6 +436,19 @@ class SimulationEngine:
         """reset() (detail/simulation_engine.cuh:1377-1386): f = w, time 0."""
         self.set_initial_pulse(np.zeros(self.dimensions), 0.0, 1.0)
 
+    def save_state(self) -> bytes:
+        """Snapshot of the state carried between steps (save_state,
+        simulation_engine.cuh:108-116) as bytes."""
+        lib = load()
+        n = C.c_int64()
+        _check(lib.naga_engine_state_size(self._live(), C.byref(n)), _ENGINE_PREFIX)
+        buf = C.create_string_buffer(n.value)
+        _check(lib.naga_engine_save_state(self._live(), buf, n.value), _ENGINE_PREFIX)
+        return buf.raw
+
+    def load_state(self, state: bytes):
+        _check(load().naga_engine_load_state(self._live(), state, len(state)), _ENGINE_PREFIX)
+
     def register_density_source(self, source: "DensitySource"):
         """register_density_source (simulation_engine.cuh:79-81) for a
         declarative source; see DensitySource."""
@@ -609,12 +662,132 @@ class SceneBatch:
         _check(load().naga_batch_step_timed(self._h, int(nsteps), C.byref(ms)), _BATCH_PREFIX)
         return ms.value
 
+    def autotune(self, trial_steps: int = 5) -> int:
+        gb = C.c_int32()
+        _check(load().naga_batch_autotune(self._h, int(trial_steps), C.byref(gb)), _BATCH_PREFIX)
+        return gb.value
+
     def __len__(self):
         return len(self._engines)
 
     def close(self):
         if self._h is not None:
             load().naga_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_SERVER_PREFIX = "naga::server: "
+
+
+def make_request(lattice, h, steps, obstacles=(), k=0, jitter=0.25, seed=0x4E414741,
+                 layer_thickness=None, peak_absorption=0.5, tau=0.6, pulse_center=None,
+                 pulse_amplitude=1e-3, pulse_width=0.05) -> Request:
+    """A solve of the unit box (square or cube) minus obstacles
+    [(centre, radius), ...]; the absorbing layer defaults to 8 spacings."""
+    D = 2 if lattice == 9 else 3
+    r = Request()
+    r.lattice, r.k, r.steps = int(lattice), int(k), int(steps)
+    r.n_obstacles = len(obstacles)
+    if r.n_obstacles > MAX_OBSTACLES:
+        raise ValueError(_SERVER_PREFIX + "too many obstacles")
+    for j, (c, rad) in enumerate(obstacles):
+        for d in range(D):
+            r.obstacles[j * (D + 1) + d] = float(c[d])
+        r.obstacles[j * (D + 1) + D] = float(rad)
+    r.h, r.jitter, r.seed = float(h), float(jitter), int(seed)
+    r.layer_thickness = 8 * h if layer_thickness is None else float(layer_thickness)
+    r.peak_absorption, r.tau = float(peak_absorption), float(tau)
+    pc = [0.5] * D if pulse_center is None else list(pulse_center)
+    for d in range(D):
+        r.pulse_center[d] = float(pc[d])
+    r.pulse_amplitude, r.pulse_width = float(pulse_amplitude), float(pulse_width)
+    return r
+
+
+def request_nodes(req: Request) -> "SimulationNodes":
+    """The scene a request describes (what the server builds for it)."""
+    D = 2 if req.lattice == 9 else 3
+    obst = [([req.obstacles[j * (D + 1) + d] for d in range(D)], req.obstacles[j * (D + 1) + D])
+            for j in range(req.n_obstacles)]
+    return ObstacleGridProvider([0] * D, [1] * D, req.h, obst, req.layer_thickness,
+                                req.peak_absorption, jitter=req.jitter, seed=req.seed).get()
+
+
+def solve_alone(req: Request) -> dict:
+    """The same solve on one engine with its own launches (reference for
+    the served result)."""
+    k = req.k or (16 if req.lattice == 9 else 32)
+    eng = SimulationEngine(lattice=req.lattice, k=k, k_boundary=k, rbf=RBF_POLYHARMONIC3,
+                           rbf_boundary=RBF_POLYHARMONIC3, pml_ref_index=True)
+    p = problem_parameters(req.lattice, req.tau or 0.6, 0.4 * req.h)
+    eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+    eng.init_domain(request_nodes(req))
+    D = 2 if req.lattice == 9 else 3
+    eng.set_initial_pulse([req.pulse_center[d] for d in range(D)], req.pulse_amplitude,
+                          req.pulse_width)
+    eng.step_forward(req.steps)
+    out = eng.macroscopic(fresh=True)
+    out["nodes"] = eng.num_nodes
+    eng.close()
+    return out
+
+
+class SolveServer:
+    """Serving loop with continuous batching (naga_server_*)."""
+
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 64, chunk_steps: int = 50,
+                 graph: bool = True):
+        h = C.c_void_p()
+        _check(load().naga_server_create(int(hbm_budget_bytes), int(max_active),
+                                         int(chunk_steps), int(bool(graph)), C.byref(h)),
+               _SERVER_PREFIX)
+        self._h = h
+        self._dims = {}
+
+    def submit(self, req: Request) -> int:
+        i = C.c_int64()
+        _check(load().naga_server_submit(self._h, C.byref(req), C.byref(i)), _SERVER_PREFIX)
+        self._dims[i.value] = 2 if req.lattice == 9 else 3
+        return i.value
+
+    def done(self, rid: int) -> bool:
+        st = C.c_int32()
+        _check(load().naga_server_status(self._h, int(rid), C.byref(st)), _SERVER_PREFIX)
+        return st.value == 1
+
+    def result(self, rid: int, release: bool = True) -> dict:
+        lib = load()
+        n = C.c_int64()
+        lat = C.c_double()
+        _check(lib.naga_server_result(self._h, int(rid), C.byref(n), None, None, None,
+                                      C.byref(lat)), _SERVER_PREFIX)
+        N, D = n.value, self._dims[rid]
+        rho = np.zeros(N, np.float32)
+        u = np.zeros(D * N, np.float32)
+        drho = np.zeros(N)
+        _check(lib.naga_server_result(self._h, int(rid), C.byref(n), _ptr(rho), _ptr(u),
+                                      _ptr(drho), C.byref(lat)), _SERVER_PREFIX)
+        if release:
+            lib.naga_server_release(self._h, int(rid))
+        return dict(rho=rho, u=u.reshape(D, N), drho=drho, latency_ms=lat.value, nodes=N)
+
+    def drain(self):
+        _check(load().naga_server_drain(self._h), _SERVER_PREFIX)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(load().naga_server_get_stats(self._h, C.byref(st)), _SERVER_PREFIX)
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h is not None:
+            load().naga_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -8,6 +8,7 @@
 
 #include "../../include/naga_amd.h"
 #include "engine.hpp"
+#include "server.hpp"
 #include "setup_kernels.hpp"
 
 using namespace naga_amd;
@@ -18,6 +19,10 @@ struct naga_engine {
 
 struct naga_batch {
     std::unique_ptr<Batch> impl;
+};
+
+struct naga_server {
+    std::unique_ptr<Server> impl;
 };
 
 namespace {
@@ -50,6 +55,11 @@ Engine& engine_of(naga_engine* e) {
 Batch& batch_of(naga_batch* b) {
     require(b != nullptr && b->impl != nullptr, "null batch handle");
     return *b->impl;
+}
+
+Server& server_of(naga_server* s) {
+    require(s != nullptr && s->impl != nullptr, "null server handle");
+    return *s->impl;
 }
 
 // ---- synthetic uniform grid (node_providers/uniform_grid_provider.cuh) ----
@@ -212,6 +222,28 @@ int naga_batch_set_graph(naga_batch* b, int32_t on) {
     return guarded([&] { batch_of(b).set_graph(on != 0); });
 }
 
+int naga_batch_autotune(naga_batch* b, int32_t trial_steps, int32_t* gather_batch) {
+    return guarded([&] {
+        const int gb = batch_of(b).autotune(trial_steps);
+        if (gather_batch) *gather_batch = gb;
+    });
+}
+
+int naga_engine_state_size(naga_engine* e, int64_t* bytes) {
+    return guarded([&] {
+        require(bytes != nullptr, "null argument");
+        *bytes = engine_of(e).state_bytes();
+    });
+}
+
+int naga_engine_save_state(naga_engine* e, void* buf, int64_t capacity) {
+    return guarded([&] { engine_of(e).save_state(buf, capacity); });
+}
+
+int naga_engine_load_state(naga_engine* e, const void* buf, int64_t bytes) {
+    return guarded([&] { engine_of(e).load_state(buf, bytes); });
+}
+
 int naga_batch_step(naga_batch* b, int32_t nsteps) {
     return guarded([&] { batch_of(b).step(nsteps); });
 }
@@ -220,6 +252,72 @@ int naga_batch_step_timed(naga_batch* b, int32_t nsteps, double* ms) {
     return guarded([&] {
         const double v = batch_of(b).step_timed(nsteps);
         if (ms) *ms = v;
+    });
+}
+
+int naga_server_create(int64_t hbm_budget_bytes, int32_t max_active, int32_t chunk_steps,
+                       int32_t use_graph, naga_server** out) {
+    return guarded([&] {
+        require(out != nullptr, "null output handle");
+        require(hbm_budget_bytes > 0, "budget must be positive");
+        auto* s = new naga_server;
+        try {
+            s->impl = std::make_unique<Server>(size_t(hbm_budget_bytes), max_active, chunk_steps,
+                                               use_graph != 0);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int naga_server_destroy(naga_server* s) {
+    return guarded([&] { delete s; });
+}
+
+int naga_server_submit(naga_server* s, const naga_request* req, int64_t* id) {
+    return guarded([&] {
+        require(req != nullptr && id != nullptr, "null argument");
+        *id = server_of(s).submit(*req);
+    });
+}
+
+int naga_server_status(naga_server* s, int64_t id, int32_t* state) {
+    return guarded([&] {
+        require(state != nullptr, "null argument");
+        const int st = server_of(s).status(id);
+        require(st >= 0, "unknown request id");
+        *state = st;
+    });
+}
+
+int naga_server_result(naga_server* s, int64_t id, int64_t* n_nodes, float* rho, float* u,
+                       double* drho, double* latency_ms) {
+    return guarded([&] {
+        ServedResult r;
+        require(server_of(s).result(id, r), "request is not done (or unknown)");
+        if (!r.ok) fail(kNumericalError, "request failed: " + r.error);
+        if (n_nodes) *n_nodes = r.nodes;
+        if (rho) std::memcpy(rho, r.rho.data(), sizeof(float) * r.rho.size());
+        if (u) std::memcpy(u, r.u.data(), sizeof(float) * r.u.size());
+        if (drho) std::memcpy(drho, r.drho.data(), sizeof(double) * r.drho.size());
+        if (latency_ms) *latency_ms = r.latency_ms;
+    });
+}
+
+int naga_server_release(naga_server* s, int64_t id) {
+    return guarded([&] { server_of(s).release(id); });
+}
+
+int naga_server_drain(naga_server* s) {
+    return guarded([&] { server_of(s).drain(); });
+}
+
+int naga_server_get_stats(naga_server* s, naga_server_stats* out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        *out = server_of(s).stats();
     });
 }
 

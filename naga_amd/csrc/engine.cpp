@@ -410,6 +410,106 @@ void Engine::prepare_batched_step(hipStream_t stream) {
                         src_field_.get(), stream);
 }
 
+EngineState Engine::snapshot() {
+    require_domain();
+    EngineState st;
+    const size_t rows = size_t(N_) * RS_;
+    auto copy = [&](DeviceBuffer<float>& dst, const DeviceBuffer<float>& src, size_t n) {
+        dst.allocate(n);
+        NAGA_HIP(hipMemcpyAsync(dst.get(), src.get(), n * sizeof(float), hipMemcpyDeviceToDevice,
+                                stream_));
+    };
+    copy(st.g, g_, rows);
+    copy(st.E, E_, rows);
+    copy(st.Qp, Qp_, rows);
+    copy(st.rho, rho_, size_t(N_));
+    copy(st.u, u_, size_t(N_) * D_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    st.time = time_;
+    st.frame = frame_;
+    return st;
+}
+
+void Engine::restore(const EngineState& st) {
+    require_domain();
+    const size_t rows = size_t(N_) * RS_;
+    require(st.g.size() == rows && st.rho.size() == size_t(N_), "state of another scene");
+    auto copy = [&](DeviceBuffer<float>& dst, const DeviceBuffer<float>& src) {
+        NAGA_HIP(hipMemcpyAsync(dst.get(), src.get(), src.size() * sizeof(float),
+                                hipMemcpyDeviceToDevice, stream_));
+    };
+    copy(g_, st.g);
+    copy(E_, st.E);
+    copy(Qp_, st.Qp);
+    copy(rho_, st.rho);
+    copy(u_, st.u);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    time_ = st.time;
+    frame_ = st.frame;
+}
+
+namespace {
+struct StateHeader {
+    char magic[8];
+    int32_t Q, D, RS, pad;
+    int64_t N, frame;
+    double time;
+};
+constexpr char kStateMagic[8] = {'N', 'A', 'G', 'A', 'S', 'T', '0', '1'};
+}  // namespace
+
+int64_t Engine::state_bytes() const {
+    require_domain();
+    const int64_t rows = N_ * RS_;
+    return int64_t(sizeof(StateHeader)) + int64_t(sizeof(float)) * (3 * rows + N_ + N_ * D_);
+}
+
+void Engine::save_state(void* buf, int64_t capacity) {
+    require(buf != nullptr && capacity >= state_bytes(), "state buffer too small");
+    StateHeader h{};
+    std::memcpy(h.magic, kStateMagic, 8);
+    h.Q = cfg_.Q;
+    h.D = D_;
+    h.RS = RS_;
+    h.N = N_;
+    h.frame = frame_;
+    h.time = time_;
+    char* p = static_cast<char*>(buf);
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    const size_t rows = size_t(N_) * RS_;
+    for (DeviceBuffer<float>* b : {&g_, &E_, &Qp_}) {
+        b->download(reinterpret_cast<float*>(p), rows, stream_);
+        p += rows * sizeof(float);
+    }
+    rho_.download(reinterpret_cast<float*>(p), size_t(N_), stream_);
+    p += size_t(N_) * sizeof(float);
+    u_.download(reinterpret_cast<float*>(p), size_t(N_) * D_, stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::load_state(const void* buf, int64_t bytes) {
+    require(buf != nullptr && bytes >= int64_t(sizeof(StateHeader)), "state buffer too small");
+    StateHeader h{};
+    std::memcpy(&h, buf, sizeof(h));
+    require(std::memcmp(h.magic, kStateMagic, 8) == 0, "not a naga_amd engine state");
+    require(h.Q == cfg_.Q && h.N == N_ && h.RS == RS_ && h.D == D_,
+            "state was saved from a different scene");
+    require(bytes >= state_bytes(), "state buffer truncated");
+    const char* p = static_cast<const char*>(buf) + sizeof(h);
+    const size_t rows = size_t(N_) * RS_;
+    for (DeviceBuffer<float>* b : {&g_, &E_, &Qp_}) {
+        b->upload(reinterpret_cast<const float*>(p), rows, stream_);
+        p += rows * sizeof(float);
+    }
+    rho_.upload(reinterpret_cast<const float*>(p), size_t(N_), stream_);
+    p += size_t(N_) * sizeof(float);
+    u_.upload(reinterpret_cast<const float*>(p), size_t(N_) * D_, stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    time_ = h.time;
+    frame_ = h.frame;
+}
+
 void Engine::finish_batched_step() {
     time_ += cfg_.dt;
     ++frame_;
@@ -519,6 +619,35 @@ void Batch::set_graph(bool on) {
     dirty_ = true;
 }
 
+int Batch::autotune(int trial_steps) {
+    require(trial_steps > 0, "trial_steps must be positive");
+    if (scenes_.empty()) return 0;
+    std::vector<EngineState> saved;
+    saved.reserve(scenes_.size());
+    for (Engine* e : scenes_) saved.push_back(e->snapshot());
+    double best = 1e300;
+    int best_gb = 8;
+    bool best_graph = true;
+    for (int gb : {8, 16, 32})
+        for (bool graph : {true, false}) {
+            gather_batch_ = gb;
+            use_graph_ = graph;
+            dirty_ = true;
+            step(1);  // builds the plan (and captures the graph)
+            const double ms = step_timed(trial_steps);
+            if (ms < best) {
+                best = ms;
+                best_gb = gb;
+                best_graph = graph;
+            }
+        }
+    gather_batch_ = best_gb;
+    use_graph_ = best_graph;
+    dirty_ = true;
+    for (size_t i = 0; i < scenes_.size(); ++i) scenes_[i]->restore(saved[i]);
+    return best_gb;
+}
+
 // Scenes are grouped by lattice and kernel variant (the four-per-lane D3Q27
 // kernels by stencil width; D2Q9 scenes share the runtime-k kernels unless
 // all have the same specialised width).
@@ -571,6 +700,7 @@ void Batch::rebuild() {
         for (int p = 0; p < kStepKernels; ++p) P.blocks[p] = bstart[size_t(p) * (n + 1) + n];
         P.unions = step_uses_unions(g.Q, g.members[0]->stencil_k(), g.members[0]->nodes());
         P.k_template = (kcommon == 32 || kcommon == 16) ? kcommon : 0;
+        P.gather_batch = gather_batch_;
         require(!P.unions || P.k_template != 0, "four-per-lane scenes need a common k");
         g.table = to_device(descs, stream_);
         g.bstart = to_device(bstart, stream_);
@@ -595,12 +725,22 @@ void Batch::step(int nsteps) {
     if (scenes_.empty() || nsteps == 0) return;
     if (dirty_) rebuild();
     // steps with density sources change kernel arguments per step: no graph
-    const bool graph = use_graph_ && !any_sources_;
+    bool graph = use_graph_ && !any_sources_;
     if (graph && !exec_) {
-        NAGA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        launch_one_step();
-        NAGA_HIP(hipStreamEndCapture(stream_, &graph_));
-        NAGA_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        // Relaxed capture: other threads (a serving loop's setup) may
+        // allocate and launch on their own streams meanwhile.  A capture
+        // that fails anyway falls back to direct launches for this call.
+        bool ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed) == hipSuccess;
+        if (ok) {
+            launch_one_step();
+            ok = hipStreamEndCapture(stream_, &graph_) == hipSuccess && graph_ != nullptr;
+        }
+        ok = ok && hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            drop_graph();
+            graph = false;
+        }
     }
     for (int n = 0; n < nsteps; ++n) {
         if (graph) {

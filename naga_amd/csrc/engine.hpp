@@ -26,6 +26,15 @@ struct SceneConfig {
 // R of the centre, for t_start <= t < t_end.
 class Batch;
 
+// The state of a scene that carries from one step to the next: the
+// distributions (as deviations), the PML history, the macroscopic values
+// held for solution(), the clock.
+struct EngineState {
+    DeviceBuffer<float> g, E, Qp, rho, u;
+    double time = 0.0;
+    int64_t frame = 0;
+};
+
 struct DensitySource {
     double center[3] = {0.0, 0.0, 0.0};
     double radius = 0.0, amplitude = 0.0, frequency = 0.0, phase = 0.0;
@@ -51,6 +60,13 @@ class Engine {
     void get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, float* bw);
     void get_pml(double* E, double* Qp);
     void add_density_source(const DensitySource& src);
+    // device-side copy of the state and its restoration
+    EngineState snapshot();
+    void restore(const EngineState& st);
+    // host serialisation of the state: header + buffers
+    int64_t state_bytes() const;
+    void save_state(void* buf, int64_t capacity);
+    void load_state(const void* buf, int64_t bytes);
     void clear_density_sources();
 
     // Descriptor of this scene for batched launches.
@@ -131,6 +147,11 @@ class Batch {
     void step(int nsteps);
     double step_timed(int nsteps);
     void set_graph(bool on);
+    // Times trial_steps steps for each gather batch (8, 16, 32) with and
+    // without graph replay, keeps the fastest and restores every scene's
+    // state (the trials do not advance the scenes).  Returns the chosen
+    // gather batch, or 0 if the batch is empty.
+    int autotune(int trial_steps);
     int size() const { return int(scenes_.size()); }
     hipStream_t stream() const { return stream_; }
 
@@ -152,6 +173,7 @@ class Batch {
     bool dirty_ = true;
     bool use_graph_ = true;
     bool any_sources_ = false;
+    int gather_batch_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };

@@ -657,6 +657,27 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     if (STAGE != 4) st4(S.acc, own, a);
 }
 
+// sum_s w_s (g_s - g_i) over the staged filter records rp ({index field, w}),
+// rows fetched by fetch(index field); four slots per lane.
+template <int K, int BATCH, class Fetch>
+__device__ __forceinline__ float4 filter_sum(const float2* rp, const float4& si, Fetch fetch) {
+    float sum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s0 = 0; s0 < K; s0 += BATCH) {
+        float2 rec[BATCH];
+        float4 gv[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) rec[u] = rp[s0 + u];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) gv[u] = fetch(__float_as_uint(rec[u].x));
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sum[e] += rec[u].y * (get(gv[u], e) - get(si, e));
+    }
+    return make_float4(sum[0], sum[1], sum[2], sum[3]);
+}
+
 // p4 for D3Q27 with the tile unions: selective filter on fluid nodes,
 // bounce-back on boundary nodes.
 template <int K, int BATCH>
@@ -691,27 +712,17 @@ __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, Batc
     float4 out;
     if (i < G.nfluid) {
         const float2* rp = srec + slot * K;
-        const bool lds = ul && U <= S.ucap;
-        float sum[4] = {0.f, 0.f, 0.f, 0.f};
+        const uint32_t lane_off = uint32_t(q) * 16u;
+        float4 sum;
+        if (!ul)
+            sum = filter_sum<K, BATCH>(rp, si, [&](uint32_t j) { return ld4(s1, j * 128u + lane_off); });
+        else if (U <= S.ucap)
+            sum = filter_sum<K, BATCH>(rp, si, [&](uint32_t j) { return urows[j * kV4Lanes + q]; });
+        else
+            sum = filter_sum<K, BATCH>(rp, si,
+                                       [&](uint32_t j) { return ld4(s1, ul[j] * 128u + lane_off); });
 #pragma unroll
-        for (int s0 = 0; s0 < K; s0 += BATCH) {
-            float2 rec[BATCH];
-            float4 gv[BATCH];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) rec[u] = rp[s0 + u];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                const uint32_t j = __float_as_uint(rec[u].x);
-                gv[u] = lds ? urows[j * kV4Lanes + q]
-                            : ld4(s1, (ul ? ul[j] : j) * 128u + uint32_t(q) * 16u);
-            }
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) sum[e] += rec[u].y * (get(gv[u], e) - get(si, e));
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) set(out, e, get(si, e) + G.filter * sum[e]);
+        for (int e = 0; e < 4; ++e) set(out, e, get(si, e) + G.filter * get(sum, e));
     } else {
         const int64_t b = i - G.nfluid;
         const float n0 = S.normals[b * 3], n1 = S.normals[b * 3 + 1], n2 = S.normals[b * 3 + 2];
@@ -747,10 +758,15 @@ __global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S, Bat
     const uint64_t rows = uint64_t(G.N) * 128u;
     const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
     // records hold node indices, or positions in the node's tile union
-    const uint32_t* ul = S.ulist ? S.ulist + S.uoff[i / kV4Nodes] : nullptr;
-    const float4 r = rate_fetch<K, BATCH>(
-        rp, [&](uint32_t j) { return ld4(gf, (ul ? ul[j] : j) * 128u + uint32_t(q) * 16u); },
-        q);
+    const uint32_t lane_off = uint32_t(q) * 16u;
+    float4 r;
+    if (!S.ulist) {
+        r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, j * 128u + lane_off); }, q);
+    } else {
+        const uint32_t* ul = S.ulist + S.uoff[i / kV4Nodes];
+        r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + lane_off); },
+                                 q);
+    }
     // the divergence of F = c Gf is the negated rate of Gf
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const float4 s = ld4(rsrc(S.A, rows), own);
@@ -835,12 +851,13 @@ struct PassCfg {
     bool unions;   // D3Q27 four-per-lane kernels with tile unions
     int ucap;      // LDS union rows (dynamic shared memory / 128 B)
     BatchArgs B;
+    int gbatch;    // gather batch of the one-per-lane stage kernels (0: default)
 };
 
 template <int Q, int STAGE, int KT>
 void launch_stage_scalar(int64_t nb, const StepGeom& G, const StepBuffers& S, BatchArgs B,
-                         hipStream_t st) {
-    const int b = KT ? stage_batch() : 8;
+                         int gbatch, hipStream_t st) {
+    const int b = gbatch > 0 ? gbatch : (KT ? stage_batch() : 8);
     if (b == 32)
         k_stage<Q, STAGE, KT, 32><<<nb, kBlock, 0, st>>>(G, S, B);
     else if (b == 16)
@@ -859,11 +876,11 @@ void launch_stage_pass(int64_t nb, const StepGeom& G, const StepBuffers& S, cons
         else
             k_stage_u<STAGE, 16, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
     } else if (c.ktmpl == 32) {
-        launch_stage_scalar<Q, STAGE, 32>(nb, G, S, c.B, st);
+        launch_stage_scalar<Q, STAGE, 32>(nb, G, S, c.B, c.gbatch, st);
     } else if (c.ktmpl == 16) {
-        launch_stage_scalar<Q, STAGE, 16>(nb, G, S, c.B, st);
+        launch_stage_scalar<Q, STAGE, 16>(nb, G, S, c.B, c.gbatch, st);
     } else {
-        launch_stage_scalar<Q, STAGE, 0>(nb, G, S, c.B, st);
+        launch_stage_scalar<Q, STAGE, 0>(nb, G, S, c.B, c.gbatch, st);
     }
 }
 
@@ -938,7 +955,7 @@ template <int Q>
 void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S, hipStream_t st) {
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
-    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}};
+    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0};
     launch_pass<Q>(kS1 + stage - 1, nb[kS1 + stage - 1], G, S, CollisionParams{}, c, st);
 }
 
@@ -947,7 +964,7 @@ void launch_step(const StepGeom& G, const StepBuffers& S, const CollisionParams&
                  hipStream_t st, hipEvent_t* ev) {
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
-    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}};
+    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0};
     for (int p = 0; p < kStepKernels; ++p) {
         if (ev) hip_check(hipEventRecord(ev[p], st), "event");
         launch_pass<Q>(p, nb[p], G, S, cp, c, st);
@@ -962,7 +979,8 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t st) {
     for (int p = 0; p < kStepKernels; ++p) {
         const PassCfg c{plan.k_template, plan.unions, plan.ucap,
                         BatchArgs{plan.dev_scenes, plan.dev_bstart + p * (plan.nscenes + 1),
-                                  plan.nscenes}};
+                                  plan.nscenes},
+                        plan.gather_batch};
         launch_pass<Q>(p, plan.blocks[p], G, S, CollisionParams{}, c, st);
     }
 }

@@ -70,3 +70,39 @@ def test_batch_with_density_sources(oracle64):
         e.step_forward(15)
     for a, b in zip(alone, mixed):
         assert rel_l2(b.macroscopic(fresh=True)["drho"], a.macroscopic(fresh=True)["drho"]) < 1e-6
+
+
+def test_state_save_load_roundtrip(oracle64):
+    """save_state/load_state: resuming from a snapshot repeats the run
+    exactly, also in a second engine built from the same nodes."""
+    s = _specs()[0]
+    a = _scene(oracle64, *s)
+    a.step_forward(10)
+    snap = a.save_state()
+    a.step_forward(10)
+    r1 = a.macroscopic(fresh=True)
+    a.load_state(snap)
+    assert abs(a.time() - 10 * 0.4 * s[1]) < 1e-12
+    a.step_forward(10)
+    assert np.array_equal(a.macroscopic(fresh=True)["drho"], r1["drho"])
+    b = _scene(oracle64, *s)
+    b.load_state(snap)
+    b.step_forward(10)
+    assert np.array_equal(b.macroscopic(fresh=True)["drho"], r1["drho"])
+    c = _scene(oracle64, *_specs()[1])
+    with pytest.raises(Exception):
+        c.load_state(snap)
+
+
+def test_autotune_leaves_scenes_unchanged(oracle64):
+    alone = [_scene(oracle64, *s) for s in _specs()]
+    mixed = [_scene(oracle64, *s) for s in _specs()]
+    batch = naga_amd.SceneBatch(mixed)
+    gb = batch.autotune(3)
+    assert gb in (8, 16, 32)
+    batch.step_forward(12)
+    for e in alone:
+        e.step_forward(12)
+    for a, b in zip(alone, mixed):
+        assert a.time() == pytest.approx(b.time(), abs=1e-12)
+        assert rel_l2(b.macroscopic(fresh=True)["drho"], a.macroscopic(fresh=True)["drho"]) < 1e-6
