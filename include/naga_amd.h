@@ -93,6 +93,17 @@ int naga_engine_add_density_source(naga_engine* e, const double* center, double 
                                    double amplitude, double frequency, double phase,
                                    double t_start, double t_end);
 int naga_engine_clear_density_sources(naga_engine* e);
+/* Density probes, the density_observer of
+ * simulation_observers/density_observer.cuh:62-113: each probe point is
+ * interpolated from its 32 nearest nodes with RPIM weights at the nodal
+ * spacing (rbf as for naga_rpim_weights; the reference uses the
+ * multiquadric): value = rho0 + sum_s w_s (rho_s - rho0).  fresh = 0 uses
+ * the density held from the start of the last step (what the observer sees
+ * in update()), fresh = 1 the current state. */
+int naga_engine_add_probes(naga_engine* e, const float* points, int64_t n, int32_t rbf,
+                           int32_t* probe_set);
+int naga_engine_sample_probes(naga_engine* e, int32_t probe_set, int32_t fresh,
+                              double* values);
 /* Snapshot/restore of the scene state that carries between steps
  * (distributions, PML history, held macroscopic values, clock) to a host
  * buffer; replaces simulation_engine::save_state/load_state

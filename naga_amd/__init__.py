@@ -133,6 +133,8 @@ def load() -> C.CDLL:
         "naga_engine_state_size": (C.c_int, [vp, C.POINTER(i64)]),
         "naga_engine_save_state": (C.c_int, [vp, vp, i64]),
         "naga_engine_load_state": (C.c_int, [vp, vp, i64]),
+        "naga_engine_add_probes": (C.c_int, [vp, vp, i64, i32, C.POINTER(i32)]),
+        "naga_engine_sample_probes": (C.c_int, [vp, i32, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -158,7 +160,7 @@ EXPORTED = [
     "naga_server_destroy", "naga_server_submit", "naga_server_status", "naga_server_result",
     "naga_server_release", "naga_server_drain", "naga_server_get_stats",
     "naga_batch_autotune", "naga_engine_state_size", "naga_engine_save_state",
-    "naga_engine_load_state",
+    "naga_engine_load_state", "naga_engine_add_probes", "naga_engine_sample_probes",
 ]
 
 STEP_KERNELS = ["p1_collide", "pl_pml_local", "pm_layer_means", "pf_pml_field",
@@ -435,6 +437,23 @@ class SimulationEngine:
     def reset(self):
         """reset() (detail/simulation_engine.cuh:1377-1386): f = w, time 0."""
         self.set_initial_pulse(np.zeros(self.dimensions), 0.0, 1.0)
+
+    def add_probes(self, points, rbf: int = RBF_MULTIQUADRIC) -> int:
+        """Density probe points (density_observer, density_observer.cuh:
+        62-113); returns the probe set's id."""
+        pts = _f32(points).reshape(-1, self.dimensions)
+        sid = C.c_int32()
+        _check(load().naga_engine_add_probes(self._live(), _ptr(pts), pts.shape[0], int(rbf),
+                                             C.byref(sid)), _ENGINE_PREFIX)
+        self._probe_sizes = getattr(self, "_probe_sizes", {})
+        self._probe_sizes[sid.value] = pts.shape[0]
+        return sid.value
+
+    def sample_probes(self, probe_set: int, fresh: bool = False) -> np.ndarray:
+        out = np.zeros(self._probe_sizes[probe_set])
+        _check(load().naga_engine_sample_probes(self._live(), int(probe_set), int(fresh),
+                                                _ptr(out)), _ENGINE_PREFIX)
+        return out
 
     def save_state(self) -> bytes:
         """Snapshot of the state carried between steps (save_state,
@@ -795,3 +814,24 @@ class SolveServer:
             self.close()
         except Exception:
             pass
+
+
+class DensityObserver:
+    """density_observer<Lattice> (simulation_observers/density_observer.cuh:
+    47-113): records the density at one location at every step, as the
+    reference's update() does at the start of each step."""
+
+    def __init__(self, engine: "SimulationEngine", location, time_multiplier: float = 1.0,
+                 rbf: int = RBF_MULTIQUADRIC):
+        self.engine = engine
+        self.set = engine.add_probes(np.asarray(location, np.float64).reshape(1, -1), rbf)
+        self.time_multiplier = float(time_multiplier)
+        self.density_values, self.time_values = [], []
+
+    def advance(self, nsteps: int = 1):
+        """Step the engine, recording the observed value of each step."""
+        for _ in range(int(nsteps)):
+            t = self.engine.time()
+            self.engine.step_forward(1)
+            self.density_values.append(float(self.engine.sample_probes(self.set)[0]))
+            self.time_values.append(t * self.time_multiplier)

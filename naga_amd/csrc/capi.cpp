@@ -229,6 +229,19 @@ int naga_batch_autotune(naga_batch* b, int32_t trial_steps, int32_t* gather_batc
     });
 }
 
+int naga_engine_add_probes(naga_engine* e, const float* points, int64_t n, int32_t rbf,
+                           int32_t* probe_set) {
+    return guarded([&] {
+        const int id = engine_of(e).add_probes(points, n, rbf);
+        if (probe_set) *probe_set = id;
+    });
+}
+
+int naga_engine_sample_probes(naga_engine* e, int32_t probe_set, int32_t fresh,
+                              double* values) {
+    return guarded([&] { engine_of(e).sample_probes(probe_set, fresh != 0, values); });
+}
+
 int naga_engine_state_size(naga_engine* e, int64_t* bytes) {
     return guarded([&] {
         require(bytes != nullptr, "null argument");

@@ -118,6 +118,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     pts_.allocate(N * D);
     pts_.upload(pts, N * D, st);
     hpts_.assign(pts, pts + N * D);
+    probes_.clear();
     sources_.clear();
     n_snodes_ = 0;
 
@@ -408,6 +409,47 @@ void Engine::prepare_batched_step(hipStream_t stream) {
     }
     launch_source_field(snodes_.get(), soff_.get(), sids_.get(), sprof_.get(), n_snodes_, amps,
                         src_field_.get(), stream);
+}
+
+// density_observer (simulation_observers/density_observer.cuh:62-113):
+// kNN of the probe points among all nodes (k = 32) and RPIM weights at the
+// nodal spacing; a sample interpolates rho - rho0 and adds rho0.
+int Engine::add_probes(const float* points, int64_t n, int rbf) {
+    require_domain();
+    require(points != nullptr && n > 0, "no probe points");
+    require(rbf == 0 || rbf == 1, "rbf must be 0 (multiquadric) or 1 (r^3)");
+    auto ps = std::make_unique<ProbeSet>();
+    ps->n = n;
+    ps->k = int(std::min<int64_t>(32, N_));
+    const int k = ps->k;
+    std::vector<float> d2(size_t(n) * k);
+    std::vector<uint32_t> idx(size_t(n) * k);
+    knn_host(D_, hpts_.data(), N_, points, n, k, d2.data(), idx.data(), stream_);
+    ps->idx = to_device(idx, stream_);
+    DeviceBuffer<float> q(size_t(n) * D_);
+    q.upload(points, size_t(n) * D_, stream_);
+    ps->w.allocate(size_t(n) * k);
+    DeviceBuffer<int> info(n);
+    launch_rpim_weights(D_, pts_.get(), q.get(), n, ps->idx.get(), k, rbf, cfg_.h, ps->w.get(),
+                        nullptr, info.get(), stream_);
+    std::vector<int> hinfo(n);
+    info.download(hinfo.data(), n, stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
+    for (int v : hinfo) require(v == 0, "singular RBF system for a probe point");
+    ps->out.allocate(n);
+    probes_.push_back(std::move(ps));
+    return int(probes_.size()) - 1;
+}
+
+void Engine::sample_probes(int set, bool fresh, double* values) {
+    require_domain();
+    require(set >= 0 && set < int(probes_.size()), "unknown probe set");
+    require(values != nullptr, "null output");
+    ProbeSet& ps = *probes_[set];
+    launch_probe(cfg_.Q, ps.n, ps.k, ps.idx.get(), ps.w.get(), fresh ? nullptr : rho_.get(),
+                 g_.get(), cfg_.rho0, ps.out.get(), stream_);
+    ps.out.download(values, ps.n, stream_);
+    NAGA_HIP(hipStreamSynchronize(stream_));
 }
 
 EngineState Engine::snapshot() {

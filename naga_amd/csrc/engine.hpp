@@ -60,6 +60,10 @@ class Engine {
     void get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, float* bw);
     void get_pml(double* E, double* Qp);
     void add_density_source(const DensitySource& src);
+    // density probes: kNN (k = 32, or fewer nodes) among all nodes and
+    // RPIM weights at spacing h; returns the probe set's index
+    int add_probes(const float* points, int64_t n, int rbf);
+    void sample_probes(int set, bool fresh, double* values);
     // device-side copy of the state and its restoration
     EngineState snapshot();
     void restore(const EngineState& st);
@@ -124,6 +128,15 @@ class Engine {
     DeviceBuffer<uint32_t> ulist_;  // tile unions (four-per-lane D3Q27 kernels)
     DeviceBuffer<int64_t> uoff_;
     int ucap_ = 0;
+
+    struct ProbeSet {
+        int64_t n = 0;
+        int k = 0;
+        DeviceBuffer<uint32_t> idx;
+        DeviceBuffer<float> w;
+        DeviceBuffer<double> out;
+    };
+    std::vector<std::unique_ptr<ProbeSet>> probes_;
 
     std::vector<float> hpts_;  // host copy of the node coordinates
     std::vector<DensitySource> sources_;

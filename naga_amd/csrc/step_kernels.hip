@@ -785,6 +785,30 @@ __global__ __launch_bounds__(kBlock) void k_source_field(const int32_t* nodes,
 }
 
 template <int Q>
+__global__ __launch_bounds__(kBlock) void k_probe(int64_t n, int k, const uint32_t* idx,
+                                                  const float* w, const float* held,
+                                                  const float* g, double rho0, double* out) {
+    using L = Lattice<Q>;
+    const int64_t p = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (p >= n) return;
+    double acc = 0.0;
+    for (int s = 0; s < k; ++s) {
+        const int64_t j = idx[p * k + s];
+        double d;
+        if (held) {
+            d = double(held[j]) - rho0;
+        } else {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < Q; ++a) t += double(g[j * L::RS + a]);
+            d = rho0 * t;
+        }
+        acc += double(w[p * k + s]) * d;
+    }
+    out[p] = rho0 + acc;
+}
+
+template <int Q>
 __global__ __launch_bounds__(kBlock) void k_macro(StepGeom G, const float* g,
                                                   float* rho, double* drho,
                                                   float* u) {
@@ -983,6 +1007,18 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t st) {
                         plan.gather_batch};
         launch_pass<Q>(p, plan.blocks[p], G, S, CollisionParams{}, c, st);
     }
+}
+
+void launch_probe(int Q, int64_t n, int k, const uint32_t* idx, const float* w,
+                  const float* held, const float* g, double rho0, double* out,
+                  hipStream_t st) {
+    if (n == 0) return;
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    if (Q == 27)
+        k_probe<27><<<nb, kBlock, 0, st>>>(n, k, idx, w, held, g, rho0, out);
+    else
+        k_probe<9><<<nb, kBlock, 0, st>>>(n, k, idx, w, held, g, rho0, out);
+    check_launch("probe");
 }
 
 void launch_source_field(const int32_t* nodes, const int32_t* off, const int32_t* ids,

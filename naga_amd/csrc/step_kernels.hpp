@@ -147,6 +147,13 @@ template <int Q>
 void launch_rk4_stage(int stage, const StepGeom& geo, const StepBuffers& buf,
                       hipStream_t stream);
 
+// Density probes (density_observer.cuh:62-113): out[p] = rho0 +
+// sum_s w[p][s] (rho(idx[p][s]) - rho0), with rho the held FP32 density
+// (held != null) or computed fresh from the deviation rows g.
+void launch_probe(int Q, int64_t n, int k, const uint32_t* idx, const float* w,
+                  const float* held, const float* g, double rho0, double* out,
+                  hipStream_t stream);
+
 // Fresh macroscopic values from the current state: rho (FP32, reference
 // units), drho = rho - rho0 in FP64 without cancellation, u (FP32).
 template <int Q>

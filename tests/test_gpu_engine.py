@@ -191,3 +191,35 @@ def test_mirror_symmetry_1m_nodes():
     e_rho, e_u = _mirror_check(nodes, mirror, 27, 32, 0.01, 20)
     print(f"mirror 1M drho={e_rho:.2e} u={e_u:.2e}")
     assert e_rho < 1e-4 and e_u < 1e-4
+
+
+@pytest.mark.parametrize("Q", [9, 27])
+def test_density_probes(oracle64, Q):
+    """Probes interpolate rho - rho0 with RPIM weights over the 32 nearest
+    nodes (density_observer.cuh:62-113); checked against the oracle's kNN
+    and RPIM weights applied to the engine's own density."""
+    import naga_amd
+    D = 2 if Q == 9 else 3
+    h = 0.02 if D == 2 else 0.1
+    k = 16 if D == 2 else 32
+    nodes = grid_nodes(oracle64, D, h, layer=3 * h)
+    gpu = gpu_engine(nodes, Q, k, h)
+    gpu.set_initial_pulse([0.45] + [0.5] * (D - 1), 1e-3, 0.1)
+    gpu.step_forward(8)
+    rng = np.random.default_rng(4)
+    probes = rng.uniform(0.1, 0.9, (7, D)).astype(np.float32)
+    sid = gpu.add_probes(probes, naga_amd.RBF_MULTIQUADRIC)
+    pts = nodes["points"]
+    _, idx = oracle64.knn(pts, probes, 32)
+    w = oracle64.rpim_weights(pts, idx, probes, 0, h)
+    fresh = gpu.macroscopic(fresh=True)["drho"]
+    want = 1.0 + (w * fresh[idx]).sum(1)
+    got = gpu.sample_probes(sid, fresh=True)
+    scale = np.abs(fresh).max()
+    assert np.abs(got - want).max() < 1e-5 * scale
+    held = gpu.macroscopic(fresh=False)["rho"].astype(np.float64)
+    want_h = 1.0 + (w * (held[idx] - 1.0)).sum(1)
+    assert np.abs(gpu.sample_probes(sid) - want_h).max() < 1e-5 * scale + 1e-7
+    obs = naga_amd.DensityObserver(gpu, probes[0])
+    obs.advance(3)
+    assert len(obs.density_values) == 3 and obs.time_values[1] > obs.time_values[0]
