@@ -155,7 +155,7 @@ def batched_workload(steps: int, n_scenes: int = 256, seed: int = 0x4E414741) ->
 
 def served_workload(n_requests: int = 48, rate: float = 50.0, seed: int = 0x4E414742) -> dict:
     """Poisson arrivals at `rate` requests/s, D2Q9/D3Q27 50/50, node counts
-    log-uniform in [5k, 200k], steps uniform in [100, 500], random pulse and
+    log-uniform in [5k, 200k], steps uniform in [100, 2000], random pulse and
     obstacles; served with continuous batching, then the same requests
     solved one at a time with their own launches (setup included in both)."""
     import naga_amd
@@ -172,7 +172,7 @@ def served_workload(n_requests: int = 48, rate: float = 50.0, seed: int = 0x4E41
         else:
             obst = [([0.5, 0.5, 0.5], 0.15)]
         reqs.append(naga_amd.make_request(
-            lat, h, int(rng.integers(100, 501)), obst, seed=int(rng.integers(1, 1 << 62)),
+            lat, h, int(rng.integers(100, 2001)), obst, seed=int(rng.integers(1, 1 << 62)),
             pulse_center=[float(v) for v in rng.uniform(0.2, 0.35, D)], pulse_width=0.05))
     gaps = rng.exponential(1.0 / rate, n_requests)
     srv = naga_amd.SolveServer(hbm_budget_bytes=96 << 30, max_active=64, chunk_steps=50)
@@ -198,7 +198,7 @@ def served_workload(n_requests: int = 48, rate: float = 50.0, seed: int = 0x4E41
             "speedup": st["node_updates_per_s"] / (work / plain_s),
             "trace_s": time.perf_counter() - t0 - plain_s,
             "workload": "Poisson 50 req/s, D2Q9/D3Q27 50/50, N log-uniform 5k-200k, "
-                        "steps 100-500, obstacles (2D: 0-5 circles; 3D: sphere r 0.15)"}
+                        "steps 100-2000, obstacles (2D: 0-5 circles; 3D: sphere r 0.15)"}
 
 
 def main() -> int:

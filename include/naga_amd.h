@@ -168,8 +168,8 @@ int naga_batch_step_timed(naga_batch* b, int32_t nsteps, double* ms);
 /* ---- serving loop: acoustic-pulse solves with continuous batching.  No
  *      counterpart in the reference.  Requests are admitted while their
  *      estimated device footprint fits hbm_budget_bytes and fewer than
- *      max_active are resident; a setup thread builds admitted scenes (host
- *      node generation, GPU setup on the scene's own stream) while a
+ *      max_active are resident; setup_threads threads build admitted scenes
+ *      (host node generation, GPU setup on each scene's own stream) while a
  *      stepping thread advances all resident scenes together in chunks of
  *      at most chunk_steps steps; finished scenes leave at once and free
  *      their memory.  Scenes are the unit box minus obstacles
@@ -201,7 +201,7 @@ typedef struct naga_server_stats {
 } naga_server_stats;
 
 int naga_server_create(int64_t hbm_budget_bytes, int32_t max_active, int32_t chunk_steps,
-                       int32_t use_graph, naga_server** out);
+                       int32_t use_graph, int32_t setup_threads, naga_server** out);
 int naga_server_destroy(naga_server* s);  /* waits for resident work */
 int naga_server_submit(naga_server* s, const naga_request* req, int64_t* id);
 /* *state: 0 queued or running, 1 done (see naga_server_result) */

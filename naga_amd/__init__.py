@@ -121,7 +121,7 @@ def load() -> C.CDLL:
         "naga_batch_step_timed": (C.c_int, [vp, i32, vp]),
         "naga_obstacle_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, i32, vp, f64, f64, i64,
                                          vp, vp, vp, vp]),
-        "naga_server_create": (C.c_int, [i64, i32, i32, i32, C.POINTER(vp)]),
+        "naga_server_create": (C.c_int, [i64, i32, i32, i32, i32, C.POINTER(vp)]),
         "naga_server_destroy": (C.c_int, [vp]),
         "naga_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(i64)]),
         "naga_server_status": (C.c_int, [vp, i64, C.POINTER(i32)]),
@@ -761,10 +761,11 @@ class SolveServer:
     """Serving loop with continuous batching (naga_server_*)."""
 
     def __init__(self, hbm_budget_bytes: int, max_active: int = 64, chunk_steps: int = 50,
-                 graph: bool = True):
+                 graph: bool = True, setup_threads: int = 4):
         h = C.c_void_p()
         _check(load().naga_server_create(int(hbm_budget_bytes), int(max_active),
-                                         int(chunk_steps), int(bool(graph)), C.byref(h)),
+                                         int(chunk_steps), int(bool(graph)), int(setup_threads),
+                                         C.byref(h)),
                _SERVER_PREFIX)
         self._h = h
         self._dims = {}

@@ -269,14 +269,14 @@ int naga_batch_step_timed(naga_batch* b, int32_t nsteps, double* ms) {
 }
 
 int naga_server_create(int64_t hbm_budget_bytes, int32_t max_active, int32_t chunk_steps,
-                       int32_t use_graph, naga_server** out) {
+                       int32_t use_graph, int32_t setup_threads, naga_server** out) {
     return guarded([&] {
         require(out != nullptr, "null output handle");
         require(hbm_budget_bytes > 0, "budget must be positive");
         auto* s = new naga_server;
         try {
             s->impl = std::make_unique<Server>(size_t(hbm_budget_bytes), max_active, chunk_steps,
-                                               use_graph != 0);
+                                               use_graph != 0, setup_threads);
         } catch (...) {
             delete s;
             throw;

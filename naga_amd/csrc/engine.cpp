@@ -743,6 +743,9 @@ void Batch::rebuild() {
         P.unions = step_uses_unions(g.Q, g.members[0]->stencil_k(), g.members[0]->nodes());
         P.k_template = (kcommon == 32 || kcommon == 16) ? kcommon : 0;
         P.gather_batch = gather_batch_;
+        P.tiles = descs[0].S.ulist != nullptr;
+        for (const SceneDesc& d : descs)
+            require((d.S.ulist != nullptr) == P.tiles, "scenes with and without tile unions");
         require(!P.unions || P.k_template != 0, "four-per-lane scenes need a common k");
         g.table = to_device(descs, stream_);
         g.bstart = to_device(bstart, stream_);

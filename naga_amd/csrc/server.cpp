@@ -40,15 +40,20 @@ void make_nodes(const naga_request& r, Nodes& out) {
                              out.nrm.data(), out.sig.data(), out.counts));
 }
 
-int64_t node_count(const naga_request& r) {
+// Upper bound of a request's node count without generating the nodes: the
+// full grid plus every obstacle's surface nodes.
+int64_t node_bound(const naga_request& r) {
     const int D = r.lattice == 9 ? 2 : 3;
     const double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {1.0, 1.0, 1.0};
-    int64_t c[3] = {0, 0, 0};
-    const int st = naga_obstacle_grid(D, lo, hi, r.h, r.jitter, r.seed, r.n_obstacles,
-                                      r.obstacles, r.layer_thickness, r.peak_absorption, 0,
-                                      nullptr, nullptr, nullptr, c);
-    if (st != NAGA_OK) fail(Status(st), naga_last_error());
-    return c[0] + c[1] + c[2];
+    const int64_t grid = naga_uniform_grid_size(D, lo, hi, r.h);
+    require(grid > 0, "bad grid spacing");
+    double surf = 0.0;
+    for (int j = 0; j < r.n_obstacles; ++j) {
+        const double R = r.obstacles[j * (D + 1) + D];
+        surf += D == 2 ? std::max(8.0, std::ceil(2.0 * M_PI * R / r.h))
+                       : std::max(16.0, std::ceil(4.0 * M_PI * R * R / (r.h * r.h)));
+    }
+    return grid + int64_t(surf);
 }
 
 int stencil_k(const naga_request& r) { return r.k > 0 ? r.k : (r.lattice == 9 ? 16 : 32); }
@@ -71,11 +76,14 @@ size_t Server::footprint(int Q, int k, int64_t n) {
     return size_t(1.25 * per * double(n)) + (size_t(8) << 20);
 }
 
-Server::Server(size_t hbm_budget, int max_active, int chunk_steps, bool graph)
+Server::Server(size_t hbm_budget, int max_active, int chunk_steps, bool graph,
+               int setup_threads)
     : budget_(hbm_budget), max_active_(max_active), chunk_(chunk_steps), graph_(graph),
       t0_(std::chrono::steady_clock::now()) {
-    require(hbm_budget > 0 && max_active > 0 && chunk_steps > 0, "bad server settings");
-    setup_thread_ = std::thread([this] { setup_loop(); });
+    require(hbm_budget > 0 && max_active > 0 && chunk_steps > 0 && setup_threads > 0,
+            "bad server settings");
+    for (int t = 0; t < setup_threads; ++t)
+        setup_threads_.emplace_back([this] { setup_loop(); });
     step_thread_ = std::thread([this] { step_loop(); });
 }
 
@@ -85,7 +93,8 @@ Server::~Server() {
         stop_ = true;
     }
     cv_.notify_all();
-    if (setup_thread_.joinable()) setup_thread_.join();
+    for (std::thread& t : setup_threads_)
+        if (t.joinable()) t.join();
     if (step_thread_.joinable()) step_thread_.join();
 }
 
@@ -154,8 +163,9 @@ naga_server_stats Server::stats() {
     return s;
 }
 
-// Admission and scene setup.  Requests are taken in arrival order; the head
-// waits until its footprint fits the budget and a slot is free.
+// Admission and scene setup (one loop per setup thread).  Requests are
+// taken in arrival order; the head waits until its footprint fits the
+// budget and a slot is free.
 void Server::setup_loop() {
     for (;;) {
         Pending p;
@@ -168,7 +178,7 @@ void Server::setup_loop() {
                 if (!queue_.empty()) {
                     const naga_request& r = queue_.front().req;
                     try {
-                        n = node_count(r);
+                        n = node_bound(r);
                         bytes = footprint(r.lattice, stencil_k(r), n);
                     } catch (const std::exception& e) {
                         Pending bad = queue_.front();

@@ -1,7 +1,7 @@
 // Serving loop for acoustic-pulse solves with continuous batching.
 //
 // Requests are admitted while their estimated device footprint fits the
-// HBM budget and a slot is free.  A setup thread builds each admitted
+// HBM budget and a slot is free.  Setup threads build each admitted
 // request's scene (node generation on the host, kNN and weights on the
 // GPU on the scene's own stream) while a stepping thread advances every
 // resident scene together through one Batch; scenes join at the next
@@ -38,7 +38,7 @@ struct ServedResult {
 
 class Server {
   public:
-    Server(size_t hbm_budget, int max_active, int chunk_steps, bool graph);
+    Server(size_t hbm_budget, int max_active, int chunk_steps, bool graph, int setup_threads);
     ~Server();
     Server(const Server&) = delete;
     Server& operator=(const Server&) = delete;
@@ -91,7 +91,8 @@ class Server {
     double t_first_submit_ = -1.0, t_last_done_ = 0.0;
     std::vector<double> latencies_;
     std::chrono::steady_clock::time_point t0_;
-    std::thread setup_thread_, step_thread_;
+    std::vector<std::thread> setup_threads_;
+    std::thread step_thread_;
 };
 
 }  // namespace naga_amd

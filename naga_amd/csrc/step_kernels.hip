@@ -583,7 +583,7 @@ __device__ __forceinline__ void stage_union(float4* urows, const uint32_t* ul, i
 
 // RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
 // union staged in LDS (unions larger than ucap rows read through the list).
-template <int STAGE, int K, int BATCH>
+template <int STAGE, int K, int BATCH, bool TU>
 __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, BatchArgs B) {
     __shared__ float4 srec[kV4Nodes * K];
     extern __shared__ float4 urows[];
@@ -603,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
     int U = 0;
-    const uint32_t* ul = S.ulist;
+    const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
     if (cnt > 0) {
         const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
         for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
@@ -680,7 +680,7 @@ __device__ __forceinline__ float4 filter_sum(const float2* rp, const float4& si,
 
 // p4 for D3Q27 with the tile unions: selective filter on fluid nodes,
 // bounce-back on boundary nodes.
-template <int K, int BATCH>
+template <int K, int BATCH, bool TU>
 __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, BatchArgs B) {
     __shared__ float2 srec[kV4Nodes * K];
     extern __shared__ float4 urows[];
@@ -696,7 +696,7 @@ __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, Batc
     const int64_t nf = G.nfluid - node0;
     const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
     int U = 0;
-    const uint32_t* ul = S.ulist;
+    const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
     if (cnt > 0) {
         const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
         for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
@@ -740,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, Batc
 
 // p2 for D3Q27 with four distributions per lane (PML divergence over the
 // listed nodes, added to s1).
-template <int K, int BATCH>
+template <int K, int BATCH, bool TU>
 __global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S, BatchArgs B) {
     __shared__ float4 srec[kV4Nodes * K];
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S, Bat
     // records hold node indices, or positions in the node's tile union
     const uint32_t lane_off = uint32_t(q) * 16u;
     float4 r;
-    if (!S.ulist) {
+    if (!TU) {
         r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, j * 128u + lane_off); }, q);
     } else {
         const uint32_t* ul = S.ulist + S.uoff[i / kV4Nodes];
@@ -876,6 +876,7 @@ struct PassCfg {
     int ucap;      // LDS union rows (dynamic shared memory / 128 B)
     BatchArgs B;
     int gbatch;    // gather batch of the one-per-lane stage kernels (0: default)
+    bool tiles;    // tile unions built (slab index fields are union positions)
 };
 
 template <int Q, int STAGE, int KT>
@@ -896,9 +897,11 @@ void launch_stage_pass(int64_t nb, const StepGeom& G, const StepBuffers& S, cons
     if (Q == 27 && c.unions) {
         const size_t lds = size_t(c.ucap) * 128u;
         if (c.ktmpl == 32)
-            k_stage_u<STAGE, 32, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
+            c.tiles ? k_stage_u<STAGE, 32, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B)
+                    : k_stage_u<STAGE, 32, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
         else
-            k_stage_u<STAGE, 16, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
+            c.tiles ? k_stage_u<STAGE, 16, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B)
+                    : k_stage_u<STAGE, 16, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
     } else if (c.ktmpl == 32) {
         launch_stage_scalar<Q, STAGE, 32>(nb, G, S, c.B, c.gbatch, st);
     } else if (c.ktmpl == 16) {
@@ -919,8 +922,10 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         case kPf: k_pml_field<Q><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
         case kP2:
             if (Q == 27 && c.unions) {
-                if (c.ktmpl == 32) k_p2_v4<32, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);
-                else k_p2_v4<16, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);
+                if (c.ktmpl == 32 && c.tiles) k_p2_v4<32, 8, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
+                else if (c.ktmpl == 32) k_p2_v4<32, 8, false><<<nb, kBlock, 0, st>>>(G, S, c.B);
+                else if (c.tiles) k_p2_v4<16, 8, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
+                else k_p2_v4<16, 8, false><<<nb, kBlock, 0, st>>>(G, S, c.B);
             } else if (c.ktmpl == 32) {
                 k_p2<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
             } else if (c.ktmpl == 16) {
@@ -933,8 +938,10 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         case kP4:
             if (Q == 27 && c.unions) {
                 const size_t lds = size_t(c.ucap) * 128u;
-                if (c.ktmpl == 32) k_p4_u<32, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
-                else k_p4_u<16, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);
+                if (c.ktmpl == 32 && c.tiles) k_p4_u<32, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B);
+                else if (c.ktmpl == 32) k_p4_u<32, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
+                else if (c.tiles) k_p4_u<16, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B);
+                else k_p4_u<16, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
             } else if (c.ktmpl == 32) {
                 k_p4<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
             } else if (c.ktmpl == 16) {
@@ -979,7 +986,8 @@ template <int Q>
 void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S, hipStream_t st) {
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
-    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0};
+    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0,
+                    S.ulist != nullptr};
     launch_pass<Q>(kS1 + stage - 1, nb[kS1 + stage - 1], G, S, CollisionParams{}, c, st);
 }
 
@@ -988,7 +996,8 @@ void launch_step(const StepGeom& G, const StepBuffers& S, const CollisionParams&
                  hipStream_t st, hipEvent_t* ev) {
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
-    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0};
+    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0,
+                    S.ulist != nullptr};
     for (int p = 0; p < kStepKernels; ++p) {
         if (ev) hip_check(hipEventRecord(ev[p], st), "event");
         launch_pass<Q>(p, nb[p], G, S, cp, c, st);
@@ -1004,7 +1013,7 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t st) {
         const PassCfg c{plan.k_template, plan.unions, plan.ucap,
                         BatchArgs{plan.dev_scenes, plan.dev_bstart + p * (plan.nscenes + 1),
                                   plan.nscenes},
-                        plan.gather_batch};
+                        plan.gather_batch, plan.tiles};
         launch_pass<Q>(p, plan.blocks[p], G, S, CollisionParams{}, c, st);
     }
 }

@@ -130,6 +130,7 @@ struct BatchPlan {
     bool unions = false;                // D3Q27 four-per-lane kernels
     int ucap = 0;                       // max LDS union rows over the scenes
     int gather_batch = 0;               // neighbours gathered per unrolled step (0: default)
+    bool tiles = false;                 // tile unions built for every scene
 };
 // Block counts of each pass for one scene (the same counts launch_step uses).
 void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* blocks);
