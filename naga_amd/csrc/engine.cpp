@@ -278,8 +278,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     rho_.allocate(N);
     u_.allocate(N * D);
     ru_.allocate(N);
-    const int64_t per = kPmlItemsPerBlock(Q);
-    nparts_ = int(std::max<int64_t>((n_rq_ + per - 1) / per, 1));
+    nparts_ = int(std::max<int64_t>(pml_parts(Q, k, N, n_rq_), 1));
     partial_.allocate(size_t(nparts_) * 2 * Q);
     partial_.zero(st);
     means_.allocate(2 * Q);

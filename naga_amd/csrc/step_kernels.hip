@@ -657,6 +657,53 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     if (STAGE != 4) st4(S.acc, own, a);
 }
 
+// RK4 stage, D3Q27, four per lane, with each node's stencil records read
+// straight from HBM by its lanes (no LDS staging, no block barrier) and the
+// neighbour rows gathered directly.
+template <int STAGE, int K, int BATCH>
+__global__ __launch_bounds__(kBlock) void k_stage_d(StepGeom G, StepBuffers S, BatchArgs B) {
+    const BlockCtx bc = locate(B, G, S);
+    const int64_t node0 = xcd_tile(bc.b, bc.nb) * kV4Nodes;
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const int64_t i = node0 + slot;
+    if (i >= G.N) return;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
+    const float4 g0 = ld4(rsrc(S.g, rows), own);
+    if (i >= G.nfluid) {  // boundary rows: constant through the stages
+        if (STAGE == 1) {
+            st4(S.A, own, g0);
+            st4(S.B, own, g0);
+        }
+        return;
+    }
+    const float* gin_p = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
+    float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
+    const float4 acc0 = STAGE != 1 ? ld4(rsrc(S.acc, rows), own) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
+    const float4* rp = reinterpret_cast<const float4*>(S.slab) + i * K;
+    const uint32_t lane_off = uint32_t(q) * 16u;
+    const float4 r =
+        rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gin, j * 128u + lane_off); }, q);
+    const float dt = G.dt;
+    if (STAGE == 1) {
+        st4(gout, own, make_float4(g0.x + (dt * 0.5f) * r.x, g0.y + (dt * 0.5f) * r.y,
+                                   g0.z + (dt * 0.5f) * r.z, g0.w + (dt * 0.5f) * r.w));
+        st4(S.acc, own, make_float4(r.x * (1.0f / 6.0f), r.y * (1.0f / 6.0f),
+                                    r.z * (1.0f / 6.0f), r.w * (1.0f / 6.0f)));
+    } else if (STAGE == 2 || STAGE == 3) {
+        const float h = STAGE == 2 ? dt * 0.5f : dt;
+        st4(gout, own, make_float4(g0.x + h * r.x, g0.y + h * r.y, g0.z + h * r.z, g0.w + h * r.w));
+        st4(S.acc, own, make_float4(acc0.x + r.x * (2.0f / 6.0f), acc0.y + r.y * (2.0f / 6.0f),
+                                    acc0.z + r.z * (2.0f / 6.0f), acc0.w + r.w * (2.0f / 6.0f)));
+    } else {
+        st4(gout, own, make_float4(g0.x + dt * (acc0.x + r.x * (1.0f / 6.0f)),
+                                   g0.y + dt * (acc0.y + r.y * (1.0f / 6.0f)),
+                                   g0.z + dt * (acc0.z + r.z * (1.0f / 6.0f)),
+                                   g0.w + dt * (acc0.w + r.w * (1.0f / 6.0f))));
+    }
+}
+
 // sum_s w_s (g_s - g_i) over the staged filter records rp ({index field, w}),
 // rows fetched by fetch(index field); four slots per lane.
 template <int K, int BATCH, class Fetch>
@@ -771,6 +818,122 @@ __global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S, Bat
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const float4 s = ld4(rsrc(S.A, rows), own);
     st4(S.A, own, make_float4(s.x - r.x, s.y - r.y, s.z - r.z, s.w - r.w));
+}
+
+// PML local terms for D3Q27, four distributions per lane (same arithmetic
+// as k_pml_local): 32 node slots per round, kPmlRounds rounds per block,
+// one row of partial layer sums per block.
+__global__ __launch_bounds__(kBlock) void k_pml_local_v4(StepGeom G, StepBuffers S, BatchArgs B) {
+    using L = Lattice<27>;
+    __shared__ double red[kV4Nodes][2][32];
+    const BlockCtx bc = locate(B, G, S);
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const double cs2 = kCsLat * kCsLat;
+    const double hdt = double(G.dt) / 2.0;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const __amdgpu_buffer_rsrc_t rE = rsrc(S.E, rows), rQp = rsrc(S.Qp, rows),
+                                 rA = rsrc(S.A, rows);
+    double s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t base = bc.b * (kV4Nodes * kPmlRounds);
+    for (int rd = 0; rd < kPmlRounds; ++rd) {
+        const int64_t item = base + rd * kV4Nodes + slot;
+        if (item >= S.n_rq) continue;
+        const int64_t i = S.rq_list[item];
+        const float4 ru = S.ru[i];
+        const double rt = double(ru.x), rho = 1.0 + rt;
+        const double u0 = ru.y, u1 = ru.z, u2 = ru.w;
+        const double uu = u2 * u2 + (u1 * u1 + u0 * u0);
+        const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
+        const float4 e4 = ld4(rE, own), p4 = ld4(rQp, own);
+        const bool layer = i >= G.nbulk && i < G.nfluid;
+        const double sig = layer ? double(S.sigma[i - G.nbulk]) : 0.0;
+        float4 a4 = layer ? ld4(rA, own) : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 eo, po, qo;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int a = 4 * q + e;
+            if (a >= 27) {  // padding slots stay zero
+                set(eo, e, 0.0f);
+                set(po, e, 0.0f);
+                set(qo, e, 0.0f);
+                continue;
+            }
+            const double uc = u0 * double(L::c(a, 0)) + u1 * double(L::c(a, 1)) +
+                              u2 * double(L::c(a, 2));
+            const double fte = L::w(a) * (rt + rho * (uc / cs2 + (uc * uc) / (2.0 * cs2 * cs2) - uu));
+            const double q1 = (fte + double(get(e4, e))) * hdt;
+            const double q2 = (q1 + double(get(p4, e))) * hdt;
+            set(eo, e, float(fte));
+            set(po, e, float(q1));
+            set(qo, e, float(q2));
+            if (layer) {
+                set(a4, e, float(double(get(a4, e)) - sig * (3.0 * fte + 3.0 * sig * q1 + sig * sig * q2)));
+                s1[e] += q1;
+                s2[e] += q2;
+            }
+        }
+        st4(S.E, own, eo);
+        st4(S.Qp, own, po);
+        st4(S.Q2v, own, qo);
+        if (layer) st4(S.A, own, a4);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        red[slot][0][4 * q + e] = s1[e];
+        red[slot][1][4 * q + e] = s2[e];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * 27) {
+        const int which = threadIdx.x / 27, aa = threadIdx.x % 27;
+        double t = 0.0;
+        for (int sl = 0; sl < kV4Nodes; ++sl) t += red[sl][which][aa];
+        S.partial[bc.b * 2 * 27 + threadIdx.x] = t;
+    }
+}
+
+// Boundary interpolation for D3Q27, four distributions per lane (same
+// arithmetic as k_binterp), records read directly by the node's lanes.
+template <bool ACCUMULATE>
+__global__ __launch_bounds__(kBlock) void k_binterp_v4(StepGeom G, StepBuffers S, BatchArgs B) {
+    const BlockCtx bc = locate(B, G, S);
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const int64_t nbnd = G.N - G.nfluid;
+    const int64_t m = xcd_tile(bc.b, bc.nb) * kV4Nodes + slot;
+    if (m >= nbnd) return;
+    float* field = ACCUMULATE ? S.A : S.g;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const __amdgpu_buffer_rsrc_t f = rsrc(field, rows), cl = rsrc(S.coll, rows);
+    const float2* rp = reinterpret_cast<const float2*>(S.bslab) + m * G.kb;
+    const uint32_t lane_off = uint32_t(q) * 16u;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    constexpr int BT = 8;
+    for (int s0 = 0; s0 < G.kb; s0 += BT) {
+        float2 rec[BT];
+        float4 v[BT];
+#pragma unroll
+        for (int u = 0; u < BT; ++u)
+            rec[u] = s0 + u < G.kb ? rp[s0 + u] : make_float2(__uint_as_float(0u), 0.0f);
+#pragma unroll
+        for (int u = 0; u < BT; ++u) {
+            const uint32_t off = __float_as_uint(rec[u].x) * 128u + lane_off;
+            v[u] = ld4(f, off);
+            if (ACCUMULATE) {
+                const float4 c = ld4(cl, off);
+                v[u] = make_float4(v[u].x - c.x, v[u].y - c.y, v[u].z - c.z, v[u].w - c.w);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BT; ++u)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[e] += rec[u].y * get(v[u], e);
+    }
+    const uint32_t own = uint32_t(G.nfluid + m) * 128u + lane_off;
+    float4 out = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    if (ACCUMULATE) {
+        const float4 b = ld4(f, own);
+        out = make_float4(b.x + out.x, b.y + out.y, b.z + out.z, b.w + out.w);
+    }
+    st4(field, own, out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_source_field(const int32_t* nodes,
@@ -891,10 +1054,25 @@ void launch_stage_scalar(int64_t nb, const StepGeom& G, const StepBuffers& S, Ba
         k_stage<Q, STAGE, KT, 8><<<nb, kBlock, 0, st>>>(G, S, B);
 }
 
+// NAGA_REC_DIRECT=1: D3Q27 stages read stencil records straight from HBM
+// (k_stage_d) instead of staging them in LDS.
+bool records_direct() {
+    static const bool on = [] {
+        const char* e = std::getenv("NAGA_REC_DIRECT");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 template <int Q, int STAGE>
 void launch_stage_pass(int64_t nb, const StepGeom& G, const StepBuffers& S, const PassCfg& c,
                        hipStream_t st) {
-    if (Q == 27 && c.unions) {
+    if (Q == 27 && c.unions && !c.tiles && records_direct()) {
+        if (c.ktmpl == 32)
+            k_stage_d<STAGE, 32, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);
+        else
+            k_stage_d<STAGE, 16, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);
+    } else if (Q == 27 && c.unions) {
         const size_t lds = size_t(c.ucap) * 128u;
         if (c.ktmpl == 32)
             c.tiles ? k_stage_u<STAGE, 32, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B)
@@ -917,7 +1095,10 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
     if (nb <= 0) return;
     switch (pass) {
         case kP1: k_p1<Q><<<nb, kBlock, 0, st>>>(G, S, cp, c.B); break;
-        case kPl: k_pml_local<Q><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+        case kPl:
+            if (Q == 27 && c.unions) k_pml_local_v4<<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else k_pml_local<Q><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            break;
         case kPm: k_means<Q><<<nb, 256, 0, st>>>(G, S, c.B); break;
         case kPf: k_pml_field<Q><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
         case kP2:
@@ -934,7 +1115,10 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
                 k_p2<Q, 0><<<nb, kBlock, 0, st>>>(G, S, c.B);
             }
             break;
-        case kP3: k_binterp<Q, true><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+        case kP3:
+            if (Q == 27 && c.unions) k_binterp_v4<true><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else k_binterp<Q, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            break;
         case kP4:
             if (Q == 27 && c.unions) {
                 const size_t lds = size_t(c.ucap) * 128u;
@@ -954,7 +1138,10 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         case kS2: launch_stage_pass<Q, 2>(nb, G, S, c, st); break;
         case kS3: launch_stage_pass<Q, 3>(nb, G, S, c, st); break;
         case kS4: launch_stage_pass<Q, 4>(nb, G, S, c, st); break;
-        default: k_binterp<Q, false><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+        default:
+            if (Q == 27 && c.unions) k_binterp_v4<false><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else k_binterp<Q, false><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            break;
     }
     check_launch("step pass");
 }
@@ -962,6 +1149,14 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
 int ktemplate(int k) { return k == 32 ? 32 : k == 16 ? 16 : 0; }
 
 }  // namespace
+
+int64_t pml_parts(int Q, int k, int64_t N, int64_t n_rq) {
+    const int64_t per = step_uses_unions(Q, k, N)
+                            ? int64_t(kV4Nodes) * kPmlRounds
+                            : int64_t(kBlock / 64) * (Q == 27 ? Lattice<27>::NPW : Lattice<9>::NPW) *
+                                  kPmlRounds;
+    return (n_rq + per - 1) / per;
+}
 
 void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* nb) {
     const bool un = step_uses_unions(Q, G.k, G.N);
@@ -975,7 +1170,7 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
     nb[kPm] = pml ? 2 * Q : 0;
     nb[kPf] = pml ? lanes(G.nlayer) : 0;
     nb[kP2] = (pml && S.n_pml > 0) ? (un ? tiles(S.n_pml) : lanes(S.n_pml)) : 0;
-    nb[kP3] = nbnd > 0 ? lanes(nbnd) : 0;
+    nb[kP3] = nbnd > 0 ? (un ? tiles(nbnd) : lanes(nbnd)) : 0;
     nb[kP4] = un ? tiles(G.N) : lanes(G.N);
     nb[kS1] = un ? tiles(G.N) : lanes(G.N);
     nb[kS2] = nb[kS3] = nb[kS4] = un ? tiles(G.nfluid) : lanes(G.nfluid);

@@ -110,7 +110,8 @@ void launch_source_field(const int32_t* nodes, const int32_t* off, const int32_t
 // events[t] is recorded before kernel t and events[kStepKernels] after the
 // last (kernels with no work still get their pair of marks).
 constexpr int kStepKernels = 12;
-constexpr int kPmlItemsPerBlock(int Q) { return (256 / 64) * (64 / (Q == 27 ? 32 : 9)) * 8; }
+// Blocks of the PML-local pass (one partial-sum row per block).
+int64_t pml_parts(int Q, int k, int64_t N, int64_t n_rq);
 template <int Q>
 void launch_step(const StepGeom& geo, const StepBuffers& buf,
                  const CollisionParams& cp, hipStream_t stream,
