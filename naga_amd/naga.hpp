@@ -163,6 +163,15 @@ struct stencil_config {
     bool reference_pml_indexing = true;
 };
 
+// Declarative density source (naga_engine_add_density_source): while
+// t_start <= t < t_end, nodes within radius of center receive
+// amplitude * cos(2 pi frequency t + phase) * P(r / radius) per step.
+struct density_source_spec {
+    std::array<double, 3> center{};
+    double radius = 0.0, amplitude = 0.0, frequency = 0.0, phase = 0.0;
+    double t_start = 0.0, t_end = 1e300;
+};
+
 template <class Lattice>
 class simulation_engine {
   public:
@@ -290,6 +299,30 @@ class simulation_engine {
         return solution_;
     }
 
+    void register_density_source(const density_source_spec& src) {
+        abi::check(naga_engine_add_density_source(live(), src.center.data(), src.radius,
+                                                  src.amplitude, src.frequency, src.phase,
+                                                  src.t_start, src.t_end),
+                   "naga::nonlocal_lbm::simulation_engine::register_density_source: ");
+    }
+
+    // State carried between steps as bytes (save_state/load_state of
+    // simulation_engine.cuh:108-116, in this library's binary format).
+    std::vector<char> save_state() const {
+        int64_t n = 0;
+        abi::check(naga_engine_state_size(live(), &n),
+                   "naga::nonlocal_lbm::simulation_engine::save_state: ");
+        std::vector<char> buf(static_cast<size_t>(n));
+        abi::check(naga_engine_save_state(live(), buf.data(), n),
+                   "naga::nonlocal_lbm::simulation_engine::save_state: ");
+        return buf;
+    }
+    void load_state(const std::vector<char>& buf) {
+        abi::check(naga_engine_load_state(live(), buf.data(), int64_t(buf.size())),
+                   "naga::nonlocal_lbm::simulation_engine::load_state: ");
+        stale_ = true;
+    }
+
     naga_engine* native_handle() const { return handle_.get(); }
 
   private:
@@ -307,6 +340,49 @@ class simulation_engine {
     std::shared_ptr<naga_engine> handle_;
     mutable solution_t solution_{};
     mutable bool stale_ = true;
+};
+
+// density_observer (simulation_observers/density_observer.cuh:47-113):
+// the density at one location, interpolated from its 32 nearest nodes
+// with multiquadric RPIM weights; update() records the value held by the
+// engine (from the start of its last step) and the time of that step.
+template <class Lattice>
+class density_observer {
+  public:
+    using value_type = typename lattice_traits<Lattice>::value_type;
+    static constexpr unsigned dimensions = lattice_traits<Lattice>::dimensions;
+
+    density_observer(const simulation_engine<Lattice>& engine,
+                     const std::array<value_type, dimensions>& location,
+                     value_type time_multiplier = 1)
+        : engine_(&engine), time_multiplier_(time_multiplier) {
+        float p[3] = {0.f, 0.f, 0.f};
+        for (unsigned d = 0; d < dimensions; ++d) p[d] = float(location[d]);
+        abi::check(naga_engine_add_probes(engine.native_handle(), p, 1, NAGA_RBF_MULTIQUADRIC,
+                                          &set_),
+                   "naga::nonlocal_lbm::density_observer: ");
+    }
+
+    // call after each step_forward()
+    void update() {
+        double v = 0.0, t = 0.0;
+        abi::check(naga_engine_sample_probes(engine_->native_handle(), set_, 0, &v),
+                   "naga::nonlocal_lbm::density_observer::update: ");
+        abi::check(naga_engine_info(engine_->native_handle(), nullptr, &t, nullptr, nullptr),
+                   "naga::nonlocal_lbm::density_observer::update: ");
+        const double dt = double(engine_->problem_parameters().time_step);
+        density_values_.push_back(value_type(v));
+        time_values_.push_back(value_type((t - dt) * time_multiplier_));
+    }
+
+    const std::vector<value_type>& density_values() const { return density_values_; }
+    const std::vector<value_type>& time_values() const { return time_values_; }
+
+  private:
+    const simulation_engine<Lattice>* engine_;
+    value_type time_multiplier_;
+    int32_t set_ = -1;
+    std::vector<value_type> density_values_, time_values_;
 };
 
 }  // namespace fluids::nonlocal_lbm

@@ -49,3 +49,10 @@ def test_facade_matches_python_mirror(facade_exe):
     ref = float(np.sum(m["rho"].astype(np.float64) - 1.0))
     assert abs(float(vals["sum_drho"]) - ref) <= 1e-6 * max(1.0, abs(ref))
     assert abs(float(vals["max_u"]) - float(np.abs(m["u"]).max())) <= 1e-7
+
+
+@pytest.mark.gpu
+def test_facade_state_source_observer(facade_exe):
+    r = subprocess.run([facade_exe, "extras"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "extras ok" in r.stdout
