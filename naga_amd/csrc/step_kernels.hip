@@ -437,15 +437,44 @@ __global__ __launch_bounds__(kBlock) void k_p4(StepGeom G, StepBuffers S, BatchA
 }
 
 // ---------------------------------------------------------------------------
+// Classic RK4 without an accumulator buffer: the stage outputs are kept,
+//   y1 = y + dt/2 k1 (A), y2 = y + dt/2 k2 (B), y3 = y + dt k3 (C),
+// and stage 4 forms y + ((y1 - y) + 2 (y2 - y) + (y3 - y)) / 3 + dt/6 k4,
+// which is y + dt/6 (k1 + 2 k2 + 2 k3 + k4) (rk4_solver.cuh:48-171).  C is
+// the buffer S.acc.  Per node this moves 11 rows per step instead of 14.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const float* stage_in(const StepBuffers& S, int stage) {
+    return stage == 1 ? S.g : stage == 2 ? S.A : stage == 3 ? S.B : S.acc;
+}
+__device__ __forceinline__ float* stage_out(const StepBuffers& S, int stage) {
+    return stage == 1 ? S.A : stage == 2 ? S.B : stage == 3 ? S.acc : S.g;
+}
+template <int STAGE>
+__device__ __forceinline__ float rk4_out(float y, float k, float dt, float y1, float y2,
+                                         float y3) {
+    if (STAGE == 1 || STAGE == 2) return y + (dt * 0.5f) * k;
+    if (STAGE == 3) return y + dt * k;
+    return y + (((y1 - y) + 2.0f * (y2 - y)) + (y3 - y)) * (1.0f / 3.0f) + (dt * (1.0f / 6.0f)) * k;
+}
+template <int STAGE>
+__device__ __forceinline__ float4 rk4_out4(const float4& y, const float4& k, float dt,
+                                           const float4& y1, const float4& y2, const float4& y3) {
+    return make_float4(rk4_out<STAGE>(y.x, k.x, dt, y1.x, y2.x, y3.x),
+                       rk4_out<STAGE>(y.y, k.y, dt, y1.y, y2.y, y3.y),
+                       rk4_out<STAGE>(y.z, k.z, dt, y1.z, y2.z, y3.z),
+                       rk4_out<STAGE>(y.w, k.w, dt, y1.w, y2.w, y3.w));
+}
+
+// ---------------------------------------------------------------------------
 // RK4 stage (rk4_solver.cuh:48-171 with pde_right_hand_side.cuh:82-176):
 // rate_alpha(i) = -sum_s (c_alpha . W(s,i)) g_in(idx(s,i))_alpha on fluid
 // nodes (rates are zero on boundary nodes, so their state is constant
 // through the stages).
-//   stage 1: A = g + dt/2 r, acc = r/6      (and A = B = g on boundary rows)
-//   stage 2: B = g + dt/2 r, acc += r/3
-//   stage 3: A = g + dt r,   acc += r/3
-//   stage 4: g = g + dt (acc + r/6)   (in place: each node reads only its
-//            own g row here; neighbours are gathered from A)
+//   stage 1: A = g + dt/2 r            (and A = B = C = g on boundary rows)
+//   stage 2: B = g + dt/2 r             (gathers A)
+//   stage 3: C = g + dt r               (gathers B)
+//   stage 4: g = g + ((A-g) + 2(B-g) + (C-g))/3 + dt/6 r   (gathers C; in
+//            place: each node reads only its own g row here)
 // ---------------------------------------------------------------------------
 template <int Q, int STAGE, int KT, int BATCH>
 __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S, BatchArgs B) {
@@ -460,12 +489,17 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S, Bat
     const int64_t i = node0 + ln.slot;
     const int a = ln.a;
     const int64_t o = i * RS + a;
-    const float* gin = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
-    float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
+    const float* gin = stage_in(S, STAGE);
+    float* gout = stage_out(S, STAGE);
     // own rows first: their HBM latency overlaps the record staging
     const bool fluid = i < G.nfluid;
     const float g0 = (i < G.N) ? S.g[o] : 0.0f;
-    const float acc0 = (STAGE != 1 && fluid) ? S.acc[o] : 0.0f;
+    float y1 = 0.0f, y2 = 0.0f, y3 = 0.0f;
+    if (STAGE == 4 && fluid) {
+        y1 = S.A[o];
+        y2 = S.B[o];
+        y3 = S.acc[o];
+    }
     const int64_t nf = G.nfluid - node0;
     stage_contiguous<REC>(S.slab, node0, nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB), K, srec);
     __syncthreads();
@@ -473,25 +507,14 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S, Bat
         if (ln.active) {
             S.A[o] = g0;
             S.B[o] = g0;
+            S.acc[o] = g0;
         }
         return;
     }
     if (!fluid) return;
     const float rate = -stencil_sum<Q, KT, BATCH>(srec + ln.slot * K * REC, K, gin, a);
     if (!ln.active) return;
-    const float dt = G.dt;
-    if (STAGE == 1) {
-        gout[o] = g0 + (dt * 0.5f) * rate;
-        S.acc[o] = rate * (1.0f / 6.0f);
-    } else if (STAGE == 2) {
-        gout[o] = g0 + (dt * 0.5f) * rate;
-        S.acc[o] = acc0 + rate * (2.0f / 6.0f);
-    } else if (STAGE == 3) {
-        gout[o] = g0 + dt * rate;
-        S.acc[o] = acc0 + rate * (2.0f / 6.0f);
-    } else {
-        gout[o] = g0 + dt * (acc0 + rate * (1.0f / 6.0f));
-    }
+    gout[o] = rk4_out<STAGE>(g0, rate, G.dt, y1, y2, y3);
 }
 
 // ---------------------------------------------------------------------------
@@ -594,11 +617,14 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
-    const float* gin_p = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
-    float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
+    const float* gin_p = stage_in(S, STAGE);
+    float* gout = stage_out(S, STAGE);
     const bool fluid = i < G.nfluid;
     const float4 g0 = ld4(rsrc(S.g, rows), own);
-    const float4 acc0 = STAGE != 1 ? ld4(rsrc(S.acc, rows), own) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 y1 = STAGE == 4 ? ld4(rsrc(S.A, rows), own) : z4;
+    const float4 y2 = STAGE == 4 ? ld4(rsrc(S.B, rows), own) : z4;
+    const float4 y3 = STAGE == 4 ? ld4(rsrc(S.acc, rows), own) : z4;
     const int64_t nf = G.nfluid - node0;
     const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
@@ -618,6 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     if (STAGE == 1 && !fluid && i < G.N) {
         st4(S.A, own, g0);
         st4(S.B, own, g0);
+        st4(S.acc, own, g0);
         return;
     }
     if (!fluid) return;
@@ -635,26 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
                                      return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
                                  },
                                  q);
-    const float dt = G.dt;
-    float4 o, a;
-    if (STAGE == 1) {
-        o = make_float4(g0.x + (dt * 0.5f) * r.x, g0.y + (dt * 0.5f) * r.y,
-                        g0.z + (dt * 0.5f) * r.z, g0.w + (dt * 0.5f) * r.w);
-        a = make_float4(r.x * (1.0f / 6.0f), r.y * (1.0f / 6.0f), r.z * (1.0f / 6.0f),
-                        r.w * (1.0f / 6.0f));
-    } else if (STAGE == 2 || STAGE == 3) {
-        const float h = STAGE == 2 ? dt * 0.5f : dt;
-        o = make_float4(g0.x + h * r.x, g0.y + h * r.y, g0.z + h * r.z, g0.w + h * r.w);
-        a = make_float4(acc0.x + r.x * (2.0f / 6.0f), acc0.y + r.y * (2.0f / 6.0f),
-                        acc0.z + r.z * (2.0f / 6.0f), acc0.w + r.w * (2.0f / 6.0f));
-    } else {
-        o = make_float4(g0.x + dt * (acc0.x + r.x * (1.0f / 6.0f)),
-                        g0.y + dt * (acc0.y + r.y * (1.0f / 6.0f)),
-                        g0.z + dt * (acc0.z + r.z * (1.0f / 6.0f)),
-                        g0.w + dt * (acc0.w + r.w * (1.0f / 6.0f)));
-    }
-    st4(gout, own, o);
-    if (STAGE != 4) st4(S.acc, own, a);
+    st4(gout, own, rk4_out4<STAGE>(g0, r, G.dt, y1, y2, y3));
 }
 
 // RK4 stage, D3Q27, four per lane, with each node's stencil records read
@@ -674,34 +682,22 @@ __global__ __launch_bounds__(kBlock) void k_stage_d(StepGeom G, StepBuffers S, B
         if (STAGE == 1) {
             st4(S.A, own, g0);
             st4(S.B, own, g0);
+            st4(S.acc, own, g0);
         }
         return;
     }
-    const float* gin_p = STAGE == 1 ? S.g : STAGE == 2 ? S.A : STAGE == 3 ? S.B : S.A;
-    float* gout = STAGE == 1 ? S.A : STAGE == 2 ? S.B : STAGE == 3 ? S.A : S.g;
-    const float4 acc0 = STAGE != 1 ? ld4(rsrc(S.acc, rows), own) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float* gin_p = stage_in(S, STAGE);
+    float* gout = stage_out(S, STAGE);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 y1 = STAGE == 4 ? ld4(rsrc(S.A, rows), own) : z4;
+    const float4 y2 = STAGE == 4 ? ld4(rsrc(S.B, rows), own) : z4;
+    const float4 y3 = STAGE == 4 ? ld4(rsrc(S.acc, rows), own) : z4;
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
     const float4* rp = reinterpret_cast<const float4*>(S.slab) + i * K;
     const uint32_t lane_off = uint32_t(q) * 16u;
     const float4 r =
         rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gin, j * 128u + lane_off); }, q);
-    const float dt = G.dt;
-    if (STAGE == 1) {
-        st4(gout, own, make_float4(g0.x + (dt * 0.5f) * r.x, g0.y + (dt * 0.5f) * r.y,
-                                   g0.z + (dt * 0.5f) * r.z, g0.w + (dt * 0.5f) * r.w));
-        st4(S.acc, own, make_float4(r.x * (1.0f / 6.0f), r.y * (1.0f / 6.0f),
-                                    r.z * (1.0f / 6.0f), r.w * (1.0f / 6.0f)));
-    } else if (STAGE == 2 || STAGE == 3) {
-        const float h = STAGE == 2 ? dt * 0.5f : dt;
-        st4(gout, own, make_float4(g0.x + h * r.x, g0.y + h * r.y, g0.z + h * r.z, g0.w + h * r.w));
-        st4(S.acc, own, make_float4(acc0.x + r.x * (2.0f / 6.0f), acc0.y + r.y * (2.0f / 6.0f),
-                                    acc0.z + r.z * (2.0f / 6.0f), acc0.w + r.w * (2.0f / 6.0f)));
-    } else {
-        st4(gout, own, make_float4(g0.x + dt * (acc0.x + r.x * (1.0f / 6.0f)),
-                                   g0.y + dt * (acc0.y + r.y * (1.0f / 6.0f)),
-                                   g0.z + dt * (acc0.z + r.z * (1.0f / 6.0f)),
-                                   g0.w + dt * (acc0.w + r.w * (1.0f / 6.0f))));
-    }
+    st4(gout, own, rk4_out4<STAGE>(g0, r, G.dt, y1, y2, y3));
 }
 
 // sum_s w_s (g_s - g_i) over the staged filter records rp ({index field, w}),

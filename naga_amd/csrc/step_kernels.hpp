@@ -49,7 +49,7 @@ struct StepBuffers {
     float* g;        // [N][RS] current deviation state
     float* A;        // [N][RS] stage ping
     float* B;        // [N][RS] stage pong
-    float* acc;      // [N][RS] RK4 accumulator
+    float* acc;      // [N][RS] RK4 stage-3 output (the fourth stage buffer)
     float4* ru;      // [N] unitless (rho - 1, u) from p1
     float* Gf;       // [N][RS] PML auxiliary field (zero outside the layer)
     const int32_t* rq_list;  // nodes whose PML Q values are maintained
