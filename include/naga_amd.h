@@ -43,6 +43,12 @@ typedef struct naga_scene_config {
     int32_t rbf_boundary;   /* RBF of the boundary interpolator */
     int32_t pml_ref_index;  /* 1: PML layer field reads Q at (j - num_bulk)
                                as pml_absorption.inl:445-448 does; 0: at j */
+    int32_t k_min;          /* 0: every node uses k neighbours; else node i
+                               uses k_i in [k_min, k] by local density,
+                               k_i = k_min d_ref / d_i (d_i: mean distance to
+                               the 4 nearest nodes, d_ref its 90th
+                               percentile); stencil entries past k_i carry
+                               zero weights */
     double nu;              /* fluid viscosity */
     double rho0;            /* nominal density */
     double dt;              /* time step */
@@ -137,6 +143,8 @@ int naga_engine_get_macroscopic(naga_engine* e, float* rho, float* u,
                                 double* drho, int32_t fresh);
 /* f[Q*N] alpha-major (FP64 = w + deviation) and/or the deviation g[Q*N]. */
 int naga_engine_get_distributions(naga_engine* e, double* f, double* g);
+/* k_i per node (uint8[N]; all k when k_min = 0). */
+int naga_engine_get_stencil_sizes(naga_engine* e, uint8_t* k);
 /* Setup products for parity tests: idx[N*k], W[N*k*D], filt[N*k],
  * bidx[Nb*kb], bw[Nb*kb]. Any may be NULL. */
 int naga_engine_get_setup(naga_engine* e, uint32_t* idx, float* W, float* filt,
@@ -234,15 +242,19 @@ int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h
  *      (same SplitMix64 stream and jitter) and are dropped within half a
  *      spacing of an obstacle; obstacle surfaces get nodes at about the grid
  *      spacing (equal angles on circles, a Fibonacci lattice on spheres)
- *      with normals pointing into the fluid.  The absorbing layer is the
+ *      with normals pointing into the fluid.  refine_width > 0 doubles the
+ *      node density within that distance of the obstacle surfaces (the
+ *      2^dims - 1 half-offset nodes of each grid cell there, jittered by
+ *      jitter_frac * h/2 from a second stream; surface nodes at h/2).  The
+ *      absorbing layer is the
  *      fluid within layer_thickness of the box faces (exact distance d),
  *      sigma = peak_absorption (1 - d/L)^2.  With points == NULL only
  *      counts[3] = {bulk, layer, boundary} are returned (size query). */
 int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double h,
                        double jitter_frac, uint64_t seed, int32_t n_obstacles,
-                       const double* obstacles, double layer_thickness, double peak_absorption,
-                       int64_t capacity, float* points, float* normals, float* sigma,
-                       int64_t* counts);
+                       const double* obstacles, double refine_width, double layer_thickness,
+                       double peak_absorption, int64_t capacity, float* points, float* normals,
+                       float* sigma, int64_t* counts);
 
 /* ---- setup primitives ---------------------------------------------------- */
 /* batched_nearest_neighbors (segmentation/nearest_neighbors.cuh:95-477),

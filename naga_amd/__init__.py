@@ -40,7 +40,7 @@ class SceneConfig(C.Structure):
     _fields_ = [
         ("lattice", C.c_int32), ("k", C.c_int32), ("k_boundary", C.c_int32),
         ("rbf", C.c_int32), ("rbf_boundary", C.c_int32),
-        ("pml_ref_index", C.c_int32),
+        ("pml_ref_index", C.c_int32), ("k_min", C.c_int32),
         ("nu", C.c_double), ("rho0", C.c_double), ("dt", C.c_double),
         ("c", C.c_double), ("h", C.c_double),
     ]
@@ -119,8 +119,8 @@ def load() -> C.CDLL:
         "naga_batch_set_graph": (C.c_int, [vp, i32]),
         "naga_batch_step": (C.c_int, [vp, i32]),
         "naga_batch_step_timed": (C.c_int, [vp, i32, vp]),
-        "naga_obstacle_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, i32, vp, f64, f64, i64,
-                                         vp, vp, vp, vp]),
+        "naga_obstacle_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, i32, vp, f64, f64, f64,
+                                         i64, vp, vp, vp, vp]),
         "naga_server_create": (C.c_int, [i64, i32, i32, i32, i32, C.POINTER(vp)]),
         "naga_server_destroy": (C.c_int, [vp]),
         "naga_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(i64)]),
@@ -135,6 +135,7 @@ def load() -> C.CDLL:
         "naga_engine_load_state": (C.c_int, [vp, vp, i64]),
         "naga_engine_add_probes": (C.c_int, [vp, vp, i64, i32, C.POINTER(i32)]),
         "naga_engine_sample_probes": (C.c_int, [vp, i32, i32, vp]),
+        "naga_engine_get_stencil_sizes": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -161,6 +162,7 @@ EXPORTED = [
     "naga_server_release", "naga_server_drain", "naga_server_get_stats",
     "naga_batch_autotune", "naga_engine_state_size", "naga_engine_save_state",
     "naga_engine_load_state", "naga_engine_add_probes", "naga_engine_sample_probes",
+    "naga_engine_get_stencil_sizes",
 ]
 
 STEP_KERNELS = ["p1_collide", "pl_pml_local", "pm_layer_means", "pf_pml_field",
@@ -311,7 +313,8 @@ class ObstacleGridProvider:
 
     def __init__(self, min_point, max_point, grid_spacing, obstacles=(),
                  absorption_layer_thickness=0.0, peak_absorption_rate=0.0,
-                 jitter=0.0, seed=0x4E414741):
+                 jitter=0.0, seed=0x4E414741, refine_width=0.0):
+        self.refine_width = float(refine_width)
         self.lo = _f64(min_point)
         self.hi = _f64(max_point)
         self.D = self.lo.size
@@ -327,7 +330,8 @@ class ObstacleGridProvider:
         counts = np.zeros(3, np.int64)
         ob = np.ascontiguousarray(self.obst.reshape(-1))
         args = (self.D, _ptr(self.lo), _ptr(self.hi), self.h, self.jitter, C.c_uint64(self.seed),
-                len(self.obst), _ptr(ob) if ob.size else None, self.layer, self.peak)
+                len(self.obst), _ptr(ob) if ob.size else None, self.refine_width, self.layer,
+                self.peak)
         _check(lib.naga_obstacle_grid(*args, 0, None, None, None, _ptr(counts)),
                "naga::nonlocal_lbm::obstacle_grid_provider::get: ")
         n = int(counts.sum())
@@ -346,9 +350,11 @@ class SimulationEngine:
 
     def __init__(self, lattice: int = 27, k: int | None = None,
                  k_boundary: int | None = None, rbf: int = RBF_POLYHARMONIC3,
-                 rbf_boundary: int | None = None, pml_ref_index: bool = True):
+                 rbf_boundary: int | None = None, pml_ref_index: bool = True,
+                 k_min: int = 0):
         if lattice not in (9, 27):
             raise ValueError(_ENGINE_PREFIX + "lattice must be D2Q9 or D3Q27")
+        self.k_min = int(k_min)
         self.lattice = lattice
         self.dimensions = 2 if lattice == 9 else 3
         self.k = 32 if k is None else int(k)
@@ -395,7 +401,7 @@ class SimulationEngine:
         self.close()
         cfg = SceneConfig(lattice=self.lattice, k=self.k, k_boundary=self.k_boundary,
                           rbf=self.rbf, rbf_boundary=self.rbf_boundary,
-                          pml_ref_index=int(self.pml_ref_index),
+                          pml_ref_index=int(self.pml_ref_index), k_min=self.k_min,
                           nu=self._params["nu"], rho0=self._params["rho0"],
                           dt=self._params["dt"], c=self._params["c"],
                           h=float(nodes.nodal_spacing))
@@ -453,6 +459,12 @@ class SimulationEngine:
         out = np.zeros(self._probe_sizes[probe_set])
         _check(load().naga_engine_sample_probes(self._live(), int(probe_set), int(fresh),
                                                 _ptr(out)), _ENGINE_PREFIX)
+        return out
+
+    def stencil_sizes(self) -> np.ndarray:
+        """k_i per node (all k unless k_min was given)."""
+        out = np.zeros(self.num_nodes, np.uint8)
+        _check(load().naga_engine_get_stencil_sizes(self._live(), _ptr(out)), _ENGINE_PREFIX)
         return out
 
     def save_state(self) -> bytes:

@@ -134,6 +134,7 @@ int naga_engine_create(const naga_scene_config* cfg, naga_engine** out) {
         c.rbf = cfg->rbf;
         c.rbf_boundary = cfg->rbf_boundary;
         c.pml_ref = cfg->pml_ref_index;
+        c.k_min = cfg->k_min;
         c.nu = cfg->nu;
         c.rho0 = cfg->rho0;
         c.dt = cfg->dt;
@@ -161,6 +162,10 @@ int naga_engine_init_domain(naga_engine* e, const float* points, int64_t n_bulk,
         engine_of(e).init_domain(points, n_bulk, n_layer, n_boundary, normals,
                                  layer_absorption);
     });
+}
+
+int naga_engine_get_stencil_sizes(naga_engine* e, uint8_t* k) {
+    return guarded([&] { engine_of(e).get_stencil_sizes(k); });
 }
 
 int naga_engine_set_pulse(naga_engine* e, const double* center, double amp, double width) {
@@ -513,9 +518,9 @@ int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h
 
 int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double h,
                        double jitter_frac, uint64_t seed, int32_t n_obstacles,
-                       const double* obstacles, double layer_thickness, double peak_absorption,
-                       int64_t capacity, float* points, float* normals, float* sigma,
-                       int64_t* counts) {
+                       const double* obstacles, double refine_width, double layer_thickness,
+                       double peak_absorption, int64_t capacity, float* points, float* normals,
+                       float* sigma, int64_t* counts) {
     return guarded([&] {
         const int D = dims;
         require(D == 2 || D == 3, "dims must be 2 or 3");
@@ -596,20 +601,72 @@ int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double 
                 bulk.push_back(nd);
             }
         }
-        // obstacle surfaces, normals into the fluid
+        // refinement: grid cells whose centre lies within refine_width of an
+        // obstacle surface (outside it) get the 2^D - 1 half-offset nodes,
+        // jittered by jitter_frac * h/2 from a second stream
         int64_t order = N;
+        if (refine_width > 0 && n_obstacles > 0) {
+            uint64_t st2 = seed ^ 0x5DEECE66DA3B1F27ull;
+            int64_t cd[3] = {1, 1, 1};
+            int64_t ncell = 1;
+            for (int d = 0; d < D; ++d) {
+                cd[d] = gd[d] - 1;
+                ncell *= cd[d];
+            }
+            for (int64_t c = 0; c < ncell; ++c) {
+                int64_t ci[3] = {0, 0, 0}, rem = c;
+                for (int d = 0; d < D; ++d) {
+                    ci[d] = rem % cd[d];
+                    rem /= cd[d];
+                }
+                double ctr[3] = {0.0, 0.0, 0.0};
+                for (int d = 0; d < D; ++d) ctr[d] = lo[d] + (double(ci[d]) + 0.5) * sp[d];
+                double gap = 1e300;
+                for (int j = 0; j < n_obstacles; ++j) {
+                    const double* o = obstacles + j * (D + 1);
+                    double r2 = 0.0;
+                    for (int d = 0; d < D; ++d) r2 += (ctr[d] - o[d]) * (ctr[d] - o[d]);
+                    gap = std::min(gap, std::sqrt(r2) - o[D]);
+                }
+                if (gap < -0.5 * h || gap > refine_width) continue;
+                for (int m = 1; m < (1 << D); ++m) {
+                    double x[3] = {0.0, 0.0, 0.0};
+                    bool ok = true;
+                    for (int d = 0; d < D; ++d) {
+                        const double u01 = double(splitmix64(st2) >> 11) * 0x1.0p-53;
+                        x[d] = lo[d] + (double(ci[d]) + 0.5 * ((m >> d) & 1)) * sp[d] +
+                               (2.0 * u01 - 1.0) * jitter_frac * 0.5 * h;
+                        ok = ok && x[d] > lo[d] && x[d] < hi[d];
+                    }
+                    if (!ok || inside(x, 0.25 * h)) continue;
+                    Node nd{qkey(x), order++, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+                    for (int d = 0; d < D; ++d) nd.x[d] = float(x[d]);
+                    double dmin = 1e300;
+                    for (int d = 0; d < D; ++d) dmin = std::min({dmin, x[d] - lo[d], hi[d] - x[d]});
+                    if (L > 0 && dmin <= L) {
+                        layer.push_back(nd);
+                        const double r = (L - std::max(dmin, 0.0)) / L;
+                        sig.push_back(float(peak_absorption * r * r));
+                    } else {
+                        bulk.push_back(nd);
+                    }
+                }
+            }
+        }
+        // obstacle surfaces (at half spacing when refined), normals into the fluid
+        const double hs = refine_width > 0 ? 0.5 * h : h;
         for (int j = 0; j < n_obstacles; ++j) {
             const double* o = obstacles + j * (D + 1);
             const double R = o[D];
             std::vector<std::array<double, 3>> dirs;
             if (D == 2) {
-                const int M = std::max(8, int(std::ceil(2.0 * M_PI * R / h)));
+                const int M = std::max(8, int(std::ceil(2.0 * M_PI * R / hs)));
                 for (int m = 0; m < M; ++m) {
                     const double t = 2.0 * M_PI * (m + 0.5) / M;
                     dirs.push_back({std::cos(t), std::sin(t), 0.0});
                 }
             } else {
-                const int M = std::max(16, int(std::ceil(4.0 * M_PI * R * R / (h * h))));
+                const int M = std::max(16, int(std::ceil(4.0 * M_PI * R * R / (hs * hs))));
                 const double ga = M_PI * (3.0 - std::sqrt(5.0));
                 for (int m = 0; m < M; ++m) {
                     const double z = 1.0 - 2.0 * (m + 0.5) / M, rr = std::sqrt(1.0 - z * z);

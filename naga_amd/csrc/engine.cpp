@@ -1,6 +1,7 @@
 // Engine: setup orchestration and the per-step schedule on one HIP stream.
 #include "engine.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -56,6 +57,7 @@ Engine::Engine(const SceneConfig& cfg) : cfg_(cfg) {
     D_ = cfg.Q == 9 ? 2 : 3;
     RS_ = cfg.Q == 9 ? Lattice<9>::RS : Lattice<27>::RS;
     require(cfg.k >= 2 && cfg.k + D_ + 1 <= 64, "k must satisfy 2 <= k and k + D + 1 <= 64");
+    require(cfg.k_min >= 0 && (cfg.k_min == 0 || cfg.k_min >= 2), "k_min must be 0 or >= 2");
     require(cfg.kb >= 1 && cfg.kb + D_ + 1 <= 64, "k_boundary must satisfy k_boundary + D + 1 <= 64");
     require(cfg.rbf == 0 || cfg.rbf == 1, "rbf must be 0 (multiquadric) or 1 (r^3)");
     require(cfg.rbf_boundary == 0 || cfg.rbf_boundary == 1, "rbf_boundary must be 0 or 1");
@@ -150,10 +152,37 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
             }
         }
         auto dhbar = to_device(hbar, st);
+        // per-node stencil widths by local density: k_i = k_min d_ref / d_i
+        // clamped to [k_min, k], d_i the mean distance to the four nearest
+        // neighbours and d_ref its 90th percentile (the coarse spacing)
+        kvar_ = DeviceBuffer<uint8_t>();
+        if (cfg_.k_min > 0 && cfg_.k_min < k) {
+            require(k >= 5 && k <= 255, "variable stencils need 5 <= k <= 255");
+            std::vector<float> hd2(size_t(N) * k);
+            d2.download(hd2.data(), hd2.size(), st);
+            NAGA_HIP(hipStreamSynchronize(st));
+            std::vector<double> dm(N);
+            for (int64_t i = 0; i < N; ++i) {
+                double a = 0.0;
+                for (int s = 1; s <= 4; ++s) a += std::sqrt(double(hd2[size_t(i) * k + s]));
+                dm[i] = a / 4.0;
+            }
+            std::vector<double> sorted_dm(dm);
+            const size_t p90 = size_t(0.9 * double(N - 1));
+            std::nth_element(sorted_dm.begin(), sorted_dm.begin() + p90, sorted_dm.end());
+            const double dref = sorted_dm[p90];
+            std::vector<uint8_t> kv(N);
+            for (int64_t i = 0; i < N; ++i) {
+                const double want = double(cfg_.k_min) * dref / std::max(dm[i], 1e-30);
+                kv[i] = uint8_t(std::min<double>(k, std::max<double>(cfg_.k_min, std::round(want))));
+            }
+            kvar_ = to_device(kv, st);
+        }
         W_.allocate(N * k * D);
         DeviceBuffer<int> info(N);
         launch_divergence_weights(D, pts_.get(), pts_.get(), N, idx_.get(), d2.get(), k,
-                                  cfg_.rbf, dhbar.get(), W_.get(), nullptr, info.get(), st);
+                                  cfg_.rbf, dhbar.get(), W_.get(), nullptr, info.get(), st,
+                                  kvar_.get());
         std::vector<int> hinfo(N);
         info.download(hinfo.data(), N, st);
         NAGA_HIP(hipStreamSynchronize(st));
@@ -163,7 +192,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
                                       "operator at node " + std::to_string(i));
     }
     filt_.allocate(N * k);
-    launch_filter_weights(D, pts_.get(), N, idx_.get(), k, cfg_.h, filt_.get(), st);
+    launch_filter_weights(D, pts_.get(), N, idx_.get(), k, cfg_.h, filt_.get(), st, kvar_.get());
     slab_.allocate(nfluid * k * (1 + D));
     launch_make_slab(nfluid, k, D, idx_.get(), W_.get(), slab_.get(), st);
     fslab_.allocate(nfluid * k * 2);
@@ -740,7 +769,7 @@ void Batch::rebuild() {
         }
         for (int p = 0; p < kStepKernels; ++p) P.blocks[p] = bstart[size_t(p) * (n + 1) + n];
         P.unions = step_uses_unions(g.Q, g.members[0]->stencil_k(), g.members[0]->nodes());
-        P.k_template = (kcommon == 32 || kcommon == 16) ? kcommon : 0;
+        P.k_template = (kcommon == 48 || kcommon == 32 || kcommon == 16) ? kcommon : 0;
         P.gather_batch = gather_batch_;
         P.tiles = descs[0].S.ulist != nullptr;
         for (const SceneDesc& d : descs)
@@ -891,6 +920,17 @@ void Engine::set_pulse(const double* center, double amp, double width) {
     NAGA_HIP(hipStreamSynchronize(stream_));
     time_ = 0.0;
     frame_ = 0;
+}
+
+void Engine::get_stencil_sizes(uint8_t* k) {
+    require_domain();
+    require(k != nullptr, "null output");
+    if (kvar_.size() == size_t(N_)) {
+        kvar_.download(k, N_, stream_);
+        NAGA_HIP(hipStreamSynchronize(stream_));
+    } else {
+        std::fill(k, k + N_, uint8_t(cfg_.k));
+    }
 }
 
 void Engine::get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, float* bw) {

@@ -18,6 +18,7 @@ struct SceneConfig {
     int rbf = 1;
     int rbf_boundary = 1;
     int pml_ref = 1;
+    int k_min = 0;  // > 0: per-node stencil widths in [k_min, k] by local density
     double nu = 0.0, rho0 = 1.0, dt = 0.0, c = kCsLat, h = 0.01;
 };
 
@@ -58,6 +59,7 @@ class Engine {
     void get_macroscopic(float* rho, float* u, double* drho, bool fresh);
     void get_distributions(double* f, double* g);
     void get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, float* bw);
+    void get_stencil_sizes(uint8_t* k);
     void get_pml(double* E, double* Qp);
     void add_density_source(const DensitySource& src);
     // density probes: kNN (k = 32, or fewer nodes) among all nodes and
@@ -111,6 +113,7 @@ class Engine {
     DeviceBuffer<float> pts_, normals_, sigma_;
     DeviceBuffer<float> slab_, fslab_, bslab_;
     DeviceBuffer<uint32_t> idx_, bidx_;
+    DeviceBuffer<uint8_t> kvar_;  // per-node stencil widths (empty: all k)
     DeviceBuffer<float> W_, filt_, bw_;
     DeviceBuffer<int32_t> pml_list_;
     DeviceBuffer<uint8_t> bsrc_;

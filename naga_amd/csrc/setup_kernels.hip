@@ -332,22 +332,29 @@ __device__ void quad_term(int D, double delta, const double* xi, const double* x
 template <int NMAX, int D>
 __global__ __launch_bounds__(64) void k_divw(const float* src, const float* query,
                                              int64_t n_q, const uint32_t* idx,
-                                             const float* d2, int k, int rbf,
-                                             const double* hbar, float* W,
-                                             double* W64, int* info) {
+                                             const float* d2, int kmax, int rbf,
+                                             const double* hbar, const uint8_t* kvar,
+                                             float* W, double* W64, int* info) {
     __shared__ double xs[64][3];
     const int lane = threadIdx.x;
     const int64_t i = blockIdx.x;
     if (i >= n_q) return;
+    // per-node stencil width k <= kmax (entries past k get zero weights)
+    const int k = kvar ? int(kvar[i]) : kmax;
     const int n = k + D + 1;
+    if (lane >= k && lane < kmax)
+        for (int d = 0; d < D; ++d) {
+            W[(i * kmax + lane) * D + d] = 0.0f;
+            if (W64) W64[(i * kmax + lane) * D + d] = 0.0;
+        }
     if (lane < k) {
-        const int64_t j = idx[i * k + lane];
+        const int64_t j = idx[i * kmax + lane];
         for (int d = 0; d < D; ++d) xs[lane][d] = double(src[j * D + d]);
     }
     __syncthreads();
     double xi[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < D; ++d) xi[d] = double(query[i * D + d]);
-    const double delta = 0.05 * sqrt(double(d2[i * k + 1]));
+    const double delta = 0.05 * sqrt(double(d2[i * kmax + 1]));
     const double spacing = hbar[i / 16384];
     double row[NMAX];
     g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row);
@@ -374,8 +381,8 @@ __global__ __launch_bounds__(64) void k_divw(const float* src, const float* quer
     if (pos < k) {
         for (int d = 0; d < D; ++d) {
             const double v = b[d] + (pos == 0 ? S[d] : 0.0);
-            W[(i * k + pos) * D + d] = float(v);
-            if (W64) W64[(i * k + pos) * D + d] = v;
+            W[(i * kmax + pos) * D + d] = float(v);
+            if (W64) W64[(i * kmax + pos) * D + d] = v;
         }
     }
 }
@@ -443,25 +450,30 @@ __global__ __launch_bounds__(64) void k_batched_lu(int n, int nrhs, int64_t batc
 }
 
 __global__ void k_filter(int D, const float* pts, int64_t n, const uint32_t* idx,
-                         int k, double sg, float* wf) {
+                         int kmax, double sg, const uint8_t* kvar, float* wf) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int k = kvar ? int(kvar[i]) : kmax;
     double xi[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < D; ++d) xi[d] = double(pts[i * D + d]);
     double sum = 0.0;
     for (int s = 0; s < k; ++s) {
         double xj[3] = {0.0, 0.0, 0.0};
-        const int64_t j = idx[i * k + s];
+        const int64_t j = idx[i * kmax + s];
         for (int d = 0; d < D; ++d) xj[d] = double(pts[j * D + d]);
         const double r = sqrt(dist2_f64(D, xi, xj)) / sg;
         sum += exp(-0.5 * r * r);
     }
-    for (int s = 0; s < k; ++s) {
-        double xj[3] = {0.0, 0.0, 0.0};
-        const int64_t j = idx[i * k + s];
-        for (int d = 0; d < D; ++d) xj[d] = double(pts[j * D + d]);
-        const double r = sqrt(dist2_f64(D, xi, xj)) / sg;
-        wf[i * k + s] = float(exp(-0.5 * r * r) / sum);
+    for (int s = 0; s < kmax; ++s) {
+        double wv = 0.0;
+        if (s < k) {
+            double xj[3] = {0.0, 0.0, 0.0};
+            const int64_t j = idx[i * kmax + s];
+            for (int d = 0; d < D; ++d) xj[d] = double(pts[j * D + d]);
+            const double r = sqrt(dist2_f64(D, xi, xj)) / sg;
+            wv = exp(-0.5 * r * r) / sum;
+        }
+        wf[i * kmax + s] = float(wv);
     }
 }
 
@@ -624,12 +636,12 @@ void upload_quadrature(int D) {
 template <int D>
 void launch_divw_d(int nmax, const float* src, const float* query, int64_t n_q,
                    const uint32_t* idx, const float* d2, int k, int rbf,
-                   const double* hbar, float* W, double* W64, int* info,
+                   const double* hbar, const uint8_t* kvar, float* W, double* W64, int* info,
                    hipStream_t st) {
 #define NAGA_DIVW(NM)                                                              \
     case NM:                                                                       \
         k_divw<NM, D><<<n_q, 64, 0, st>>>(src, query, n_q, idx, d2, k, rbf, hbar, \
-                                           W, W64, info);                          \
+                                           kvar, W, W64, info);                    \
         break;
     switch (nmax) {
         NAGA_DIVW(20) NAGA_DIVW(28) NAGA_DIVW(36) NAGA_DIVW(44) NAGA_DIVW(52) NAGA_DIVW(64)
@@ -729,16 +741,16 @@ void launch_batched_lu_solve(int n, int nrhs, int64_t batch, double* A, double* 
 void launch_divergence_weights(int D, const float* src, const float* query,
                                int64_t n_q, const uint32_t* idx, const float* d2,
                                int k, int rbf, const double* hbar, float* W,
-                               double* W64, int* info, hipStream_t st) {
+                               double* W64, int* info, hipStream_t st, const uint8_t* kvar) {
     const int nmax = pick_nmax(k + D + 1);
     require(nmax > 0, "stencil too wide (k + D + 1 must be <= 64)");
     require(k >= 2, "k must be >= 2");
     if (n_q == 0) return;
     upload_quadrature(D);
     if (D == 2)
-        launch_divw_d<2>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, W, W64, info, st);
+        launch_divw_d<2>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, kvar, W, W64, info, st);
     else
-        launch_divw_d<3>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, W, W64, info, st);
+        launch_divw_d<3>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, kvar, W, W64, info, st);
     hip_check(hipGetLastError(), "divergence weights");
 }
 
@@ -756,9 +768,9 @@ void launch_rpim_weights(int D, const float* src, const float* query, int64_t n_
 }
 
 void launch_filter_weights(int D, const float* pts, int64_t n, const uint32_t* idx,
-                           int k, double h, float* wf, hipStream_t st) {
+                           int k, double h, float* wf, hipStream_t st, const uint8_t* kvar) {
     if (n == 0) return;
-    k_filter<<<(n + 255) / 256, 256, 0, st>>>(D, pts, n, idx, k, h / 2.5, wf);
+    k_filter<<<(n + 255) / 256, 256, 0, st>>>(D, pts, n, idx, k, h / 2.5, kvar, wf);
     hip_check(hipGetLastError(), "filter weights");
 }
 

@@ -38,11 +38,14 @@ void launch_batched_lu_solve(int n, int nrhs, int64_t batch, double* A,
 // W[(i*k + s)*D + d] (FP32 and, if W64 != nullptr, FP64).
 // rbf: 0 multiquadric (shape 0.1*hbar), 1 polyharmonic r^3.
 // hbar[b] is the mean nearest-neighbour distance of query batch b (16384).
+// kvar (optional, device): per-query stencil width <= k; stencil entries
+// past it get zero weights (idx/d2/W keep the row stride k).
 void launch_divergence_weights(int D, const float* src, const float* query,
                                int64_t n_q, const uint32_t* idx,
                                const float* d2, int k, int rbf,
                                const double* hbar, float* W, double* W64,
-                               int* info, hipStream_t stream);
+                               int* info, hipStream_t stream,
+                               const uint8_t* kvar = nullptr);
 
 // RPIM interpolation weights, group size 1: w[q*k + s].
 void launch_rpim_weights(int D, const float* src, const float* query,
@@ -53,7 +56,7 @@ void launch_rpim_weights(int D, const float* src, const float* query,
 // Normalised Gaussian filter weights over the stencil (sigma_g = h/2.5).
 void launch_filter_weights(int D, const float* pts, int64_t n,
                            const uint32_t* idx, int k, double h, float* wf,
-                           hipStream_t stream);
+                           hipStream_t stream, const uint8_t* kvar = nullptr);
 
 // Interleave indices and weights into slab records:
 // rec[(i*k + s)*(1+ncomp) + 0] = idx bits, then ncomp weights.

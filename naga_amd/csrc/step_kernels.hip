@@ -1012,14 +1012,14 @@ int stage_batch() {
     return b;
 }
 
-// The four-per-lane D3Q27 kernels need k in {16, 32} and 32-bit row offsets.
+// The four-per-lane D3Q27 kernels need k in {16, 32, 48} and 32-bit row offsets.
 // NAGA_V4=0 selects the one-per-lane kernels.
 bool step_uses_unions(int Q, int k, int64_t N) {
     static const bool on = [] {
         const char* e = std::getenv("NAGA_V4");
         return !(e && e[0] == '0');
     }();
-    return Q == 27 && on && (k == 32 || k == 16) && N < (int64_t(1) << 25);
+    return Q == 27 && on && (k == 48 || k == 32 || k == 16) && N < (int64_t(1) << 25);
 }
 
 namespace {
@@ -1060,23 +1060,38 @@ bool records_direct() {
     return on;
 }
 
+// The D3Q27 four-per-lane kernels of the stencil passes for one stencil
+// width K (16, 32 or 48).
+template <int K>
+void launch_v4(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S, const PassCfg& c,
+               hipStream_t st) {
+    const size_t lds = size_t(c.ucap) * 128u;
+    const bool direct = !c.tiles && records_direct();
+    switch (pass) {
+        case kP2:
+            if (c.tiles) k_p2_v4<K, 8, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else k_p2_v4<K, 8, false><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            break;
+        case kP4:
+            if (c.tiles) k_p4_u<K, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B);
+            else k_p4_u<K, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
+            break;
+#define NAGA_V4_STAGE(ST)                                                            \
+    case kS1 + ST - 1:                                                               \
+        if (direct) k_stage_d<ST, K, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);          \
+        else if (c.tiles) k_stage_u<ST, K, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B); \
+        else k_stage_u<ST, K, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);      \
+        break;
+        NAGA_V4_STAGE(1) NAGA_V4_STAGE(2) NAGA_V4_STAGE(3) NAGA_V4_STAGE(4)
+#undef NAGA_V4_STAGE
+        default: fail(kInvalidArgument, "not a stencil pass");
+    }
+}
+
 template <int Q, int STAGE>
 void launch_stage_pass(int64_t nb, const StepGeom& G, const StepBuffers& S, const PassCfg& c,
                        hipStream_t st) {
-    if (Q == 27 && c.unions && !c.tiles && records_direct()) {
-        if (c.ktmpl == 32)
-            k_stage_d<STAGE, 32, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);
-        else
-            k_stage_d<STAGE, 16, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);
-    } else if (Q == 27 && c.unions) {
-        const size_t lds = size_t(c.ucap) * 128u;
-        if (c.ktmpl == 32)
-            c.tiles ? k_stage_u<STAGE, 32, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B)
-                    : k_stage_u<STAGE, 32, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
-        else
-            c.tiles ? k_stage_u<STAGE, 16, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B)
-                    : k_stage_u<STAGE, 16, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
-    } else if (c.ktmpl == 32) {
+    if (c.ktmpl == 32) {
         launch_stage_scalar<Q, STAGE, 32>(nb, G, S, c.B, c.gbatch, st);
     } else if (c.ktmpl == 16) {
         launch_stage_scalar<Q, STAGE, 16>(nb, G, S, c.B, c.gbatch, st);
@@ -1089,46 +1104,35 @@ template <int Q>
 void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
                  const CollisionParams& cp, const PassCfg& c, hipStream_t st) {
     if (nb <= 0) return;
+    const bool v4 = Q == 27 && c.unions;
+    if (v4 && (pass == kP2 || pass == kP4 || (pass >= kS1 && pass <= kS4))) {
+        if (c.ktmpl == 48) launch_v4<48>(pass, nb, G, S, c, st);
+        else if (c.ktmpl == 32) launch_v4<32>(pass, nb, G, S, c, st);
+        else launch_v4<16>(pass, nb, G, S, c, st);
+        check_launch("step pass");
+        return;
+    }
     switch (pass) {
         case kP1: k_p1<Q><<<nb, kBlock, 0, st>>>(G, S, cp, c.B); break;
         case kPl:
-            if (Q == 27 && c.unions) k_pml_local_v4<<<nb, kBlock, 0, st>>>(G, S, c.B);
+            if (v4) k_pml_local_v4<<<nb, kBlock, 0, st>>>(G, S, c.B);
             else k_pml_local<Q><<<nb, kBlock, 0, st>>>(G, S, c.B);
             break;
         case kPm: k_means<Q><<<nb, 256, 0, st>>>(G, S, c.B); break;
         case kPf: k_pml_field<Q><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
         case kP2:
-            if (Q == 27 && c.unions) {
-                if (c.ktmpl == 32 && c.tiles) k_p2_v4<32, 8, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
-                else if (c.ktmpl == 32) k_p2_v4<32, 8, false><<<nb, kBlock, 0, st>>>(G, S, c.B);
-                else if (c.tiles) k_p2_v4<16, 8, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
-                else k_p2_v4<16, 8, false><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            } else if (c.ktmpl == 32) {
-                k_p2<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            } else if (c.ktmpl == 16) {
-                k_p2<Q, 16><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            } else {
-                k_p2<Q, 0><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            }
+            if (c.ktmpl == 32) k_p2<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else if (c.ktmpl == 16) k_p2<Q, 16><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else k_p2<Q, 0><<<nb, kBlock, 0, st>>>(G, S, c.B);
             break;
         case kP3:
-            if (Q == 27 && c.unions) k_binterp_v4<true><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            if (v4) k_binterp_v4<true><<<nb, kBlock, 0, st>>>(G, S, c.B);
             else k_binterp<Q, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
             break;
         case kP4:
-            if (Q == 27 && c.unions) {
-                const size_t lds = size_t(c.ucap) * 128u;
-                if (c.ktmpl == 32 && c.tiles) k_p4_u<32, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B);
-                else if (c.ktmpl == 32) k_p4_u<32, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
-                else if (c.tiles) k_p4_u<16, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B);
-                else k_p4_u<16, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
-            } else if (c.ktmpl == 32) {
-                k_p4<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            } else if (c.ktmpl == 16) {
-                k_p4<Q, 16><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            } else {
-                k_p4<Q, 0><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            }
+            if (c.ktmpl == 32) k_p4<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else if (c.ktmpl == 16) k_p4<Q, 16><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else k_p4<Q, 0><<<nb, kBlock, 0, st>>>(G, S, c.B);
             break;
         case kS1: launch_stage_pass<Q, 1>(nb, G, S, c, st); break;
         case kS2: launch_stage_pass<Q, 2>(nb, G, S, c, st); break;
@@ -1142,7 +1146,7 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
     check_launch("step pass");
 }
 
-int ktemplate(int k) { return k == 32 ? 32 : k == 16 ? 16 : 0; }
+int ktemplate(int k) { return (k == 48 || k == 32 || k == 16) ? k : 0; }
 
 }  // namespace
 

@@ -161,6 +161,7 @@ struct stencil_config {
     int rbf = NAGA_RBF_POLYHARMONIC3;
     int boundary_rbf = NAGA_RBF_MULTIQUADRIC;
     bool reference_pml_indexing = true;
+    int min_support_size = 0;  // > 0: per-node widths in [min, support_size] by density
 };
 
 // Declarative density source (naga_engine_add_density_source): while
@@ -212,6 +213,7 @@ class simulation_engine {
         cfg.rbf = stencil_.rbf;
         cfg.rbf_boundary = stencil_.boundary_rbf;
         cfg.pml_ref_index = stencil_.reference_pml_indexing ? 1 : 0;
+        cfg.k_min = stencil_.min_support_size;
         cfg.nu = double(params_.fluid_viscosity);
         cfg.rho0 = double(params_.nominal_density);
         cfg.dt = double(params_.time_step);

@@ -28,6 +28,7 @@ class Config(C.Structure):
         ("pml_ref_index", C.c_int), ("threads", C.c_int),
         ("nu", C.c_double), ("rho0", C.c_double), ("dt", C.c_double),
         ("c", C.c_double), ("h", C.c_double), ("filter", C.c_double),
+        ("kvar", C.c_void_p),
     ]
 
 
@@ -81,7 +82,7 @@ class Oracle:
                          C.c_int, C.c_double, _f64p])
         self._divw = fn("divergence_weights", C.c_int,
                         [C.c_int, _f32p, C.c_int64, _f32p, C.c_int64, _u32p,
-                         _f32p, C.c_int, C.c_int, _f64p, _f64p])
+                         _f32p, C.c_int, C.c_int, C.c_void_p, _f64p, _f64p])
         self._diva = fn("divergence_apply", C.c_int,
                         [C.c_int, C.c_int64, C.c_int, _u32p, _f64p, _f64p,
                          C.c_double, _f64p])
@@ -184,7 +185,7 @@ class Oracle:
             raise ValueError("singular RPIM system")
         return w.reshape(nq, k)
 
-    def divergence_weights(self, src, query, idx, d2, rbf):
+    def divergence_weights(self, src, query, idx, d2, rbf, kvar=None):
         src = np.ascontiguousarray(src, np.float32)
         query = np.ascontiguousarray(query, np.float32)
         idx = np.ascontiguousarray(idx, np.uint32)
@@ -193,7 +194,9 @@ class Oracle:
         D = src.shape[1]
         W = np.zeros(nq * k * D)
         delta = np.zeros(nq)
-        if self._divw(D, src, src.shape[0], query, nq, idx, d2, k, rbf, W, delta):
+        kv = None if kvar is None else np.ascontiguousarray(kvar, np.uint8)
+        if self._divw(D, src, src.shape[0], query, nq, idx, d2, k, rbf,
+                      None if kv is None else kv.ctypes.data, W, delta):
             raise ValueError("singular divergence system")
         return W.reshape(nq, k, D), delta
 
@@ -233,7 +236,7 @@ class OracleEngine:
                  rbf_boundary: int | None = None, tau: float = 0.6,
                  dt: float | None = None, h: float = 0.01, rho0: float = 1.0,
                  pml_ref_index: int = 1, threads: int = 0, c=None,
-                 filter_strength: float = 0.75):
+                 filter_strength: float = 0.75, kvar=None):
         self.o = oracle
         self.Q = Q
         self.D = 2 if Q == 9 else 3
@@ -248,6 +251,9 @@ class OracleEngine:
                      pml_ref_index=pml_ref_index, threads=threads,
                      nu=ph["nu"], rho0=rho0, dt=ph["dt"], c=ph["c"], h=h,
                      filter=filter_strength)
+        # per-node stencil widths; kept alive for the engine's setup
+        self._kvar = None if kvar is None else np.ascontiguousarray(kvar, np.uint8)
+        cfg.kvar = None if self._kvar is None else self._kvar.ctypes.data
         self.cfg = cfg
         pts = np.ascontiguousarray(nodes["points"], np.float32).reshape(-1)
         nrm = np.ascontiguousarray(nodes["normals"], np.float32).reshape(-1)

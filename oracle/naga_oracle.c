@@ -947,9 +947,11 @@ static void div_kernel(int D, double delta, const double* x, const double* y,
 int NO_FN(divergence_weights)(int D, const float* src, int64_t n_src,
                               const float* query, int64_t n_q,
                               const uint32_t* idx, const float* d2, int k,
-                              int rbf, double* W_out, double* delta_out) {
+                              int rbf, const unsigned char* kvar, double* W_out,
+                              double* delta_out) {
     (void)n_src;
-    const int n = k + D + 1;
+    const int kmax = k;
+    const int nmax = kmax + D + 1;
     const int Nq = D == 2 ? 16 : 64;
     /* operators.cuh:84-117: delta = 0.05 * nearest-neighbour distance; the
      * multiquadric shape parameter uses the mean nearest distance over each
@@ -961,23 +963,27 @@ int NO_FN(divergence_weights)(int D, const float* src, int64_t n_src,
     for (int64_t b = 0; b < nbatch; ++b) {
         int64_t e = (b + 1) * batch < n_q ? (b + 1) * batch : n_q;
         double s = 0;
-        for (int64_t i = b * batch; i < e; ++i) s += sqrt((double)d2[i * k + 1]);
+        for (int64_t i = b * batch; i < e; ++i) s += sqrt((double)d2[i * kmax + 1]);
         hbar[b] = s / (double)(e - b * batch);
     }
     int err = 0;
 #pragma omp parallel
     {
-        double* G0 = (double*)malloc(sizeof(double) * n * n);
-        double* B = (double*)malloc(sizeof(double) * n * Nq);
+        double* G0 = (double*)malloc(sizeof(double) * nmax * nmax);
+        double* B = (double*)malloc(sizeof(double) * nmax * Nq);
         double xs[64 * 3];
 #pragma omp for schedule(dynamic, 64)
         for (int64_t i = 0; i < n_q; ++i) {
+            /* per-query stencil width (variable stencils): the first k of
+             * the kmax neighbours, zero weights after them */
+            const int k = kvar ? (int)kvar[i] : kmax;
+            const int n = k + D + 1;
             double xi[3], xq[64][3];
             for (int d = 0; d < D; ++d) xi[d] = query[i * D + d];
             for (int s = 0; s < k; ++s)
                 for (int d = 0; d < D; ++d)
-                    xs[s * D + d] = src[(int64_t)idx[i * k + s] * D + d];
-            double delta = 0.05 * sqrt((double)d2[i * k + 1]);
+                    xs[s * D + d] = src[(int64_t)idx[i * kmax + s] * D + d];
+            double delta = 0.05 * sqrt((double)d2[i * kmax + 1]);
             double spacing = hbar[i / batch];
             if (delta_out) delta_out[i] = delta;
             /* quadrature_point_map (quadrature_points.cuh:238-263) */
@@ -999,8 +1005,8 @@ int NO_FN(divergence_weights)(int D, const float* src, int64_t n_src,
                 err = 2;
             }
             /* compute_divergence_weights (divergence.inl:161-222) */
-            double* Wi = W_out + i * (int64_t)k * D;
-            for (int t = 0; t < k * D; ++t) Wi[t] = 0.0;
+            double* Wi = W_out + i * (int64_t)kmax * D;
+            for (int t = 0; t < kmax * D; ++t) Wi[t] = 0.0;
             for (int q = 0; q < Nq; ++q) {
                 double a[3];
                 div_kernel(D, delta, xi, xq[q], a);
@@ -1412,7 +1418,7 @@ no_engine* NO_FN(engine_create)(const no_config* cfg, const float* pts,
     NO_FN(knn)(D, pts, N, pts, N, k, d2, e->idx);
     e->W64 = (double*)malloc(sizeof(double) * N * k * D);
     int err = NO_FN(divergence_weights)(D, pts, N, pts, N, e->idx, d2, k,
-                                        cfg->rbf, e->W64, NULL);
+                                        cfg->rbf, cfg->kvar, e->W64, NULL);
     e->W = (no_real*)malloc(sizeof(no_real) * N * k * D);
     for (int64_t t = 0; t < N * k * D; ++t) e->W[t] = (no_real)e->W64[t];
     /* filter weights: Gaussian of sigma_g = h/2.5 over the divergence
@@ -1423,15 +1429,17 @@ no_engine* NO_FN(engine_create)(const no_config* cfg, const float* pts,
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < N; ++i) {
             double gsum = 0, g[128], xi[3], xj[3];
+            const int ki = cfg->kvar ? (int)cfg->kvar[i] : k;
             for (int d = 0; d < D; ++d) xi[d] = pts[i * D + d];
-            for (int s = 0; s < k; ++s) {
+            for (int s = 0; s < ki; ++s) {
                 for (int d = 0; d < D; ++d)
                     xj[d] = pts[(int64_t)e->idx[i * k + s] * D + d];
                 double r = sqrt(d2_f64(D, xi, xj));
                 g[s] = exp(-0.5 * (r / sg) * (r / sg));
                 gsum += g[s];
             }
-            for (int s = 0; s < k; ++s) e->filt[i * k + s] = (no_real)(g[s] / gsum);
+            for (int s = 0; s < k; ++s)
+                e->filt[i * k + s] = s < ki ? (no_real)(g[s] / gsum) : (no_real)0;
         }
     }
     free(d2);

@@ -107,7 +107,8 @@ int NO_FN(rpim_weights)(int D, const float* src, const uint32_t* idx, int k,
 int NO_FN(divergence_weights)(int D, const float* src, int64_t n_src,
                               const float* query, int64_t n_q,
                               const uint32_t* idx, const float* d2, int k,
-                              int rbf, double* W_out, double* delta_out);
+                              int rbf, const unsigned char* kvar, double* W_out,
+                              double* delta_out);
 
 /* Apply a divergence operator to a vector field F[j*D + d]:
  * out[i] = tree-sum over s of sum_d W(d,s,i) (F_d(idx(s,i)) - off),
@@ -138,6 +139,7 @@ typedef struct no_config {
     double filter;      /* selective filter strength: 0.75 in the reference
                            (simulation_engine.cuh:1242); other values exist
                            only for tests of the scheme without it */
+    const unsigned char* kvar; /* optional per-node stencil widths <= k */
 } no_config;
 
 no_engine* NO_FN(engine_create)(const no_config* cfg, const float* pts,

@@ -223,3 +223,36 @@ def test_density_probes(oracle64, Q):
     obs = naga_amd.DensityObserver(gpu, probes[0])
     obs.advance(3)
     assert len(obs.density_values) == 3 and obs.time_values[1] > obs.time_values[0]
+
+
+@pytest.mark.parametrize("Q,kmax,kmin", [(27, 48, 24), (9, 32, 16)])
+def test_variable_stencils_parity(oracle64, Q, kmax, kmin):
+    """Box minus a sphere/circle with a refined shell and per-node stencil
+    widths by local density (k in [kmin, kmax]): the step matches the oracle
+    given the same widths."""
+    import naga_amd
+    D = 2 if Q == 9 else 3
+    h = 0.04 if D == 2 else 0.1
+    centre = [0.5] * D
+    nodes = naga_amd.ObstacleGridProvider([0] * D, [1] * D, h, [(centre, 0.2)], 3 * h, 0.5,
+                                          jitter=0.25, refine_width=0.1).get()
+    eng = naga_amd.SimulationEngine(lattice=Q, k=kmax, k_boundary=32 if D == 3 else 16,
+                                    rbf=naga_amd.RBF_POLYHARMONIC3,
+                                    rbf_boundary=naga_amd.RBF_POLYHARMONIC3, k_min=kmin)
+    p = naga_amd.problem_parameters(Q, 0.6, 0.4 * h)
+    eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+    eng.init_domain(nodes)
+    kv = eng.stencil_sizes()
+    assert kv.min() >= kmin and kv.max() <= kmax and len(np.unique(kv)) > 3
+    nd = nodes.as_dict()
+    ora = oracle64.engine(nd, Q=Q, k=kmax, k_boundary=32 if D == 3 else 16, rbf=1,
+                          rbf_boundary=1, h=h, kvar=kv)
+    pc = [0.2] + [0.5] * (D - 1)
+    eng.set_initial_pulse(pc, 1e-3, 0.1)
+    ora.set_pulse(pc, 1e-3, 0.1)
+    eng.step_forward(12)
+    ora.step(12)
+    g, o = eng.macroscopic(fresh=True), ora.get(fresh=True)
+    err_rho, err_u = rel_l2(g["drho"], o["drho"]), rel_l2(g["u"], o["u"])
+    print(f"variable k Q={Q}: rel_l2 drho={err_rho:.3e} u={err_u:.3e}")
+    assert err_rho < TOL and err_u < TOL

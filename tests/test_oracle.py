@@ -532,3 +532,20 @@ def test_density_source(oracle64):
     assert np.abs(late.get()["drho"]).max() < 1e-13
     late.step(1)
     assert np.abs(late.get()["drho"]).max() > 1e-6
+
+
+def test_variable_stencil_weights(oracle64):
+    """Per-node stencil widths: node i's weights are those of a fixed-width
+    operator over its first k_i neighbours, zero after them."""
+    nd = oracle64.uniform_grid(3, [0] * 3, [1] * 3, 0.125, jitter=0.25)
+    pts = nd["points"]
+    d2, idx = oracle64.knn(pts, pts, 32)
+    rng = np.random.default_rng(8)
+    kv = rng.integers(20, 33, len(pts)).astype(np.uint8)
+    W, _ = oracle64.divergence_weights(pts, pts, idx, d2, 1, kvar=kv)
+    for i in rng.choice(len(pts), 6, replace=False):
+        ki = int(kv[i])
+        Wi, _ = oracle64.divergence_weights(pts, pts[i:i + 1], idx[i:i + 1, :ki],
+                                            d2[i:i + 1, :ki], 1)
+        assert np.allclose(W[i, :ki], Wi[0], rtol=1e-12, atol=1e-12)
+        assert np.all(W[i, ki:] == 0)

@@ -87,3 +87,29 @@ def test_obstacle_grid_without_obstacles_matches_uniform_grid_nodes():
     pts = {tuple(p) for p in a.points.tolist()}
     exp = {tuple(np.float32(x).tolist()) for f, x in _grid_stream_points(2, h, 0.25, 0x4E414741)}
     assert pts == exp
+
+
+def test_obstacle_grid_refinement():
+    """refine_width doubles the node density in a shell around obstacles."""
+    h, R, w = 0.04, 0.15, 0.1
+    base = naga_amd.ObstacleGridProvider([0, 0], [1, 1], h, [([0.5, 0.5], R)], 0.0, 0.0,
+                                         jitter=0.25).get()
+    ref = naga_amd.ObstacleGridProvider([0, 0], [1, 1], h, [([0.5, 0.5], R)], 0.0, 0.0,
+                                        jitter=0.25, refine_width=w).get()
+    pb = base.points.astype(np.float64)
+    pr = ref.points.astype(np.float64)
+    gap_b = np.linalg.norm(pb - 0.5, axis=1) - R
+    gap_r = np.linalg.norm(pr - 0.5, axis=1) - R
+    far = 2 * w
+    # away from the shell nothing changes
+    assert np.sum(gap_b > far) == np.sum(gap_r > far)
+    # inside the shell the fluid density roughly quadruples (2D)
+    shell_b = np.sum((gap_b > 0.25 * h) & (gap_b < w - h))
+    shell_r = np.sum((gap_r > 0.25 * h) & (gap_r < w - h))
+    assert 3.0 < shell_r / shell_b < 5.0
+    # no fluid node inside the obstacle or closer than h/4 to it
+    nf = ref.num_bulk_points + ref.num_layer_points
+    assert gap_r[:nf].min() >= 0.25 * h - 1e-6
+    # surface nodes at half spacing
+    on = np.abs(gap_r[nf:]) < 1e-5
+    assert on.sum() == int(np.ceil(2 * np.pi * R / (0.5 * h)))
