@@ -64,9 +64,9 @@ def copy_calibration(nbytes: int) -> int:
 def stage_bytes_per_node(Q: int, D: int, k: int, stage: int) -> int:
     """Algorithmic bytes of one RK4 stage per fluid node: the stencil slab
     (k records of 4 + 4D bytes) plus FP32 rows of Q values: the gathered
-    input read once, the f0 row, the accumulator (read except stage 1,
-    written except stage 4) and the stage output."""
-    rows = {1: 4, 2: 5, 3: 5, 4: 4}[stage]  # gather, f0, acc r/w, out
+    input read once, the state row (the same array as the input in stage
+    1), the stage output, and in stage 4 the two earlier stage outputs."""
+    rows = {1: 2, 2: 3, 3: 3, 4: 5}[stage]
     return k * (4 + 4 * D) + rows * 4 * Q
 
 

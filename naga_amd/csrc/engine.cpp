@@ -224,7 +224,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         launch_tile_unions(nfluid, k, idx_.get(), nullptr, uoff_.get(), ulist_.get(),
                            slab_.get(), D, fslab_.get(), 1, st);
         const char* env = std::getenv("NAGA_UCAP");
-        const int cap = env ? std::max(0, std::min(std::atoi(env), 384)) : kUnionCapDefault;
+        const int cap = env ? std::max(0, std::min(std::atoi(env), 12 * 32)) : kUnionCapDefault;
         ucap_ = std::min(maxu, cap);
     }
 

@@ -598,10 +598,29 @@ __device__ __forceinline__ float4 rate_fetch(const float4* rp, Fetch fetch, int 
 }
 
 // Stage the union rows of a tile into LDS (8 lanes per 128-byte row).
+// All index loads are issued first, then all row loads, then the LDS
+// stores, so a thread has its whole share of rows in flight at once (one
+// round trip for the indices and one for the rows, not one per row).
+constexpr int kUnionPerThread = 12;  // rows per thread: ucap <= 12 * 32 = 384
 __device__ __forceinline__ void stage_union(float4* urows, const uint32_t* ul, int U,
                                             __amdgpu_buffer_rsrc_t src, int q) {
-    for (int r = threadIdx.x / kV4Lanes; r < U; r += kBlock / kV4Lanes)
-        urows[r * kV4Lanes + q] = ld4(src, ul[r] * 128u + uint32_t(q) * 16u);
+    constexpr int stride = kBlock / kV4Lanes;
+    const int r0 = threadIdx.x / kV4Lanes;
+    uint32_t j[kUnionPerThread];
+    float4 v[kUnionPerThread];
+#pragma unroll
+    for (int t = 0; t < kUnionPerThread; ++t) {
+        const int r = r0 + t * stride;
+        j[t] = r < U ? ul[r] : 0u;
+    }
+#pragma unroll
+    for (int t = 0; t < kUnionPerThread; ++t)
+        if (r0 + t * stride < U) v[t] = ld4(src, j[t] * 128u + uint32_t(q) * 16u);
+#pragma unroll
+    for (int t = 0; t < kUnionPerThread; ++t) {
+        const int r = r0 + t * stride;
+        if (r < U) urows[r * kV4Lanes + q] = v[t];
+    }
 }
 
 // RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
@@ -662,41 +681,6 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
                                      return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
                                  },
                                  q);
-    st4(gout, own, rk4_out4<STAGE>(g0, r, G.dt, y1, y2, y3));
-}
-
-// RK4 stage, D3Q27, four per lane, with each node's stencil records read
-// straight from HBM by its lanes (no LDS staging, no block barrier) and the
-// neighbour rows gathered directly.
-template <int STAGE, int K, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_stage_d(StepGeom G, StepBuffers S, BatchArgs B) {
-    const BlockCtx bc = locate(B, G, S);
-    const int64_t node0 = xcd_tile(bc.b, bc.nb) * kV4Nodes;
-    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
-    const int64_t i = node0 + slot;
-    if (i >= G.N) return;
-    const uint64_t rows = uint64_t(G.N) * 128u;
-    const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
-    const float4 g0 = ld4(rsrc(S.g, rows), own);
-    if (i >= G.nfluid) {  // boundary rows: constant through the stages
-        if (STAGE == 1) {
-            st4(S.A, own, g0);
-            st4(S.B, own, g0);
-            st4(S.acc, own, g0);
-        }
-        return;
-    }
-    const float* gin_p = stage_in(S, STAGE);
-    float* gout = stage_out(S, STAGE);
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 y1 = STAGE == 4 ? ld4(rsrc(S.A, rows), own) : z4;
-    const float4 y2 = STAGE == 4 ? ld4(rsrc(S.B, rows), own) : z4;
-    const float4 y3 = STAGE == 4 ? ld4(rsrc(S.acc, rows), own) : z4;
-    const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
-    const float4* rp = reinterpret_cast<const float4*>(S.slab) + i * K;
-    const uint32_t lane_off = uint32_t(q) * 16u;
-    const float4 r =
-        rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gin, j * 128u + lane_off); }, q);
     st4(gout, own, rk4_out4<STAGE>(g0, r, G.dt, y1, y2, y3));
 }
 
@@ -1050,23 +1034,12 @@ void launch_stage_scalar(int64_t nb, const StepGeom& G, const StepBuffers& S, Ba
         k_stage<Q, STAGE, KT, 8><<<nb, kBlock, 0, st>>>(G, S, B);
 }
 
-// NAGA_REC_DIRECT=1: D3Q27 stages read stencil records straight from HBM
-// (k_stage_d) instead of staging them in LDS.
-bool records_direct() {
-    static const bool on = [] {
-        const char* e = std::getenv("NAGA_REC_DIRECT");
-        return e && e[0] == '1';
-    }();
-    return on;
-}
-
 // The D3Q27 four-per-lane kernels of the stencil passes for one stencil
 // width K (16, 32 or 48).
 template <int K>
 void launch_v4(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S, const PassCfg& c,
                hipStream_t st) {
     const size_t lds = size_t(c.ucap) * 128u;
-    const bool direct = !c.tiles && records_direct();
     switch (pass) {
         case kP2:
             if (c.tiles) k_p2_v4<K, 8, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
@@ -1078,8 +1051,7 @@ void launch_v4(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S, co
             break;
 #define NAGA_V4_STAGE(ST)                                                            \
     case kS1 + ST - 1:                                                               \
-        if (direct) k_stage_d<ST, K, 8><<<nb, kBlock, 0, st>>>(G, S, c.B);          \
-        else if (c.tiles) k_stage_u<ST, K, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B); \
+        if (c.tiles) k_stage_u<ST, K, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B); \
         else k_stage_u<ST, K, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);      \
         break;
         NAGA_V4_STAGE(1) NAGA_V4_STAGE(2) NAGA_V4_STAGE(3) NAGA_V4_STAGE(4)
