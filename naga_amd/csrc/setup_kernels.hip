@@ -4,6 +4,10 @@
 
 #include "setup_kernels.hpp"
 
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 namespace naga_amd {
 
 namespace {
@@ -244,7 +248,10 @@ __device__ int wave_lu_solve(double (&row)[NMAX], double (&rhs)[NR], int n) {
     return lane < n ? pos : 64;
 }
 
-__constant__ double c_quad_unit[64][3];  // unscaled quadrature points
+// unscaled quadrature points, [0]: 2D (16 used), [1]: 3D (64); both
+// tables are written once per device and never change afterwards, so
+// scenes of either dimension can be set up concurrently
+__constant__ double c_quad_unit[2][64][3];
 __constant__ double c_quad_rw[4];        // radial Gauss weights
 
 __device__ __forceinline__ double dist2_f64(int D, const double* a, const double* b) {
@@ -363,7 +370,7 @@ __global__ __launch_bounds__(64) void k_divw(const float* src, const float* quer
     const int Nq = D == 2 ? 16 : 64;
     for (int q = 0; q < Nq; ++q) {
         double xq[3] = {0.0, 0.0, 0.0};
-        for (int d = 0; d < D; ++d) xq[d] = c_quad_unit[q][d] * delta + xi[d];
+        for (int d = 0; d < D; ++d) xq[d] = c_quad_unit[D - 2][q][d] * delta + xi[d];
         double aw[3];
         quad_term(D, delta, xi, xq, q, aw);
         const double gx = lane < n ? gx_entry(D, xs, k, rbf, spacing, lane, xq) : 0.0;
@@ -608,11 +615,11 @@ int pick_nmax(int n) {
     return -1;
 }
 
-void upload_quadrature(int D) {
+void fill_quadrature(int D, double (&tab)[64][3]) {
     static const double xr[4] = {0.339981, -0.339981, 0.861136, -0.861136};
-    static const double wr[4] = {0.652145, 0.652145, 0.347855, 0.347855};
-    double tab[64][3] = {};
     const int Nq = D == 2 ? 16 : 64;
+    for (int q = 0; q < 64; ++q)
+        for (int d = 0; d < 3; ++d) tab[q][d] = 0.0;
     for (int q = 0; q < Nq; ++q) {
         // nonlocal_calculus/detail/quadrature_points.cuh:74-91, :127-152
         const double r = (xr[q % 4] + 1.0) / 2.0;
@@ -629,8 +636,23 @@ void upload_quadrature(int D) {
             tab[q][2] = r * cos(ph);
         }
     }
+}
+
+// Uploads both quadrature tables once per device (thread-safe).
+void upload_quadrature(int /*D*/) {
+    static std::mutex mu;
+    static std::vector<int> done;  // devices already holding the tables
+    int dev = 0;
+    NAGA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if (std::find(done.begin(), done.end(), dev) != done.end()) return;
+    static const double wr[4] = {0.652145, 0.652145, 0.347855, 0.347855};
+    double tab[2][64][3];
+    fill_quadrature(2, tab[0]);
+    fill_quadrature(3, tab[1]);
     NAGA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_quad_unit), tab, sizeof(tab)));
     NAGA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_quad_rw), wr, sizeof(wr)));
+    done.push_back(dev);
 }
 
 template <int D>
