@@ -283,6 +283,51 @@ int naga_divergence_apply(const uint32_t* idx, const float* W, int64_t n,
                           int32_t k, int32_t dims, const float* F, int64_t n_f,
                           float off, float* out);
 
+/* ---- Standalone advection operator -------------------------------------
+ * advection_operator::create(domain) (nonlocal_calculus/advection.cuh:
+ * 143-151): exact kNN stencils of width k over the points (n x dims) and
+ * their divergence weights (rbf as in naga_scene_config).  The operator's
+ * state stays on the device. */
+typedef struct naga_advection naga_advection;
+int naga_advection_create(const float* points, int64_t n, int32_t dims, int32_t k,
+                          int32_t rbf, naga_advection** out);
+/* advection_operator::create(divergence_op) (advection.cuh:153-159) and the
+ * load half of save/load (:498-523): an operator over existing stencils
+ * idx (n x k, indices < n) and FP32 weights W (n x k x dims), e.g. those of
+ * naga_advection_weights or naga_divergence_weights. */
+int naga_advection_from_weights(const uint32_t* idx, const float* W, int64_t n,
+                                int32_t dims, int32_t k, naga_advection** out);
+int naga_advection_destroy(naga_advection* a);
+/* nsteps of advection_task (advection.cuh:162-276): classic RK4 of
+ * df/dt = div(-v (f - centering_offset)) with the divergence field map of
+ * advection.cuh:102-141.  velocity: dims values (constant field) or n x dims
+ * (per_node != 0).  f (n values) is read as f0 and overwritten by the result.
+ * boundary_start > 0: only nodes [0, boundary_start) are updated, the rest
+ * keep their value (the reference's end_slice); 0: every node. */
+int naga_advection_step(naga_advection* a, const float* velocity, int32_t per_node,
+                        float* f, double dt, double centering_offset,
+                        int64_t boundary_start, int32_t nsteps);
+/* stencil indices (n x k) and FP32 weights (n x k x dims); either may be
+ * NULL (the save half of save/load, advection.cuh:498-514) */
+int naga_advection_weights(naga_advection* a, uint32_t* idx, float* W);
+
+/* ---- CSR matrices -------------------------------------------------------
+ * linalg::matrix<T, sparse_csr> (linalg/matrix.cuh:231-437), device
+ * resident, FP32 values.  naga_csr_from_stencil is
+ * create_from_index_stencil(columns, index_stencil, values)
+ * (matrix.cuh:300-368): row r holds the k entries idx[r*k + i] with values
+ * values[r*k + i], in stencil order (the reference's per-row sort leaves the
+ * order unchanged).  Column indices must be < cols; rows and cols < 2^31. */
+typedef struct naga_csr naga_csr;
+int naga_csr_create(int64_t rows, int64_t cols, const int64_t* row_offsets,
+                    const uint32_t* column_indices, const float* values, naga_csr** out);
+int naga_csr_from_stencil(int64_t rows, int64_t cols, int32_t k, const uint32_t* idx,
+                          const float* values, naga_csr** out);
+int naga_csr_destroy(naga_csr* m);
+/* matrix_mult<csr, dense, dense> (matrix.cuh:438-629): y = alpha A x + beta y; x has cols
+ * entries, y rows; y is only read when beta != 0. */
+int naga_csr_mult(naga_csr* m, float alpha, float beta, const float* x, float* y);
+
 #ifdef __cplusplus
 }
 #endif

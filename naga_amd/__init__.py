@@ -136,6 +136,15 @@ def load() -> C.CDLL:
         "naga_engine_add_probes": (C.c_int, [vp, vp, i64, i32, C.POINTER(i32)]),
         "naga_engine_sample_probes": (C.c_int, [vp, i32, i32, vp]),
         "naga_engine_get_stencil_sizes": (C.c_int, [vp, vp]),
+        "naga_advection_create": (C.c_int, [vp, i64, i32, i32, i32, C.POINTER(vp)]),
+        "naga_advection_from_weights": (C.c_int, [vp, vp, i64, i32, i32, C.POINTER(vp)]),
+        "naga_advection_destroy": (C.c_int, [vp]),
+        "naga_advection_step": (C.c_int, [vp, vp, i32, vp, f64, f64, i64, i32]),
+        "naga_advection_weights": (C.c_int, [vp, vp, vp]),
+        "naga_csr_create": (C.c_int, [i64, i64, vp, vp, vp, C.POINTER(vp)]),
+        "naga_csr_from_stencil": (C.c_int, [i64, i64, i32, vp, vp, C.POINTER(vp)]),
+        "naga_csr_destroy": (C.c_int, [vp]),
+        "naga_csr_mult": (C.c_int, [vp, C.c_float, C.c_float, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -162,7 +171,9 @@ EXPORTED = [
     "naga_server_release", "naga_server_drain", "naga_server_get_stats",
     "naga_batch_autotune", "naga_engine_state_size", "naga_engine_save_state",
     "naga_engine_load_state", "naga_engine_add_probes", "naga_engine_sample_probes",
-    "naga_engine_get_stencil_sizes",
+    "naga_engine_get_stencil_sizes", "naga_advection_create", "naga_advection_from_weights",
+    "naga_advection_destroy", "naga_advection_step", "naga_advection_weights",
+    "naga_csr_create", "naga_csr_from_stencil", "naga_csr_destroy", "naga_csr_mult",
 ]
 
 STEP_KERNELS = ["p1_collide", "pl_pml_local", "pm_layer_means", "pf_pml_field",
@@ -656,6 +667,144 @@ class DivergenceOperator:
         _check(load().naga_divergence_apply(_ptr(self._idx), _ptr(W32), n, k, D, _ptr(F),
                                             F.shape[0], float(centering_offset), _ptr(out)),
                "naga::nonlocal_calculus::divergence_operator::")
+        return out
+
+
+class ConstantVelocityField:
+    """constant_velocity_field (nonlocal_calculus/advection.cuh:58-85): the
+    same velocity at every node."""
+
+    def __init__(self, velocity):
+        self.velocity = _f32(velocity).reshape(-1)
+
+
+class AdvectionOperator:
+    """nonlocal_calculus::advection_operator<T, D> (nonlocal_calculus/
+    advection.cuh:88-524): RK4 steps of df/dt = div(-v (f - offset)) over a
+    divergence operator; stencils and weights stay on the device."""
+
+    _PREFIX = "naga::nonlocal_calculus::advection_operator::"
+
+    def __init__(self, handle, n, dims, k):
+        self._h = handle
+        self._n = int(n)
+        self._D = int(dims)
+        self._k = int(k)
+
+    @classmethod
+    def create(cls, domain, k: int = 32, rbf: int = RBF_POLYHARMONIC3):
+        """advection_operator::create(domain) (advection.cuh:143-151)."""
+        domain = _f32(domain)
+        n, D = domain.shape
+        h = C.c_void_p()
+        _check(load().naga_advection_create(_ptr(domain), n, D, int(k), int(rbf), C.byref(h)),
+               cls._PREFIX)
+        return cls(h, n, D, k)
+
+    @classmethod
+    def from_divergence(cls, div_op: "DivergenceOperator"):
+        """advection_operator::create(divergence_op) (advection.cuh:153-159)."""
+        W = _f32(div_op.weights())
+        idx = np.ascontiguousarray(div_op.support_indices(), np.uint32)
+        n, k, D = W.shape
+        h = C.c_void_p()
+        _check(load().naga_advection_from_weights(_ptr(idx), _ptr(W), n, D, k, C.byref(h)),
+               cls._PREFIX)
+        return cls(h, n, D, k)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.naga_advection_destroy(h)
+            self._h = None
+
+    def divergence_op(self) -> "DivergenceOperator":
+        """The operator's stencils and FP32 weights (advection.cuh:494-496)."""
+        n, k, D = self._n, self._k, self._D
+        W = np.zeros(n * k * D, np.float32)
+        idx = np.zeros(n * k, np.uint32)
+        _check(load().naga_advection_weights(self._h, _ptr(idx), _ptr(W)), self._PREFIX)
+        return DivergenceOperator(W.reshape(n, k, D), idx.reshape(n, k))
+
+    def step_forward(self, velocity_field, f0, dt, centering_offset=0.0,
+                     boundary_idx_start=0, steps=1):
+        """step_forward (advection.cuh:460-492): `steps` advection_task RK4
+        steps (advection.cuh:207-276) from f0; returns f.  velocity_field:
+        ConstantVelocityField or an (n, D) per-node array.  Nodes at or past boundary_idx_start (> 0)
+        keep their f0 value."""
+        f = _f32(f0).copy()
+        if f.shape != (self._n,):
+            raise ValueError(self._PREFIX + "f0 must have the same number of elements as the "
+                             "domain for which this operator was created.")
+        if isinstance(velocity_field, ConstantVelocityField):
+            v, per = velocity_field.velocity, 0
+            if v.shape != (self._D,):
+                raise ValueError(self._PREFIX + "velocity has incorrect dimensions.")
+        else:
+            v, per = _f32(velocity_field), 1
+            if v.shape != (self._n, self._D):
+                raise ValueError(self._PREFIX + "velocity field has incorrect shape.")
+        _check(load().naga_advection_step(self._h, _ptr(v), per, _ptr(f), float(dt),
+                                          float(centering_offset), int(boundary_idx_start),
+                                          int(steps)), self._PREFIX)
+        return f
+
+
+class CsrMatrix:
+    """linalg::matrix<T, sparse_csr> (linalg/matrix.cuh:231-437), device
+    resident, FP32 values."""
+
+    _PREFIX = "naga::linalg::matrix::"
+
+    def __init__(self, handle, rows, cols):
+        self._h = handle
+        self.rows = int(rows)
+        self.columns = int(cols)
+
+    @classmethod
+    def create(cls, rows, cols, row_offsets, column_indices, values):
+        rp = np.ascontiguousarray(row_offsets, np.int64)
+        ci = np.ascontiguousarray(column_indices, np.uint32)
+        v = _f32(values)
+        if rp.shape != (rows + 1,) or ci.shape != v.shape or ci.size != rp[-1]:
+            raise ValueError(cls._PREFIX + "inconsistent CSR arrays")
+        h = C.c_void_p()
+        _check(load().naga_csr_create(rows, cols, _ptr(rp), _ptr(ci), _ptr(v), C.byref(h)),
+               cls._PREFIX)
+        return cls(h, rows, cols)
+
+    @classmethod
+    def create_from_index_stencil(cls, index_stencil, values, columns=None):
+        """create_from_index_stencil (matrix.cuh:300-380): row r holds the
+        stencil entries index_stencil[r] with values[r], in stencil order;
+        square unless `columns` is given."""
+        idx = np.ascontiguousarray(index_stencil, np.uint32)
+        v = _f32(values)
+        if idx.ndim != 2 or v.shape != idx.shape:
+            raise ValueError(cls._PREFIX + "index stencil and values must have the same shape")
+        rows, k = idx.shape
+        cols = rows if columns is None else int(columns)
+        h = C.c_void_p()
+        _check(load().naga_csr_from_stencil(rows, cols, k, _ptr(idx), _ptr(v), C.byref(h)),
+               cls._PREFIX)
+        return cls(h, rows, cols)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.naga_csr_destroy(h)
+            self._h = None
+
+    def mult(self, x, y=None, alpha=1.0, beta=0.0):
+        """matrix_mult (matrix.cuh:438-629): alpha A x + beta y."""
+        x = _f32(x)
+        if x.shape != (self.columns,):
+            raise ValueError(self._PREFIX + "x has incorrect size")
+        out = np.zeros(self.rows, np.float32) if y is None else _f32(y).copy()
+        if out.shape != (self.rows,):
+            raise ValueError(self._PREFIX + "y has incorrect size")
+        _check(load().naga_csr_mult(self._h, float(alpha), float(beta), _ptr(x), _ptr(out)),
+               self._PREFIX)
         return out
 
 

@@ -2,11 +2,13 @@
 // to status translation, thread-local error text.
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
 
 #include "../../include/naga_amd.h"
+#include "calculus.hpp"
 #include "engine.hpp"
 #include "server.hpp"
 #include "setup_kernels.hpp"
@@ -23,6 +25,14 @@ struct naga_batch {
 
 struct naga_server {
     std::unique_ptr<Server> impl;
+};
+
+struct naga_advection {
+    std::unique_ptr<Advection> impl;
+};
+
+struct naga_csr {
+    std::unique_ptr<Csr> impl;
 };
 
 namespace {
@@ -853,6 +863,88 @@ int naga_divergence_apply(const uint32_t* idx, const float* W, int64_t n, int32_
         dout.download(out, n, s);
         NAGA_HIP(hipStreamSynchronize(s));
         (void)hipStreamDestroy(s);
+    });
+}
+
+int naga_advection_create(const float* points, int64_t n, int32_t dims, int32_t k, int32_t rbf,
+                          naga_advection** out) {
+    return guarded([&] {
+        require(points && out, "null argument");
+        *out = nullptr;
+        auto a = std::make_unique<naga_advection>();
+        a->impl = std::make_unique<Advection>(points, n, dims, k, rbf);
+        *out = a.release();
+    });
+}
+
+int naga_advection_from_weights(const uint32_t* idx, const float* W, int64_t n, int32_t dims,
+                                int32_t k, naga_advection** out) {
+    return guarded([&] {
+        require(out, "null argument");
+        *out = nullptr;
+        auto a = std::make_unique<naga_advection>();
+        a->impl = std::make_unique<Advection>(idx, W, n, dims, k);
+        *out = a.release();
+    });
+}
+
+int naga_advection_destroy(naga_advection* a) {
+    return guarded([&] { delete a; });
+}
+
+int naga_advection_step(naga_advection* a, const float* velocity, int32_t per_node, float* f,
+                        double dt, double centering_offset, int64_t boundary_start,
+                        int32_t nsteps) {
+    return guarded([&] {
+        require(a && a->impl, "null advection operator");
+        require(boundary_start >= 0 && boundary_start <= a->impl->size(),
+                "boundary_start out of range");
+        a->impl->step(velocity, per_node != 0, f, dt, centering_offset, boundary_start, nsteps);
+    });
+}
+
+int naga_advection_weights(naga_advection* a, uint32_t* idx, float* W) {
+    return guarded([&] {
+        require(a && a->impl, "null advection operator");
+        a->impl->weights(idx, W);
+    });
+}
+
+int naga_csr_create(int64_t rows, int64_t cols, const int64_t* row_offsets,
+                    const uint32_t* column_indices, const float* values, naga_csr** out) {
+    return guarded([&] {
+        require(out, "null argument");
+        *out = nullptr;
+        auto m = std::make_unique<naga_csr>();
+        m->impl = std::make_unique<Csr>(rows, cols, row_offsets, column_indices, values);
+        *out = m.release();
+    });
+}
+
+int naga_csr_from_stencil(int64_t rows, int64_t cols, int32_t k, const uint32_t* idx,
+                          const float* values, naga_csr** out) {
+    return guarded([&] {
+        require(idx && values && out, "null argument");
+        require(k >= 1, "stencil width must be >= 1");
+        require(rows <= int64_t(INT32_MAX) && cols <= int64_t(INT32_MAX),
+                "rows and columns must fit in int");
+        *out = nullptr;
+        std::vector<int64_t> rp(rows + 1);
+        for (int64_t r = 0; r <= rows; ++r) rp[r] = r * k;
+        auto m = std::make_unique<naga_csr>();
+        m->impl = std::make_unique<Csr>(rows, cols, rp.data(), idx, values);
+        *out = m.release();
+    });
+}
+
+int naga_csr_destroy(naga_csr* m) {
+    return guarded([&] { delete m; });
+}
+
+int naga_csr_mult(naga_csr* m, float alpha, float beta, const float* x, float* y) {
+    return guarded([&] {
+        require(m && m->impl, "null matrix");
+        m->impl->mult(alpha, beta, x, y);
     });
 }
 

@@ -209,6 +209,62 @@ class Oracle:
         self._diva(D, n, k, idx, W, F, off, out)
         return out
 
+    def advection_rate(self, idx, W, v, f, off):
+        """rate = div(F), F_j = v_j (-(f_j - off)): the divergence field map of
+        nonlocal_calculus/advection.cuh:102-141 applied with
+        divergence_operator::apply."""
+        v = np.asarray(v, np.float64)
+        n = idx.shape[0]
+        if v.ndim == 1:
+            v = np.broadcast_to(v, (n, v.shape[0]))
+        F = v * (-(np.asarray(f, np.float64) - off))[:, None]
+        return self.divergence_apply(idx, W, F)
+
+    def advection_step(self, idx, W, v, f0, dt, off=0.0, boundary_start=0, steps=1):
+        """advection_task::impl (nonlocal_calculus/advection.cuh:207-276) in
+        FP64: k1 = rate(f0); f = f0 + dt a_i k_i on [0, end) with
+        a = (1/2, 1/2, 1) (:231-237, :568), k_{i+1} = rate(f) (:246);
+        f = f0 + dt sum b_i k_i on [0, end), b = (1, 2, 2, 1)/6 (:245-276,
+        :569-570); end = boundary_start when > 0, else every node.  Nodes at
+        and past end keep f0 (assign_array(f0, f), :223)."""
+        a = (0.5, 0.5, 1.0)
+        b = (1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0)
+        f0 = np.asarray(f0, np.float64).copy()
+        n = f0.shape[0]
+        end = boundary_start if boundary_start > 0 else n
+        for _ in range(steps):
+            ks = [self.advection_rate(idx, W, v, f0, off)]
+            f = f0.copy()
+            for i in range(3):
+                f[:end] = f0[:end] + dt * a[i] * ks[i][:end]
+                ks.append(self.advection_rate(idx, W, v, f, off))
+            total = sum(bi * ki for bi, ki in zip(b, ks))
+            f[:end] = f0[:end] + dt * total[:end]
+            f0 = f
+        return f0
+
+    @staticmethod
+    def csr_from_stencil(idx, values):
+        """create_from_index_stencil (linalg/matrix.cuh:300-368): row offsets
+        k r, row r's entries in stencil order (the per-row sort writes entry i
+        back to position i, :337-351)."""
+        n, k = idx.shape
+        rp = np.arange(n + 1, dtype=np.int64) * k
+        return rp, np.asarray(idx, np.int64).reshape(-1), np.asarray(values, np.float64).reshape(-1)
+
+    @staticmethod
+    def csr_mult(rp, ci, vals, x, alpha=1.0, beta=0.0, y=None):
+        """y = alpha A x + beta y (matrix_mult, linalg/matrix.cuh:438-629)."""
+        rows = rp.shape[0] - 1
+        out = np.zeros(rows)
+        for r in range(rows):
+            e0, e1 = rp[r], rp[r + 1]
+            out[r] = np.dot(vals[e0:e1], np.asarray(x, np.float64)[ci[e0:e1]])
+        out *= alpha
+        if beta != 0.0:
+            out += beta * np.asarray(y, np.float64)
+        return out
+
     # --- engine ---------------------------------------------------------
     def engine(self, nodes, **kw):
         return OracleEngine(self, nodes, **kw)

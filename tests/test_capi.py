@@ -76,3 +76,22 @@ def test_engine_shape_errors():
                                     np.zeros(0), 8, 0, 2, 0.01)
     with pytest.raises(ValueError, match="boundary normals"):
         eng.init_domain(bad2)
+
+
+def test_csr_and_advection_argument_errors():
+    """Shape and index errors are reported before any device work."""
+    import ctypes as C
+    lib = naga_amd.load()
+    idx = np.array([[0, 5]], np.uint32)
+    vals = np.ones((1, 2), np.float32)
+    h = C.c_void_p()
+    st = lib.naga_csr_from_stencil(1, 5, 2, idx.ctypes.data_as(C.c_void_p),
+                                   vals.ctypes.data_as(C.c_void_p), C.byref(h))
+    assert st == 1 and b"column index" in lib.naga_last_error()
+    assert not h.value
+    W = np.ones((1, 2, 2), np.float32)
+    st = lib.naga_advection_from_weights(idx.ctypes.data_as(C.c_void_p),
+                                         W.ctypes.data_as(C.c_void_p), 1, 2, 2, C.byref(h))
+    assert st == 1 and b"stencil index" in lib.naga_last_error()
+    with pytest.raises(ValueError):
+        naga_amd.CsrMatrix.create_from_index_stencil(idx, np.ones((2, 2)))

@@ -549,3 +549,92 @@ def test_variable_stencil_weights(oracle64):
                                             d2[i:i + 1, :ki], 1)
         assert np.allclose(W[i, :ki], Wi[0], rtol=1e-12, atol=1e-12)
         assert np.all(W[i, ki:] == 0)
+
+
+# -------------------------------------------------------------- advection
+def _advection_matrix(idx, W, v):
+    """Dense L with rate(f) = L (f - off), assembled entry by entry."""
+    n, k, D = W.shape
+    L = np.zeros((n, n))
+    for i in range(n):
+        for s in range(k):
+            L[i, idx[i, s]] -= np.dot(W[i, s], v)
+    return L
+
+
+@pytest.mark.parametrize("end", [0, 70])
+def test_advection_rk4_closed_form(oracle64, end):
+    """For a constant velocity the advection task (advection.cuh:207-276) is
+    RK4 on the linear system dg/dt = P L g, g = f - off, P the projection
+    onto [0, end): one step is the degree-4 Taylor polynomial of exp(dt P L),
+    and nodes past end keep f0."""
+    o = oracle64
+    nd = o.uniform_grid(2, [0, 0], [1, 1], 0.1, jitter=0.25)
+    pts = nd["points"]
+    d2, idx = o.knn(pts, pts, 12)
+    W, _ = o.divergence_weights(pts, pts, idx, d2, 1)
+    n = len(pts)
+    v = np.array([0.3, -0.2])
+    rng = np.random.default_rng(5)
+    f0 = rng.standard_normal(n)
+    dt, off = 0.02, 0.25
+    P = np.eye(n)
+    if end:
+        P[end:, end:] = 0.0
+    M = dt * P @ _advection_matrix(idx, W, v)
+    g = f0 - off
+    want, term = g.copy(), g.copy()
+    for m in range(1, 5):
+        term = M @ term / m
+        want += term
+    want += off
+    got = o.advection_step(idx, W, v, f0, dt, off, boundary_start=end)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    if end:
+        assert np.array_equal(got[end:], f0[end:])
+    # two steps are two applications of the same polynomial
+    got2 = o.advection_step(idx, W, v, f0, dt, off, boundary_start=end, steps=2)
+    one = o.advection_step(idx, W, v, got, dt, off, boundary_start=end)
+    assert np.abs(got2 - one).max() < 1e-12 * np.abs(one).max()
+
+
+def test_advection_translates_a_pulse(oracle64):
+    """df/dt = -v . grad f carries a smooth pulse with velocity v."""
+    o = oracle64
+    nd = o.uniform_grid(2, [0, 0], [1, 1], 0.02, jitter=0.25)
+    pts = nd["points"].astype(np.float64)
+    d2, idx = o.knn(nd["points"], nd["points"], 16)
+    W, _ = o.divergence_weights(nd["points"], nd["points"], idx, d2, 1)
+    v = np.array([0.5, 0.25])
+    width, c0 = 0.08, np.array([0.35, 0.4])
+    f0 = np.exp(-np.sum((pts - c0) ** 2, 1) / width ** 2)
+    T, dt = 0.4, 0.01
+    f = o.advection_step(idx, W, v, f0, dt, 0.0, steps=int(round(T / dt)))
+    c1 = c0 + v * T
+    want = np.exp(-np.sum((pts - c1) ** 2, 1) / width ** 2)
+    inside = np.all((pts > 0.15) & (pts < 0.85), axis=1)
+    err = np.linalg.norm((f - want)[inside]) / np.linalg.norm(want[inside])
+    assert err < 0.06
+    com = (f[inside, None] * pts[inside]).sum(0) / f[inside].sum()
+    assert np.abs(com - c1).max() < 0.01
+
+
+# -------------------------------------------------------------------- CSR
+def test_csr_from_stencil_vs_scipy(oracle64):
+    """create_from_index_stencil keeps stencil order (duplicate columns
+    included) and its product equals SciPy's CSR product."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(9)
+    n, k, cols = 200, 7, 150
+    idx = rng.integers(0, cols, (n, k)).astype(np.uint32)
+    idx[3, :3] = 11  # duplicate columns in one row
+    vals = rng.standard_normal((n, k))
+    rp, ci, cv = oracle64.csr_from_stencil(idx, vals)
+    assert np.array_equal(rp, np.arange(n + 1) * k)
+    assert np.array_equal(ci, idx.reshape(-1).astype(np.int64))
+    A = sp.csr_matrix((vals.reshape(-1), idx.reshape(-1).astype(np.int64), rp),
+                      shape=(n, cols))
+    x = rng.standard_normal(cols)
+    y = rng.standard_normal(n)
+    got = oracle64.csr_mult(rp, ci, cv, x, alpha=0.5, beta=-2.0, y=y)
+    assert np.allclose(got, 0.5 * (A @ x) - 2.0 * y, rtol=1e-12, atol=1e-12)
