@@ -23,6 +23,7 @@
 
 #include <array>
 #include <cstdint>
+#include <future>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -420,6 +421,62 @@ inline std::vector<int32_t> batched_lu_solve(int32_t n, int64_t batch,
                "naga::linalg::batched_matrix_inverse: ");
     return info;
 }
+
+// linalg::matrix<T, sparse_csr> (linalg/matrix.cuh:231-437): device-resident
+// CSR matrix with FP32 values.
+class sparse_csr_matrix {
+  public:
+    // create_from_index_stencil(columns, index_stencil, values)
+    // (matrix.cuh:300-368): index_stencil and values are [row][k]; row r
+    // keeps its k entries in stencil order.
+    static sparse_csr_matrix create_from_index_stencil(int64_t columns,
+                                                       const std::vector<uint32_t>& index_stencil,
+                                                       const std::vector<float>& values,
+                                                       int32_t k) {
+        if (k <= 0 || index_stencil.size() % size_t(k) != 0 ||
+            values.size() != index_stencil.size())
+            throw std::invalid_argument(
+                "naga::linalg::matrix::create_from_index_stencil: index stencil and values "
+                "must be [rows][k]");
+        sparse_csr_matrix m;
+        m.rows_ = int64_t(index_stencil.size() / size_t(k));
+        m.cols_ = columns;
+        naga_csr* h = nullptr;
+        abi::check(naga_csr_from_stencil(m.rows_, columns, k, index_stencil.data(),
+                                         values.data(), &h),
+                   "naga::linalg::matrix::create_from_index_stencil: ");
+        m.h_.reset(h, [](naga_csr* p) { naga_csr_destroy(p); });
+        return m;
+    }
+    // square form (matrix.cuh:370-380)
+    static sparse_csr_matrix create_from_index_stencil(const std::vector<uint32_t>& index_stencil,
+                                                       const std::vector<float>& values,
+                                                       int32_t k) {
+        return create_from_index_stencil(int64_t(index_stencil.size() / size_t(k > 0 ? k : 1)),
+                                         index_stencil, values, k);
+    }
+    int64_t rows() const { return rows_; }
+    int64_t columns() const { return cols_; }
+    naga_csr* handle() const { return h_.get(); }
+
+  private:
+    std::shared_ptr<naga_csr> h_;
+    int64_t rows_ = 0, cols_ = 0;
+};
+
+// matrix_mult (matrix.cuh:438-629): y = alpha A x + beta y
+inline void matrix_mult(float alpha, float beta, const sparse_csr_matrix& A,
+                        const std::vector<float>& x, std::vector<float>& y) {
+    if (int64_t(x.size()) != A.columns())
+        throw std::invalid_argument("naga::linalg::matrix_mult: x has incorrect size");
+    y.resize(size_t(A.rows()));
+    abi::check(naga_csr_mult(A.handle(), alpha, beta, x.data(), y.data()),
+               "naga::linalg::matrix_mult: ");
+}
+inline void matrix_mult(const sparse_csr_matrix& A, const std::vector<float>& x,
+                        std::vector<float>& y) {
+    matrix_mult(1.0f, 0.0f, A, x, y);
+}
 }  // namespace linalg
 
 namespace interpolation {
@@ -472,12 +529,118 @@ class divergence_operator {
 
     const std::vector<float>& weights() const { return W_; }
     const std::vector<uint32_t>& support_indices() const { return idx_; }
+    int64_t size() const { return n_; }
+    uint32_t support_size() const { return k_; }
+
+    // from stencils [i][k] and weights [i][k][d] (the operator's load half)
+    static divergence_operator from_weights(std::vector<uint32_t> idx, std::vector<float> W,
+                                            uint32_t k) {
+        divergence_operator op;
+        op.k_ = k;
+        op.n_ = k ? int64_t(idx.size() / k) : 0;
+        op.idx_ = std::move(idx);
+        op.W_ = std::move(W);
+        return op;
+    }
 
   private:
     uint32_t k_ = 0;
     int64_t n_ = 0;
     std::vector<uint32_t> idx_;
     std::vector<float> W_;
+};
+
+// constant_velocity_field (nonlocal_calculus/advection.cuh:58-85)
+template <class T, uint32_t Dimensions>
+class constant_velocity_field {
+  public:
+    template <class Velocity>
+    static constant_velocity_field create(Velocity&& v) {
+        constant_velocity_field m;
+        for (uint32_t d = 0; d < Dimensions; ++d) m.v_[d] = float(v[d]);
+        return m;
+    }
+    const float* data() const { return v_; }
+
+  private:
+    float v_[Dimensions] = {};
+};
+
+// advection_operator (nonlocal_calculus/advection.cuh:88-524).  Stencils and
+// weights live on the device; step_forward runs synchronously and returns a
+// ready future so callers written against the reference's asynchronous
+// executor keep working.
+template <class T, uint32_t Dimensions>
+class advection_operator {
+  public:
+    static advection_operator create(const std::vector<float>& domain, uint32_t k = 32,
+                                     int rbf = NAGA_RBF_POLYHARMONIC3) {
+        advection_operator op;
+        op.n_ = int64_t(domain.size() / Dimensions);
+        op.k_ = k;
+        naga_advection* h = nullptr;
+        abi::check(naga_advection_create(domain.data(), op.n_, int32_t(Dimensions), int32_t(k),
+                                         rbf, &h),
+                   "naga::nonlocal_calculus::advection_operator::create: ");
+        op.h_.reset(h, [](naga_advection* p) { naga_advection_destroy(p); });
+        return op;
+    }
+    static advection_operator create(const divergence_operator<T, Dimensions>& div) {
+        advection_operator op;
+        op.n_ = div.size();
+        op.k_ = div.support_size();
+        naga_advection* h = nullptr;
+        abi::check(naga_advection_from_weights(div.support_indices().data(),
+                                               div.weights().data(), op.n_,
+                                               int32_t(Dimensions), int32_t(op.k_), &h),
+                   "naga::nonlocal_calculus::advection_operator::create: ");
+        op.h_.reset(h, [](naga_advection* p) { naga_advection_destroy(p); });
+        return op;
+    }
+
+    // constant velocity
+    std::future<void> step_forward(const constant_velocity_field<T, Dimensions>& v,
+                                   const std::vector<float>& f0, std::vector<float>& f, T dt,
+                                   T centering_offset = T(0), size_t boundary_idx_start = 0) {
+        return run(v.data(), 0, f0, f, dt, centering_offset, boundary_idx_start);
+    }
+    // per-node velocity [i][d]
+    std::future<void> step_forward(const std::vector<float>& velocity,
+                                   const std::vector<float>& f0, std::vector<float>& f, T dt,
+                                   T centering_offset = T(0), size_t boundary_idx_start = 0) {
+        if (velocity.size() != size_t(n_) * Dimensions)
+            throw std::invalid_argument(
+                "naga::nonlocal_calculus::advection_operator::velocity field has incorrect size");
+        return run(velocity.data(), 1, f0, f, dt, centering_offset, boundary_idx_start);
+    }
+
+    divergence_operator<T, Dimensions> divergence_op() const {
+        std::vector<uint32_t> idx(size_t(n_) * k_);
+        std::vector<float> W(size_t(n_) * k_ * Dimensions);
+        abi::check(naga_advection_weights(h_.get(), idx.data(), W.data()),
+                   "naga::nonlocal_calculus::advection_operator::divergence_op: ");
+        return divergence_operator<T, Dimensions>::from_weights(std::move(idx), std::move(W), k_);
+    }
+
+  private:
+    std::future<void> run(const float* v, int per_node, const std::vector<float>& f0,
+                          std::vector<float>& f, T dt, T off, size_t end) {
+        if (int64_t(f0.size()) != n_)
+            throw std::invalid_argument(
+                "naga::nonlocal_calculus::advection_operator::f0 must have the same number of "
+                "elements as the domain for which this operator was created.");
+        f = f0;
+        abi::check(naga_advection_step(h_.get(), v, per_node, f.data(), double(dt), double(off),
+                                       int64_t(end), 1),
+                   "naga::nonlocal_calculus::advection_operator::step_forward: ");
+        std::promise<void> done;
+        done.set_value();
+        return done.get_future();
+    }
+
+    std::shared_ptr<naga_advection> h_;
+    int64_t n_ = 0;
+    uint32_t k_ = 0;
 };
 }  // namespace nonlocal_calculus
 

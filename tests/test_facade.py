@@ -56,3 +56,10 @@ def test_facade_state_source_observer(facade_exe):
     r = subprocess.run([facade_exe, "extras"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "extras ok" in r.stdout
+
+
+@pytest.mark.gpu
+def test_facade_advection_and_csr(facade_exe):
+    r = subprocess.run([facade_exe, "operators"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "operators ok" in r.stdout
