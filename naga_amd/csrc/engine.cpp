@@ -200,11 +200,12 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     ucap_ = 0;
     ulist_ = DeviceBuffer<uint32_t>();
     uoff_ = DeviceBuffer<int64_t>();
-    // Tile unions staged in LDS measured slower than direct gathers on the
-    // 1M-node D3Q27 scene (0.61 vs 0.31 ms per RK4 stage: three blocks per
-    // CU and a load-then-compute phase per tile); NAGA_UNIONS=1 enables them.
+    // Tile unions: the gather kernels copy each 32-node tile's distinct
+    // stencil neighbours into LDS by LDS DMA and read them from there
+    // (0.22 against 0.27 ms per RK4 stage with direct gathers on the
+    // 1M-node D3Q27 scene); NAGA_UNIONS=0 disables them.
     const char* un_env = std::getenv("NAGA_UNIONS");
-    if (un_env && un_env[0] == '1' && step_uses_unions(Q, k, N) && nfluid > 0) {
+    if (!(un_env && un_env[0] == '0') && step_uses_unions(Q, k, N) && nfluid > 0) {
         // tile unions; slab index fields become union positions
         const int64_t ntiles = (nfluid + kTileNodes - 1) / kTileNodes;
         DeviceBuffer<int32_t> counts(ntiles);
@@ -266,6 +267,8 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     }
     // fluid nodes whose stencil reaches the absorbing layer
     n_pml_ = 0;
+    n_ptiles_ = 0;
+    ptiles_ = DeviceBuffer<int32_t>();
     if (nlayer > 0) {
         DeviceBuffer<uint8_t> flag(nfluid);
         launch_flag_neighbours(nfluid, k, idx_.get(), nbulk, nfluid, flag.get(), st);
@@ -277,6 +280,21 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
             if (hflag[i]) list.push_back(int32_t(i));
         n_pml_ = int64_t(list.size());
         pml_list_ = to_device(list, st);
+        // with tile unions the PML divergence runs over whole tiles: every
+        // tile holding a listed node (the others add an exact zero)
+        if (ulist_.get()) {
+            std::vector<int32_t> tl;
+            for (int64_t t0 = 0; t0 < nfluid; t0 += kTileNodes) {
+                const int64_t t1 = std::min<int64_t>(t0 + kTileNodes, nfluid);
+                for (int64_t i = t0; i < t1; ++i)
+                    if (hflag[i]) {
+                        tl.push_back(int32_t(t0 / kTileNodes));
+                        break;
+                    }
+            }
+            n_ptiles_ = int64_t(tl.size());
+            ptiles_ = to_device(tl.empty() ? std::vector<int32_t>{0} : tl, st);
+        }
         sigma_.allocate(nlayer);
         sigma_.upload(sigma, nlayer, st);
     }
@@ -347,6 +365,8 @@ StepBuffers Engine::buffers() {
     S.bsrc = bsrc_.get();
     S.pml_list = pml_list_.get();
     S.n_pml = n_pml_;
+    S.ptiles = ptiles_.get();
+    S.n_ptiles = n_ptiles_;
     S.rho = rho_.get();
     S.u = u_.get();
     S.partial = partial_.get();

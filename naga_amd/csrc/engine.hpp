@@ -116,6 +116,7 @@ class Engine {
     DeviceBuffer<uint8_t> kvar_;  // per-node stencil widths (empty: all k)
     DeviceBuffer<float> W_, filt_, bw_;
     DeviceBuffer<int32_t> pml_list_;
+    DeviceBuffer<int32_t> ptiles_;
     DeviceBuffer<uint8_t> bsrc_;
     DeviceBuffer<float> coll_, Gf_;
     DeviceBuffer<float4> ru_;
@@ -127,6 +128,7 @@ class Engine {
     DeviceBuffer<float> means_;
     int nparts_ = 1;
     int64_t n_pml_ = 0;
+    int64_t n_ptiles_ = 0;
 
     DeviceBuffer<uint32_t> ulist_;  // tile unions (four-per-lane D3Q27 kernels)
     DeviceBuffer<int64_t> uoff_;

@@ -531,7 +531,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S, Bat
 // zero instead of faulting.
 // ---------------------------------------------------------------------------
 constexpr int kV4Lanes = 8;                   // lanes per node
-constexpr int kV4Nodes = kBlock / kV4Lanes;  // nodes per block
+constexpr int kV4Nodes = kBlock / kV4Lanes;  // nodes per 256-thread block
+constexpr int kV4DefaultNodes = 32;          // nodes per block of the stencil kernels
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint64_t bytes) {
     const uint32_t n = bytes > 0xffffffffull ? 0xffffffffu : uint32_t(bytes);
@@ -597,41 +598,84 @@ __device__ __forceinline__ float4 rate_fetch(const float4* rp, Fetch fetch, int 
     return r;
 }
 
-// Stage the union rows of a tile into LDS (8 lanes per 128-byte row).
-// All index loads are issued first, then all row loads, then the LDS
-// stores, so a thread has its whole share of rows in flight at once (one
-// round trip for the indices and one for the rows, not one per row).
-constexpr int kUnionPerThread = 12;  // rows per thread: ucap <= 12 * 32 = 384
+// LDS DMA (gfx950 buffer_load ... lds): one wave instruction copies 64
+// 16-byte pieces, lane l's piece landing at lds_base + 16 l whatever its
+// source offset, with no VGPR round trip; the base must be wave-uniform.
+// Offsets past the buffer resource's size read as zero.
+typedef __attribute__((address_space(3))) void lds_void_t;
+__device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t src, uint32_t off, float4* lds_base) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(src, (lds_void_t*)(lds_base), 16, off,
+                                            0, 0, 0);
+}
+
+// Copy n16 <= NMAX contiguous 16-byte pieces of src (from offset 0) to dst.
+template <int NMAX>
+__device__ __forceinline__ void dma_contiguous(__amdgpu_buffer_rsrc_t src, float4* dst, int n16) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int nw = kBlock / 64;
+    constexpr int per_wave = (NMAX + 64 * nw - 1) / (64 * nw);
+#pragma unroll
+    for (int t = 0; t < per_wave; ++t) {
+        const int c = wave + t * nw;
+        if (c * 64 + lane < n16) dma16(src, uint32_t(c * 64 + lane) * 16u, dst + c * 64);
+    }
+}
+
+// The union range [u0, u0 + U) of tile t, loaded through the scalar cache
+// (wave-uniform values in SGPRs, off the vector memory counter).
+__device__ __forceinline__ void union_range(const int64_t* uoff, int64_t t, int64_t& u0, int& U) {
+    typedef const __attribute__((address_space(4))) int64_t const_i64;
+    const_i64* p = (const_i64*)(uoff);
+    u0 = p[t];
+    U = int(p[t + 1] - u0);
+}
+
+// Stage the U union rows of a tile into LDS (urows[r * 8 + q] = quarter q
+// of row ul[r]): eight rows per wave instruction.  Every index is loaded
+// before the first copy is issued, so a wave waits for one index round
+// trip and keeps all its copies in flight together.
+constexpr int kUnionChunks = 12;  // 8-row chunks per wave: ucap <= 12 * 8 * 4 = 384
 __device__ __forceinline__ void stage_union(float4* urows, const uint32_t* ul, int U,
-                                            __amdgpu_buffer_rsrc_t src, int q) {
-    constexpr int stride = kBlock / kV4Lanes;
-    const int r0 = threadIdx.x / kV4Lanes;
-    uint32_t j[kUnionPerThread];
-    float4 v[kUnionPerThread];
+                                            __amdgpu_buffer_rsrc_t src) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int nw = kBlock / 64;
+    const uint32_t q16 = uint32_t(lane & 7) * 16u;
+    // indices past U read as zero through the buffer resource: the loads
+    // are unconditional and all in flight before the first wait
+    const __amdgpu_buffer_rsrc_t ur = rsrc(ul, uint64_t(U) * 4u);
+    uint32_t j[kUnionChunks];
 #pragma unroll
-    for (int t = 0; t < kUnionPerThread; ++t) {
-        const int r = r0 + t * stride;
-        j[t] = r < U ? ul[r] : 0u;
+    for (int t = 0; t < kUnionChunks; ++t) {
+        const int r = (wave + t * nw) * 8 + (lane >> 3);
+        j[t] = __builtin_amdgcn_raw_buffer_load_b32(ur, uint32_t(r) * 4u, 0, 0);
     }
 #pragma unroll
-    for (int t = 0; t < kUnionPerThread; ++t)
-        if (r0 + t * stride < U) v[t] = ld4(src, j[t] * 128u + uint32_t(q) * 16u);
-#pragma unroll
-    for (int t = 0; t < kUnionPerThread; ++t) {
-        const int r = r0 + t * stride;
-        if (r < U) urows[r * kV4Lanes + q] = v[t];
+    for (int t = 0; t < kUnionChunks; ++t) {
+        const int c = wave + t * nw;
+        if (c * 8 < U && c * 8 + (lane >> 3) < U) dma16(src, j[t] * 128u + q16, urows + c * 64);
     }
+}
+
+// Wait for this wave's LDS DMA copies, then for the whole block.
+__device__ __forceinline__ void dma_fence() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
 }
 
 // RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
 // union staged in LDS (unions larger than ucap rows read through the list).
-template <int STAGE, int K, int BATCH, bool TU>
-__global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, BatchArgs B) {
-    __shared__ float4 srec[kV4Nodes * K];
+template <int STAGE, int K, int BATCH, bool TU, int NPB>
+__global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuffers S,
+                                                            BatchArgs B) {
+    static_assert(!TU || NPB == kV4Nodes, "tile unions are built for 32-node tiles");
+    // record rows padded by one entry: the eight nodes of a wavefront read
+    // their records from distinct LDS banks
+    constexpr int KP = K + 1;
+    __shared__ float4 srec[NPB * KP];
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
     const int64_t tile = xcd_tile(bc.b, bc.nb);
-    const int64_t node0 = tile * kV4Nodes;
+    const int64_t node0 = tile * NPB;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
@@ -645,21 +689,28 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     const float4 y2 = STAGE == 4 ? ld4(rsrc(S.B, rows), own) : z4;
     const float4 y3 = STAGE == 4 ? ld4(rsrc(S.acc, rows), own) : z4;
     const int64_t nf = G.nfluid - node0;
-    const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+    const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
     int U = 0;
     const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
+    // records: padded rows (distinct banks per node) copied through VGPRs,
+    // or, with tile unions, unpadded rows copied by LDS DMA with the rows
+    const int kp = TU ? K : KP;
     if (cnt > 0) {
-        const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
-        for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
-        if (ul) {
-            const int64_t u0 = S.uoff[tile];
-            U = int(S.uoff[tile + 1] - u0);
+        if (!TU) {
+            const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
+            for (int t = threadIdx.x; t < cnt * K; t += NPB * kV4Lanes) srec[t + t / K] = src[t];
+        } else {
+            int64_t u0;
+            union_range(S.uoff, tile, u0, U);
             ul += u0;
-            if (U <= S.ucap) stage_union(urows, ul, U, gin, q);
+            dma_contiguous<NPB * K>(rsrc(S.slab + node0 * K * 4, uint64_t(cnt) * K * 16u), srec,
+                                    cnt * K);
+            if (U <= S.ucap) stage_union(urows, ul, U, gin);
         }
     }
-    __syncthreads();
+    if (TU) dma_fence();
+    else __syncthreads();
     if (STAGE == 1 && !fluid && i < G.N) {
         st4(S.A, own, g0);
         st4(S.B, own, g0);
@@ -670,13 +721,13 @@ __global__ __launch_bounds__(kBlock) void k_stage_u(StepGeom G, StepBuffers S, B
     float4 r;
     if (!ul)  // records hold node indices
         r = rate_fetch<K, BATCH>(
-            srec + slot * K,
+            srec + slot * kp,
             [&](uint32_t j) { return ld4(gin, j * 128u + uint32_t(q) * 16u); }, q);
     else if (U <= S.ucap)
-        r = rate_fetch<K, BATCH>(srec + slot * K,
+        r = rate_fetch<K, BATCH>(srec + slot * kp,
                                  [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
     else
-        r = rate_fetch<K, BATCH>(srec + slot * K,
+        r = rate_fetch<K, BATCH>(srec + slot * kp,
                                  [&](uint32_t j) {
                                      return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
                                  },
@@ -707,13 +758,16 @@ __device__ __forceinline__ float4 filter_sum(const float2* rp, const float4& si,
 
 // p4 for D3Q27 with the tile unions: selective filter on fluid nodes,
 // bounce-back on boundary nodes.
-template <int K, int BATCH, bool TU>
-__global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, BatchArgs B) {
-    __shared__ float2 srec[kV4Nodes * K];
+template <int K, int BATCH, bool TU, int NPB>
+__global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers S, BatchArgs B) {
+    static_assert(!TU || NPB == kV4Nodes, "tile unions are built for 32-node tiles");
+    constexpr int KP = K + 1;  // padded record rows (distinct banks per node)
+    __shared__ float4 srec4[(NPB * KP + 1) / 2];
+    float2* srec = reinterpret_cast<float2*>(srec4);
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
     const int64_t tile = xcd_tile(bc.b, bc.nb);
-    const int64_t node0 = tile * kV4Nodes;
+    const int64_t node0 = tile * NPB;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
@@ -721,24 +775,31 @@ __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, Batc
     const __amdgpu_buffer_rsrc_t s1 = rsrc(S.A, rows);
     const float4 si = ld4(s1, own);
     const int64_t nf = G.nfluid - node0;
-    const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+    const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     int U = 0;
     const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
+    // records: padded rows through VGPRs, or with tile unions unpadded rows
+    // by LDS DMA (two 8-byte records per 16-byte piece; K is even)
+    const int kp = TU ? K : KP;
     if (cnt > 0) {
-        const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
-        for (int t = threadIdx.x; t < cnt * K; t += kBlock) srec[t] = src[t];
-        if (ul) {
-            const int64_t u0 = S.uoff[tile];
-            U = int(S.uoff[tile + 1] - u0);
+        if (!TU) {
+            const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
+            for (int t = threadIdx.x; t < cnt * K; t += NPB * kV4Lanes) srec[t + t / K] = src[t];
+        } else {
+            int64_t u0;
+            union_range(S.uoff, tile, u0, U);
             ul += u0;
-            if (U <= S.ucap) stage_union(urows, ul, U, s1, q);
+            dma_contiguous<NPB * K / 2>(rsrc(S.fslab + node0 * K * 2, uint64_t(cnt) * K * 8u),
+                                        srec4, cnt * K / 2);
+            if (U <= S.ucap) stage_union(urows, ul, U, s1);
         }
     }
-    __syncthreads();
+    if (TU) dma_fence();
+    else __syncthreads();
     if (i >= G.N) return;
     float4 out;
     if (i < G.nfluid) {
-        const float2* rp = srec + slot * K;
+        const float2* rp = srec + slot * kp;
         const uint32_t lane_off = uint32_t(q) * 16u;
         float4 sum;
         if (!ul)
@@ -767,15 +828,17 @@ __global__ __launch_bounds__(kBlock) void k_p4_u(StepGeom G, StepBuffers S, Batc
 
 // p2 for D3Q27 with four distributions per lane (PML divergence over the
 // listed nodes, added to s1).
-template <int K, int BATCH, bool TU>
-__global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S, BatchArgs B) {
-    __shared__ float4 srec[kV4Nodes * K];
+template <int K, int BATCH, bool TU, int NPB>
+__global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_v4(StepGeom G, StepBuffers S, BatchArgs B) {
+    static_assert(!TU || NPB == kV4Nodes, "tile unions are built for 32-node tiles");
+    constexpr int KP = K + 1;  // padded record rows (distinct banks per node)
+    __shared__ float4 srec[NPB * KP];
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const BlockCtx bc = locate(B, G, S);
-    const int64_t item = xcd_tile(bc.b, bc.nb) * kV4Nodes + slot;
+    const int64_t item = xcd_tile(bc.b, bc.nb) * NPB + slot;
     const bool valid = item < S.n_pml;
     const int64_t i = valid ? S.pml_list[item] : 0;
-    float4* rp = srec + slot * K;
+    float4* rp = srec + slot * KP;
     if (valid) {
         const float4* src = reinterpret_cast<const float4*>(S.slab) + i * K;
         for (int r = q; r < K; r += kV4Lanes) rp[r] = src[r];
@@ -797,6 +860,49 @@ __global__ __launch_bounds__(kBlock) void k_p2_v4(StepGeom G, StepBuffers S, Bat
     // the divergence of F = c Gf is the negated rate of Gf
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const float4 s = ld4(rsrc(S.A, rows), own);
+    st4(S.A, own, make_float4(s.x - r.x, s.y - r.y, s.z - r.z, s.w - r.w));
+}
+
+// p2 for D3Q27 with tile unions: the PML divergence over every fluid node
+// of each listed tile (tiles holding a node with a layer neighbour; for the
+// other nodes of such a tile every neighbour's auxiliary row is zero and
+// the added divergence is an exact zero), the tile's union rows of Gf and
+// its records staged by LDS DMA.
+template <int K, int BATCH>
+__global__ __launch_bounds__(kBlock) void k_p2_t(StepGeom G, StepBuffers S, BatchArgs B) {
+    __shared__ float4 srec[kV4Nodes * K];
+    extern __shared__ float4 urows[];
+    const BlockCtx bc = locate(B, G, S);
+    typedef const __attribute__((address_space(4))) int32_t const_i32;
+    const int64_t tile = ((const_i32*)S.ptiles)[xcd_tile(bc.b, bc.nb)];
+    const int64_t node0 = tile * kV4Nodes;
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const int64_t i = node0 + slot;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
+    const int64_t nf = G.nfluid - node0;
+    const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+    const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
+    const __amdgpu_buffer_rsrc_t ar = rsrc(S.A, rows);
+    int64_t u0;
+    int U;
+    union_range(S.uoff, tile, u0, U);
+    const uint32_t* ul = S.ulist + u0;
+    dma_contiguous<kV4Nodes * K>(rsrc(S.slab + node0 * K * 4, uint64_t(cnt) * K * 16u), srec,
+                                 cnt * K);
+    if (U <= S.ucap) stage_union(urows, ul, U, gf);
+    const float4 s = ld4(ar, own);
+    dma_fence();
+    if (slot >= cnt) return;
+    const float4* rp = srec + slot * K;
+    const uint32_t lane_off = uint32_t(q) * 16u;
+    float4 r;
+    if (U <= S.ucap)
+        r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
+    else
+        r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + lane_off); },
+                                 q);
+    // the divergence of F = c Gf is the negated rate of Gf
     st4(S.A, own, make_float4(s.x - r.x, s.y - r.y, s.z - r.z, s.w - r.w));
 }
 
@@ -1006,6 +1112,21 @@ bool step_uses_unions(int Q, int k, int64_t N) {
     return Q == 27 && on && (k == 48 || k == 32 || k == 16) && N < (int64_t(1) << 25);
 }
 
+// Nodes per block of the D3Q27 stencil kernels: 32, 64 or 128 (records of
+// all of them staged in at most 64 KB of LDS); 32 when tile unions are
+// built.  NAGA_V4_NPB overrides the default.
+int v4_nodes_per_block(int k, bool tiles) {
+    static const int env = [] {
+        const char* e = std::getenv("NAGA_V4_NPB");
+        const int v = e ? std::atoi(e) : kV4DefaultNodes;
+        return (v == 32 || v == 64 || v == 128) ? v : kV4DefaultNodes;
+    }();
+    if (tiles) return kV4Nodes;
+    int v = env;
+    while (v > kV4Nodes && v * (k + 1) * 16 > 65536) v /= 2;
+    return v;
+}
+
 namespace {
 
 // Passes of one step, in launch order.
@@ -1035,29 +1156,45 @@ void launch_stage_scalar(int64_t nb, const StepGeom& G, const StepBuffers& S, Ba
 }
 
 // The D3Q27 four-per-lane kernels of the stencil passes for one stencil
-// width K (16, 32 or 48).
+// width K (16, 32 or 48) and NPB nodes per block.
+template <int K, int NPB>
+void launch_v4_npb(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
+                   const PassCfg& c, hipStream_t st) {
+    constexpr int threads = NPB * kV4Lanes;
+    const size_t lds = size_t(c.ucap) * 128u;
+    if constexpr (NPB == kV4Nodes) {
+        if (c.tiles) {
+            switch (pass) {
+                case kP2: k_p2_t<K, 8><<<nb, threads, lds, st>>>(G, S, c.B); return;
+                case kP4: k_p4_u<K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+                case kS1: k_stage_u<1, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+                case kS2: k_stage_u<2, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+                case kS3: k_stage_u<3, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+                case kS4: k_stage_u<4, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+                default: fail(kInvalidArgument, "not a stencil pass");
+            }
+        }
+    }
+    switch (pass) {
+        case kP2: k_p2_v4<K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
+        case kP4: k_p4_u<K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
+        case kS1: k_stage_u<1, K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
+        case kS2: k_stage_u<2, K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
+        case kS3: k_stage_u<3, K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
+        case kS4: k_stage_u<4, K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
+        default: fail(kInvalidArgument, "not a stencil pass");
+    }
+}
+
 template <int K>
 void launch_v4(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S, const PassCfg& c,
                hipStream_t st) {
-    const size_t lds = size_t(c.ucap) * 128u;
-    switch (pass) {
-        case kP2:
-            if (c.tiles) k_p2_v4<K, 8, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            else k_p2_v4<K, 8, false><<<nb, kBlock, 0, st>>>(G, S, c.B);
-            break;
-        case kP4:
-            if (c.tiles) k_p4_u<K, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B);
-            else k_p4_u<K, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);
-            break;
-#define NAGA_V4_STAGE(ST)                                                            \
-    case kS1 + ST - 1:                                                               \
-        if (c.tiles) k_stage_u<ST, K, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B); \
-        else k_stage_u<ST, K, 8, false><<<nb, kBlock, lds, st>>>(G, S, c.B);      \
-        break;
-        NAGA_V4_STAGE(1) NAGA_V4_STAGE(2) NAGA_V4_STAGE(3) NAGA_V4_STAGE(4)
-#undef NAGA_V4_STAGE
-        default: fail(kInvalidArgument, "not a stencil pass");
-    }
+    const int npb = v4_nodes_per_block(K, c.tiles);
+    if constexpr ((K + 1) * 128 * 16 <= 65536)
+        if (npb == 128) return launch_v4_npb<K, 128>(pass, nb, G, S, c, st);
+    if constexpr ((K + 1) * 64 * 16 <= 65536)
+        if (npb == 64) return launch_v4_npb<K, 64>(pass, nb, G, S, c, st);
+    launch_v4_npb<K, kV4Nodes>(pass, nb, G, S, c, st);
 }
 
 template <int Q, int STAGE>
@@ -1134,15 +1271,19 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
     const bool un = step_uses_unions(Q, G.k, G.N);
     const int npb = kBlock / 64 * (Q == 27 ? Lattice<27>::NPW : Lattice<9>::NPW);
     auto lanes = [&](int64_t n) { return (n + npb - 1) / npb; };
-    auto tiles = [&](int64_t n) { return (n + kV4Nodes - 1) / kV4Nodes; };
+    const int64_t tn = un ? v4_nodes_per_block(G.k, S.ulist != nullptr) : kV4Nodes;
+    auto tiles = [&](int64_t n) { return (n + tn - 1) / tn; };
+    auto tiles32 = [&](int64_t n) { return (n + kV4Nodes - 1) / kV4Nodes; };
     const bool pml = G.nlayer > 0 && S.n_rq > 0;
     const int64_t nbnd = G.N - G.nfluid;
     nb[kP1] = (G.N + kBlock - 1) / kBlock;
     nb[kPl] = pml ? S.nparts : 0;
     nb[kPm] = pml ? 2 * Q : 0;
     nb[kPf] = pml ? lanes(G.nlayer) : 0;
-    nb[kP2] = (pml && S.n_pml > 0) ? (un ? tiles(S.n_pml) : lanes(S.n_pml)) : 0;
-    nb[kP3] = nbnd > 0 ? (un ? tiles(nbnd) : lanes(nbnd)) : 0;
+    nb[kP2] = (pml && S.n_pml > 0)
+                  ? (un ? (S.ulist ? S.n_ptiles : tiles(S.n_pml)) : lanes(S.n_pml))
+                  : 0;
+    nb[kP3] = nbnd > 0 ? (un ? tiles32(nbnd) : lanes(nbnd)) : 0;
     nb[kP4] = un ? tiles(G.N) : lanes(G.N);
     nb[kS1] = un ? tiles(G.N) : lanes(G.N);
     nb[kS2] = nb[kS3] = nb[kS4] = un ? tiles(G.nfluid) : lanes(G.nfluid);

@@ -66,6 +66,8 @@ struct StepBuffers {
     const uint8_t* bsrc;  // [nfluid] 1 if the node feeds the boundary interpolator
     const int32_t* pml_list;  // fluid nodes with a layer neighbour
     int64_t n_pml;
+    const int32_t* ptiles;    // with tile unions: the tiles holding a pml_list node
+    int64_t n_ptiles;
     float* rho;      // [N] density (reference semantics, FP32)
     float* u;        // [D][N] velocity
     double* partial; // [nparts][2Q] layer column sums (one row per pl block)
@@ -142,7 +144,7 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 // tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
 // positions: D3Q27, k in {16, 32}, fewer than 2^25 nodes.
 bool step_uses_unions(int Q, int k, int64_t N);
-constexpr int kUnionCapDefault = 296;  // LDS rows per block (3 blocks per CU)
+constexpr int kUnionCapDefault = 272;  // LDS rows per block (3 blocks per CU)
 
 // Per-kernel entry points (used by the profiler and the standalone C-ABI).
 template <int Q>
