@@ -493,6 +493,18 @@ __global__ void k_make_slab(int64_t n, int k, int nc, const uint32_t* idx,
     for (int c = 0; c < nc; ++c) r[1 + c] = w[t * nc + c];
 }
 
+// dst row i = src row i (k records of rec floats) followed by kstride - k
+// zero records.
+__global__ void k_pad_records(int64_t n, int k, int rec, int kstride, const float* src,
+                              float* dst) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n * kstride * rec) return;
+    const int64_t i = t / (int64_t(kstride) * rec);
+    const int r = int(t - i * kstride * rec);
+    const int s = r / rec, c = r - s * rec;
+    dst[t] = s < k ? src[(i * k + s) * rec + c] : 0.0f;
+}
+
 // One block per tile: the tile's k * kTileNodes (<= 2048) neighbour indices
 // are sorted in LDS (bitonic), flagged where they differ from their
 // predecessor and compacted by a block-wide scan of the flags.
@@ -801,6 +813,14 @@ void launch_make_slab(int64_t n, int k, int nc, const uint32_t* idx, const float
     if (n == 0) return;
     k_make_slab<<<(n * k + 255) / 256, 256, 0, st>>>(n, k, nc, idx, w, slab);
     hip_check(hipGetLastError(), "slab");
+}
+
+void launch_pad_records(int64_t n, int k, int rec, int kstride, const float* src, float* dst,
+                        hipStream_t st) {
+    if (n == 0) return;
+    const int64_t tot = n * kstride * rec;
+    k_pad_records<<<(tot + 255) / 256, 256, 0, st>>>(n, k, rec, kstride, src, dst);
+    hip_check(hipGetLastError(), "pad records");
 }
 
 void launch_tile_unions(int64_t nfluid, int k, const uint32_t* idx, int32_t* counts,

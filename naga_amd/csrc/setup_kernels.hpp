@@ -63,6 +63,12 @@ void launch_filter_weights(int D, const float* pts, int64_t n,
 void launch_make_slab(int64_t n, int k, int ncomp, const uint32_t* idx,
                       const float* w, float* slab, hipStream_t stream);
 
+// Copy n rows of k records (rec floats each) into rows of kstride >= k
+// records, the extra records zero (padded rows for bank-conflict-free LDS
+// reads of the records of consecutive nodes).
+void launch_pad_records(int64_t n, int k, int rec, int kstride, const float* src, float* dst,
+                        hipStream_t stream);
+
 // Tile unions: fluid nodes are grouped in tiles of kTileNodes consecutive
 // nodes; a tile's union is the sorted list of the distinct neighbour
 // indices of its nodes' k-stencils.  Counting pass (ulist == nullptr):

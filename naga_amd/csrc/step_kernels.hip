@@ -693,9 +693,10 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuff
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
     int U = 0;
     const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
-    // records: padded rows (distinct banks per node) copied through VGPRs,
-    // or, with tile unions, unpadded rows copied by LDS DMA with the rows
-    const int kp = TU ? K : KP;
+    // records: rows padded by one record (distinct banks per node), copied
+    // through VGPRs, or with tile unions by LDS DMA from rows padded so in HBM
+    static_assert(KP == K + kSlabPad, "padded slab rows");
+    const int kp = KP;
     if (cnt > 0) {
         if (!TU) {
             const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
@@ -704,8 +705,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuff
             int64_t u0;
             union_range(S.uoff, tile, u0, U);
             ul += u0;
-            dma_contiguous<NPB * K>(rsrc(S.slab + node0 * K * 4, uint64_t(cnt) * K * 16u), srec,
-                                    cnt * K);
+            dma_contiguous<NPB * KP>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u),
+                                     srec, cnt * KP);
             if (U <= S.ucap) stage_union(urows, ul, U, gin);
         }
     }
@@ -761,8 +762,11 @@ __device__ __forceinline__ float4 filter_sum(const float2* rp, const float4& si,
 template <int K, int BATCH, bool TU, int NPB>
 __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers S, BatchArgs B) {
     static_assert(!TU || NPB == kV4Nodes, "tile unions are built for 32-node tiles");
-    constexpr int KP = K + 1;  // padded record rows (distinct banks per node)
-    __shared__ float4 srec4[(NPB * KP + 1) / 2];
+    // padded record rows (distinct banks per node): K + 1 records through
+    // VGPRs, or K + kFslabPad by LDS DMA from rows padded so in HBM
+    constexpr int KP = K + 1, KF = K + kFslabPad;
+    static_assert(KF % 2 == 0, "16-byte record pairs");
+    __shared__ float4 srec4[(NPB * KF) / 2];
     float2* srec = reinterpret_cast<float2*>(srec4);
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
@@ -778,9 +782,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
     const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     int U = 0;
     const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
-    // records: padded rows through VGPRs, or with tile unions unpadded rows
-    // by LDS DMA (two 8-byte records per 16-byte piece; K is even)
-    const int kp = TU ? K : KP;
+    const int kp = TU ? KF : KP;
     if (cnt > 0) {
         if (!TU) {
             const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
@@ -789,8 +791,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
             int64_t u0;
             union_range(S.uoff, tile, u0, U);
             ul += u0;
-            dma_contiguous<NPB * K / 2>(rsrc(S.fslab + node0 * K * 2, uint64_t(cnt) * K * 8u),
-                                        srec4, cnt * K / 2);
+            dma_contiguous<NPB * KF / 2>(rsrc(S.fslab + node0 * KF * 2, uint64_t(cnt) * KF * 8u),
+                                         srec4, cnt * KF / 2);
             if (U <= S.ucap) stage_union(urows, ul, U, s1);
         }
     }
@@ -870,7 +872,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_v4(StepGeom G, StepBuffer
 // its records staged by LDS DMA.
 template <int K, int BATCH>
 __global__ __launch_bounds__(kBlock) void k_p2_t(StepGeom G, StepBuffers S, BatchArgs B) {
-    __shared__ float4 srec[kV4Nodes * K];
+    constexpr int KP = K + kSlabPad;  // padded slab rows
+    __shared__ float4 srec[kV4Nodes * KP];
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
     typedef const __attribute__((address_space(4))) int32_t const_i32;
@@ -888,13 +891,13 @@ __global__ __launch_bounds__(kBlock) void k_p2_t(StepGeom G, StepBuffers S, Batc
     int U;
     union_range(S.uoff, tile, u0, U);
     const uint32_t* ul = S.ulist + u0;
-    dma_contiguous<kV4Nodes * K>(rsrc(S.slab + node0 * K * 4, uint64_t(cnt) * K * 16u), srec,
-                                 cnt * K);
+    dma_contiguous<kV4Nodes * KP>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u), srec,
+                                  cnt * KP);
     if (U <= S.ucap) stage_union(urows, ul, U, gf);
     const float4 s = ld4(ar, own);
     dma_fence();
     if (slot >= cnt) return;
-    const float4* rp = srec + slot * K;
+    const float4* rp = srec + slot * KP;
     const uint32_t lane_off = uint32_t(q) * 16u;
     float4 r;
     if (U <= S.ucap)

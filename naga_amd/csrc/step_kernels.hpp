@@ -77,8 +77,9 @@ struct StepBuffers {
     // Tile unions (D3Q27 four-per-lane kernels): the fluid nodes form tiles
     // of 32; ulist[uoff[t] .. uoff[t+1]) are the distinct stencil neighbours
     // of tile t, and the index field of slab / fslab records holds the
-    // neighbour's position in that list.  Unions of at most ucap rows are
-    // staged in LDS by the gather kernels.
+    // neighbour's position in that list; slab rows then hold k + kSlabPad
+    // records and fslab rows k + kFslabPad (the extra ones zero).  Unions of
+    // at most ucap rows are staged in LDS by the gather kernels.
     const uint32_t* ulist;
     const int64_t* uoff;
     int ucap;
@@ -144,6 +145,8 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 // tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
 // positions: D3Q27, k in {16, 32}, fewer than 2^25 nodes.
 bool step_uses_unions(int Q, int k, int64_t N);
+constexpr int kSlabPad = 1;   // padding records per slab row with tile unions
+constexpr int kFslabPad = 2;  // padding records per fslab row with tile unions
 constexpr int kUnionCapDefault = 272;  // LDS rows per block (3 blocks per CU)
 
 // Per-kernel entry points (used by the profiler and the standalone C-ABI).
