@@ -3,6 +3,10 @@
 
 #include "step_kernels.hpp"
 
+#ifndef NAGA_RECORD_POLICY
+#define NAGA_RECORD_POLICY 2
+#endif
+
 namespace naga_amd {
 
 namespace {
@@ -539,8 +543,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint64_t b
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), int(n), 0x00020000);
 }
 
+template <int AUX = 0>
 __device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX);
     return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
                        __uint_as_float(v[2]), __uint_as_float(v[3]));
 }
@@ -603,10 +608,14 @@ __device__ __forceinline__ float4 rate_fetch(const float4* rp, Fetch fetch, int 
 // source offset, with no VGPR round trip; the base must be wave-uniform.
 // Offsets past the buffer resource's size read as zero.
 typedef __attribute__((address_space(3))) void lds_void_t;
+template <int AUX = 0>
 __device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t src, uint32_t off, float4* lds_base) {
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(src, (lds_void_t*)(lds_base), 16, off,
-                                            0, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(src, (lds_void_t*)(lds_base), 16, off, 0, 0, AUX);
 }
+
+// Cache policy of the record streams (read once per pass): non-temporal,
+// so they do not displace the state rows in L2 and the last-level cache.
+constexpr int kStreamPolicy = NAGA_RECORD_POLICY;
 
 // Copy n16 <= NMAX contiguous 16-byte pieces of src (from offset 0) to dst.
 template <int NMAX>
@@ -617,7 +626,8 @@ __device__ __forceinline__ void dma_contiguous(__amdgpu_buffer_rsrc_t src, float
 #pragma unroll
     for (int t = 0; t < per_wave; ++t) {
         const int c = wave + t * nw;
-        if (c * 64 + lane < n16) dma16(src, uint32_t(c * 64 + lane) * 16u, dst + c * 64);
+        if (c * 64 + lane < n16)
+            dma16<kStreamPolicy>(src, uint32_t(c * 64 + lane) * 16u, dst + c * 64);
     }
 }
 
@@ -631,28 +641,36 @@ __device__ __forceinline__ void union_range(const int64_t* uoff, int64_t t, int6
 }
 
 // Stage the U union rows of a tile into LDS (urows[r * 8 + q] = quarter q
-// of row ul[r]): eight rows per wave instruction.  Every index is loaded
-// before the first copy is issued, so a wave waits for one index round
-// trip and keeps all its copies in flight together.
+// of row ul[r]): eight rows per wave instruction.  union_indices loads every
+// index a wave needs (past U they read as zero through the buffer resource,
+// so the loads are unconditional); union_copy then issues all the copies,
+// so a wave waits for one index round trip and keeps its copies in flight
+// together.
 constexpr int kUnionChunks = 12;  // 8-row chunks per wave: ucap <= 12 * 8 * 4 = 384
-__device__ __forceinline__ void stage_union(float4* urows, const uint32_t* ul, int U,
-                                            __amdgpu_buffer_rsrc_t src) {
+struct UnionIdx {
+    uint32_t j[kUnionChunks];
+};
+__device__ __forceinline__ UnionIdx union_indices(const uint32_t* ul, int U) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int nw = kBlock / 64;
-    const uint32_t q16 = uint32_t(lane & 7) * 16u;
-    // indices past U read as zero through the buffer resource: the loads
-    // are unconditional and all in flight before the first wait
     const __amdgpu_buffer_rsrc_t ur = rsrc(ul, uint64_t(U) * 4u);
-    uint32_t j[kUnionChunks];
+    UnionIdx x;
 #pragma unroll
     for (int t = 0; t < kUnionChunks; ++t) {
         const int r = (wave + t * nw) * 8 + (lane >> 3);
-        j[t] = __builtin_amdgcn_raw_buffer_load_b32(ur, uint32_t(r) * 4u, 0, 0);
+        x.j[t] = __builtin_amdgcn_raw_buffer_load_b32(ur, uint32_t(r) * 4u, 0, 0);
     }
+    return x;
+}
+__device__ __forceinline__ void union_copy(float4* urows, const UnionIdx& x, int U,
+                                           __amdgpu_buffer_rsrc_t src) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int nw = kBlock / 64;
+    const uint32_t q16 = uint32_t(lane & 7) * 16u;
 #pragma unroll
     for (int t = 0; t < kUnionChunks; ++t) {
         const int c = wave + t * nw;
-        if (c * 8 < U && c * 8 + (lane >> 3) < U) dma16(src, j[t] * 128u + q16, urows + c * 64);
+        if (c * 8 < U && c * 8 + (lane >> 3) < U) dma16(src, x.j[t] * 128u + q16, urows + c * 64);
     }
 }
 
@@ -685,9 +703,10 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuff
     const bool fluid = i < G.nfluid;
     const float4 g0 = ld4(rsrc(S.g, rows), own);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 y1 = STAGE == 4 ? ld4(rsrc(S.A, rows), own) : z4;
-    const float4 y2 = STAGE == 4 ? ld4(rsrc(S.B, rows), own) : z4;
-    const float4 y3 = STAGE == 4 ? ld4(rsrc(S.acc, rows), own) : z4;
+    // stage outputs read for the last time (non-temporal)
+    const float4 y1 = STAGE == 4 ? ld4<kStreamPolicy>(rsrc(S.A, rows), own) : z4;
+    const float4 y2 = STAGE == 4 ? ld4<kStreamPolicy>(rsrc(S.B, rows), own) : z4;
+    const float4 y3 = STAGE == 4 ? ld4<kStreamPolicy>(rsrc(S.acc, rows), own) : z4;
     const int64_t nf = G.nfluid - node0;
     const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
@@ -702,12 +721,18 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuff
             const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
             for (int t = threadIdx.x; t < cnt * K; t += NPB * kV4Lanes) srec[t + t / K] = src[t];
         } else {
+            // order: union indices, record copies, then the row copies (which
+            // wait for the indices only)
             int64_t u0;
             union_range(S.uoff, tile, u0, U);
             ul += u0;
+            const bool fits = U <= S.ucap;
+            const UnionIdx ux = union_indices(ul, U);  // unconditional: no wait here
+            __builtin_amdgcn_sched_barrier(0);
             dma_contiguous<NPB * KP>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u),
                                      srec, cnt * KP);
-            if (U <= S.ucap) stage_union(urows, ul, U, gin);
+            __builtin_amdgcn_sched_barrier(0);
+            if (fits) union_copy(urows, ux, U, gin);
         }
     }
     if (TU) dma_fence();
@@ -791,9 +816,12 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
             int64_t u0;
             union_range(S.uoff, tile, u0, U);
             ul += u0;
+            const UnionIdx ux = union_indices(ul, U);
+            __builtin_amdgcn_sched_barrier(0);
             dma_contiguous<NPB * KF / 2>(rsrc(S.fslab + node0 * KF * 2, uint64_t(cnt) * KF * 8u),
                                          srec4, cnt * KF / 2);
-            if (U <= S.ucap) stage_union(urows, ul, U, s1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (U <= S.ucap) union_copy(urows, ux, U, s1);
         }
     }
     if (TU) dma_fence();
@@ -891,9 +919,12 @@ __global__ __launch_bounds__(kBlock) void k_p2_t(StepGeom G, StepBuffers S, Batc
     int U;
     union_range(S.uoff, tile, u0, U);
     const uint32_t* ul = S.ulist + u0;
+    const UnionIdx ux = union_indices(ul, U);
+    __builtin_amdgcn_sched_barrier(0);
     dma_contiguous<kV4Nodes * KP>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u), srec,
                                   cnt * KP);
-    if (U <= S.ucap) stage_union(urows, ul, U, gf);
+    __builtin_amdgcn_sched_barrier(0);
+    if (U <= S.ucap) union_copy(urows, ux, U, gf);
     const float4 s = ld4(ar, own);
     dma_fence();
     if (slot >= cnt) return;
