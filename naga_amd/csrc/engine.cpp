@@ -200,16 +200,19 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     ucap_ = 0;
     ulist_ = DeviceBuffer<uint32_t>();
     uoff_ = DeviceBuffer<int64_t>();
-    // Tile unions: the gather kernels copy each 32-node tile's distinct
-    // stencil neighbours into LDS by LDS DMA and read them from there
-    // (0.22 against 0.27 ms per RK4 stage with direct gathers on the
-    // 1M-node D3Q27 scene); NAGA_UNIONS=0 disables them.
+    // Tile unions: the gather kernels copy each tile's distinct stencil
+    // neighbours into LDS by LDS DMA and read them from there (fewer bytes
+    // through the vector L1 than direct gathers: 0.2 against 0.27 ms per
+    // RK4 stage on the 1M-node D3Q27 scene); NAGA_UNIONS=0 disables them.
     const char* un_env = std::getenv("NAGA_UNIONS");
+    tile_nodes_ = 0;
     if (!(un_env && un_env[0] == '0') && step_uses_unions(Q, k, N) && nfluid > 0) {
         // tile unions; slab index fields become union positions
-        const int64_t ntiles = (nfluid + kTileNodes - 1) / kTileNodes;
+        const int T = union_tile_nodes(k);
+        tile_nodes_ = T;
+        const int64_t ntiles = (nfluid + T - 1) / T;
         DeviceBuffer<int32_t> counts(ntiles);
-        launch_tile_unions(nfluid, k, idx_.get(), counts.get(), nullptr, nullptr, nullptr, 0,
+        launch_tile_unions(nfluid, k, T, idx_.get(), counts.get(), nullptr, nullptr, nullptr, 0,
                            nullptr, 0, st);
         std::vector<int32_t> hc(ntiles);
         counts.download(hc.data(), ntiles, st);
@@ -222,7 +225,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         }
         uoff_ = to_device(off, st);
         ulist_.allocate(std::max<int64_t>(off[ntiles], 1));
-        launch_tile_unions(nfluid, k, idx_.get(), nullptr, uoff_.get(), ulist_.get(),
+        launch_tile_unions(nfluid, k, T, idx_.get(), nullptr, uoff_.get(), ulist_.get(),
                            slab_.get(), D, fslab_.get(), 1, st);
         // the union kernels copy records into LDS as they lie: rows padded to
         // k + 1 divergence records and k + 2 filter records keep the records
@@ -234,7 +237,8 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         slab_ = std::move(ps);
         fslab_ = std::move(pf);
         const char* env = std::getenv("NAGA_UCAP");
-        const int cap = env ? std::max(0, std::min(std::atoi(env), 12 * 32)) : kUnionCapDefault;
+        const int cap = env ? std::max(0, std::min(std::atoi(env), kUnionRowsMax))
+                            : union_cap_default(k);
         ucap_ = std::min(maxu, cap);
     }
 
@@ -293,11 +297,12 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         // tile holding a listed node (the others add an exact zero)
         if (ulist_.get()) {
             std::vector<int32_t> tl;
-            for (int64_t t0 = 0; t0 < nfluid; t0 += kTileNodes) {
-                const int64_t t1 = std::min<int64_t>(t0 + kTileNodes, nfluid);
+            const int T = tile_nodes_;
+            for (int64_t t0 = 0; t0 < nfluid; t0 += T) {
+                const int64_t t1 = std::min<int64_t>(t0 + T, nfluid);
                 for (int64_t i = t0; i < t1; ++i)
                     if (hflag[i]) {
-                        tl.push_back(int32_t(t0 / kTileNodes));
+                        tl.push_back(int32_t(t0 / T));
                         break;
                     }
             }

@@ -129,6 +129,7 @@ class Engine {
     int nparts_ = 1;
     int64_t n_pml_ = 0;
     int64_t n_ptiles_ = 0;
+    int tile_nodes_ = 0;  // nodes per union tile (0: no unions)
 
     DeviceBuffer<uint32_t> ulist_;  // tile unions (four-per-lane D3Q27 kernels)
     DeviceBuffer<int64_t> uoff_;

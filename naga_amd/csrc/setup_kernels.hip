@@ -505,10 +505,10 @@ __global__ void k_pad_records(int64_t n, int k, int rec, int kstride, const floa
     dst[t] = s < k ? src[(i * k + s) * rec + c] : 0.0f;
 }
 
-// One block per tile: the tile's k * kTileNodes (<= 2048) neighbour indices
+// One block per tile: the tile's k * tile_nodes (<= 4096) neighbour indices
 // are sorted in LDS (bitonic), flagged where they differ from their
 // predecessor and compacted by a block-wide scan of the flags.
-constexpr int kUnionMax = 2048;
+constexpr int kUnionMax = 4096;
 
 __device__ __forceinline__ int block_exclusive_scan(int* v, int n, int* tmp) {
     // v[0..n) in LDS, n <= kUnionMax, blockDim.x == 256: per-thread chunks of
@@ -536,7 +536,8 @@ __device__ __forceinline__ int block_exclusive_scan(int* v, int n, int* tmp) {
     return total;
 }
 
-__global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, const uint32_t* idx,
+__global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, int tile_nodes,
+                                                    const uint32_t* idx,
                                                     int32_t* counts, const int64_t* uoff,
                                                     uint32_t* ulist, float* slab_a, int nca,
                                                     float* slab_b, int ncb) {
@@ -544,9 +545,9 @@ __global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, const
     __shared__ uint32_t uniq[kUnionMax];
     __shared__ int rank[kUnionMax];
     __shared__ int tmp[256];
-    const int64_t node0 = int64_t(blockIdx.x) * kTileNodes;
+    const int64_t node0 = int64_t(blockIdx.x) * tile_nodes;
     const int64_t left = nfluid - node0;
-    const int cnt = left < kTileNodes ? int(left) : kTileNodes;
+    const int cnt = left < tile_nodes ? int(left) : tile_nodes;
     const int M = cnt * k;
     int P = 256;
     while (P < M) P <<= 1;
@@ -823,14 +824,14 @@ void launch_pad_records(int64_t n, int k, int rec, int kstride, const float* src
     hip_check(hipGetLastError(), "pad records");
 }
 
-void launch_tile_unions(int64_t nfluid, int k, const uint32_t* idx, int32_t* counts,
-                        const int64_t* uoff, uint32_t* ulist, float* slab_a, int ncomp_a,
-                        float* slab_b, int ncomp_b, hipStream_t st) {
-    require(k * kTileNodes <= kUnionMax, "stencil too wide for tile unions");
-    const int64_t ntiles = (nfluid + kTileNodes - 1) / kTileNodes;
+void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx,
+                        int32_t* counts, const int64_t* uoff, uint32_t* ulist, float* slab_a,
+                        int ncomp_a, float* slab_b, int ncomp_b, hipStream_t st) {
+    require(tile_nodes > 0 && k * tile_nodes <= kUnionMax, "stencil too wide for tile unions");
+    const int64_t ntiles = (nfluid + tile_nodes - 1) / tile_nodes;
     if (ntiles == 0) return;
-    k_tile_union<<<ntiles, 256, 0, st>>>(nfluid, k, idx, counts, uoff, ulist, slab_a, ncomp_a,
-                                          slab_b, ncomp_b);
+    k_tile_union<<<ntiles, 256, 0, st>>>(nfluid, k, tile_nodes, idx, counts, uoff, ulist, slab_a,
+                                          ncomp_a, slab_b, ncomp_b);
     hip_check(hipGetLastError(), "tile unions");
 }
 

@@ -69,17 +69,17 @@ void launch_make_slab(int64_t n, int k, int ncomp, const uint32_t* idx,
 void launch_pad_records(int64_t n, int k, int rec, int kstride, const float* src, float* dst,
                         hipStream_t stream);
 
-// Tile unions: fluid nodes are grouped in tiles of kTileNodes consecutive
-// nodes; a tile's union is the sorted list of the distinct neighbour
-// indices of its nodes' k-stencils.  Counting pass (ulist == nullptr):
-// counts[t] = union size.  Filling pass (uoff = exclusive scan of counts,
-// ntiles + 1 entries): ulist[uoff[t] ..] = the union and, for each slab
-// given (records of 1 + ncomp floats, field 0 the neighbour index), the
-// index field is replaced by the neighbour's position in its tile's union.
-constexpr int kTileNodes = 32;
-void launch_tile_unions(int64_t nfluid, int k, const uint32_t* idx, int32_t* counts,
-                        const int64_t* uoff, uint32_t* ulist, float* slab_a, int ncomp_a,
-                        float* slab_b, int ncomp_b, hipStream_t stream);
+// Tile unions: fluid nodes are grouped in tiles of tile_nodes consecutive
+// nodes (k * tile_nodes <= 4096); a tile's union is the sorted list of the
+// distinct neighbour indices of its nodes' k-stencils.  Counting pass
+// (ulist == nullptr): counts[t] = union size.  Filling pass (uoff =
+// exclusive scan of counts, ntiles + 1 entries): ulist[uoff[t] ..] = the
+// union and, for each slab given (records of 1 + ncomp floats, field 0 the
+// neighbour index), the index field is replaced by the neighbour's
+// position in its tile's union.
+void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx,
+                        int32_t* counts, const int64_t* uoff, uint32_t* ulist, float* slab_a,
+                        int ncomp_a, float* slab_b, int ncomp_b, hipStream_t stream);
 
 // flag[i] = 1 if any stencil neighbour of fluid node i is in [lo, hi).
 void launch_flag_neighbours(int64_t n, int k, const uint32_t* idx, int64_t lo,

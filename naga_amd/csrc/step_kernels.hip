@@ -12,6 +12,12 @@ namespace naga_amd {
 namespace {
 
 constexpr int kBlock = 256;  // 4 wavefronts
+#ifndef NAGA_P1_BLOCK
+#define NAGA_P1_BLOCK 256
+#endif
+// Collision block: one node per thread, rows transposed through LDS
+// ((RS + 1) floats per node), so the block size sets the LDS per block.
+constexpr int kP1Block = NAGA_P1_BLOCK;
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD works
 // on one contiguous range of tiles (neighbouring nodes share its L2).
@@ -76,18 +82,18 @@ __device__ __forceinline__ float lane_c(int a, int d) {
 // (collision), :876-877 (boundary velocity reset).
 // ---------------------------------------------------------------------------
 template <int Q>
-__global__ __launch_bounds__(kBlock) void k_p1(StepGeom G, StepBuffers S,
-                                               CollisionParams cp, BatchArgs B) {
+__global__ __launch_bounds__(kP1Block) void k_p1(StepGeom G, StepBuffers S,
+                                                 CollisionParams cp, BatchArgs B) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS, LS = RS + 1;
-    __shared__ float rows[kBlock * LS];
+    __shared__ float rows[kP1Block * LS];
     const BlockCtx bc = locate(B, G, S);
     if (B.scenes) cp = B.scenes[bc.scene].cp;
-    const int64_t node0 = bc.b * kBlock;
+    const int64_t node0 = bc.b * kP1Block;
     const int64_t left = G.N - node0;
-    const int cnt = left < kBlock ? int(left) : kBlock;
+    const int cnt = left < kP1Block ? int(left) : kP1Block;
     const float* src = S.g + node0 * RS;
-    for (int e = threadIdx.x; e < cnt * RS; e += kBlock) rows[(e / RS) * LS + e % RS] = src[e];
+    for (int e = threadIdx.x; e < cnt * RS; e += kP1Block) rows[(e / RS) * LS + e % RS] = src[e];
     __syncthreads();
     const int t = threadIdx.x;
     if (t < cnt) {
@@ -133,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_p1(StepGeom G, StepBuffers S,
     }
     __syncthreads();
     float* dst = S.A + node0 * RS;
-    for (int e = threadIdx.x; e < cnt * RS; e += kBlock) dst[e] = rows[(e / RS) * LS + e % RS];
+    for (int e = threadIdx.x; e < cnt * RS; e += kP1Block) dst[e] = rows[(e / RS) * LS + e % RS];
 }
 
 // ---------------------------------------------------------------------------
@@ -617,11 +623,12 @@ __device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t src, uint32_t off, 
 // so they do not displace the state rows in L2 and the last-level cache.
 constexpr int kStreamPolicy = NAGA_RECORD_POLICY;
 
-// Copy n16 <= NMAX contiguous 16-byte pieces of src (from offset 0) to dst.
-template <int NMAX>
+// Copy n16 <= NMAX contiguous 16-byte pieces of src (from offset 0) to dst
+// with the NT threads of the block.
+template <int NMAX, int NT>
 __device__ __forceinline__ void dma_contiguous(__amdgpu_buffer_rsrc_t src, float4* dst, int n16) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int nw = kBlock / 64;
+    constexpr int nw = NT / 64;
     constexpr int per_wave = (NMAX + 64 * nw - 1) / (64 * nw);
 #pragma unroll
     for (int t = 0; t < per_wave; ++t) {
@@ -640,37 +647,41 @@ __device__ __forceinline__ void union_range(const int64_t* uoff, int64_t t, int6
     U = int(p[t + 1] - u0);
 }
 
-// Stage the U union rows of a tile into LDS (urows[r * 8 + q] = quarter q
-// of row ul[r]): eight rows per wave instruction.  union_indices loads every
-// index a wave needs (past U they read as zero through the buffer resource,
-// so the loads are unconditional); union_copy then issues all the copies,
-// so a wave waits for one index round trip and keeps its copies in flight
-// together.
-constexpr int kUnionChunks = 12;  // 8-row chunks per wave: ucap <= 12 * 8 * 4 = 384
+// Stage the first Uc union rows of a tile into LDS (urows[r * 8 + q] =
+// quarter q of row ul[r]): eight rows per wave instruction, NT threads per
+// block.  union_indices loads every index a wave needs (past Uc they read
+// as zero through the buffer resource, so the loads are unconditional);
+// union_copy then issues all the copies, so a wave waits for one index
+// round trip and keeps its copies in flight together.
+template <int NT>
 struct UnionIdx {
-    uint32_t j[kUnionChunks];
+    static constexpr int kChunks = kUnionRowsMax / (8 * (NT / 64));  // 8-row chunks per wave
+    uint32_t j[kChunks];
 };
-__device__ __forceinline__ UnionIdx union_indices(const uint32_t* ul, int U) {
+template <int NT>
+__device__ __forceinline__ UnionIdx<NT> union_indices(const uint32_t* ul, int Uc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int nw = kBlock / 64;
-    const __amdgpu_buffer_rsrc_t ur = rsrc(ul, uint64_t(U) * 4u);
-    UnionIdx x;
+    constexpr int nw = NT / 64;
+    const __amdgpu_buffer_rsrc_t ur = rsrc(ul, uint64_t(Uc) * 4u);
+    UnionIdx<NT> x;
 #pragma unroll
-    for (int t = 0; t < kUnionChunks; ++t) {
+    for (int t = 0; t < UnionIdx<NT>::kChunks; ++t) {
         const int r = (wave + t * nw) * 8 + (lane >> 3);
         x.j[t] = __builtin_amdgcn_raw_buffer_load_b32(ur, uint32_t(r) * 4u, 0, 0);
     }
     return x;
 }
-__device__ __forceinline__ void union_copy(float4* urows, const UnionIdx& x, int U,
+template <int NT>
+__device__ __forceinline__ void union_copy(float4* urows, const UnionIdx<NT>& x, int Uc,
                                            __amdgpu_buffer_rsrc_t src) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int nw = kBlock / 64;
+    constexpr int nw = NT / 64;
     const uint32_t q16 = uint32_t(lane & 7) * 16u;
 #pragma unroll
-    for (int t = 0; t < kUnionChunks; ++t) {
+    for (int t = 0; t < UnionIdx<NT>::kChunks; ++t) {
         const int c = wave + t * nw;
-        if (c * 8 < U && c * 8 + (lane >> 3) < U) dma16(src, x.j[t] * 128u + q16, urows + c * 64);
+        if (c * 8 < Uc && c * 8 + (lane >> 3) < Uc)
+            dma16(src, x.j[t] * 128u + q16, urows + c * 64);
     }
 }
 
@@ -685,7 +696,7 @@ __device__ __forceinline__ void dma_fence() {
 template <int STAGE, int K, int BATCH, bool TU, int NPB>
 __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuffers S,
                                                             BatchArgs B) {
-    static_assert(!TU || NPB == kV4Nodes, "tile unions are built for 32-node tiles");
+    constexpr int NT = NPB * kV4Lanes;
     // record rows padded by one entry: the eight nodes of a wavefront read
     // their records from distinct LDS banks
     constexpr int KP = K + 1;
@@ -719,20 +730,20 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuff
     if (cnt > 0) {
         if (!TU) {
             const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
-            for (int t = threadIdx.x; t < cnt * K; t += NPB * kV4Lanes) srec[t + t / K] = src[t];
+            for (int t = threadIdx.x; t < cnt * K; t += NT) srec[t + t / K] = src[t];
         } else {
             // order: union indices, record copies, then the row copies (which
             // wait for the indices only)
             int64_t u0;
             union_range(S.uoff, tile, u0, U);
             ul += u0;
-            const bool fits = U <= S.ucap;
-            const UnionIdx ux = union_indices(ul, U);  // unconditional: no wait here
+            const int Uc = U < S.ucap ? U : S.ucap;
+            const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);  // unconditional: no wait here
             __builtin_amdgcn_sched_barrier(0);
-            dma_contiguous<NPB * KP>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u),
-                                     srec, cnt * KP);
+            dma_contiguous<NPB * KP, NT>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u),
+                                         srec, cnt * KP);
             __builtin_amdgcn_sched_barrier(0);
-            if (fits) union_copy(urows, ux, U, gin);
+            union_copy<NT>(urows, ux, Uc, gin);
         }
     }
     if (TU) dma_fence();
@@ -752,7 +763,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuff
     else if (U <= S.ucap)
         r = rate_fetch<K, BATCH>(srec + slot * kp,
                                  [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
-    else
+    else  // union over the cap: every row read from HBM through the list
         r = rate_fetch<K, BATCH>(srec + slot * kp,
                                  [&](uint32_t j) {
                                      return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
@@ -786,7 +797,7 @@ __device__ __forceinline__ float4 filter_sum(const float2* rp, const float4& si,
 // bounce-back on boundary nodes.
 template <int K, int BATCH, bool TU, int NPB>
 __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers S, BatchArgs B) {
-    static_assert(!TU || NPB == kV4Nodes, "tile unions are built for 32-node tiles");
+    constexpr int NT = NPB * kV4Lanes;
     // padded record rows (distinct banks per node): K + 1 records through
     // VGPRs, or K + kFslabPad by LDS DMA from rows padded so in HBM
     constexpr int KP = K + 1, KF = K + kFslabPad;
@@ -811,17 +822,18 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
     if (cnt > 0) {
         if (!TU) {
             const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
-            for (int t = threadIdx.x; t < cnt * K; t += NPB * kV4Lanes) srec[t + t / K] = src[t];
+            for (int t = threadIdx.x; t < cnt * K; t += NT) srec[t + t / K] = src[t];
         } else {
             int64_t u0;
             union_range(S.uoff, tile, u0, U);
             ul += u0;
-            const UnionIdx ux = union_indices(ul, U);
+            const int Uc = U < S.ucap ? U : S.ucap;
+            const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);
             __builtin_amdgcn_sched_barrier(0);
-            dma_contiguous<NPB * KF / 2>(rsrc(S.fslab + node0 * KF * 2, uint64_t(cnt) * KF * 8u),
-                                         srec4, cnt * KF / 2);
+            dma_contiguous<NPB * KF / 2, NT>(
+                rsrc(S.fslab + node0 * KF * 2, uint64_t(cnt) * KF * 8u), srec4, cnt * KF / 2);
             __builtin_amdgcn_sched_barrier(0);
-            if (U <= S.ucap) union_copy(urows, ux, U, s1);
+            union_copy<NT>(urows, ux, Uc, s1);
         }
     }
     if (TU) dma_fence();
@@ -836,7 +848,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
             sum = filter_sum<K, BATCH>(rp, si, [&](uint32_t j) { return ld4(s1, j * 128u + lane_off); });
         else if (U <= S.ucap)
             sum = filter_sum<K, BATCH>(rp, si, [&](uint32_t j) { return urows[j * kV4Lanes + q]; });
-        else
+        else  // union over the cap: every row read from HBM through the list
             sum = filter_sum<K, BATCH>(rp, si,
                                        [&](uint32_t j) { return ld4(s1, ul[j] * 128u + lane_off); });
 #pragma unroll
@@ -857,10 +869,9 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
 }
 
 // p2 for D3Q27 with four distributions per lane (PML divergence over the
-// listed nodes, added to s1).
-template <int K, int BATCH, bool TU, int NPB>
+// listed nodes, added to s1), records holding node indices.
+template <int K, int BATCH, int NPB>
 __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_v4(StepGeom G, StepBuffers S, BatchArgs B) {
-    static_assert(!TU || NPB == kV4Nodes, "tile unions are built for 32-node tiles");
     constexpr int KP = K + 1;  // padded record rows (distinct banks per node)
     __shared__ float4 srec[NPB * KP];
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
@@ -877,16 +888,9 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_v4(StepGeom G, StepBuffer
     if (!valid) return;
     const uint64_t rows = uint64_t(G.N) * 128u;
     const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
-    // records hold node indices, or positions in the node's tile union
     const uint32_t lane_off = uint32_t(q) * 16u;
-    float4 r;
-    if (!TU) {
-        r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, j * 128u + lane_off); }, q);
-    } else {
-        const uint32_t* ul = S.ulist + S.uoff[i / kV4Nodes];
-        r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + lane_off); },
-                                 q);
-    }
+    const float4 r =
+        rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, j * 128u + lane_off); }, q);
     // the divergence of F = c Gf is the negated rate of Gf
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const float4 s = ld4(rsrc(S.A, rows), own);
@@ -898,33 +902,35 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_v4(StepGeom G, StepBuffer
 // other nodes of such a tile every neighbour's auxiliary row is zero and
 // the added divergence is an exact zero), the tile's union rows of Gf and
 // its records staged by LDS DMA.
-template <int K, int BATCH>
-__global__ __launch_bounds__(kBlock) void k_p2_t(StepGeom G, StepBuffers S, BatchArgs B) {
+template <int K, int BATCH, int NPB>
+__global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_t(StepGeom G, StepBuffers S, BatchArgs B) {
     constexpr int KP = K + kSlabPad;  // padded slab rows
-    __shared__ float4 srec[kV4Nodes * KP];
+    constexpr int NT = NPB * kV4Lanes;
+    __shared__ float4 srec[NPB * KP];
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
     typedef const __attribute__((address_space(4))) int32_t const_i32;
     const int64_t tile = ((const_i32*)S.ptiles)[xcd_tile(bc.b, bc.nb)];
-    const int64_t node0 = tile * kV4Nodes;
+    const int64_t node0 = tile * NPB;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const int64_t nf = G.nfluid - node0;
-    const int cnt = nf <= 0 ? 0 : (nf < kV4Nodes ? int(nf) : kV4Nodes);
+    const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
     const __amdgpu_buffer_rsrc_t ar = rsrc(S.A, rows);
     int64_t u0;
     int U;
     union_range(S.uoff, tile, u0, U);
     const uint32_t* ul = S.ulist + u0;
-    const UnionIdx ux = union_indices(ul, U);
+    const int Uc = U < S.ucap ? U : S.ucap;
+    const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);
     __builtin_amdgcn_sched_barrier(0);
-    dma_contiguous<kV4Nodes * KP>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u), srec,
-                                  cnt * KP);
+    dma_contiguous<NPB * KP, NT>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u), srec,
+                                 cnt * KP);
     __builtin_amdgcn_sched_barrier(0);
-    if (U <= S.ucap) union_copy(urows, ux, U, gf);
+    union_copy<NT>(urows, ux, Uc, gf);
     const float4 s = ld4(ar, own);
     dma_fence();
     if (slot >= cnt) return;
@@ -933,11 +939,37 @@ __global__ __launch_bounds__(kBlock) void k_p2_t(StepGeom G, StepBuffers S, Batc
     float4 r;
     if (U <= S.ucap)
         r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
-    else
+    else  // union over the cap: every row read from HBM through the list
         r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + lane_off); },
                                  q);
     // the divergence of F = c Gf is the negated rate of Gf
     st4(S.A, own, make_float4(s.x - r.x, s.y - r.y, s.z - r.z, s.w - r.w));
+}
+
+// PML auxiliary field rows for D3Q27, four distributions per lane (the
+// arithmetic of k_pml_field; padding slots stay zero).
+__global__ __launch_bounds__(kBlock) void k_pml_field_v4(StepGeom G, StepBuffers S, BatchArgs B) {
+    const BlockCtx bc = locate(B, G, S);
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const int64_t m = xcd_tile(bc.b, bc.nb) * kV4Nodes + slot;
+    if (m >= G.nlayer) return;
+    const int64_t j = G.nbulk + m;
+    const int64_t p = G.pml_ref ? j - G.nbulk : j;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const uint32_t lane_off = uint32_t(q) * 16u;
+    const float sig = S.sigma[m];
+    const float4 q1 = ld4(rsrc(S.Qp, rows), uint32_t(p) * 128u + lane_off);
+    const float4 q2 = ld4(rsrc(S.Q2v, rows), uint32_t(p) * 128u + lane_off);
+    float4 out;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int a = 4 * q + e;
+        const float v = a < 27 ? -2.0f * sig * (get(q1, e) - S.means[a]) +
+                                     sig * sig * (get(q2, e) - S.means[27 + a])
+                               : 0.0f;
+        set(out, e, v);
+    }
+    st4(S.Gf, uint32_t(j) * 128u + lane_off, out);
 }
 
 // PML local terms for D3Q27, four distributions per lane (same arithmetic
@@ -1146,20 +1178,30 @@ bool step_uses_unions(int Q, int k, int64_t N) {
     return Q == 27 && on && (k == 48 || k == 32 || k == 16) && N < (int64_t(1) << 25);
 }
 
-// Nodes per block of the D3Q27 stencil kernels: 32, 64 or 128 (records of
-// all of them staged in at most 64 KB of LDS); 32 when tile unions are
-// built.  NAGA_V4_NPB overrides the default.
+// Nodes per block of the D3Q27 stencil kernels: the union tile size with
+// tile unions, else 32, 64 or 128 (records of all of them staged in at
+// most 64 KB of LDS; NAGA_V4_NPB overrides the default 32).
 int v4_nodes_per_block(int k, bool tiles) {
     static const int env = [] {
         const char* e = std::getenv("NAGA_V4_NPB");
         const int v = e ? std::atoi(e) : kV4DefaultNodes;
         return (v == 32 || v == 64 || v == 128) ? v : kV4DefaultNodes;
     }();
-    if (tiles) return kV4Nodes;
+    if (tiles) return union_tile_nodes(k);
     int v = env;
     while (v > kV4Nodes && v * (k + 1) * 16 > 65536) v /= 2;
     return v;
 }
+
+int union_tile_nodes(int k) {
+    static const bool want64 = [] {
+        const char* e = std::getenv("NAGA_TILE");
+        return e && std::atoi(e) == 64;
+    }();
+    return (k <= 32 && want64) ? 2 * kV4Nodes : kV4Nodes;
+}
+
+int union_cap_default(int k) { return union_tile_nodes(k) == kV4Nodes ? 272 : 368; }
 
 namespace {
 
@@ -1196,21 +1238,19 @@ void launch_v4_npb(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S
                    const PassCfg& c, hipStream_t st) {
     constexpr int threads = NPB * kV4Lanes;
     const size_t lds = size_t(c.ucap) * 128u;
-    if constexpr (NPB == kV4Nodes) {
-        if (c.tiles) {
-            switch (pass) {
-                case kP2: k_p2_t<K, 8><<<nb, threads, lds, st>>>(G, S, c.B); return;
-                case kP4: k_p4_u<K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
-                case kS1: k_stage_u<1, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
-                case kS2: k_stage_u<2, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
-                case kS3: k_stage_u<3, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
-                case kS4: k_stage_u<4, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
-                default: fail(kInvalidArgument, "not a stencil pass");
-            }
+    if (c.tiles) {
+        switch (pass) {
+            case kP2: k_p2_t<K, 8, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kP4: k_p4_u<K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kS1: k_stage_u<1, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kS2: k_stage_u<2, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kS3: k_stage_u<3, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kS4: k_stage_u<4, K, 8, true, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            default: fail(kInvalidArgument, "not a stencil pass");
         }
     }
     switch (pass) {
-        case kP2: k_p2_v4<K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
+        case kP2: k_p2_v4<K, 8, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
         case kP4: k_p4_u<K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
         case kS1: k_stage_u<1, K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
         case kS2: k_stage_u<2, K, 8, false, NPB><<<nb, threads, 0, st>>>(G, S, c.B); return;
@@ -1256,13 +1296,16 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         return;
     }
     switch (pass) {
-        case kP1: k_p1<Q><<<nb, kBlock, 0, st>>>(G, S, cp, c.B); break;
+        case kP1: k_p1<Q><<<nb, kP1Block, 0, st>>>(G, S, cp, c.B); break;
         case kPl:
             if (v4) k_pml_local_v4<<<nb, kBlock, 0, st>>>(G, S, c.B);
             else k_pml_local<Q><<<nb, kBlock, 0, st>>>(G, S, c.B);
             break;
         case kPm: k_means<Q><<<nb, 256, 0, st>>>(G, S, c.B); break;
-        case kPf: k_pml_field<Q><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+        case kPf:
+            if (v4) k_pml_field_v4<<<nb, kBlock, 0, st>>>(G, S, c.B);
+            else k_pml_field<Q><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            break;
         case kP2:
             if (c.ktmpl == 32) k_p2<Q, 32><<<nb, kBlock, 0, st>>>(G, S, c.B);
             else if (c.ktmpl == 16) k_p2<Q, 16><<<nb, kBlock, 0, st>>>(G, S, c.B);
@@ -1310,10 +1353,10 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
     auto tiles32 = [&](int64_t n) { return (n + kV4Nodes - 1) / kV4Nodes; };
     const bool pml = G.nlayer > 0 && S.n_rq > 0;
     const int64_t nbnd = G.N - G.nfluid;
-    nb[kP1] = (G.N + kBlock - 1) / kBlock;
+    nb[kP1] = (G.N + kP1Block - 1) / kP1Block;
     nb[kPl] = pml ? S.nparts : 0;
     nb[kPm] = pml ? 2 * Q : 0;
-    nb[kPf] = pml ? lanes(G.nlayer) : 0;
+    nb[kPf] = pml ? (un ? tiles32(G.nlayer) : lanes(G.nlayer)) : 0;
     nb[kP2] = (pml && S.n_pml > 0)
                   ? (un ? (S.ulist ? S.n_ptiles : tiles(S.n_pml)) : lanes(S.n_pml))
                   : 0;

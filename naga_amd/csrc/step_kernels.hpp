@@ -141,13 +141,22 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
 template <int Q>
 void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 
-// Whether the step runs the D3Q27 four-per-lane kernels, which need the
-// tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
-// positions: D3Q27, k in {16, 32}, fewer than 2^25 nodes.
+// Whether the step runs the D3Q27 four-per-lane kernels, which may use
+// the tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
+// positions: D3Q27, k in {16, 32, 48}, fewer than 2^25 nodes.
 bool step_uses_unions(int Q, int k, int64_t N);
 constexpr int kSlabPad = 1;   // padding records per slab row with tile unions
 constexpr int kFslabPad = 2;  // padding records per fslab row with tile unions
-constexpr int kUnionCapDefault = 272;  // LDS rows per block (3 blocks per CU)
+// Nodes per union tile (= per block of the union kernels) for stencil width
+// k: 32, or with NAGA_TILE=64 and k <= 32, 64 (fewer union rows per node,
+// but two blocks per CU and 9% of unions over the LDS cap: 0.23 against
+// 0.20 ms per RK4 stage on the 1M-node D3Q27 scene).
+int union_tile_nodes(int k);
+// Default LDS union rows per block for that tile size: 368 at 64 nodes
+// (two blocks per CU), 272 at 32 nodes (three blocks per CU).  A tile with
+// a larger union reads its rows from HBM through the union list.
+int union_cap_default(int k);
+constexpr int kUnionRowsMax = 512;  // LDS union rows per block, upper bound
 
 // Per-kernel entry points (used by the profiler and the standalone C-ABI).
 template <int Q>
