@@ -54,6 +54,9 @@ typedef struct naga_scene_config {
     double dt;              /* time step */
     double c;               /* speed of sound */
     double h;               /* nodal spacing (simulation_nodes::nodal_spacing) */
+    double filter_strength; /* selective filter strength per step: 0 gives the
+                               reference's 0.75 (detail/simulation_engine.cuh:
+                               1242), > 0 that strength, < 0 no filter */
 } naga_scene_config;
 
 const char* naga_last_error(void);

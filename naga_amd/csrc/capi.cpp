@@ -150,6 +150,7 @@ int naga_engine_create(const naga_scene_config* cfg, naga_engine** out) {
         c.dt = cfg->dt;
         c.c = cfg->c;
         c.h = cfg->h;
+        c.filter = cfg->filter_strength;
         auto* e = new naga_engine{};
         try {
             e->impl = std::make_unique<Engine>(c);

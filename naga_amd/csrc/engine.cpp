@@ -72,7 +72,8 @@ Engine::Engine(const SceneConfig& cfg) : cfg_(cfg) {
     geo_.dt = float(dt_lat);
     geo_.rho0 = float(cfg.rho0);
     geo_.vel_scale = float(cfg.c / kCsLat);
-    geo_.filter = 0.75f;
+    // selective filter strength (detail/simulation_engine.cuh:1242: 0.75)
+    geo_.filter = cfg.filter == 0.0 ? 0.75f : (cfg.filter < 0.0 ? 0.0f : float(cfg.filter));
     geo_.k = cfg.k;
     geo_.kb = cfg.kb;
     geo_.pml_ref = cfg.pml_ref ? 1 : 0;

@@ -20,6 +20,7 @@ struct SceneConfig {
     int pml_ref = 1;
     int k_min = 0;  // > 0: per-node stencil widths in [k_min, k] by local density
     double nu = 0.0, rho0 = 1.0, dt = 0.0, c = kCsLat, h = 0.01;
+    double filter = 0.0;  // 0: the reference's 0.75; > 0: that strength; < 0: off
 };
 
 // A declarative density source: amplitude * cos(2 pi frequency t + phase)

@@ -163,6 +163,7 @@ struct stencil_config {
     int boundary_rbf = NAGA_RBF_MULTIQUADRIC;
     bool reference_pml_indexing = true;
     int min_support_size = 0;  // > 0: per-node widths in [min, support_size] by density
+    double filter_strength = 0.0;  // 0: the reference's 0.75 per step; < 0: no filter
 };
 
 // Declarative density source (naga_engine_add_density_source): while
@@ -220,6 +221,7 @@ class simulation_engine {
         cfg.dt = double(params_.time_step);
         cfg.c = double(params_.speed_of_sound);
         cfg.h = double(domain.nodal_spacing);
+        cfg.filter_strength = stencil_.filter_strength;
         naga_engine* e = nullptr;
         abi::check(naga_engine_create(&cfg, &e), where);
         handle_.reset(e, [](naga_engine* p) { naga_engine_destroy(p); });

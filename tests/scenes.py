@@ -19,10 +19,11 @@ def to_sim_nodes(nodes, h):
 
 
 def gpu_engine(nodes, Q, k, h, tau=0.6, dt=None, rbf=1, kb=None, rbf_b=None,
-               pml_ref_index=True):
+               pml_ref_index=True, filter_strength=0.0):
     dt = 0.4 * h if dt is None else dt
     eng = naga_amd.SimulationEngine(lattice=Q, k=k, k_boundary=kb, rbf=rbf,
-                                    rbf_boundary=rbf_b, pml_ref_index=pml_ref_index)
+                                    rbf_boundary=rbf_b, pml_ref_index=pml_ref_index,
+                                    filter_strength=filter_strength)
     p = naga_amd.problem_parameters(Q, tau, dt)
     eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
     eng.init_domain(to_sim_nodes(nodes, h))
