@@ -179,7 +179,9 @@ def emit(path):
         rhs = (rho * e[j] - sum(v * m[mon] for mon, v in coeffs[j].items())).subs(subs)
         acc = sp.expand(rhs) - sum(offv[(j, l)] * kp[l]
                                    for l in R if (j, l) in offv)
-        w(f"    const double kp{j} = ({ccode(acc)}) / {ccode(B[j][j])}.0;")
+        # multiply by the reciprocal (a compile-time constant): FP64 division
+        # is a long instruction sequence on the GPU
+        w(f"    const double kp{j} = ({ccode(acc)}) * (1.0 / {ccode(B[j][j])}.0);")
     w("    double k[27];")
     w("    k[0] = k[1] = k[2] = k[3] = 0.0;")
     for j in R:

@@ -110,8 +110,9 @@ __global__ __launch_bounds__(kP1Block) void k_p1(StepGeom G, StepBuffers S,
             f[a] = L::w(a) + gv;
         }
         double u[3] = {0.0, 0.0, 0.0};
+        const double inv_rho = 1.0 / (1.0 + rt);
 #pragma unroll
-        for (int d = 0; d < D; ++d) u[d] = mom[d] / (1.0 + rt);
+        for (int d = 0; d < D; ++d) u[d] = mom[d] * inv_rho;
         // density source (simulation_engine.cuh:1057-1107, once per node):
         // f += w s / rho0, rho <- (rho + (rho + s)) / 2, u unchanged
         const double ws = S.src ? double(S.src[i]) / double(G.rho0) : 0.0;
@@ -166,6 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S,
     const int a = al < Q ? al : Q - 1;
     const BlockCtx bc = locate(B, G, S);
     const double cs2 = kCsLat * kCsLat;
+    const double inv_cs2 = 1.0 / cs2, inv_2cs4 = 1.0 / (2.0 * cs2 * cs2);
     const double hdt = double(G.dt) / 2.0;
     const double ca[3] = {double(L::c(a, 0)), double(L::c(a, 1)), D == 3 ? double(L::c(a, 2)) : 0.0};
     double s1 = 0.0, s2 = 0.0;
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S,
         const double uc = D == 3 ? u0 * ca[0] + u1 * ca[1] + u2 * ca[2] : u0 * ca[0] + u1 * ca[1];
         const double uu = D == 3 ? u2 * u2 + (u1 * u1 + u0 * u0) : u1 * u1 + u0 * u0;
         // feq - w (equilibrium_distribution.cuh:52-63), cancellation-free
-        const double fte = L::w(a) * (rt + rho * (uc / cs2 + (uc * uc) / (2.0 * cs2 * cs2) - uu));
+        const double fte = L::w(a) * (rt + rho * (uc * inv_cs2 + (uc * uc) * inv_2cs4 - uu));
         const int64_t o = i * RS + a;
         const double q1 = (fte + double(S.E[o])) * hdt;
         const double q2 = (q1 + double(S.Qp[o])) * hdt;
@@ -981,6 +983,7 @@ __global__ __launch_bounds__(kBlock) void k_pml_local_v4(StepGeom G, StepBuffers
     const BlockCtx bc = locate(B, G, S);
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const double cs2 = kCsLat * kCsLat;
+    const double inv_cs2 = 1.0 / cs2, inv_2cs4 = 1.0 / (2.0 * cs2 * cs2);
     const double hdt = double(G.dt) / 2.0;
     const uint64_t rows = uint64_t(G.N) * 128u;
     const __amdgpu_buffer_rsrc_t rE = rsrc(S.E, rows), rQp = rsrc(S.Qp, rows),
@@ -1012,7 +1015,7 @@ __global__ __launch_bounds__(kBlock) void k_pml_local_v4(StepGeom G, StepBuffers
             }
             const double uc = u0 * double(L::c(a, 0)) + u1 * double(L::c(a, 1)) +
                               u2 * double(L::c(a, 2));
-            const double fte = L::w(a) * (rt + rho * (uc / cs2 + (uc * uc) / (2.0 * cs2 * cs2) - uu));
+            const double fte = L::w(a) * (rt + rho * (uc * inv_cs2 + (uc * uc) * inv_2cs4 - uu));
             const double q1 = (fte + double(get(e4, e))) * hdt;
             const double q2 = (q1 + double(get(p4, e))) * hdt;
             set(eo, e, float(fte));
