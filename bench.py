@@ -153,27 +153,35 @@ def batched_workload(steps: int, n_scenes: int = 256, seed: int = 0x4E414741) ->
                         "k 12-24, PML 8h, tau 0.6, dt 0.4h"}
 
 
-def served_workload(n_requests: int = 48, rate: float = 50.0, seed: int = 0x4E414742) -> dict:
+def served_workload(n_requests: int = 32, rate: float = 50.0, seed: int = 0x4E414742) -> dict:
     """Poisson arrivals at `rate` requests/s, D2Q9/D3Q27 50/50, node counts
-    log-uniform in [5k, 200k], steps uniform in [100, 2000], random pulse and
-    obstacles; served with continuous batching, then the same requests
-    solved one at a time with their own launches (setup included in both)."""
+    log-uniform in [5k, 2M], steps uniform in [100, 2000], random pulse
+    position.  D3Q27 requests are the irregular domain: the unit cube minus
+    a central sphere of radius 0.15, 2x node density within 0.05 of it, k
+    from 24 to 48 by local density; D2Q9 requests the unit square with 0-5
+    circles of radius 0.03-0.1, k = 16.  Served with continuous batching,
+    then the same requests solved one at a time with their own launches
+    (setup included in both)."""
     import naga_amd
     rng = np.random.default_rng(seed)
     reqs = []
     for _ in range(n_requests):
         lat = 9 if rng.random() < 0.5 else 27
         D = 2 if lat == 9 else 3
-        n = float(np.exp(rng.uniform(np.log(5e3), np.log(2e5))))
+        n = float(np.exp(rng.uniform(np.log(5e3), np.log(2e6))))
         h = 1.0 / (n ** (1.0 / D) - 1.0)
+        pc = [float(v) for v in rng.uniform(0.2, 0.3, 1)] + \
+             [float(v) for v in rng.uniform(0.35, 0.65, D - 1)]
         if D == 2:
             obst = [([float(v) for v in rng.uniform(0.15, 0.85, 2)], float(rng.uniform(0.03, 0.1)))
                     for _ in range(int(rng.integers(0, 6)))]
+            kw = dict(k=16)
         else:
             obst = [([0.5, 0.5, 0.5], 0.15)]
+            kw = dict(k=48, k_min=24, refine_width=0.05)
         reqs.append(naga_amd.make_request(
             lat, h, int(rng.integers(100, 2001)), obst, seed=int(rng.integers(1, 1 << 62)),
-            pulse_center=[float(v) for v in rng.uniform(0.2, 0.35, D)], pulse_width=0.05))
+            pulse_center=pc, pulse_width=0.05, **kw))
     gaps = rng.exponential(1.0 / rate, n_requests)
     srv = naga_amd.SolveServer(hbm_budget_bytes=96 << 30, max_active=64, chunk_steps=50)
     t0 = time.perf_counter()
@@ -191,14 +199,16 @@ def served_workload(n_requests: int = 48, rate: float = 50.0, seed: int = 0x4E41
         out = naga_amd.solve_alone(r)
         work += out["nodes"] * r.steps
     plain_s = time.perf_counter() - t1
+    print(f"served: plain loop {plain_s:.1f} s", file=sys.stderr, flush=True)
     return {"requests": n_requests, "rate_per_s": rate, "p50_ms": st["p50_ms"],
             "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "served_node_updates_per_s": st["node_updates_per_s"],
             "plain_node_updates_per_s": work / plain_s,
             "speedup": st["node_updates_per_s"] / (work / plain_s),
             "trace_s": time.perf_counter() - t0 - plain_s,
-            "workload": "Poisson 50 req/s, D2Q9/D3Q27 50/50, N log-uniform 5k-200k, "
-                        "steps 100-2000, obstacles (2D: 0-5 circles; 3D: sphere r 0.15)"}
+            "workload": "Poisson 50 req/s, D2Q9/D3Q27 50/50, N log-uniform 5k-2M, "
+                        "steps 100-2000; 3D: cube minus sphere r 0.15, 2x refinement "
+                        "within 0.05, k 24-48; 2D: 0-5 circles, k 16"}
 
 
 def main() -> int:
@@ -213,7 +223,7 @@ def main() -> int:
                     help="only run one device copy of BYTES (counter calibration)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched and served workloads")
-    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--batched-steps", type=int, default=500)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)

@@ -202,6 +202,10 @@ typedef struct naga_request {
     double pulse_center[3];
     double pulse_amplitude;
     double pulse_width;
+    int32_t k_min;             /* 0: every node uses k; else per-node widths in
+                                  [k_min, k] by local density (naga_scene_config) */
+    double refine_width;       /* > 0: 2x node density within this distance of
+                                  the obstacle surfaces (naga_obstacle_grid) */
 } naga_request;
 
 typedef struct naga_server_stats {

@@ -58,6 +58,7 @@ class Request(C.Structure):
         ("layer_thickness", C.c_double), ("peak_absorption", C.c_double),
         ("tau", C.c_double), ("pulse_center", C.c_double * 3),
         ("pulse_amplitude", C.c_double), ("pulse_width", C.c_double),
+        ("k_min", C.c_int32), ("refine_width", C.c_double),
     ]
 
 
@@ -871,9 +872,12 @@ _SERVER_PREFIX = "naga::server: "
 
 def make_request(lattice, h, steps, obstacles=(), k=0, jitter=0.25, seed=0x4E414741,
                  layer_thickness=None, peak_absorption=0.5, tau=0.6, pulse_center=None,
-                 pulse_amplitude=1e-3, pulse_width=0.05) -> Request:
+                 pulse_amplitude=1e-3, pulse_width=0.05, k_min=0,
+                 refine_width=0.0) -> Request:
     """A solve of the unit box (square or cube) minus obstacles
-    [(centre, radius), ...]; the absorbing layer defaults to 8 spacings."""
+    [(centre, radius), ...]; the absorbing layer defaults to 8 spacings.
+    refine_width > 0 doubles the node density within that distance of the
+    obstacles; k_min > 0 gives per-node stencil widths in [k_min, k]."""
     D = 2 if lattice == 9 else 3
     r = Request()
     r.lattice, r.k, r.steps = int(lattice), int(k), int(steps)
@@ -891,6 +895,7 @@ def make_request(lattice, h, steps, obstacles=(), k=0, jitter=0.25, seed=0x4E414
     for d in range(D):
         r.pulse_center[d] = float(pc[d])
     r.pulse_amplitude, r.pulse_width = float(pulse_amplitude), float(pulse_width)
+    r.k_min, r.refine_width = int(k_min), float(refine_width)
     return r
 
 
@@ -900,7 +905,8 @@ def request_nodes(req: Request) -> "SimulationNodes":
     obst = [([req.obstacles[j * (D + 1) + d] for d in range(D)], req.obstacles[j * (D + 1) + D])
             for j in range(req.n_obstacles)]
     return ObstacleGridProvider([0] * D, [1] * D, req.h, obst, req.layer_thickness,
-                                req.peak_absorption, jitter=req.jitter, seed=req.seed).get()
+                                req.peak_absorption, jitter=req.jitter, seed=req.seed,
+                                refine_width=req.refine_width).get()
 
 
 def solve_alone(req: Request) -> dict:
@@ -908,7 +914,8 @@ def solve_alone(req: Request) -> dict:
     the served result)."""
     k = req.k or (16 if req.lattice == 9 else 32)
     eng = SimulationEngine(lattice=req.lattice, k=k, k_boundary=k, rbf=RBF_POLYHARMONIC3,
-                           rbf_boundary=RBF_POLYHARMONIC3, pml_ref_index=True)
+                           rbf_boundary=RBF_POLYHARMONIC3, pml_ref_index=True,
+                           k_min=req.k_min)
     p = problem_parameters(req.lattice, req.tau or 0.6, 0.4 * req.h)
     eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
     eng.init_domain(request_nodes(req))

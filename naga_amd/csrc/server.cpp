@@ -28,32 +28,43 @@ void make_nodes(const naga_request& r, Nodes& out) {
     auto check = [](int st) {
         if (st != NAGA_OK) fail(Status(st), naga_last_error());
     };
-    check(naga_obstacle_grid(D, lo, hi, r.h, r.jitter, r.seed, r.n_obstacles, r.obstacles, 0.0,
-                             r.layer_thickness, r.peak_absorption, 0, nullptr, nullptr, nullptr,
-                             out.counts));
+    check(naga_obstacle_grid(D, lo, hi, r.h, r.jitter, r.seed, r.n_obstacles, r.obstacles,
+                             r.refine_width, r.layer_thickness, r.peak_absorption, 0, nullptr,
+                             nullptr, nullptr, out.counts));
     const int64_t n = out.counts[0] + out.counts[1] + out.counts[2];
     out.pts.resize(size_t(n) * D);
     out.nrm.resize(size_t(std::max<int64_t>(out.counts[2], 1)) * D);
     out.sig.resize(size_t(std::max<int64_t>(out.counts[1], 1)));
-    check(naga_obstacle_grid(D, lo, hi, r.h, r.jitter, r.seed, r.n_obstacles, r.obstacles, 0.0,
-                             r.layer_thickness, r.peak_absorption, n, out.pts.data(),
-                             out.nrm.data(), out.sig.data(), out.counts));
+    check(naga_obstacle_grid(D, lo, hi, r.h, r.jitter, r.seed, r.n_obstacles, r.obstacles,
+                             r.refine_width, r.layer_thickness, r.peak_absorption, n,
+                             out.pts.data(), out.nrm.data(), out.sig.data(), out.counts));
 }
 
 // Upper bound of a request's node count without generating the nodes: the
-// full grid plus every obstacle's surface nodes.
+// full grid, every obstacle's surface nodes (at half spacing when refined)
+// and, with refinement, 2^D - 1 extra nodes per grid cell of the shell
+// around each obstacle.
 int64_t node_bound(const naga_request& r) {
     const int D = r.lattice == 9 ? 2 : 3;
     const double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {1.0, 1.0, 1.0};
     const int64_t grid = naga_uniform_grid_size(D, lo, hi, r.h);
     require(grid > 0, "bad grid spacing");
-    double surf = 0.0;
+    require(r.refine_width >= 0.0, "refine_width must be >= 0");
+    const double hs = r.refine_width > 0 ? 0.5 * r.h : r.h;
+    double extra = 0.0;
     for (int j = 0; j < r.n_obstacles; ++j) {
         const double R = r.obstacles[j * (D + 1) + D];
-        surf += D == 2 ? std::max(8.0, std::ceil(2.0 * M_PI * R / r.h))
-                       : std::max(16.0, std::ceil(4.0 * M_PI * R * R / (r.h * r.h)));
+        extra += D == 2 ? std::max(8.0, std::ceil(2.0 * M_PI * R / hs))
+                        : std::max(16.0, std::ceil(4.0 * M_PI * R * R / (hs * hs)));
+        if (r.refine_width > 0) {
+            const double ro = R + r.refine_width + r.h, ri = std::max(R - r.h, 0.0);
+            const double vol = D == 2 ? M_PI * (ro * ro - ri * ri)
+                                      : 4.0 / 3.0 * M_PI * (ro * ro * ro - ri * ri * ri);
+            const double cells = vol / std::pow(r.h, D) + 4.0 * std::pow(ro / r.h + 2.0, D - 1);
+            extra += double((1 << D) - 1) * cells;
+        }
     }
-    return grid + int64_t(surf);
+    return grid + int64_t(std::ceil(extra));
 }
 
 int stencil_k(const naga_request& r) { return r.k > 0 ? r.k : (r.lattice == 9 ? 16 : 32); }
@@ -215,6 +226,7 @@ void Server::setup_loop() {
             SceneConfig cfg;
             cfg.Q = r.lattice;
             cfg.k = cfg.kb = stencil_k(r);
+            cfg.k_min = r.k_min;
             cfg.rbf = cfg.rbf_boundary = 1;
             cfg.pml_ref = 1;
             cfg.h = r.h;

@@ -1204,7 +1204,10 @@ int union_tile_nodes(int k) {
     return (k <= 32 && want64) ? 2 * kV4Nodes : kV4Nodes;
 }
 
-int union_cap_default(int k) { return union_tile_nodes(k) == kV4Nodes ? 272 : 368; }
+int union_cap_default(int k) {
+    if (union_tile_nodes(k) != kV4Nodes) return 368;  // 64-node tiles, two blocks per CU
+    return k <= 32 ? 272 : 440;  // 32-node tiles: three blocks per CU, two for k = 48
+}
 
 namespace {
 

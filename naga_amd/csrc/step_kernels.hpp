@@ -152,9 +152,10 @@ constexpr int kFslabPad = 2;  // padding records per fslab row with tile unions
 // but two blocks per CU and 9% of unions over the LDS cap: 0.23 against
 // 0.20 ms per RK4 stage on the 1M-node D3Q27 scene).
 int union_tile_nodes(int k);
-// Default LDS union rows per block for that tile size: 368 at 64 nodes
-// (two blocks per CU), 272 at 32 nodes (three blocks per CU).  A tile with
-// a larger union reads its rows from HBM through the union list.
+// Default LDS union rows per block: 272 for 32-node tiles and k <= 32
+// (three blocks per CU), 440 for k = 48 (two blocks per CU, the records
+// taking 25 KB), 368 for 64-node tiles (two blocks per CU).  A tile with a
+// larger union reads its rows from HBM through the union list.
 int union_cap_default(int k);
 constexpr int kUnionRowsMax = 512;  // LDS union rows per block, upper bound
 

@@ -54,3 +54,27 @@ def test_small_budget_admits_one_at_a_time():
     for r, i in zip(reqs, ids):
         assert rel_l2(srv.result(i)["drho"], naga_amd.solve_alone(r)["drho"]) < 1e-6
     srv.close()
+
+
+def test_served_refined_variable_stencils_equal_alone():
+    """Requests laid out as the irregular-domain workload (box minus a
+    central sphere or circle, 2x refinement near it, per-node stencil widths
+    by density) served together equal the same solves alone."""
+    reqs = []
+    for i, (lat, h, steps) in enumerate([(27, 0.08, 20), (9, 0.02, 35), (27, 0.1, 12),
+                                         (9, 0.025, 25)]):
+        D = 2 if lat == 9 else 3
+        k, kmin = (48, 24) if D == 3 else (24, 12)
+        reqs.append(naga_amd.make_request(
+            lat, h, steps, [([0.5] * D, 0.15)], k=k, k_min=kmin, refine_width=1.5 * h,
+            seed=100 + i, pulse_center=[0.3] + [0.5] * (D - 1), pulse_width=0.1))
+    srv = naga_amd.SolveServer(hbm_budget_bytes=8 << 30, max_active=8, chunk_steps=9)
+    ids = [srv.submit(r) for r in reqs]
+    srv.drain()
+    assert srv.stats()["completed"] == len(reqs)
+    for r, i in zip(reqs, ids):
+        got, ref = srv.result(i), naga_amd.solve_alone(r)
+        assert got["nodes"] == ref["nodes"]
+        assert rel_l2(got["drho"], ref["drho"]) < 1e-6, (r.lattice, r.h)
+        assert rel_l2(got["u"], ref["u"]) < 1e-6
+    srv.close()
