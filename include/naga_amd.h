@@ -184,7 +184,8 @@ int naga_batch_step_timed(naga_batch* b, int32_t nsteps, double* ms);
  *      stepping thread advances all resident scenes together in chunks of
  *      at most chunk_steps steps; finished scenes leave at once and free
  *      their memory.  Scenes are the unit box minus obstacles
- *      (naga_obstacle_grid), dt = 0.4 h, c = c_s, rho0 = 1, r^3 RBFs. */
+ *      (naga_obstacle_grid), dt = 0.4 of the finest spacing (h, or h/2
+ *      with refinement), c = c_s, rho0 = 1, r^3 RBFs. */
 typedef struct naga_server naga_server;
 #define NAGA_MAX_OBSTACLES 8
 typedef struct naga_request {

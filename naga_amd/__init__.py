@@ -735,8 +735,8 @@ class AdvectionOperator:
                      boundary_idx_start=0, steps=1):
         """step_forward (advection.cuh:460-492): `steps` advection_task RK4
         steps (advection.cuh:207-276) from f0; returns f.  velocity_field:
-        ConstantVelocityField or an (n, D) per-node array.  Nodes at or past boundary_idx_start (> 0)
-        keep their f0 value."""
+        ConstantVelocityField or an (n, D) per-node array.  Nodes at or past
+        boundary_idx_start (> 0) keep their f0 value."""
         f = _f32(f0).copy()
         if f.shape != (self._n,):
             raise ValueError(self._PREFIX + "f0 must have the same number of elements as the "
@@ -877,7 +877,8 @@ def make_request(lattice, h, steps, obstacles=(), k=0, jitter=0.25, seed=0x4E414
     """A solve of the unit box (square or cube) minus obstacles
     [(centre, radius), ...]; the absorbing layer defaults to 8 spacings.
     refine_width > 0 doubles the node density within that distance of the
-    obstacles; k_min > 0 gives per-node stencil widths in [k_min, k]."""
+    obstacles (the time step is then 0.4 of the refined spacing h/2);
+    k_min > 0 gives per-node stencil widths in [k_min, k]."""
     D = 2 if lattice == 9 else 3
     r = Request()
     r.lattice, r.k, r.steps = int(lattice), int(k), int(steps)
@@ -916,7 +917,8 @@ def solve_alone(req: Request) -> dict:
     eng = SimulationEngine(lattice=req.lattice, k=k, k_boundary=k, rbf=RBF_POLYHARMONIC3,
                            rbf_boundary=RBF_POLYHARMONIC3, pml_ref_index=True,
                            k_min=req.k_min)
-    p = problem_parameters(req.lattice, req.tau or 0.6, 0.4 * req.h)
+    hmin = 0.5 * req.h if req.refine_width > 0 else req.h  # finest spacing
+    p = problem_parameters(req.lattice, req.tau or 0.6, 0.4 * hmin)
     eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
     eng.init_domain(request_nodes(req))
     D = 2 if req.lattice == 9 else 3

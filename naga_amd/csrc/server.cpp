@@ -230,7 +230,8 @@ void Server::setup_loop() {
             cfg.rbf = cfg.rbf_boundary = 1;
             cfg.pml_ref = 1;
             cfg.h = r.h;
-            cfg.dt = 0.4 * r.h;
+            // dt = 0.4 of the finest spacing (h / 2 in a refined shell)
+            cfg.dt = 0.4 * (r.refine_width > 0 ? 0.5 * r.h : r.h);
             cfg.c = kCsLat;
             cfg.rho0 = 1.0;
             cfg.nu = viscosity_for(r.lattice, r.tau > 0 ? r.tau : 0.6, cfg.dt);

@@ -75,6 +75,7 @@ def test_served_refined_variable_stencils_equal_alone():
     for r, i in zip(reqs, ids):
         got, ref = srv.result(i), naga_amd.solve_alone(r)
         assert got["nodes"] == ref["nodes"]
+        assert np.all(np.isfinite(got["drho"])) and np.all(np.isfinite(got["u"]))
         assert rel_l2(got["drho"], ref["drho"]) < 1e-6, (r.lattice, r.h)
         assert rel_l2(got["u"], ref["u"]) < 1e-6
     srv.close()
