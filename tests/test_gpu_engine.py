@@ -43,7 +43,7 @@ def test_cfg1_initial_state(cfg1):
     assert rel_l2(g["drho"], o["drho"]) < 1e-7
 
 
-@pytest.mark.parametrize("nsteps", [1, 20, 100])
+@pytest.mark.parametrize("nsteps", [1, 20, 100, 1000])
 def test_cfg1_steps(cfg1, nsteps):
     gpu, ora = cfg1["gpu"], cfg1["ora"]
     done = int(round(ora.time / 0.004))
@@ -89,6 +89,27 @@ def test_3d_small_steps(oracle64):
     err_rho = rel_l2(g["drho"], o["drho"])
     err_u = rel_l2(g["u"], o["u"])
     print(f"3d rel_l2 drho={err_rho:.3e} u={err_u:.3e}")
+    assert err_rho < TOL and err_u < TOL
+
+
+def test_3d_200_steps(oracle64):
+    """The 3D configuration's physics and run length (D3Q27, k = 32, r^3
+    RBF, PML 8h, tau 0.6, dt 0.4h, pulse 1e-3 / 0.05 at (0.3, 0.5, 0.5),
+    200 steps) on the h = 0.02 cube (132,651 nodes) against the FP64
+    oracle."""
+    import os
+    h = 0.02
+    nodes = grid_nodes(oracle64, 3, h, layer=8 * h)
+    gpu = gpu_engine(nodes, 27, 32, h)
+    ora = oracle64.engine(nodes, Q=27, k=32, k_boundary=32, rbf=1, rbf_boundary=1, h=h,
+                          threads=min(16, os.cpu_count() or 1))
+    gpu.set_initial_pulse([0.3, 0.5, 0.5], 1e-3, 0.05)
+    ora.set_pulse([0.3, 0.5, 0.5], 1e-3, 0.05)
+    gpu.step_forward(200)
+    ora.step(200)
+    g, o = gpu.macroscopic(fresh=True), ora.get(fresh=True)
+    err_rho, err_u = rel_l2(g["drho"], o["drho"]), rel_l2(g["u"], o["u"])
+    print(f"3d 200 steps rel_l2 drho={err_rho:.3e} u={err_u:.3e}")
     assert err_rho < TOL and err_u < TOL
 
 
