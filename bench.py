@@ -211,6 +211,65 @@ def served_workload(n_requests: int = 32, rate: float = 50.0, seed: int = 0x4E41
                         "within 0.05, k 24-48; 2D: 0-5 circles, k 16"}
 
 
+def other_configs(threads: int) -> dict:
+    """Throughput of the 2D plain scene (D2Q9, 10,201 nodes, k = 16, PML
+    10h, 1000 steps) and the irregular 3D scene (cube minus a refined
+    sphere, 2.24M nodes, k 24-48, 300 steps) on the GPU, and of the 2D
+    scene on the oracle's FP32 OpenMP port; setup times beside them."""
+    import naga_amd
+    from oracle import Oracle
+    out = {}
+    h = 0.01
+    o = Oracle(32)
+    nd = o.uniform_grid(2, [0, 0], [1, 1], h, jitter=0.25, layer=10 * h, sigma_max=0.5)
+    nodes = naga_amd.SimulationNodes(nd["points"], nd["normals"], nd["sigma"], nd["n_bulk"],
+                                     nd["n_layer"], nd["n_boundary"], h)
+    eng = naga_amd.SimulationEngine(lattice=9, k=16, rbf=naga_amd.RBF_POLYHARMONIC3)
+    p = naga_amd.problem_parameters(9, 0.6, 0.4 * h)
+    eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+    t0 = time.perf_counter()
+    eng.init_domain(nodes)
+    setup = time.perf_counter() - t0
+    eng.set_initial_pulse([0.5, 0.5], 1e-3, 0.05)
+    eng.step_forward(10)
+    t0 = time.perf_counter()
+    eng.step_forward(1000)
+    wall = time.perf_counter() - t0
+    out["2d_plain"] = {"nodes": eng.num_nodes, "steps": 1000, "setup_s": setup,
+                       "node_updates_per_s": eng.num_nodes * 1000 / wall}
+    eng.close()
+    t0 = time.perf_counter()
+    ora = o.engine(nd, Q=9, k=16, h=h, tau=0.6, threads=threads)
+    cpu_setup = time.perf_counter() - t0
+    ora.set_pulse([0.5, 0.5], 1e-3, 0.05)
+    ora.step(1)
+    t0 = time.perf_counter()
+    ora.step(200)
+    out["2d_plain"]["cpu_node_updates_per_s"] = ora.N * 200 / (time.perf_counter() - t0)
+    out["2d_plain"]["cpu_setup_s"] = cpu_setup
+    out["2d_plain"]["cpu_cores"] = threads
+    h = 0.008
+    nodes = naga_amd.ObstacleGridProvider([0, 0, 0], [1, 1, 1], h, [([0.5, 0.5, 0.5], 0.15)],
+                                          8 * h, 0.5, jitter=0.25, refine_width=0.05).get()
+    eng = naga_amd.SimulationEngine(lattice=27, k=48, k_boundary=32,
+                                    rbf=naga_amd.RBF_POLYHARMONIC3,
+                                    rbf_boundary=naga_amd.RBF_POLYHARMONIC3, k_min=24)
+    p = naga_amd.problem_parameters(27, 0.6, 0.4 * (h / 2))
+    eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+    t0 = time.perf_counter()
+    eng.init_domain(nodes)
+    setup = time.perf_counter() - t0
+    eng.set_initial_pulse([0.3, 0.5, 0.5], 1e-3, 0.05)
+    eng.step_forward(5)
+    t0 = time.perf_counter()
+    eng.step_forward(300)
+    wall = time.perf_counter() - t0
+    out["3d_irregular"] = {"nodes": eng.num_nodes, "steps": 300, "setup_s": setup,
+                           "node_updates_per_s": eng.num_nodes * 300 / wall}
+    eng.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +283,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched and served workloads")
     ap.add_argument("--batched-steps", type=int, default=500)
+    ap.add_argument("--other-configs", action="store_true",
+                    help="also time the 2D plain and irregular 3D scenes")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -296,8 +357,10 @@ def main() -> int:
                 out[name] = fn()
             except Exception as exc:  # reported, not fatal to the headline line
                 out[name] = {"error": f"{type(exc).__name__}: {exc}"}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if args.other_configs:
+        out["other_configs"] = other_configs(threads)
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(threads)
     print(json.dumps(out))
     return 0
