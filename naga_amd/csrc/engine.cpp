@@ -760,10 +760,13 @@ void Batch::rebuild() {
     drop_graph();
     groups_.clear();
     any_sources_ = false;
+    // groups: lattice, four-per-lane stencil width, and whether the tile
+    // unions (and their padded slab layout) were built
     auto key_of = [](Engine* e) {
         const int Q = e->lattice();
         const bool un = step_uses_unions(Q, e->stencil_k(), e->nodes());
-        return Q * 1000 + (un ? e->stencil_k() : 0);
+        const bool tiles = e->scene_desc().S.ulist != nullptr;
+        return Q * 1000 + (un ? e->stencil_k() : 0) + (tiles ? 500 : 0);
     };
     std::vector<int> keys;
     for (Engine* e : scenes_) {

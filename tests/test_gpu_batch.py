@@ -106,3 +106,22 @@ def test_autotune_leaves_scenes_unchanged(oracle64):
     for a, b in zip(alone, mixed):
         assert a.time() == pytest.approx(b.time(), abs=1e-12)
         assert rel_l2(b.macroscopic(fresh=True)["drho"], a.macroscopic(fresh=True)["drho"]) < 1e-6
+
+
+def test_batch_mixes_union_and_direct_scenes(oracle64, monkeypatch):
+    """D3Q27 scenes built with and without tile unions (different slab
+    layouts) step in separate groups of one batch, each as it does alone."""
+    specs = [(3, 0.1, 32, 2, [0.5, 0.4, 0.5]), (3, 0.125, 32, 2, [0.4, 0.5, 0.5])]
+    scenes, alone = [], []
+    for flag, s in zip(("1", "0"), specs):
+        monkeypatch.setenv("NAGA_UNIONS", flag)
+        scenes.append(_scene(oracle64, *s))
+        alone.append(_scene(oracle64, *s))
+    batch = naga_amd.SceneBatch(scenes)
+    batch.step_forward(15)
+    for e in alone:
+        e.step_forward(15)
+    for a, b in zip(alone, scenes):
+        ma, mb = a.macroscopic(fresh=True), b.macroscopic(fresh=True)
+        assert rel_l2(mb["drho"], ma["drho"]) < 1e-6
+    batch.close()
