@@ -81,6 +81,7 @@ Engine::Engine(const SceneConfig& cfg) : cfg_(cfg) {
 
 Engine::~Engine() {
     if (batch) batch->remove(this);
+    drop_step_graph();
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
@@ -94,6 +95,7 @@ void Engine::require_domain() const {
 void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_t nbnd,
                          const float* normals, const float* sigma) {
     require(batch == nullptr, "remove the engine from its batch before init_domain");
+    drop_step_graph();  // the captured step refers to the old buffers
     Timer timer;
     const int D = D_, Q = cfg_.Q, k = cfg_.k, kb = cfg_.kb;
     // shape checks of detail/simulation_engine.cuh:128-167
@@ -642,10 +644,55 @@ void Engine::launch_one_step(hipEvent_t* ev) {
     ++frame_;
 }
 
+void Engine::drop_step_graph() {
+    if (step_exec_) (void)hipGraphExecDestroy(step_exec_);
+    if (step_graph_) (void)hipGraphDestroy(step_graph_);
+    step_exec_ = nullptr;
+    step_graph_ = nullptr;
+}
+
+void Engine::launch_steps(int nsteps) {
+    static const bool graphs = [] {
+        const char* e = std::getenv("NAGA_ENGINE_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    // density sources change kernel arguments every step: direct launches
+    if (!graphs || !sources_.empty() || nsteps < 2) {
+        for (int n = 0; n < nsteps; ++n) launch_one_step(nullptr);
+        return;
+    }
+    if (!step_exec_) {
+        // the step's kernels take the same arguments every step (buffers,
+        // geometry, rates); the clock is advanced on the host per replay
+        bool ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed) == hipSuccess;
+        if (ok) {
+            const StepBuffers S = buffers();
+            if (cfg_.Q == 27)
+                launch_step<27>(geo_, S, cp_, stream_, nullptr);
+            else
+                launch_step<9>(geo_, S, cp_, stream_, nullptr);
+            ok = hipStreamEndCapture(stream_, &step_graph_) == hipSuccess && step_graph_;
+        }
+        ok = ok && hipGraphInstantiate(&step_exec_, step_graph_, nullptr, nullptr, 0) ==
+                       hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            drop_step_graph();
+            for (int n = 0; n < nsteps; ++n) launch_one_step(nullptr);
+            return;
+        }
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        NAGA_HIP(hipGraphLaunch(step_exec_, stream_));
+        time_ += cfg_.dt;
+        ++frame_;
+    }
+}
+
 void Engine::step(int nsteps) {
     require_domain();
     require(nsteps >= 0, "nsteps must be >= 0");
-    for (int n = 0; n < nsteps; ++n) launch_one_step(nullptr);
+    launch_steps(nsteps);
     NAGA_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -655,7 +702,7 @@ double Engine::step_timed(int nsteps) {
     NAGA_HIP(hipEventCreate(&a));
     NAGA_HIP(hipEventCreate(&b));
     NAGA_HIP(hipEventRecord(a, stream_));
-    for (int n = 0; n < nsteps; ++n) launch_one_step(nullptr);
+    launch_steps(nsteps);
     NAGA_HIP(hipEventRecord(b, stream_));
     NAGA_HIP(hipEventSynchronize(b));
     float ms = 0.0f;

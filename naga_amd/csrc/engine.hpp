@@ -98,8 +98,15 @@ class Engine {
   private:
     void require_domain() const;
     void launch_one_step(hipEvent_t* ev);
+    // nsteps steps on the stream: one step captured once into a hipGraph and
+    // replayed (no density sources; NAGA_ENGINE_GRAPH=0 disables), else
+    // direct launches
+    void launch_steps(int nsteps);
+    void drop_step_graph();
     void refresh_macroscopic_state();
     StepBuffers buffers();
+    hipGraph_t step_graph_ = nullptr;
+    hipGraphExec_t step_exec_ = nullptr;
 
     SceneConfig cfg_;
     int D_ = 3, RS_ = 32;

@@ -331,3 +331,23 @@ def test_irregular_domain_2m_nodes(monkeypatch):
     _, m = run(300)
     assert np.all(np.isfinite(m["drho"])) and np.all(np.isfinite(m["u"]))
     assert np.abs(m["drho"]).max() < 1.5e-3
+
+
+@pytest.mark.parametrize("Q", [9, 27])
+def test_step_graph_replays_match_direct_launches(oracle64, Q):
+    """step_forward(n >= 2) replays one captured step; step_forward(1)
+    launches directly.  Replays interleaved with direct steps and host
+    reads give the direct-launch state bit for bit after every call."""
+    D = 2 if Q == 9 else 3
+    h = 0.02 if D == 2 else 0.1
+    nodes = grid_nodes(oracle64, D, h, layer=3 * h)
+    a = gpu_engine(nodes, Q, 16 if D == 2 else 32, h)
+    b = gpu_engine(nodes, Q, 16 if D == 2 else 32, h)
+    for e in (a, b):
+        e.set_initial_pulse([0.4] + [0.5] * (D - 1), 1e-3, 0.1)
+    for n in (3, 1, 5, 1, 2, 20):
+        b.step_forward(n)
+        for _ in range(n):
+            a.step_forward(1)
+        assert np.array_equal(a.distributions()["f"], b.distributions()["f"]), n
+        assert abs(a.time() - b.time()) < 1e-12
