@@ -81,11 +81,64 @@ __device__ __forceinline__ float lane_c(int a, int d) {
 // Reference: simulation_engine.cuh:366-433 (macroscopic), :469-531
 // (collision), :876-877 (boundary velocity reset).
 // ---------------------------------------------------------------------------
+// A node's deviation row as contiguous floats.
+struct FlatRow {
+    float* p;
+    __device__ float get(int a) const { return p[a]; }
+    __device__ void set(int a, float v) const { p[a] = v; }
+};
+
+// Macroscopic values, density source and FP64 collision of node i, its
+// deviation row r[0..Q) updated in place.
+template <int Q, class Row>
+__device__ __forceinline__ void p1_node(const StepGeom& G, const StepBuffers& S,
+                                        const CollisionParams& cp, int64_t i, Row r) {
+    using L = Lattice<Q>;
+    constexpr int D = L::D, RS = L::RS;
+    double f[Q];
+    double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        const double gv = double(r.get(a));
+        rt += gv;
+#pragma unroll
+        for (int d = 0; d < D; ++d) mom[d] += gv * double(L::c(a, d));
+        f[a] = L::w(a) + gv;
+    }
+    double u[3] = {0.0, 0.0, 0.0};
+    const double inv_rho = 1.0 / (1.0 + rt);
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = mom[d] * inv_rho;
+    // density source (simulation_engine.cuh:1057-1107, once per node):
+    // f += w s / rho0, rho <- (rho + (rho + s)) / 2, u unchanged
+    const double ws = S.src ? double(S.src[i]) / double(G.rho0) : 0.0;
+    if (ws != 0.0) {
+#pragma unroll
+        for (int a = 0; a < Q; ++a) f[a] += L::w(a) * ws;
+        rt += 0.5 * ws;
+    }
+    const double rho = 1.0 + rt;
+    const bool bnd = i >= G.nfluid;
+    S.rho[i] = float(rho * double(G.rho0));
+#pragma unroll
+    for (int d = 0; d < D; ++d) S.u[d * G.N + i] = bnd ? 0.0f : float(u[d] * double(G.vel_scale));
+    S.ru[i] = make_float4(float(rt), float(u[0]), float(u[1]), float(u[2]));
+    double df[Q];
+    collide<Q>(f, rho, u, cp, df);
+    const bool src_node = !bnd && S.bsrc[i];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        const double v = bnd ? df[a] : double(r.get(a)) + L::w(a) * ws + df[a];
+        r.set(a, float(v));
+        if (src_node) S.coll[i * RS + a] = float(df[a]);
+    }
+}
+
 template <int Q>
 __global__ __launch_bounds__(kP1Block) void k_p1(StepGeom G, StepBuffers S,
                                                  CollisionParams cp, BatchArgs B) {
     using L = Lattice<Q>;
-    constexpr int D = L::D, RS = L::RS, LS = RS + 1;
+    constexpr int RS = L::RS, LS = RS + 1;
     __shared__ float rows[kP1Block * LS];
     const BlockCtx bc = locate(B, G, S);
     if (B.scenes) cp = B.scenes[bc.scene].cp;
@@ -96,48 +149,7 @@ __global__ __launch_bounds__(kP1Block) void k_p1(StepGeom G, StepBuffers S,
     for (int e = threadIdx.x; e < cnt * RS; e += kP1Block) rows[(e / RS) * LS + e % RS] = src[e];
     __syncthreads();
     const int t = threadIdx.x;
-    if (t < cnt) {
-        const int64_t i = node0 + t;
-        float* r = rows + t * LS;
-        double f[Q];
-        double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int a = 0; a < Q; ++a) {
-            const double gv = double(r[a]);
-            rt += gv;
-#pragma unroll
-            for (int d = 0; d < D; ++d) mom[d] += gv * double(L::c(a, d));
-            f[a] = L::w(a) + gv;
-        }
-        double u[3] = {0.0, 0.0, 0.0};
-        const double inv_rho = 1.0 / (1.0 + rt);
-#pragma unroll
-        for (int d = 0; d < D; ++d) u[d] = mom[d] * inv_rho;
-        // density source (simulation_engine.cuh:1057-1107, once per node):
-        // f += w s / rho0, rho <- (rho + (rho + s)) / 2, u unchanged
-        const double ws = S.src ? double(S.src[i]) / double(G.rho0) : 0.0;
-        if (ws != 0.0) {
-#pragma unroll
-            for (int a = 0; a < Q; ++a) f[a] += L::w(a) * ws;
-            rt += 0.5 * ws;
-        }
-        const double rho = 1.0 + rt;
-        const bool bnd = i >= G.nfluid;
-        S.rho[i] = float(rho * double(G.rho0));
-#pragma unroll
-        for (int d = 0; d < D; ++d)
-            S.u[d * G.N + i] = bnd ? 0.0f : float(u[d] * double(G.vel_scale));
-        S.ru[i] = make_float4(float(rt), float(u[0]), float(u[1]), float(u[2]));
-        double df[Q];
-        collide<Q>(f, rho, u, cp, df);
-        const bool src_node = !bnd && S.bsrc[i];
-#pragma unroll
-        for (int a = 0; a < Q; ++a) {
-            const double v = bnd ? df[a] : double(r[a]) + L::w(a) * ws + df[a];
-            r[a] = float(v);
-            if (src_node) S.coll[i * RS + a] = float(df[a]);
-        }
-    }
+    if (t < cnt) p1_node<Q>(G, S, cp, node0 + t, FlatRow{rows + t * LS});
     __syncthreads();
     float* dst = S.A + node0 * RS;
     for (int e = threadIdx.x; e < cnt * RS; e += kP1Block) dst[e] = rows[(e / RS) * LS + e % RS];
@@ -691,6 +703,56 @@ __device__ __forceinline__ void union_copy(float4* urows, const UnionIdx<NT>& x,
 __device__ __forceinline__ void dma_fence() {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+}
+
+// Collision for one D3Q27 scene: one wave per block of 64 nodes, whose
+// rows arrive in LDS by DMA (no VGPR round trip), are collided in place
+// and stored as coalesced 16-byte pieces.  Piece j of chunk node n sits at
+// slot j * 64 + (n ^ j): the DMA writes the wave's 64 slots of one piece
+// contiguously, and both a lane's reads of its own row and the store of
+// the chunk are free of bank conflicts.  Three waves per SIMD (168 VGPRs)
+// measured faster than the VGPR-staged kernel at any occupancy.
+constexpr int kP1Chunk = 64;
+#ifndef NAGA_P1_DMA
+#define NAGA_P1_DMA 1
+#endif
+
+// The DMA collision runs with the four-per-lane D3Q27 kernels (their
+// 2^25-node bound keeps its 32-bit row offsets exact); the flag is uniform
+// over a batch group.
+bool p1_uses_dma(int Q, bool unions) { return NAGA_P1_DMA && Q == 27 && unions; }
+
+struct XorRow {
+    float* buf;  // pieces of the chunk, swizzled as above
+    int t;       // node within the chunk
+    __device__ float& at(int a) const {
+        const int j = a >> 2;
+        return buf[(j * kP1Chunk + (t ^ j)) * 4 + (a & 3)];
+    }
+    __device__ float get(int a) const { return at(a); }
+    __device__ void set(int a, float v) const { at(a) = v; }
+};
+
+__global__ __launch_bounds__(kP1Chunk) __attribute__((amdgpu_waves_per_eu(3))) void k_p1_dma(
+    StepGeom G, StepBuffers S, CollisionParams cp, BatchArgs B) {
+    __shared__ float4 buf[8 * kP1Chunk];
+    const int t = threadIdx.x;
+    const BlockCtx bc = locate(B, G, S);
+    if (B.scenes) cp = B.scenes[bc.scene].cp;
+    const __amdgpu_buffer_rsrc_t gsrc = rsrc(S.g, uint64_t(G.N) * 128u);
+    const int64_t node0 = bc.b * kP1Chunk;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        dma16(gsrc, uint32_t(node0 + (t ^ j)) * 128u + uint32_t(j) * 16u, buf + j * kP1Chunk);
+    dma_fence();
+    if (node0 + t < G.N) p1_node<27>(G, S, cp, node0 + t, XorRow{(float*)buf, t});
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int p = r * kP1Chunk + t, n = p >> 3, j = p & 7;
+        if (node0 + n < G.N)
+            st4(S.A, uint32_t(node0 + n) * 128u + uint32_t(j) * 16u, buf[j * kP1Chunk + (n ^ j)]);
+    }
 }
 
 // RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
@@ -1302,7 +1364,13 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         return;
     }
     switch (pass) {
-        case kP1: k_p1<Q><<<nb, kP1Block, 0, st>>>(G, S, cp, c.B); break;
+        case kP1:
+            if (p1_uses_dma(Q, c.unions)) {
+                k_p1_dma<<<nb, kP1Chunk, 0, st>>>(G, S, cp, c.B);
+            } else {
+                k_p1<Q><<<nb, kP1Block, 0, st>>>(G, S, cp, c.B);
+            }
+            break;
         case kPl:
             if (v4) k_pml_local_v4<<<nb, kBlock, 0, st>>>(G, S, c.B);
             else k_pml_local<Q><<<nb, kBlock, 0, st>>>(G, S, c.B);
@@ -1359,7 +1427,7 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
     auto tiles32 = [&](int64_t n) { return (n + kV4Nodes - 1) / kV4Nodes; };
     const bool pml = G.nlayer > 0 && S.n_rq > 0;
     const int64_t nbnd = G.N - G.nfluid;
-    nb[kP1] = (G.N + kP1Block - 1) / kP1Block;
+    nb[kP1] = p1_uses_dma(Q, un) ? (G.N + kP1Chunk - 1) / kP1Chunk : (G.N + kP1Block - 1) / kP1Block;
     nb[kPl] = pml ? S.nparts : 0;
     nb[kPm] = pml ? 2 * Q : 0;
     nb[kPf] = pml ? (un ? tiles32(G.nlayer) : lanes(G.nlayer)) : 0;
