@@ -200,8 +200,9 @@ __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S,
         const double q2 = (q1 + double(S.Qp[o])) * hdt;
         S.E[o] = float(fte);
         S.Qp[o] = float(q1);
-        S.Q2v[o] = float(q2);
-        if (i >= G.nbulk && i < G.nfluid) {
+        const bool layer = i >= G.nbulk && i < G.nfluid;
+        if (G.pml_ref ? i < G.nlayer : layer) S.Q2v[o] = float(q2);  // rows the layer field reads
+        if (layer) {
             const double sig = double(S.sigma[i - G.nbulk]);
             S.A[o] = float(double(S.A[o]) - sig * (3.0 * fte + 3.0 * sig * q1 + sig * sig * q2));
             s1 += q1;
@@ -1091,7 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_pml_local_v4(StepGeom G, StepBuffers
         }
         st4(S.E, own, eo);
         st4(S.Qp, own, po);
-        st4(S.Q2v, own, qo);
+        if (G.pml_ref ? i < G.nlayer : layer) st4(S.Q2v, own, qo);  // rows the layer field reads
         if (layer) st4(S.A, own, a4);
     }
 #pragma unroll
