@@ -230,11 +230,12 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         ulist_.allocate(std::max<int64_t>(off[ntiles], 1));
         launch_tile_unions(nfluid, k, T, idx_.get(), nullptr, uoff_.get(), ulist_.get(),
                            slab_.get(), D, fslab_.get(), 1, st);
-        // the union kernels copy records into LDS as they lie: rows padded to
-        // k + 1 divergence records and k + 2 filter records keep the records
-        // of consecutive nodes in distinct LDS banks
-        DeviceBuffer<float> ps(nfluid * (k + kSlabPad) * (1 + D)), pf(nfluid * (k + kFslabPad) * 2);
-        launch_pad_records(nfluid, k, 1 + D, k + kSlabPad, slab_.get(), ps.get(), st);
+        // the union kernels copy records into LDS as they lie: packed
+        // divergence records (16-bit union positions) and filter rows padded
+        // to k + 2 records keep the records of consecutive nodes in distinct
+        // LDS banks
+        DeviceBuffer<float> ps(nfluid * packed_div_units(k) * 4), pf(nfluid * (k + kFslabPad) * 2);
+        launch_pack_div_records(nfluid, k, slab_.get(), ps.get(), st);
         launch_pad_records(nfluid, k, 2, k + kFslabPad, fslab_.get(), pf.get(), st);
         NAGA_HIP(hipStreamSynchronize(st));
         slab_ = std::move(ps);

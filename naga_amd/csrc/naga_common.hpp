@@ -43,6 +43,14 @@ inline void require(bool ok, const std::string& msg) {
     if (!ok) fail(kInvalidArgument, msg);
 }
 
+// Divergence records of a node with tile unions, packed: per group of 8
+// records, six 16-byte units of weights (record r's x, y, z at floats 3r ..
+// 3r + 2) and one unit of eight 16-bit union positions (record 2m in the
+// low half of dword m, 2m + 1 in the high half); then one zero unit, which
+// makes the row length odd in units so the records of consecutive nodes
+// fall in distinct LDS banks.  k must be a multiple of 8.
+constexpr int packed_div_units(int k) { return 7 * (k / 8) + 1; }
+
 #if defined(__HIPCC__)
 inline void hip_check(hipError_t e, const char* what) {
     if (e == hipSuccess) return;

@@ -505,6 +505,30 @@ __global__ void k_pad_records(int64_t n, int k, int rec, int kstride, const floa
     dst[t] = s < k ? src[(i * k + s) * rec + c] : 0.0f;
 }
 
+// One thread per (node, 32-bit word) of the packed divergence records.
+__global__ void k_pack_div_records(int64_t n, int k, const float* src, uint32_t* dst) {
+    const int units = packed_div_units(k);
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n * units * 4) return;
+    const int64_t i = t / (int64_t(units) * 4);
+    const int word = int(t - i * units * 4);
+    const int u = word / 4, g = u / 7, w = u % 7;
+    const float* rec = src + (i * k + 8 * g) * 4;  // the group's 8 records
+    uint32_t v = 0;
+    if (u == units - 1) {
+        v = 0;  // padding unit
+    } else if (w < 6) {
+        const int f = 4 * w + word % 4;  // float f of the group's 24 weights
+        v = __float_as_uint(rec[(f / 3) * 4 + 1 + f % 3]);
+    } else {
+        const int m = word % 4;  // records 2m, 2m + 1
+        const uint32_t lo = __float_as_uint(rec[(2 * m) * 4]);
+        const uint32_t hi = __float_as_uint(rec[(2 * m + 1) * 4]);
+        v = (lo & 0xffffu) | (hi << 16);
+    }
+    dst[t] = v;
+}
+
 // One block per tile: the tile's k * tile_nodes (<= 4096) neighbour indices
 // are sorted in LDS (bitonic), flagged where they differ from their
 // predecessor and compacted by a block-wide scan of the flags.
@@ -822,6 +846,15 @@ void launch_pad_records(int64_t n, int k, int rec, int kstride, const float* src
     const int64_t tot = n * kstride * rec;
     k_pad_records<<<(tot + 255) / 256, 256, 0, st>>>(n, k, rec, kstride, src, dst);
     hip_check(hipGetLastError(), "pad records");
+}
+
+void launch_pack_div_records(int64_t n, int k, const float* src, float* dst, hipStream_t st) {
+    require(k % 8 == 0 && k <= 64, "packed divergence records need k a multiple of 8");
+    if (n == 0) return;
+    const int64_t tot = n * packed_div_units(k) * 4;
+    k_pack_div_records<<<(tot + 255) / 256, 256, 0, st>>>(n, k, src,
+                                                          reinterpret_cast<uint32_t*>(dst));
+    hip_check(hipGetLastError(), "pack divergence records");
 }
 
 void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx,

@@ -624,6 +624,58 @@ __device__ __forceinline__ float4 rate_fetch(const float4* rp, Fetch fetch, int 
     return r;
 }
 
+// rate_fetch over packed records (packed_div_units layout): per group of 8
+// records six weight units and one unit of 16-bit union positions.  The
+// same products in the same order as rate_fetch, so the two agree bit for
+// bit.
+#ifndef NAGA_PACK_SUB
+#define NAGA_PACK_SUB 4
+#endif
+template <int K, class Fetch>
+__device__ __forceinline__ float4 rate_fetch_packed(const float4* rp, Fetch fetch, int q) {
+    static_assert(K % 8 == 0, "packed records come in groups of 8");
+    constexpr int H = NAGA_PACK_SUB;  // records gathered per sub-batch: 8 or 4
+    static_assert(H == 8 || H == 4, "sub-batch of 8 or 4 records");
+    float tx[4] = {0.f, 0.f, 0.f, 0.f}, ty[4] = {0.f, 0.f, 0.f, 0.f},
+          tz[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = 0; g < K / 8; ++g) {
+        const float4 iu = rp[7 * g + 6];
+#pragma unroll
+        for (int h = 0; h < 8; h += H) {
+            // records h .. h + H - 1: weights in units 3h/4 .. 3(h + H)/4 - 1
+            float4 wu[3 * H / 4], gv[H];
+#pragma unroll
+            for (int r = 0; r < H; ++r) {
+                const uint32_t word = __float_as_uint(get(iu, (h + r) >> 1));
+                gv[r] = fetch(((h + r) & 1) ? word >> 16 : word & 0xffffu);
+            }
+#pragma unroll
+            for (int w = 0; w < 3 * H / 4; ++w) wu[w] = rp[7 * g + 3 * h / 4 + w];
+#pragma unroll
+            for (int r = 0; r < H; ++r) {
+                const float wx = get(wu[(3 * r) >> 2], (3 * r) & 3);
+                const float wy = get(wu[(3 * r + 1) >> 2], (3 * r + 1) & 3);
+                const float wz = get(wu[(3 * r + 2) >> 2], (3 * r + 2) & 3);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float v = get(gv[r], e);
+                    tx[e] += wx * v;
+                    ty[e] += wy * v;
+                    tz[e] += wz * v;
+                }
+            }
+        }
+    }
+    float4 r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int a = 4 * q + e;
+        set(r, e, -(c27(a, 0) * tx[e] + c27(a, 1) * ty[e] + c27(a, 2) * tz[e]));
+    }
+    return r;
+}
+
 // LDS DMA (gfx950 buffer_load ... lds): one wave instruction copies 64
 // 16-byte pieces, lane l's piece landing at lds_base + 16 l whatever its
 // source offset, with no VGPR round trip; the base must be wave-uniform.
@@ -759,12 +811,15 @@ __global__ __launch_bounds__(kP1Chunk) __attribute__((amdgpu_waves_per_eu(3))) v
 // RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
 // union staged in LDS (unions larger than ucap rows read through the list).
 template <int STAGE, int K, int BATCH, bool TU, int NPB>
-__global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuffers S,
+#ifndef NAGA_STAGE_WAVES
+#define NAGA_STAGE_WAVES 3
+#endif
+__global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(NAGA_STAGE_WAVES))) void k_stage_u(StepGeom G, StepBuffers S,
                                                             BatchArgs B) {
     constexpr int NT = NPB * kV4Lanes;
-    // record rows padded by one entry: the eight nodes of a wavefront read
-    // their records from distinct LDS banks
-    constexpr int KP = K + 1;
+    // record rows padded by one entry (or packed, with tile unions): the
+    // eight nodes of a wavefront read their records from distinct LDS banks
+    constexpr int KP = TU ? packed_div_units(K) : K + 1;
     __shared__ float4 srec[NPB * KP];
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
@@ -789,8 +844,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuff
     int U = 0;
     const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
     // records: rows padded by one record (distinct banks per node), copied
-    // through VGPRs, or with tile unions by LDS DMA from rows padded so in HBM
-    static_assert(KP == K + kSlabPad, "padded slab rows");
+    // through VGPRs, or with tile unions by LDS DMA from packed rows
     const int kp = KP;
     if (cnt > 0) {
         if (!TU) {
@@ -826,10 +880,10 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_stage_u(StepGeom G, StepBuff
             srec + slot * kp,
             [&](uint32_t j) { return ld4(gin, j * 128u + uint32_t(q) * 16u); }, q);
     else if (U <= S.ucap)
-        r = rate_fetch<K, BATCH>(srec + slot * kp,
+        r = rate_fetch_packed<K>(srec + slot * kp,
                                  [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
     else  // union over the cap: every row read from HBM through the list
-        r = rate_fetch<K, BATCH>(srec + slot * kp,
+        r = rate_fetch_packed<K>(srec + slot * kp,
                                  [&](uint32_t j) {
                                      return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
                                  },
@@ -969,7 +1023,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_v4(StepGeom G, StepBuffer
 // its records staged by LDS DMA.
 template <int K, int BATCH, int NPB>
 __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_t(StepGeom G, StepBuffers S, BatchArgs B) {
-    constexpr int KP = K + kSlabPad;  // padded slab rows
+    constexpr int KP = packed_div_units(K);  // packed record rows
     constexpr int NT = NPB * kV4Lanes;
     __shared__ float4 srec[NPB * KP];
     extern __shared__ float4 urows[];
@@ -1003,9 +1057,9 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_t(StepGeom G, StepBuffers
     const uint32_t lane_off = uint32_t(q) * 16u;
     float4 r;
     if (U <= S.ucap)
-        r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
+        r = rate_fetch_packed<K>(rp, [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
     else  // union over the cap: every row read from HBM through the list
-        r = rate_fetch<K, BATCH>(rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + lane_off); },
+        r = rate_fetch_packed<K>(rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + lane_off); },
                                  q);
     // the divergence of F = c Gf is the negated rate of Gf
     st4(S.A, own, make_float4(s.x - r.x, s.y - r.y, s.z - r.z, s.w - r.w));
@@ -1269,7 +1323,7 @@ int union_tile_nodes(int k) {
 
 int union_cap_default(int k) {
     if (union_tile_nodes(k) != kV4Nodes) return 368;  // 64-node tiles, two blocks per CU
-    return k <= 32 ? 272 : 440;  // 32-node tiles: three blocks per CU, two for k = 48
+    return k <= 32 ? 296 : 440;  // 32-node tiles: three blocks per CU, two for k = 48
 }
 
 namespace {

@@ -28,6 +28,7 @@
 #pragma once
 
 #include "collision.hpp"
+#include "naga_common.hpp"
 
 namespace naga_amd {
 
@@ -77,9 +78,10 @@ struct StepBuffers {
     // Tile unions (D3Q27 four-per-lane kernels): the fluid nodes form tiles
     // of 32; ulist[uoff[t] .. uoff[t+1]) are the distinct stencil neighbours
     // of tile t, and the index field of slab / fslab records holds the
-    // neighbour's position in that list; slab rows then hold k + kSlabPad
-    // records and fslab rows k + kFslabPad (the extra ones zero).  Unions of
-    // at most ucap rows are staged in LDS by the gather kernels.
+    // neighbour's position in that list; slab rows are then packed
+    // (packed_div_units) and fslab rows hold k + kFslabPad records (the
+    // extra ones zero).  Unions of at most ucap rows are staged in LDS by
+    // the gather kernels.
     const uint32_t* ulist;
     const int64_t* uoff;
     int ucap;
@@ -145,16 +147,15 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 // the tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
 // positions: D3Q27, k in {16, 32, 48}, fewer than 2^25 nodes.
 bool step_uses_unions(int Q, int k, int64_t N);
-constexpr int kSlabPad = 1;   // padding records per slab row with tile unions
 constexpr int kFslabPad = 2;  // padding records per fslab row with tile unions
 // Nodes per union tile (= per block of the union kernels) for stencil width
 // k: 32, or with NAGA_TILE=64 and k <= 32, 64 (fewer union rows per node,
 // but two blocks per CU and 9% of unions over the LDS cap: 0.23 against
 // 0.20 ms per RK4 stage on the 1M-node D3Q27 scene).
 int union_tile_nodes(int k);
-// Default LDS union rows per block: 272 for 32-node tiles and k <= 32
+// Default LDS union rows per block: 296 for 32-node tiles and k <= 32
 // (three blocks per CU), 440 for k = 48 (two blocks per CU, the records
-// taking 25 KB), 368 for 64-node tiles (two blocks per CU).  A tile with a
+// taking 22 KB), 368 for 64-node tiles (two blocks per CU).  A tile with a
 // larger union reads its rows from HBM through the union list.
 int union_cap_default(int k);
 constexpr int kUnionRowsMax = 512;  // LDS union rows per block, upper bound
