@@ -230,13 +230,13 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         ulist_.allocate(std::max<int64_t>(off[ntiles], 1));
         launch_tile_unions(nfluid, k, T, idx_.get(), nullptr, uoff_.get(), ulist_.get(),
                            slab_.get(), D, fslab_.get(), 1, st);
-        // the union kernels copy records into LDS as they lie: packed
-        // divergence records (16-bit union positions) and filter rows padded
-        // to k + 2 records keep the records of consecutive nodes in distinct
-        // LDS banks
-        DeviceBuffer<float> ps(nfluid * packed_div_units(k) * 4), pf(nfluid * (k + kFslabPad) * 2);
+        // the union kernels copy records into LDS as they lie: packed rows
+        // (16-bit union positions, an odd number of 16-byte units per node)
+        // keep the records of consecutive nodes in distinct LDS banks
+        DeviceBuffer<float> ps(nfluid * packed_div_units(k) * 4),
+            pf(nfluid * packed_filter_units(k) * 4);
         launch_pack_div_records(nfluid, k, slab_.get(), ps.get(), st);
-        launch_pad_records(nfluid, k, 2, k + kFslabPad, fslab_.get(), pf.get(), st);
+        launch_pack_filter_records(nfluid, k, fslab_.get(), pf.get(), st);
         NAGA_HIP(hipStreamSynchronize(st));
         slab_ = std::move(ps);
         fslab_ = std::move(pf);

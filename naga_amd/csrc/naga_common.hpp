@@ -50,6 +50,10 @@ inline void require(bool ok, const std::string& msg) {
 // makes the row length odd in units so the records of consecutive nodes
 // fall in distinct LDS banks.  k must be a multiple of 8.
 constexpr int packed_div_units(int k) { return 7 * (k / 8) + 1; }
+// Filter records of a node with tile unions, packed the same way: per group
+// of 8 records two units of weights and one unit of 16-bit positions, then
+// one zero unit.
+constexpr int packed_filter_units(int k) { return 3 * (k / 8) + 1; }
 
 #if defined(__HIPCC__)
 inline void hip_check(hipError_t e, const char* what) {

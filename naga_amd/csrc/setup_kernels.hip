@@ -493,18 +493,6 @@ __global__ void k_make_slab(int64_t n, int k, int nc, const uint32_t* idx,
     for (int c = 0; c < nc; ++c) r[1 + c] = w[t * nc + c];
 }
 
-// dst row i = src row i (k records of rec floats) followed by kstride - k
-// zero records.
-__global__ void k_pad_records(int64_t n, int k, int rec, int kstride, const float* src,
-                              float* dst) {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n * kstride * rec) return;
-    const int64_t i = t / (int64_t(kstride) * rec);
-    const int r = int(t - i * kstride * rec);
-    const int s = r / rec, c = r - s * rec;
-    dst[t] = s < k ? src[(i * k + s) * rec + c] : 0.0f;
-}
-
 // One thread per (node, 32-bit word) of the packed divergence records.
 __global__ void k_pack_div_records(int64_t n, int k, const float* src, uint32_t* dst) {
     const int units = packed_div_units(k);
@@ -524,6 +512,29 @@ __global__ void k_pack_div_records(int64_t n, int k, const float* src, uint32_t*
         const int m = word % 4;  // records 2m, 2m + 1
         const uint32_t lo = __float_as_uint(rec[(2 * m) * 4]);
         const uint32_t hi = __float_as_uint(rec[(2 * m + 1) * 4]);
+        v = (lo & 0xffffu) | (hi << 16);
+    }
+    dst[t] = v;
+}
+
+// One thread per (node, 32-bit word) of the packed filter records.
+__global__ void k_pack_filter_records(int64_t n, int k, const float* src, uint32_t* dst) {
+    const int units = packed_filter_units(k);
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n * units * 4) return;
+    const int64_t i = t / (int64_t(units) * 4);
+    const int word = int(t - i * units * 4);
+    const int u = word / 4, g = u / 3, w = u % 3;
+    const float* rec = src + (i * k + 8 * g) * 2;  // the group's 8 {position, weight}
+    uint32_t v = 0;
+    if (u == units - 1) {
+        v = 0;  // padding unit
+    } else if (w < 2) {
+        v = __float_as_uint(rec[(4 * w + word % 4) * 2 + 1]);
+    } else {
+        const int m = word % 4;  // records 2m, 2m + 1
+        const uint32_t lo = __float_as_uint(rec[(2 * m) * 2]);
+        const uint32_t hi = __float_as_uint(rec[(2 * m + 1) * 2]);
         v = (lo & 0xffffu) | (hi << 16);
     }
     dst[t] = v;
@@ -840,14 +851,6 @@ void launch_make_slab(int64_t n, int k, int nc, const uint32_t* idx, const float
     hip_check(hipGetLastError(), "slab");
 }
 
-void launch_pad_records(int64_t n, int k, int rec, int kstride, const float* src, float* dst,
-                        hipStream_t st) {
-    if (n == 0) return;
-    const int64_t tot = n * kstride * rec;
-    k_pad_records<<<(tot + 255) / 256, 256, 0, st>>>(n, k, rec, kstride, src, dst);
-    hip_check(hipGetLastError(), "pad records");
-}
-
 void launch_pack_div_records(int64_t n, int k, const float* src, float* dst, hipStream_t st) {
     require(k % 8 == 0 && k <= 64, "packed divergence records need k a multiple of 8");
     if (n == 0) return;
@@ -855,6 +858,16 @@ void launch_pack_div_records(int64_t n, int k, const float* src, float* dst, hip
     k_pack_div_records<<<(tot + 255) / 256, 256, 0, st>>>(n, k, src,
                                                           reinterpret_cast<uint32_t*>(dst));
     hip_check(hipGetLastError(), "pack divergence records");
+}
+
+void launch_pack_filter_records(int64_t n, int k, const float* src, float* dst,
+                                hipStream_t st) {
+    require(k % 8 == 0 && k <= 64, "packed filter records need k a multiple of 8");
+    if (n == 0) return;
+    const int64_t tot = n * packed_filter_units(k) * 4;
+    k_pack_filter_records<<<(tot + 255) / 256, 256, 0, st>>>(n, k, src,
+                                                             reinterpret_cast<uint32_t*>(dst));
+    hip_check(hipGetLastError(), "pack filter records");
 }
 
 void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx,

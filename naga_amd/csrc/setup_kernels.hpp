@@ -63,14 +63,13 @@ void launch_filter_weights(int D, const float* pts, int64_t n,
 void launch_make_slab(int64_t n, int k, int ncomp, const uint32_t* idx,
                       const float* w, float* slab, hipStream_t stream);
 
-// Copy n rows of k records (rec floats each) into rows of kstride >= k
-// records, the extra records zero (padded rows for bank-conflict-free LDS
-// reads of the records of consecutive nodes).
 // Packs D3Q27 divergence records ({union position bits, wx, wy, wz} per
 // stencil entry, k per node) into the packed_div_units layout.
 void launch_pack_div_records(int64_t n, int k, const float* src, float* dst, hipStream_t st);
-void launch_pad_records(int64_t n, int k, int rec, int kstride, const float* src, float* dst,
-                        hipStream_t stream);
+// Packs filter records ({union position bits, w} per stencil entry) into the
+// packed_filter_units layout.
+void launch_pack_filter_records(int64_t n, int k, const float* src, float* dst,
+                                hipStream_t st);
 
 // Tile unions: fluid nodes are grouped in tiles of tile_nodes consecutive
 // nodes (k * tile_nodes <= 4096); a tile's union is the sorted list of the

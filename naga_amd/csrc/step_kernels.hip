@@ -912,16 +912,52 @@ __device__ __forceinline__ float4 filter_sum(const float2* rp, const float4& si,
     return make_float4(sum[0], sum[1], sum[2], sum[3]);
 }
 
+// filter_sum over packed filter records (packed_filter_units layout): the
+// same products in the same order, so the two agree bit for bit.
+#ifndef NAGA_FILTER_SUB
+#define NAGA_FILTER_SUB 4
+#endif
+template <int K, class Fetch>
+__device__ __forceinline__ float4 filter_sum_packed(const float4* rp, const float4& si,
+                                                    Fetch fetch) {
+    static_assert(K % 8 == 0, "packed records come in groups of 8");
+    constexpr int H = NAGA_FILTER_SUB;  // records gathered per sub-batch: 8 or 4
+    static_assert(H == 8 || H == 4, "sub-batch of 8 or 4 records");
+    float sum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = 0; g < K / 8; ++g) {
+        const float4 iu = rp[3 * g + 2];
+#pragma unroll
+        for (int h = 0; h < 8; h += H) {  // records h .. h + H - 1, weights in units h/4 ..
+            float4 gv[H];
+#pragma unroll
+            for (int r = 0; r < H; ++r) {
+                const uint32_t word = __float_as_uint(get(iu, (h + r) >> 1));
+                gv[r] = fetch(((h + r) & 1) ? word >> 16 : word & 0xffffu);
+            }
+            float4 wu[H / 4];
+#pragma unroll
+            for (int w = 0; w < H / 4; ++w) wu[w] = rp[3 * g + h / 4 + w];
+#pragma unroll
+            for (int r = 0; r < H; ++r)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    sum[e] += get(wu[r >> 2], r & 3) * (get(gv[r], e) - get(si, e));
+        }
+    }
+    return make_float4(sum[0], sum[1], sum[2], sum[3]);
+}
+
 // p4 for D3Q27 with the tile unions: selective filter on fluid nodes,
 // bounce-back on boundary nodes.
 template <int K, int BATCH, bool TU, int NPB>
 __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers S, BatchArgs B) {
     constexpr int NT = NPB * kV4Lanes;
-    // padded record rows (distinct banks per node): K + 1 records through
-    // VGPRs, or K + kFslabPad by LDS DMA from rows padded so in HBM
-    constexpr int KP = K + 1, KF = K + kFslabPad;
-    static_assert(KF % 2 == 0, "16-byte record pairs");
-    __shared__ float4 srec4[(NPB * KF) / 2];
+    // record rows with distinct banks per node: K + 1 records through
+    // VGPRs, or packed rows of KF units by LDS DMA
+    constexpr int KP = K + 1, KF = packed_filter_units(K);
+    constexpr int SU = TU ? NPB * KF : (NPB * KP + 1) / 2;  // 16-byte units
+    __shared__ float4 srec4[SU];
     float2* srec = reinterpret_cast<float2*>(srec4);
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
@@ -937,7 +973,6 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
     const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     int U = 0;
     const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
-    const int kp = TU ? KF : KP;
     if (cnt > 0) {
         if (!TU) {
             const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
@@ -949,8 +984,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
             const int Uc = U < S.ucap ? U : S.ucap;
             const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);
             __builtin_amdgcn_sched_barrier(0);
-            dma_contiguous<NPB * KF / 2, NT>(
-                rsrc(S.fslab + node0 * KF * 2, uint64_t(cnt) * KF * 8u), srec4, cnt * KF / 2);
+            dma_contiguous<NPB * KF, NT>(
+                rsrc(S.fslab + node0 * KF * 4, uint64_t(cnt) * KF * 16u), srec4, cnt * KF);
             __builtin_amdgcn_sched_barrier(0);
             union_copy<NT>(urows, ux, Uc, s1);
         }
@@ -960,16 +995,18 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
     if (i >= G.N) return;
     float4 out;
     if (i < G.nfluid) {
-        const float2* rp = srec + slot * kp;
         const uint32_t lane_off = uint32_t(q) * 16u;
         float4 sum;
         if (!ul)
-            sum = filter_sum<K, BATCH>(rp, si, [&](uint32_t j) { return ld4(s1, j * 128u + lane_off); });
+            sum = filter_sum<K, BATCH>(srec + slot * KP, si,
+                                       [&](uint32_t j) { return ld4(s1, j * 128u + lane_off); });
         else if (U <= S.ucap)
-            sum = filter_sum<K, BATCH>(rp, si, [&](uint32_t j) { return urows[j * kV4Lanes + q]; });
+            sum = filter_sum_packed<K>(srec4 + slot * KF, si,
+                                       [&](uint32_t j) { return urows[j * kV4Lanes + q]; });
         else  // union over the cap: every row read from HBM through the list
-            sum = filter_sum<K, BATCH>(rp, si,
-                                       [&](uint32_t j) { return ld4(s1, ul[j] * 128u + lane_off); });
+            sum = filter_sum_packed<K>(srec4 + slot * KF, si, [&](uint32_t j) {
+                return ld4(s1, ul[j] * 128u + lane_off);
+            });
 #pragma unroll
         for (int e = 0; e < 4; ++e) set(out, e, get(si, e) + G.filter * get(sum, e));
     } else {

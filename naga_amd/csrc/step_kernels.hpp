@@ -78,10 +78,9 @@ struct StepBuffers {
     // Tile unions (D3Q27 four-per-lane kernels): the fluid nodes form tiles
     // of 32; ulist[uoff[t] .. uoff[t+1]) are the distinct stencil neighbours
     // of tile t, and the index field of slab / fslab records holds the
-    // neighbour's position in that list; slab rows are then packed
-    // (packed_div_units) and fslab rows hold k + kFslabPad records (the
-    // extra ones zero).  Unions of at most ucap rows are staged in LDS by
-    // the gather kernels.
+    // neighbour's position in that list; slab and fslab rows are then
+    // packed (packed_div_units, packed_filter_units).  Unions of at most
+    // ucap rows are staged in LDS by the gather kernels.
     const uint32_t* ulist;
     const int64_t* uoff;
     int ucap;
@@ -147,7 +146,6 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 // the tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
 // positions: D3Q27, k in {16, 32, 48}, fewer than 2^25 nodes.
 bool step_uses_unions(int Q, int k, int64_t N);
-constexpr int kFslabPad = 2;  // padding records per fslab row with tile unions
 // Nodes per union tile (= per block of the union kernels) for stencil width
 // k: 32, or with NAGA_TILE=64 and k <= 32, 64 (fewer union rows per node,
 // but two blocks per CU and 9% of unions over the LDS cap: 0.23 against
