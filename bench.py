@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
-ROOFLINE_KERNEL = "k_stage_u<2, 32, 8, true, 32>"
+ROOFLINE_KERNEL = "k_stage_u<2, 32, 8, true, 64>"
 # HBM bytes per launch of ROOFLINE_KERNEL on this workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) summarised by
 # naga_amd.profsum; the file records the commands.

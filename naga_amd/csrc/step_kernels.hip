@@ -808,13 +808,15 @@ __global__ __launch_bounds__(kP1Chunk) __attribute__((amdgpu_waves_per_eu(3))) v
     }
 }
 
+// Waves per SIMD the stage kernels are compiled for: three 256-thread
+// blocks per CU with 32-node tiles, two 512-thread blocks with 64.
+template <int NPB>
+constexpr int stage_waves() { return NPB >= 64 ? 4 : 3; }
+
 // RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
 // union staged in LDS (unions larger than ucap rows read through the list).
 template <int STAGE, int K, int BATCH, bool TU, int NPB>
-#ifndef NAGA_STAGE_WAVES
-#define NAGA_STAGE_WAVES 3
-#endif
-__global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(NAGA_STAGE_WAVES))) void k_stage_u(StepGeom G, StepBuffers S,
+__global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(stage_waves<NPB>()))) void k_stage_u(StepGeom G, StepBuffers S,
                                                             BatchArgs B) {
     constexpr int NT = NPB * kV4Lanes;
     // record rows padded by one entry (or packed, with tile unions): the
@@ -834,10 +836,14 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     const bool fluid = i < G.nfluid;
     const float4 g0 = ld4(rsrc(S.g, rows), own);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    // stage outputs read for the last time (non-temporal)
-    const float4 y1 = STAGE == 4 ? ld4<kStreamPolicy>(rsrc(S.A, rows), own) : z4;
-    const float4 y2 = STAGE == 4 ? ld4<kStreamPolicy>(rsrc(S.B, rows), own) : z4;
-    const float4 y3 = STAGE == 4 ? ld4<kStreamPolicy>(rsrc(S.acc, rows), own) : z4;
+    // stage outputs read for the last time (non-temporal); with 64-node
+    // tiles after the gather, so they hold no registers through it (the
+    // kernel then fits 128 registers: four waves per SIMD)
+    constexpr bool late = NPB >= 64;
+    constexpr bool early = STAGE == 4 && !late;
+    const float4 y1 = early ? ld4<kStreamPolicy>(rsrc(S.A, rows), own) : z4;
+    const float4 y2 = early ? ld4<kStreamPolicy>(rsrc(S.B, rows), own) : z4;
+    const float4 y3 = early ? ld4<kStreamPolicy>(rsrc(S.acc, rows), own) : z4;
     const int64_t nf = G.nfluid - node0;
     const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
@@ -888,6 +894,13 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
                                      return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
                                  },
                                  q);
+    if (STAGE == 4 && late) {
+        const float4 l1 = ld4<kStreamPolicy>(rsrc(S.A, rows), own);
+        const float4 l2 = ld4<kStreamPolicy>(rsrc(S.B, rows), own);
+        const float4 l3 = ld4<kStreamPolicy>(rsrc(S.acc, rows), own);
+        st4(gout, own, rk4_out4<STAGE>(g0, r, G.dt, l1, l2, l3));
+        return;
+    }
     st4(gout, own, rk4_out4<STAGE>(g0, r, G.dt, y1, y2, y3));
 }
 
@@ -1351,15 +1364,15 @@ int v4_nodes_per_block(int k, bool tiles) {
 }
 
 int union_tile_nodes(int k) {
-    static const bool want64 = [] {
+    static const bool want32 = [] {
         const char* e = std::getenv("NAGA_TILE");
-        return e && std::atoi(e) == 64;
+        return e && std::atoi(e) == 32;
     }();
-    return (k <= 32 && want64) ? 2 * kV4Nodes : kV4Nodes;
+    return (k <= 32 && !want32) ? 2 * kV4Nodes : kV4Nodes;
 }
 
 int union_cap_default(int k) {
-    if (union_tile_nodes(k) != kV4Nodes) return 368;  // 64-node tiles, two blocks per CU
+    if (union_tile_nodes(k) != kV4Nodes) return 400;  // 64-node tiles, two blocks per CU
     return k <= 32 ? 296 : 440;  // 32-node tiles: three blocks per CU, two for k = 48
 }
 

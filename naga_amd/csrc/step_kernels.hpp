@@ -147,13 +147,13 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 // positions: D3Q27, k in {16, 32, 48}, fewer than 2^25 nodes.
 bool step_uses_unions(int Q, int k, int64_t N);
 // Nodes per union tile (= per block of the union kernels) for stencil width
-// k: 32, or with NAGA_TILE=64 and k <= 32, 64 (fewer union rows per node,
-// but two blocks per CU and 9% of unions over the LDS cap: 0.23 against
-// 0.20 ms per RK4 stage on the 1M-node D3Q27 scene).
+// k: 64 for k <= 32 (5.4 union rows per node against 7.0 with 32-node
+// tiles, two blocks per CU; 1.40 against 1.44 ms per step on the 1M-node
+// D3Q27 scene), else 32; NAGA_TILE=32 selects 32-node tiles for every k.
 int union_tile_nodes(int k);
-// Default LDS union rows per block: 296 for 32-node tiles and k <= 32
-// (three blocks per CU), 440 for k = 48 (two blocks per CU, the records
-// taking 22 KB), 368 for 64-node tiles (two blocks per CU).  A tile with a
+// Default LDS union rows per block: 400 for 64-node tiles (two blocks per
+// CU), 296 for 32-node tiles and k <= 32 (three blocks per CU), 440 for
+// k = 48 (two blocks per CU, the records taking 22 KB).  A tile with a
 // larger union reads its rows from HBM through the union list.
 int union_cap_default(int k);
 constexpr int kUnionRowsMax = 512;  // LDS union rows per block, upper bound
