@@ -481,6 +481,22 @@ __device__ __forceinline__ float rk4_out(float y, float k, float dt, float y1, f
     if (STAGE == 3) return y + dt * k;
     return y + (((y1 - y) + 2.0f * (y2 - y)) + (y3 - y)) * (1.0f / 3.0f) + (dt * (1.0f / 6.0f)) * k;
 }
+// Stage 4 in two parts with the arithmetic of rk4_out<4>: the part without
+// the rate, y + ((y1 - y) + 2 (y2 - y) + (y3 - y)) / 3, formed before the
+// gather (four live values instead of sixteen), then + dt/6 k.
+__device__ __forceinline__ float rk4_pre(float y, float y1, float y2, float y3) {
+    return y + (((y1 - y) + 2.0f * (y2 - y)) + (y3 - y)) * (1.0f / 3.0f);
+}
+__device__ __forceinline__ float4 rk4_pre4(const float4& y, const float4& y1, const float4& y2,
+                                           const float4& y3) {
+    return make_float4(rk4_pre(y.x, y1.x, y2.x, y3.x), rk4_pre(y.y, y1.y, y2.y, y3.y),
+                       rk4_pre(y.z, y1.z, y2.z, y3.z), rk4_pre(y.w, y1.w, y2.w, y3.w));
+}
+__device__ __forceinline__ float4 rk4_post4(const float4& pre, const float4& k, float dt) {
+    const float c = dt * (1.0f / 6.0f);
+    return make_float4(pre.x + c * k.x, pre.y + c * k.y, pre.z + c * k.z, pre.w + c * k.w);
+}
+
 template <int STAGE>
 __device__ __forceinline__ float4 rk4_out4(const float4& y, const float4& k, float dt,
                                            const float4& y1, const float4& y2, const float4& y3) {
@@ -836,14 +852,6 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     const bool fluid = i < G.nfluid;
     const float4 g0 = ld4(rsrc(S.g, rows), own);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    // stage outputs read for the last time (non-temporal); with 64-node
-    // tiles after the gather, so they hold no registers through it (the
-    // kernel then fits 128 registers: four waves per SIMD)
-    constexpr bool late = NPB >= 64;
-    constexpr bool early = STAGE == 4 && !late;
-    const float4 y1 = early ? ld4<kStreamPolicy>(rsrc(S.A, rows), own) : z4;
-    const float4 y2 = early ? ld4<kStreamPolicy>(rsrc(S.B, rows), own) : z4;
-    const float4 y3 = early ? ld4<kStreamPolicy>(rsrc(S.acc, rows), own) : z4;
     const int64_t nf = G.nfluid - node0;
     const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
@@ -871,6 +879,15 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
             union_copy<NT>(urows, ux, Uc, gin);
         }
     }
+    // stage 4: the stage outputs, read for the last time (non-temporal),
+    // are loaded once the copies are issued (no registers held through
+    // them) and folded before the gather
+    float4 y1 = z4, y2 = z4, y3 = z4;
+    if (STAGE == 4) {
+        y1 = ld4<kStreamPolicy>(rsrc(S.A, rows), own);
+        y2 = ld4<kStreamPolicy>(rsrc(S.B, rows), own);
+        y3 = ld4<kStreamPolicy>(rsrc(S.acc, rows), own);
+    }
     if (TU) dma_fence();
     else __syncthreads();
     if (STAGE == 1 && !fluid && i < G.N) {
@@ -880,6 +897,10 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
         return;
     }
     if (!fluid) return;
+    float4 pre = STAGE == 4 ? rk4_pre4(g0, y1, y2, y3) : g0;
+    // keep the fold here (the compiler would otherwise sink it to its use
+    // and hold the sixteen inputs through the gather)
+    if (STAGE == 4) asm volatile("" : "+v"(pre.x), "+v"(pre.y), "+v"(pre.z), "+v"(pre.w));
     float4 r;
     if (!ul)  // records hold node indices
         r = rate_fetch<K, BATCH>(
@@ -894,14 +915,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
                                      return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
                                  },
                                  q);
-    if (STAGE == 4 && late) {
-        const float4 l1 = ld4<kStreamPolicy>(rsrc(S.A, rows), own);
-        const float4 l2 = ld4<kStreamPolicy>(rsrc(S.B, rows), own);
-        const float4 l3 = ld4<kStreamPolicy>(rsrc(S.acc, rows), own);
-        st4(gout, own, rk4_out4<STAGE>(g0, r, G.dt, l1, l2, l3));
-        return;
-    }
-    st4(gout, own, rk4_out4<STAGE>(g0, r, G.dt, y1, y2, y3));
+    st4(gout, own, STAGE == 4 ? rk4_post4(pre, r, G.dt) : rk4_out4<STAGE>(g0, r, G.dt, z4, z4, z4));
 }
 
 // sum_s w_s (g_s - g_i) over the staged filter records rp ({index field, w}),
