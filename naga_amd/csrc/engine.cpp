@@ -203,6 +203,10 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     ucap_ = 0;
     ulist_ = DeviceBuffer<uint32_t>();
     uoff_ = DeviceBuffer<int64_t>();
+    usplit_ = DeviceBuffer<int32_t>();
+    split_tiles_ = DeviceBuffer<int32_t>();
+    split_of_.clear();
+    n_split_ = 0;
     // Tile unions: the gather kernels copy each tile's distinct stencil
     // neighbours into LDS by LDS DMA and read them from there (fewer bytes
     // through the vector L1 than direct gathers: 0.2 against 0.27 ms per
@@ -214,9 +218,17 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         const int T = union_tile_nodes(k);
         tile_nodes_ = T;
         const int64_t ntiles = (nfluid + T - 1) / T;
+        const char* env = std::getenv("NAGA_UCAP");
+        const int cap = env ? std::max(0, std::min(std::atoi(env), kUnionRowsMax))
+                            : union_cap_default(k);
+        // tiles over the cap are split into two half-tile unions when
+        // those fit (NAGA_SPLIT=0: never)
+        const char* sp_env = std::getenv("NAGA_SPLIT");
+        const bool split = !(sp_env && sp_env[0] == '0');
         DeviceBuffer<int32_t> counts(ntiles);
-        launch_tile_unions(nfluid, k, T, idx_.get(), counts.get(), nullptr, nullptr, nullptr, 0,
-                           nullptr, 0, st);
+        usplit_.allocate(ntiles);
+        launch_tile_unions(nfluid, k, T, idx_.get(), split ? cap : 0, counts.get(), usplit_.get(),
+                           nullptr, nullptr, nullptr, 0, nullptr, 0, st);
         std::vector<int32_t> hc(ntiles);
         counts.download(hc.data(), ntiles, st);
         NAGA_HIP(hipStreamSynchronize(st));
@@ -228,8 +240,8 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         }
         uoff_ = to_device(off, st);
         ulist_.allocate(std::max<int64_t>(off[ntiles], 1));
-        launch_tile_unions(nfluid, k, T, idx_.get(), nullptr, uoff_.get(), ulist_.get(),
-                           slab_.get(), D, fslab_.get(), 1, st);
+        launch_tile_unions(nfluid, k, T, idx_.get(), split ? cap : 0, nullptr, usplit_.get(),
+                           uoff_.get(), ulist_.get(), slab_.get(), D, fslab_.get(), 1, st);
         // the union kernels copy records into LDS as they lie: packed rows
         // (16-bit union positions, an odd number of 16-byte units per node)
         // keep the records of consecutive nodes in distinct LDS banks
@@ -240,10 +252,19 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         NAGA_HIP(hipStreamSynchronize(st));
         slab_ = std::move(ps);
         fslab_ = std::move(pf);
-        const char* env = std::getenv("NAGA_UCAP");
-        const int cap = env ? std::max(0, std::min(std::atoi(env), kUnionRowsMax))
-                            : union_cap_default(k);
         ucap_ = std::min(maxu, cap);
+        // split tiles: one more block each serves the second half
+        std::vector<int32_t> hs(ntiles), splits;
+        usplit_.download(hs.data(), ntiles, st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        split_of_.assign(ntiles, -1);
+        for (int64_t t = 0; t < ntiles; ++t)
+            if (hs[t] > 0) {
+                split_of_[t] = int32_t(splits.size());
+                splits.push_back(int32_t(t));
+            }
+        n_split_ = int64_t(splits.size());
+        split_tiles_ = to_device(splits.empty() ? std::vector<int32_t>{0} : splits, st);
     }
 
     // boundary interpolator (init_domain :223-258): kNN of boundary nodes
@@ -306,7 +327,11 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
                 const int64_t t1 = std::min<int64_t>(t0 + T, nfluid);
                 for (int64_t i = t0; i < t1; ++i)
                     if (hflag[i]) {
-                        tl.push_back(int32_t(t0 / T));
+                        const int64_t t = t0 / T;
+                        tl.push_back(int32_t(t));
+                        // a split tile's second half: entry -1 - s
+                        if (t < int64_t(split_of_.size()) && split_of_[t] >= 0)
+                            tl.push_back(-1 - split_of_[t]);
                         break;
                     }
             }
@@ -393,6 +418,9 @@ StepBuffers Engine::buffers() {
     S.src = sources_.empty() ? nullptr : src_field_.get();
     S.ulist = ulist_.get();
     S.uoff = uoff_.get();
+    S.usplit = usplit_.get();
+    S.split_tiles = split_tiles_.get();
+    S.n_split = n_split_;
     S.ucap = ucap_;
     return S;
 }

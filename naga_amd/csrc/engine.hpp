@@ -141,6 +141,10 @@ class Engine {
 
     DeviceBuffer<uint32_t> ulist_;  // tile unions (four-per-lane D3Q27 kernels)
     DeviceBuffer<int64_t> uoff_;
+    DeviceBuffer<int32_t> usplit_;  // per tile: first half-union's length if split, else 0
+    DeviceBuffer<int32_t> split_tiles_;  // the split tiles, in order
+    std::vector<int32_t> split_of_;      // per tile: its index among the split tiles, or -1
+    int64_t n_split_ = 0;
     int ucap_ = 0;
 
     struct ProbeSet {

@@ -571,22 +571,12 @@ __device__ __forceinline__ int block_exclusive_scan(int* v, int n, int* tmp) {
     return total;
 }
 
-__global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, int tile_nodes,
-                                                    const uint32_t* idx,
-                                                    int32_t* counts, const int64_t* uoff,
-                                                    uint32_t* ulist, float* slab_a, int nca,
-                                                    float* slab_b, int ncb) {
-    __shared__ uint32_t keys[kUnionMax];
-    __shared__ uint32_t uniq[kUnionMax];
-    __shared__ int rank[kUnionMax];
-    __shared__ int tmp[256];
-    const int64_t node0 = int64_t(blockIdx.x) * tile_nodes;
-    const int64_t left = nfluid - node0;
-    const int cnt = left < tile_nodes ? int(left) : tile_nodes;
-    const int M = cnt * k;
+// Sorted distinct keys of src[0..M) (M <= kUnionMax) into uniq[0..U) by
+// the whole block; returns U.
+__device__ int block_union(const uint32_t* src, int M, uint32_t* keys, uint32_t* uniq,
+                           int* rank, int* tmp) {
     int P = 256;
     while (P < M) P <<= 1;
-    const uint32_t* src = idx + node0 * k;
     for (int e = threadIdx.x; e < P; e += 256) keys[e] = e < M ? src[e] : 0xffffffffu;
     __syncthreads();
     for (int size = 2; size <= P; size <<= 1)
@@ -606,15 +596,17 @@ __global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, int t
         rank[e] = (e < M && (e == 0 || keys[e] != keys[e - 1])) ? 1 : 0;
     __syncthreads();
     const int U = block_exclusive_scan(rank, P, tmp);
-    if (!ulist) {
-        if (threadIdx.x == 0) counts[blockIdx.x] = U;
-        return;
-    }
     // compact the distinct keys (rank is the exclusive scan of the flags)
     for (int e = threadIdx.x; e < M; e += 256)
         if (e == 0 || keys[e] != keys[e - 1]) uniq[rank[e]] = keys[e];
     __syncthreads();
-    uint32_t* out = ulist + uoff[blockIdx.x];
+    return U;
+}
+
+// Writes uniq[0..U) to out and, for the M stencil entries src[0..M), the
+// entry's position in uniq into the index field of the slab records.
+__device__ void write_union(const uint32_t* src, int M, const uint32_t* uniq, int U,
+                            uint32_t* out, float* slab_a, int nca, float* slab_b, int ncb) {
     for (int e = threadIdx.x; e < U; e += 256) out[e] = uniq[e];
     for (int r = threadIdx.x; r < M; r += 256) {
         const uint32_t j = src[r];
@@ -625,9 +617,68 @@ __global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, int t
             else hi = mid;
         }
         const float slot = __uint_as_float(uint32_t(lo));
-        slab_a[(node0 * k + r) * (1 + nca)] = slot;
-        if (slab_b) slab_b[(node0 * k + r) * (1 + ncb)] = slot;
+        slab_a[r * (1 + nca)] = slot;
+        if (slab_b) slab_b[r * (1 + ncb)] = slot;
     }
+    __syncthreads();
+}
+
+// One block per tile: the tile's k * tile_nodes (<= 4096) neighbour indices
+// are sorted in LDS (bitonic), flagged where they differ from their
+// predecessor and compacted by a block-wide scan of the flags.  A tile
+// whose union exceeds cap (> 0) while each half-tile's union fits is
+// split: its list is the first half's union followed by the second
+// half's, each half's records hold positions in its own part, and
+// usplit[t] is the first part's length (0 for tiles not split).
+// Counting pass (ulist == nullptr): counts[t] = list length, usplit[t].
+__global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, int tile_nodes,
+                                                    const uint32_t* idx, int cap,
+                                                    int32_t* counts, int32_t* usplit,
+                                                    const int64_t* uoff,
+                                                    uint32_t* ulist, float* slab_a, int nca,
+                                                    float* slab_b, int ncb) {
+    __shared__ uint32_t keys[kUnionMax];
+    __shared__ uint32_t uniq[kUnionMax];
+    __shared__ int rank[kUnionMax];
+    __shared__ int tmp[256];
+    const int64_t node0 = int64_t(blockIdx.x) * tile_nodes;
+    const int64_t left = nfluid - node0;
+    const int cnt = left < tile_nodes ? int(left) : tile_nodes;
+    const int M = cnt * k;
+    const int h = tile_nodes / 2;  // nodes of the first half
+    const uint32_t* src = idx + node0 * k;
+    if (!ulist) {
+        const int U = block_union(src, M, keys, uniq, rank, tmp);
+        int total = U, first = 0;
+        if (cap > 0 && U > cap && cnt > h) {
+            const int UA = block_union(src, h * k, keys, uniq, rank, tmp);
+            const int UB = block_union(src + h * k, (cnt - h) * k, keys, uniq, rank, tmp);
+            if (UA <= cap && UB <= cap) {
+                total = UA + UB;
+                first = UA;
+            }
+        }
+        if (threadIdx.x == 0) {
+            counts[blockIdx.x] = total;
+            if (usplit) usplit[blockIdx.x] = first;
+        }
+        return;
+    }
+    uint32_t* out = ulist + uoff[blockIdx.x];
+    float* sa = slab_a + node0 * k * (1 + nca);
+    float* sb = slab_b ? slab_b + node0 * k * (1 + ncb) : nullptr;
+    const int first = usplit ? usplit[blockIdx.x] : 0;
+    if (first == 0) {
+        const int U = block_union(src, M, keys, uniq, rank, tmp);
+        write_union(src, M, uniq, U, out, sa, nca, sb, ncb);
+        return;
+    }
+    const int UA = block_union(src, h * k, keys, uniq, rank, tmp);
+    write_union(src, h * k, uniq, UA, out, sa, nca, sb, ncb);
+    const int MB = (cnt - h) * k;
+    const int UB = block_union(src + h * k, MB, keys, uniq, rank, tmp);
+    write_union(src + h * k, MB, uniq, UB, out + UA, sa + int64_t(h) * k * (1 + nca), nca,
+                sb ? sb + int64_t(h) * k * (1 + ncb) : nullptr, ncb);
 }
 
 __global__ void k_flag(int64_t n, int k, const uint32_t* idx, int64_t lo,
@@ -870,14 +921,15 @@ void launch_pack_filter_records(int64_t n, int k, const float* src, float* dst,
     hip_check(hipGetLastError(), "pack filter records");
 }
 
-void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx,
-                        int32_t* counts, const int64_t* uoff, uint32_t* ulist, float* slab_a,
-                        int ncomp_a, float* slab_b, int ncomp_b, hipStream_t st) {
-    require(tile_nodes > 0 && k * tile_nodes <= kUnionMax, "stencil too wide for tile unions");
+void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx, int cap,
+                        int32_t* counts, int32_t* usplit, const int64_t* uoff, uint32_t* ulist,
+                        float* slab_a, int ncomp_a, float* slab_b, int ncomp_b, hipStream_t st) {
+    require(tile_nodes > 0 && tile_nodes % 2 == 0 && k * tile_nodes <= kUnionMax,
+            "stencil too wide for tile unions");
     const int64_t ntiles = (nfluid + tile_nodes - 1) / tile_nodes;
     if (ntiles == 0) return;
-    k_tile_union<<<ntiles, 256, 0, st>>>(nfluid, k, tile_nodes, idx, counts, uoff, ulist, slab_a,
-                                          ncomp_a, slab_b, ncomp_b);
+    k_tile_union<<<ntiles, 256, 0, st>>>(nfluid, k, tile_nodes, idx, cap, counts, usplit, uoff,
+                                          ulist, slab_a, ncomp_a, slab_b, ncomp_b);
     hip_check(hipGetLastError(), "tile unions");
 }
 

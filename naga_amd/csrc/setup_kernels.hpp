@@ -73,15 +73,19 @@ void launch_pack_filter_records(int64_t n, int k, const float* src, float* dst,
 
 // Tile unions: fluid nodes are grouped in tiles of tile_nodes consecutive
 // nodes (k * tile_nodes <= 4096); a tile's union is the sorted list of the
-// distinct neighbour indices of its nodes' k-stencils.  Counting pass
-// (ulist == nullptr): counts[t] = union size.  Filling pass (uoff =
-// exclusive scan of counts, ntiles + 1 entries): ulist[uoff[t] ..] = the
-// union and, for each slab given (records of 1 + ncomp floats, field 0 the
-// neighbour index), the index field is replaced by the neighbour's
-// position in its tile's union.
-void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx,
-                        int32_t* counts, const int64_t* uoff, uint32_t* ulist, float* slab_a,
-                        int ncomp_a, float* slab_b, int ncomp_b, hipStream_t stream);
+// distinct neighbour indices of its nodes' k-stencils.  A tile whose union
+// exceeds cap (> 0) while each half-tile's union fits is split: its list
+// is the two half-tiles' unions one after the other and usplit[t] the
+// first one's length (0: not split).  Counting pass (ulist == nullptr):
+// counts[t] = list length and usplit[t].  Filling pass (uoff = exclusive
+// scan of counts, ntiles + 1 entries; usplit from the counting pass):
+// ulist[uoff[t] ..] = the list and, for each slab given (records of
+// 1 + ncomp floats, field 0 the neighbour index), the index field is
+// replaced by the neighbour's position in its (half-)tile's union.
+void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx, int cap,
+                        int32_t* counts, int32_t* usplit, const int64_t* uoff, uint32_t* ulist,
+                        float* slab_a, int ncomp_a, float* slab_b, int ncomp_b,
+                        hipStream_t stream);
 
 // flag[i] = 1 if any stencil neighbour of fluid node i is in [lo, hi).
 void launch_flag_neighbours(int64_t n, int k, const uint32_t* idx, int64_t lo,

@@ -774,6 +774,56 @@ __device__ __forceinline__ void dma_fence() {
     __syncthreads();
 }
 
+// The work of one block of a union kernel.  Tile entry e >= 0 is tile e
+// (only its first half if the tile was split at setup); e < 0 is the
+// second half of split tile split_tiles[-1 - e].  The block serves node
+// slots [lo, hi) of the tile's cnt fluid nodes (first: it also serves the
+// tile's non-fluid nodes) and stages the union rows [u0, u0 + U) of the
+// list, in which those nodes' records hold positions.
+struct UnionWork {
+    int64_t tile, u0;
+    int lo, hi, U;
+    bool first;
+};
+
+template <int NPB, bool TU>
+__device__ __forceinline__ UnionWork union_work(const StepGeom& G, const StepBuffers& S,
+                                                int64_t e) {
+    typedef const __attribute__((address_space(4))) int32_t const_i32;
+    UnionWork w;
+    w.first = e >= 0;
+    w.tile = e >= 0 ? e : ((const_i32*)S.split_tiles)[-1 - e];
+    const int64_t nf = G.nfluid - w.tile * NPB;
+    const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
+    w.lo = 0;
+    w.hi = cnt;
+    w.u0 = 0;
+    w.U = 0;
+    if (TU && cnt > 0) {
+        union_range(S.uoff, w.tile, w.u0, w.U);
+        if (w.U > S.ucap && S.usplit) {
+            const int A = ((const_i32*)S.usplit)[w.tile];
+            if (A > 0 && e >= 0) {
+                w.hi = NPB / 2;
+                w.U = A;
+            } else if (A > 0) {
+                w.lo = NPB / 2;
+                w.u0 += A;
+                w.U -= A;
+            }
+        }
+    }
+    return w;
+}
+
+// Tile entry of block b of nb of a kernel over the tiles: the regular
+// tiles first, then (with tile unions) the second halves of split tiles.
+template <bool TU>
+__device__ __forceinline__ int64_t union_entry(const StepBuffers& S, int64_t b, int64_t nb) {
+    const int64_t t = xcd_tile(b, nb), nreg = TU ? nb - S.n_split : nb;
+    return t < nreg ? t : -1 - (t - nreg);
+}
+
 // Collision for one D3Q27 scene: one wave per block of 64 nodes, whose
 // rows arrive in LDS by DMA (no VGPR round trip), are collided in place
 // and stored as coalesced 16-byte pieces.  Piece j of chunk node n sits at
@@ -841,8 +891,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     __shared__ float4 srec[NPB * KP];
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
-    const int64_t tile = xcd_tile(bc.b, bc.nb);
-    const int64_t node0 = tile * NPB;
+    const UnionWork w = union_work<NPB, TU>(G, S, union_entry<TU>(S, bc.b, bc.nb));
+    const int64_t node0 = w.tile * NPB;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
@@ -850,31 +900,29 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     const float* gin_p = stage_in(S, STAGE);
     float* gout = stage_out(S, STAGE);
     const bool fluid = i < G.nfluid;
+    const bool mine = slot >= w.lo && slot < w.hi;  // a fluid node this block serves
     const float4 g0 = ld4(rsrc(S.g, rows), own);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int64_t nf = G.nfluid - node0;
-    const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     const __amdgpu_buffer_rsrc_t gin = rsrc(gin_p, rows);
-    int U = 0;
-    const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
+    const int U = w.U;
+    const uint32_t* ul = TU ? S.ulist + w.u0 : nullptr;  // TU: tile unions built
     // records: rows padded by one record (distinct banks per node), copied
     // through VGPRs, or with tile unions by LDS DMA from packed rows
     const int kp = KP;
-    if (cnt > 0) {
+    if (w.hi > w.lo) {
         if (!TU) {
             const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
-            for (int t = threadIdx.x; t < cnt * K; t += NT) srec[t + t / K] = src[t];
+            for (int t = threadIdx.x; t < w.hi * K; t += NT) srec[t + t / K] = src[t];
         } else {
             // order: union indices, record copies, then the row copies (which
             // wait for the indices only)
-            int64_t u0;
-            union_range(S.uoff, tile, u0, U);
-            ul += u0;
             const int Uc = U < S.ucap ? U : S.ucap;
             const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);  // unconditional: no wait here
             __builtin_amdgcn_sched_barrier(0);
-            dma_contiguous<NPB * KP, NT>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u),
-                                         srec, cnt * KP);
+            const int n = w.hi - w.lo;
+            dma_contiguous<NPB * KP, NT>(
+                rsrc(S.slab + (node0 + w.lo) * KP * 4, uint64_t(n) * KP * 16u),
+                srec + w.lo * KP, n * KP);
             __builtin_amdgcn_sched_barrier(0);
             union_copy<NT>(urows, ux, Uc, gin);
         }
@@ -890,13 +938,13 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     }
     if (TU) dma_fence();
     else __syncthreads();
-    if (STAGE == 1 && !fluid && i < G.N) {
+    if (STAGE == 1 && w.first && !fluid && i < G.N) {
         st4(S.A, own, g0);
         st4(S.B, own, g0);
         st4(S.acc, own, g0);
         return;
     }
-    if (!fluid) return;
+    if (!mine) return;
     float4 pre = STAGE == 4 ? rk4_pre4(g0, y1, y2, y3) : g0;
     // keep the fold here (the compiler would otherwise sink it to its use
     // and hold the sixteen inputs through the gather)
@@ -988,31 +1036,28 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
     float2* srec = reinterpret_cast<float2*>(srec4);
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
-    const int64_t tile = xcd_tile(bc.b, bc.nb);
-    const int64_t node0 = tile * NPB;
+    const UnionWork w = union_work<NPB, TU>(G, S, union_entry<TU>(S, bc.b, bc.nb));
+    const int64_t node0 = w.tile * NPB;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
     const __amdgpu_buffer_rsrc_t s1 = rsrc(S.A, rows);
     const float4 si = ld4(s1, own);
-    const int64_t nf = G.nfluid - node0;
-    const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
-    int U = 0;
-    const uint32_t* ul = TU ? S.ulist : nullptr;  // TU: tile unions built
-    if (cnt > 0) {
+    const int U = w.U;
+    const uint32_t* ul = TU ? S.ulist + w.u0 : nullptr;  // TU: tile unions built
+    if (w.hi > w.lo) {
         if (!TU) {
             const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
-            for (int t = threadIdx.x; t < cnt * K; t += NT) srec[t + t / K] = src[t];
+            for (int t = threadIdx.x; t < w.hi * K; t += NT) srec[t + t / K] = src[t];
         } else {
-            int64_t u0;
-            union_range(S.uoff, tile, u0, U);
-            ul += u0;
             const int Uc = U < S.ucap ? U : S.ucap;
             const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);
             __builtin_amdgcn_sched_barrier(0);
+            const int n = w.hi - w.lo;
             dma_contiguous<NPB * KF, NT>(
-                rsrc(S.fslab + node0 * KF * 4, uint64_t(cnt) * KF * 16u), srec4, cnt * KF);
+                rsrc(S.fslab + (node0 + w.lo) * KF * 4, uint64_t(n) * KF * 16u),
+                srec4 + w.lo * KF, n * KF);
             __builtin_amdgcn_sched_barrier(0);
             union_copy<NT>(urows, ux, Uc, s1);
         }
@@ -1020,8 +1065,11 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
     if (TU) dma_fence();
     else __syncthreads();
     if (i >= G.N) return;
+    const bool fluid = i < G.nfluid;
+    // fluid nodes: those this block serves; other nodes: the tile's block
+    if (fluid ? (slot < w.lo || slot >= w.hi) : !w.first) return;
     float4 out;
-    if (i < G.nfluid) {
+    if (fluid) {
         const uint32_t lane_off = uint32_t(q) * 16u;
         float4 sum;
         if (!ul)
@@ -1093,30 +1141,28 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_t(StepGeom G, StepBuffers
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
     typedef const __attribute__((address_space(4))) int32_t const_i32;
-    const int64_t tile = ((const_i32*)S.ptiles)[xcd_tile(bc.b, bc.nb)];
-    const int64_t node0 = tile * NPB;
+    const UnionWork w =
+        union_work<NPB, true>(G, S, ((const_i32*)S.ptiles)[xcd_tile(bc.b, bc.nb)]);
+    const int64_t node0 = w.tile * NPB;
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
     const int64_t i = node0 + slot;
     const uint64_t rows = uint64_t(G.N) * 128u;
     const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
-    const int64_t nf = G.nfluid - node0;
-    const int cnt = nf <= 0 ? 0 : (nf < NPB ? int(nf) : NPB);
     const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
     const __amdgpu_buffer_rsrc_t ar = rsrc(S.A, rows);
-    int64_t u0;
-    int U;
-    union_range(S.uoff, tile, u0, U);
-    const uint32_t* ul = S.ulist + u0;
+    const int U = w.U;
+    const uint32_t* ul = S.ulist + w.u0;
     const int Uc = U < S.ucap ? U : S.ucap;
     const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);
     __builtin_amdgcn_sched_barrier(0);
-    dma_contiguous<NPB * KP, NT>(rsrc(S.slab + node0 * KP * 4, uint64_t(cnt) * KP * 16u), srec,
-                                 cnt * KP);
+    const int n = w.hi - w.lo;
+    dma_contiguous<NPB * KP, NT>(rsrc(S.slab + (node0 + w.lo) * KP * 4, uint64_t(n) * KP * 16u),
+                                 srec + w.lo * KP, n * KP);
     __builtin_amdgcn_sched_barrier(0);
     union_copy<NT>(urows, ux, Uc, gf);
     const float4 s = ld4(ar, own);
     dma_fence();
-    if (slot >= cnt) return;
+    if (slot < w.lo || slot >= w.hi) return;
     const float4* rp = srec + slot * KP;
     const uint32_t lane_off = uint32_t(q) * 16u;
     float4 r;
@@ -1554,9 +1600,11 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
                   ? (un ? (S.ulist ? S.n_ptiles : tiles(S.n_pml)) : lanes(S.n_pml))
                   : 0;
     nb[kP3] = nbnd > 0 ? (un ? tiles32(nbnd) : lanes(nbnd)) : 0;
-    nb[kP4] = un ? tiles(G.N) : lanes(G.N);
-    nb[kS1] = un ? tiles(G.N) : lanes(G.N);
-    nb[kS2] = nb[kS3] = nb[kS4] = un ? tiles(G.nfluid) : lanes(G.nfluid);
+    // with tile unions, one more block per split tile (its second half)
+    const int64_t ns = (un && S.ulist) ? S.n_split : 0;
+    nb[kP4] = un ? tiles(G.N) + ns : lanes(G.N);
+    nb[kS1] = un ? tiles(G.N) + ns : lanes(G.N);
+    nb[kS2] = nb[kS3] = nb[kS4] = un ? tiles(G.nfluid) + ns : lanes(G.nfluid);
     nb[kP5] = nb[kP3];
 }
 

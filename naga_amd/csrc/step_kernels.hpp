@@ -76,13 +76,20 @@ struct StepBuffers {
     int nparts;
     const float* src;  // [N] density source of this step (physical units), or null
     // Tile unions (D3Q27 four-per-lane kernels): the fluid nodes form tiles
-    // of 32; ulist[uoff[t] .. uoff[t+1]) are the distinct stencil neighbours
+    // of 64 or 32; ulist[uoff[t] .. uoff[t+1]) are the distinct stencil neighbours
     // of tile t, and the index field of slab / fslab records holds the
     // neighbour's position in that list; slab and fslab rows are then
     // packed (packed_div_units, packed_filter_units).  Unions of at most
-    // ucap rows are staged in LDS by the gather kernels.
+    // ucap rows are staged in LDS by the gather kernels.  A tile over the
+    // cap with usplit[t] > 0 holds two half-tile unions (lengths usplit[t]
+    // and the rest, each at most ucap): its block serves the first half,
+    // and one more block per split tile, after the regular ones (tile
+    // entries -1 - s in ptiles), serves the second half of split_tiles[s].
     const uint32_t* ulist;
     const int64_t* uoff;
+    const int32_t* usplit;
+    const int32_t* split_tiles;
+    int64_t n_split;
     int ucap;
 };
 

@@ -295,6 +295,39 @@ def test_tile_unions_bit_identical(oracle64, monkeypatch):
     assert np.array_equal(runs[0], runs[1])
 
 
+def test_split_tiles_bit_identical(oracle64, monkeypatch):
+    """With a union cap below most 64-node tiles' unions, tiles whose two
+    half-tile unions fit are split (a second block per tile serves the
+    second half); the others read their rows through the list.  Split,
+    unsplit over-cap and direct-gather runs agree bit for bit."""
+    h = 0.04
+    nodes = grid_nodes(oracle64, 3, h, layer=3 * h)
+    nf = nodes["n_bulk"] + nodes["n_layer"]
+    # the setup's split decision, from the oracle's kNN (bit-exact with the GPU's)
+    pts = np.ascontiguousarray(nodes["points"], dtype=np.float32)
+    _, idx = oracle64.knn(pts, pts[:nf], 32)
+    cap, n_split = 300, 0
+    for t0 in range(0, nf - 32, 64):
+        tile = idx[t0:t0 + 64]
+        if len(np.unique(tile)) > cap and max(len(np.unique(tile[:32])),
+                                              len(np.unique(tile[32:]))) <= cap:
+            n_split += 1
+    assert n_split > 10
+    runs = []
+    for env in ({"NAGA_UCAP": str(cap)}, {"NAGA_UCAP": str(cap), "NAGA_SPLIT": "0"},
+                {"NAGA_UNIONS": "0"}):
+        for key in ("NAGA_UCAP", "NAGA_SPLIT", "NAGA_UNIONS"):
+            monkeypatch.delenv(key, raising=False)
+        for key, val in env.items():
+            monkeypatch.setenv(key, val)
+        eng = gpu_engine(nodes, 27, 32, h)
+        eng.set_initial_pulse([0.4, 0.5, 0.5], 1e-3, 0.15)
+        eng.step_forward(20)
+        runs.append(eng.distributions()["f"])
+    assert np.array_equal(runs[0], runs[2])
+    assert np.array_equal(runs[1], runs[2])
+
+
 def test_irregular_domain_2m_nodes(monkeypatch):
     """The irregular-domain scene at full size: the unit cube minus a
     central sphere of radius 0.15 at h = 0.008 with 2x refinement within
