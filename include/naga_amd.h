@@ -117,9 +117,10 @@ int naga_engine_sample_probes(naga_engine* e, int32_t probe_set, int32_t fresh,
  * (distributions, PML history, held macroscopic values, clock) to a host
  * buffer; replaces simulation_engine::save_state/load_state
  * (fluids/nonlocal_lattice_boltzmann/simulation_engine.cuh:108-116) with a
- * binary format: a header (magic "NAGAST01", lattice, sizes, time, frame)
- * then FP32 arrays.  Setup products are rebuilt by init_domain and are not
- * part of the state. */
+ * binary format: a header (magic "NAGAST02", lattice, sizes, time, frame)
+ * then FP32 arrays (distributions, the previous step's (rho - 1, u) for the
+ * PML terms, PML Q1, density, velocity).  Setup products are rebuilt by
+ * init_domain and are not part of the state. */
 int naga_engine_state_size(naga_engine* e, int64_t* bytes);
 int naga_engine_save_state(naga_engine* e, void* buf, int64_t capacity);
 int naga_engine_load_state(naga_engine* e, const void* buf, int64_t bytes);

@@ -361,10 +361,14 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         std::vector<uint8_t> none(std::max<int64_t>(nfluid, 1), 0);
         bsrc_ = to_device(none, st);
     }
-    for (DeviceBuffer<float>* b : {&g_, &A_, &B_, &acc_, &E_, &Qp_, &Q2v_, &coll_, &Gf_}) {
+    for (DeviceBuffer<float>* b : {&g_, &A_, &B_, &acc_, &Qp_, &Q2v_, &coll_, &Gf_}) {
         b->allocate(rows);
         b->zero(st);
     }
+    // the previous step's (rho - 1, u) for the PML terms: zero is the rest
+    // state, whose equilibrium deviation is zero
+    E_.allocate(size_t(N) * 4);
+    E_.zero(st);
     rho_.allocate(N);
     u_.allocate(N * D);
     ru_.allocate(N);
@@ -556,7 +560,7 @@ EngineState Engine::snapshot() {
                                 stream_));
     };
     copy(st.g, g_, rows);
-    copy(st.E, E_, rows);
+    copy(st.E, E_, size_t(N_) * 4);
     copy(st.Qp, Qp_, rows);
     copy(st.rho, rho_, size_t(N_));
     copy(st.u, u_, size_t(N_) * D_);
@@ -591,13 +595,15 @@ struct StateHeader {
     int64_t N, frame;
     double time;
 };
-constexpr char kStateMagic[8] = {'N', 'A', 'G', 'A', 'S', 'T', '0', '1'};
+// version 02: the PML history is the previous step's (rho - 1, u) per node
+constexpr char kStateMagic[8] = {'N', 'A', 'G', 'A', 'S', 'T', '0', '2'};
 }  // namespace
 
 int64_t Engine::state_bytes() const {
     require_domain();
     const int64_t rows = N_ * RS_;
-    return int64_t(sizeof(StateHeader)) + int64_t(sizeof(float)) * (3 * rows + N_ + N_ * D_);
+    return int64_t(sizeof(StateHeader)) +
+           int64_t(sizeof(float)) * (2 * rows + 4 * N_ + N_ + N_ * D_);
 }
 
 void Engine::save_state(void* buf, int64_t capacity) {
@@ -613,10 +619,9 @@ void Engine::save_state(void* buf, int64_t capacity) {
     char* p = static_cast<char*>(buf);
     std::memcpy(p, &h, sizeof(h));
     p += sizeof(h);
-    const size_t rows = size_t(N_) * RS_;
     for (DeviceBuffer<float>* b : {&g_, &E_, &Qp_}) {
-        b->download(reinterpret_cast<float*>(p), rows, stream_);
-        p += rows * sizeof(float);
+        b->download(reinterpret_cast<float*>(p), b->size(), stream_);
+        p += b->size() * sizeof(float);
     }
     rho_.download(reinterpret_cast<float*>(p), size_t(N_), stream_);
     p += size_t(N_) * sizeof(float);
@@ -633,10 +638,9 @@ void Engine::load_state(const void* buf, int64_t bytes) {
             "state was saved from a different scene");
     require(bytes >= state_bytes(), "state buffer truncated");
     const char* p = static_cast<const char*>(buf) + sizeof(h);
-    const size_t rows = size_t(N_) * RS_;
     for (DeviceBuffer<float>* b : {&g_, &E_, &Qp_}) {
-        b->upload(reinterpret_cast<const float*>(p), rows, stream_);
-        p += rows * sizeof(float);
+        b->upload(reinterpret_cast<const float*>(p), b->size(), stream_);
+        p += b->size() * sizeof(float);
     }
     rho_.upload(reinterpret_cast<const float*>(p), size_t(N_), stream_);
     p += size_t(N_) * sizeof(float);
@@ -1058,16 +1062,29 @@ void Engine::get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, flo
     NAGA_HIP(hipStreamSynchronize(stream_));
 }
 
+// feq_a - w_a at ru = (rho - 1, u), as the PML kernels form it.
+template <int Q>
+static double feq_dev_host(const float* ru, int a) {
+    using L = Lattice<Q>;
+    const double cs2 = kCsLat * kCsLat;
+    const double rt = double(ru[0]), rho = 1.0 + rt;
+    double uc = 0.0, uu = 0.0;
+    for (int d = L::D - 1; d >= 0; --d) uu += double(ru[1 + d]) * double(ru[1 + d]);
+    for (int d = 0; d < L::D; ++d) uc += double(ru[1 + d]) * double(L::c(a, d));
+    return L::w(a) * (rt + rho * (uc / cs2 + (uc * uc) / (2.0 * cs2 * cs2) - uu));
+}
+
 void Engine::get_pml(double* E, double* Qp) {
     require_domain();
     const int Q = cfg_.Q;
-    std::vector<float> e(size_t(N_) * RS_), q(size_t(N_) * RS_);
+    std::vector<float> e(size_t(N_) * 4), q(size_t(N_) * RS_);
     E_.download(e.data(), e.size(), stream_);
     Qp_.download(q.data(), q.size(), stream_);
     NAGA_HIP(hipStreamSynchronize(stream_));
     for (int a = 0; a < Q; ++a)
         for (int64_t i = 0; i < N_; ++i) {
-            if (E) E[a * N_ + i] = double(e[size_t(i) * RS_ + a]);
+            if (E) E[a * N_ + i] = Q == 27 ? feq_dev_host<27>(&e[size_t(i) * 4], a)
+                                           : feq_dev_host<9>(&e[size_t(i) * 4], a);
             if (Qp) Qp[a * N_ + i] = double(q[size_t(i) * RS_ + a]);
         }
 }

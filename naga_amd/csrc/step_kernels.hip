@@ -166,10 +166,26 @@ __global__ __launch_bounds__(kP1Block) void k_p1(StepGeom G, StepBuffers S,
 // ---------------------------------------------------------------------------
 constexpr int kPmlRounds = 8;
 
+// feq_a - w_a (equilibrium_distribution.cuh:52-63, cancellation-free) at
+// rho = 1 + rt and velocity u, ru = (rt, u).
+template <int Q>
+__device__ __forceinline__ double feq_dev(const float4& ru, int a) {
+    using L = Lattice<Q>;
+    constexpr double cs2 = kCsLat * kCsLat;
+    constexpr double inv_cs2 = 1.0 / cs2, inv_2cs4 = 1.0 / (2.0 * cs2 * cs2);
+    const double rt = double(ru.x), rho = 1.0 + rt;
+    const double u0 = ru.y, u1 = ru.z, u2 = ru.w;
+    const double uc = L::D == 3 ? u0 * double(L::c(a, 0)) + u1 * double(L::c(a, 1)) +
+                                      u2 * double(L::c(a, 2))
+                                : u0 * double(L::c(a, 0)) + u1 * double(L::c(a, 1));
+    const double uu = L::D == 3 ? u2 * u2 + (u1 * u1 + u0 * u0) : u1 * u1 + u0 * u0;
+    return L::w(a) * (rt + rho * (uc * inv_cs2 + (uc * uc) * inv_2cs4 - uu));
+}
+
 template <int Q>
 __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S, BatchArgs B) {
     using L = Lattice<Q>;
-    constexpr int D = L::D, RS = L::RS;
+    constexpr int RS = L::RS;
     constexpr int NPB = kBlock / 64 * L::NPW;
     __shared__ double red[NPB][2][Q];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -178,10 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S,
     const int slot = wave * L::NPW + (sub < L::NPW ? sub : 0);
     const int a = al < Q ? al : Q - 1;
     const BlockCtx bc = locate(B, G, S);
-    const double cs2 = kCsLat * kCsLat;
-    const double inv_cs2 = 1.0 / cs2, inv_2cs4 = 1.0 / (2.0 * cs2 * cs2);
     const double hdt = double(G.dt) / 2.0;
-    const double ca[3] = {double(L::c(a, 0)), double(L::c(a, 1)), D == 3 ? double(L::c(a, 2)) : 0.0};
     double s1 = 0.0, s2 = 0.0;
     const int64_t base = bc.b * (NPB * kPmlRounds);
     for (int rd = 0; rd < kPmlRounds; ++rd) {
@@ -189,16 +202,18 @@ __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S,
         if (!active || item >= S.n_rq) continue;
         const int64_t i = S.rq_list[item];
         const float4 ru = S.ru[i];
-        const double rt = double(ru.x), rho = 1.0 + rt;
-        const double u0 = ru.y, u1 = ru.z, u2 = ru.w;
-        const double uc = D == 3 ? u0 * ca[0] + u1 * ca[1] + u2 * ca[2] : u0 * ca[0] + u1 * ca[1];
-        const double uu = D == 3 ? u2 * u2 + (u1 * u1 + u0 * u0) : u1 * u1 + u0 * u0;
-        // feq - w (equilibrium_distribution.cuh:52-63), cancellation-free
-        const double fte = L::w(a) * (rt + rho * (uc * inv_cs2 + (uc * uc) * inv_2cs4 - uu));
+        float4* prev = reinterpret_cast<float4*>(S.E) + i;
+        // feq - w now and at the previous step's macroscopic values
+        const double fte = feq_dev<Q>(ru, a);
+        const double ftp = feq_dev<Q>(*prev, a);
         const int64_t o = i * RS + a;
-        const double q1 = (fte + double(S.E[o])) * hdt;
+        const double q1 = (fte + ftp) * hdt;
         const double q2 = (q1 + double(S.Qp[o])) * hdt;
-        S.E[o] = float(fte);
+        // the node's lanes (one wavefront) have read prev: the store waits
+        // for q1, which needs the loaded value
+        float4 keep = ru;
+        asm volatile("" : "+v"(keep.x) : "v"(q1));
+        if (al == 0) *prev = keep;
         S.Qp[o] = float(q1);
         const bool layer = i >= G.nbulk && i < G.nfluid;
         if (G.pml_ref ? i < G.nlayer : layer) S.Q2v[o] = float(q2);  // rows the layer field reads
@@ -1205,16 +1220,12 @@ __global__ __launch_bounds__(kBlock) void k_pml_field_v4(StepGeom G, StepBuffers
 // as k_pml_local): 32 node slots per round, kPmlRounds rounds per block,
 // one row of partial layer sums per block.
 __global__ __launch_bounds__(kBlock) void k_pml_local_v4(StepGeom G, StepBuffers S, BatchArgs B) {
-    using L = Lattice<27>;
     __shared__ double red[kV4Nodes][2][32];
     const BlockCtx bc = locate(B, G, S);
     const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
-    const double cs2 = kCsLat * kCsLat;
-    const double inv_cs2 = 1.0 / cs2, inv_2cs4 = 1.0 / (2.0 * cs2 * cs2);
     const double hdt = double(G.dt) / 2.0;
     const uint64_t rows = uint64_t(G.N) * 128u;
-    const __amdgpu_buffer_rsrc_t rE = rsrc(S.E, rows), rQp = rsrc(S.Qp, rows),
-                                 rA = rsrc(S.A, rows);
+    const __amdgpu_buffer_rsrc_t rQp = rsrc(S.Qp, rows), rA = rsrc(S.A, rows);
     double s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
     const int64_t base = bc.b * (kV4Nodes * kPmlRounds);
     for (int rd = 0; rd < kPmlRounds; ++rd) {
@@ -1222,30 +1233,26 @@ __global__ __launch_bounds__(kBlock) void k_pml_local_v4(StepGeom G, StepBuffers
         if (item >= S.n_rq) continue;
         const int64_t i = S.rq_list[item];
         const float4 ru = S.ru[i];
-        const double rt = double(ru.x), rho = 1.0 + rt;
-        const double u0 = ru.y, u1 = ru.z, u2 = ru.w;
-        const double uu = u2 * u2 + (u1 * u1 + u0 * u0);
+        float4* prev = reinterpret_cast<float4*>(S.E) + i;
+        const float4 pr = *prev;  // the previous step's (rho - 1, u)
         const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
-        const float4 e4 = ld4(rE, own), p4 = ld4(rQp, own);
+        const float4 p4 = ld4(rQp, own);
         const bool layer = i >= G.nbulk && i < G.nfluid;
         const double sig = layer ? double(S.sigma[i - G.nbulk]) : 0.0;
         float4 a4 = layer ? ld4(rA, own) : make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 eo, po, qo;
+        float4 po, qo;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int a = 4 * q + e;
             if (a >= 27) {  // padding slots stay zero
-                set(eo, e, 0.0f);
                 set(po, e, 0.0f);
                 set(qo, e, 0.0f);
                 continue;
             }
-            const double uc = u0 * double(L::c(a, 0)) + u1 * double(L::c(a, 1)) +
-                              u2 * double(L::c(a, 2));
-            const double fte = L::w(a) * (rt + rho * (uc * inv_cs2 + (uc * uc) * inv_2cs4 - uu));
-            const double q1 = (fte + double(get(e4, e))) * hdt;
+            // feq - w now and at the previous step's macroscopic values
+            const double fte = feq_dev<27>(ru, a);
+            const double q1 = (fte + feq_dev<27>(pr, a)) * hdt;
             const double q2 = (q1 + double(get(p4, e))) * hdt;
-            set(eo, e, float(fte));
             set(po, e, float(q1));
             set(qo, e, float(q2));
             if (layer) {
@@ -1254,7 +1261,11 @@ __global__ __launch_bounds__(kBlock) void k_pml_local_v4(StepGeom G, StepBuffers
                 s2[e] += q2;
             }
         }
-        st4(S.E, own, eo);
+        // the node's eight lanes (one wavefront) have read prev: the store
+        // waits for the products of the loaded value
+        float4 keep = ru;
+        asm volatile("" : "+v"(keep.x) : "v"(get(po, 0)));
+        if (q == 0) *prev = keep;
         st4(S.Qp, own, po);
         if (G.pml_ref ? i < G.nlayer : layer) st4(S.Q2v, own, qo);  // rows the layer field reads
         if (layer) st4(S.A, own, a4);

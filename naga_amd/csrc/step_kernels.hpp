@@ -55,7 +55,9 @@ struct StepBuffers {
     float* Gf;       // [N][RS] PML auxiliary field (zero outside the layer)
     const int32_t* rq_list;  // nodes whose PML Q values are maintained
     int64_t n_rq;
-    float* E;        // [N][RS] previous equilibrium deviation (PML Q1 - w)
+    float* E;        // [N][4] (rho - 1, u) of the previous step's PML local
+                     // terms, from which the previous equilibrium deviation
+                     // (PML Q1 - w) is recomputed
     float* Qp;       // [N][RS] previous Q1 value (PML Q2 source)
     float* Q2v;      // [N][RS] current Q2 value
     const float* slab;    // [nfluid][k][1+D] stencil records
