@@ -207,6 +207,13 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     split_tiles_ = DeviceBuffer<int32_t>();
     split_of_.clear();
     n_split_ = 0;
+    p2ulist_ = DeviceBuffer<uint32_t>();
+    p2uoff_ = DeviceBuffer<int64_t>();
+    p2usplit_ = DeviceBuffer<int32_t>();
+    p2split_tiles_ = DeviceBuffer<int32_t>();
+    p2slab_ = DeviceBuffer<float>();
+    p2split_of_.clear();
+    p2n_split_ = 0;
     // Tile unions: the gather kernels copy each tile's distinct stencil
     // neighbours into LDS by LDS DMA and read them from there (fewer bytes
     // through the vector L1 than direct gathers: 0.2 against 0.27 ms per
@@ -225,23 +232,59 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         // those fit (NAGA_SPLIT=0: never)
         const char* sp_env = std::getenv("NAGA_SPLIT");
         const bool split = !(sp_env && sp_env[0] == '0');
-        DeviceBuffer<int32_t> counts(ntiles);
-        usplit_.allocate(ntiles);
-        launch_tile_unions(nfluid, k, T, idx_.get(), split ? cap : 0, counts.get(), usplit_.get(),
-                           nullptr, nullptr, nullptr, 0, nullptr, 0, st);
-        std::vector<int32_t> hc(ntiles);
-        counts.download(hc.data(), ntiles, st);
-        NAGA_HIP(hipStreamSynchronize(st));
-        std::vector<int64_t> off(ntiles + 1, 0);
-        int maxu = 0;
-        for (int64_t t = 0; t < ntiles; ++t) {
-            off[t + 1] = off[t] + hc[t];
-            maxu = std::max(maxu, hc[t]);
+        // one union set over the stencils idx_in: lists, offsets, split
+        // tiles; the index fields of slab_a / slab_b become union positions
+        auto build = [&](const uint32_t* idx_in, float* slab_a, int nca, float* slab_b, int ncb,
+                         DeviceBuffer<uint32_t>& ulist, DeviceBuffer<int64_t>& uoff,
+                         DeviceBuffer<int32_t>& usplit, DeviceBuffer<int32_t>& split_tiles,
+                         std::vector<int32_t>& split_of, int64_t& n_split) {
+            DeviceBuffer<int32_t> counts(ntiles);
+            usplit.allocate(ntiles);
+            launch_tile_unions(nfluid, k, T, idx_in, split ? cap : 0, counts.get(), usplit.get(),
+                               nullptr, nullptr, nullptr, 0, nullptr, 0, st);
+            std::vector<int32_t> hc(ntiles), hs(ntiles), splits;
+            counts.download(hc.data(), ntiles, st);
+            usplit.download(hs.data(), ntiles, st);
+            NAGA_HIP(hipStreamSynchronize(st));
+            std::vector<int64_t> off(ntiles + 1, 0);
+            int maxu = 0;
+            split_of.assign(ntiles, -1);
+            for (int64_t t = 0; t < ntiles; ++t) {
+                off[t + 1] = off[t] + hc[t];
+                maxu = std::max(maxu, hc[t]);
+                if (hs[t] > 0) {  // split: one more block serves the second half
+                    split_of[t] = int32_t(splits.size());
+                    splits.push_back(int32_t(t));
+                }
+            }
+            uoff = to_device(off, st);
+            ulist.allocate(std::max<int64_t>(off[ntiles], 1));
+            launch_tile_unions(nfluid, k, T, idx_in, split ? cap : 0, nullptr, usplit.get(),
+                               uoff.get(), ulist.get(), slab_a, nca, slab_b, ncb, st);
+            n_split = int64_t(splits.size());
+            split_tiles = to_device(splits.empty() ? std::vector<int32_t>{0} : splits, st);
+            return maxu;
+        };
+        // The PML divergence reads Gf, zero outside the layer: its unions
+        // hold the layer neighbours and one zero row (every other neighbour
+        // mapped to a node outside the layer), about 60% of the full union.
+        DeviceBuffer<float> slab2;
+        if (nlayer > 0) {
+            DeviceBuffer<uint32_t> idx2(nfluid * k);
+            const uint32_t zero_row = nbnd > 0 ? uint32_t(nfluid) : 0u;
+            launch_layer_only(nfluid * k, idx_.get(), uint32_t(nbulk), uint32_t(nfluid), zero_row,
+                              idx2.get(), st);
+            slab2.allocate(nfluid * k * (1 + D));
+            NAGA_HIP(hipMemcpyAsync(slab2.get(), slab_.get(), slab2.size() * sizeof(float),
+                                    hipMemcpyDeviceToDevice, st));
+            build(idx2.get(), slab2.get(), D, nullptr, 0, p2ulist_, p2uoff_, p2usplit_,
+                  p2split_tiles_, p2split_of_, p2n_split_);
+            p2slab_.allocate(nfluid * packed_div_units(k) * 4);
+            launch_pack_div_records(nfluid, k, slab2.get(), p2slab_.get(), st);
+            NAGA_HIP(hipStreamSynchronize(st));
         }
-        uoff_ = to_device(off, st);
-        ulist_.allocate(std::max<int64_t>(off[ntiles], 1));
-        launch_tile_unions(nfluid, k, T, idx_.get(), split ? cap : 0, nullptr, usplit_.get(),
-                           uoff_.get(), ulist_.get(), slab_.get(), D, fslab_.get(), 1, st);
+        const int maxu = build(idx_.get(), slab_.get(), D, fslab_.get(), 1, ulist_, uoff_,
+                               usplit_, split_tiles_, split_of_, n_split_);
         // the union kernels copy records into LDS as they lie: packed rows
         // (16-bit union positions, an odd number of 16-byte units per node)
         // keep the records of consecutive nodes in distinct LDS banks
@@ -253,18 +296,6 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         slab_ = std::move(ps);
         fslab_ = std::move(pf);
         ucap_ = std::min(maxu, cap);
-        // split tiles: one more block each serves the second half
-        std::vector<int32_t> hs(ntiles), splits;
-        usplit_.download(hs.data(), ntiles, st);
-        NAGA_HIP(hipStreamSynchronize(st));
-        split_of_.assign(ntiles, -1);
-        for (int64_t t = 0; t < ntiles; ++t)
-            if (hs[t] > 0) {
-                split_of_[t] = int32_t(splits.size());
-                splits.push_back(int32_t(t));
-            }
-        n_split_ = int64_t(splits.size());
-        split_tiles_ = to_device(splits.empty() ? std::vector<int32_t>{0} : splits, st);
     }
 
     // boundary interpolator (init_domain :223-258): kNN of boundary nodes
@@ -330,8 +361,8 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
                         const int64_t t = t0 / T;
                         tl.push_back(int32_t(t));
                         // a split tile's second half: entry -1 - s
-                        if (t < int64_t(split_of_.size()) && split_of_[t] >= 0)
-                            tl.push_back(-1 - split_of_[t]);
+                        if (t < int64_t(p2split_of_.size()) && p2split_of_[t] >= 0)
+                            tl.push_back(-1 - p2split_of_[t]);
                         break;
                     }
             }
@@ -425,6 +456,12 @@ StepBuffers Engine::buffers() {
     S.usplit = usplit_.get();
     S.split_tiles = split_tiles_.get();
     S.n_split = n_split_;
+    S.p2slab = p2slab_.get();
+    S.p2ulist = p2ulist_.get();
+    S.p2uoff = p2uoff_.get();
+    S.p2usplit = p2usplit_.get();
+    S.p2split_tiles = p2split_tiles_.get();
+    S.p2n_split = p2n_split_;
     S.ucap = ucap_;
     return S;
 }

@@ -145,6 +145,14 @@ class Engine {
     DeviceBuffer<int32_t> split_tiles_;  // the split tiles, in order
     std::vector<int32_t> split_of_;      // per tile: its index among the split tiles, or -1
     int64_t n_split_ = 0;
+    // the PML divergence's unions (layer neighbours and one zero row) and
+    // its packed records holding positions in them
+    DeviceBuffer<uint32_t> p2ulist_;
+    DeviceBuffer<int64_t> p2uoff_;
+    DeviceBuffer<int32_t> p2usplit_, p2split_tiles_;
+    std::vector<int32_t> p2split_of_;
+    int64_t p2n_split_ = 0;
+    DeviceBuffer<float> p2slab_;
     int ucap_ = 0;
 
     struct ProbeSet {

@@ -681,6 +681,15 @@ __global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, int t
                 sb ? sb + int64_t(h) * k * (1 + ncb) : nullptr, ncb);
 }
 
+// out[e] = idx[e] if it is in [lo, hi), else zero_row.
+__global__ void k_layer_only(int64_t n, const uint32_t* idx, uint32_t lo, uint32_t hi,
+                             uint32_t zero_row, uint32_t* out) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t j = idx[e];
+    out[e] = (j >= lo && j < hi) ? j : zero_row;
+}
+
 __global__ void k_flag(int64_t n, int k, const uint32_t* idx, int64_t lo,
                        int64_t hi, uint8_t* flag) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -931,6 +940,13 @@ void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* i
     k_tile_union<<<ntiles, 256, 0, st>>>(nfluid, k, tile_nodes, idx, cap, counts, usplit, uoff,
                                           ulist, slab_a, ncomp_a, slab_b, ncomp_b);
     hip_check(hipGetLastError(), "tile unions");
+}
+
+void launch_layer_only(int64_t n, const uint32_t* idx, uint32_t lo, uint32_t hi,
+                       uint32_t zero_row, uint32_t* out, hipStream_t st) {
+    if (n == 0) return;
+    k_layer_only<<<(n + 255) / 256, 256, 0, st>>>(n, idx, lo, hi, zero_row, out);
+    hip_check(hipGetLastError(), "layer-only stencil");
 }
 
 void launch_flag_neighbours(int64_t n, int k, const uint32_t* idx, int64_t lo,

@@ -71,6 +71,10 @@ void launch_pack_div_records(int64_t n, int k, const float* src, float* dst, hip
 void launch_pack_filter_records(int64_t n, int k, const float* src, float* dst,
                                 hipStream_t st);
 
+// out[e] = idx[e] where it lies in [lo, hi), else zero_row (n entries).
+void launch_layer_only(int64_t n, const uint32_t* idx, uint32_t lo, uint32_t hi,
+                       uint32_t zero_row, uint32_t* out, hipStream_t stream);
+
 // Tile unions: fluid nodes are grouped in tiles of tile_nodes consecutive
 // nodes (k * tile_nodes <= 4096); a tile's union is the sorted list of the
 // distinct neighbour indices of its nodes' k-stencils.  A tile whose union

@@ -1155,6 +1155,13 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_t(StepGeom G, StepBuffers
     __shared__ float4 srec[NPB * KP];
     extern __shared__ float4 urows[];
     const BlockCtx bc = locate(B, G, S);
+    // this pass's unions: the layer neighbours and one zero row
+    S.slab = S.p2slab;
+    S.ulist = S.p2ulist;
+    S.uoff = S.p2uoff;
+    S.usplit = S.p2usplit;
+    S.split_tiles = S.p2split_tiles;
+    S.n_split = S.p2n_split;
     typedef const __attribute__((address_space(4))) int32_t const_i32;
     const UnionWork w =
         union_work<NPB, true>(G, S, ((const_i32*)S.ptiles)[xcd_tile(bc.b, bc.nb)]);

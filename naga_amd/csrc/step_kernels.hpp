@@ -93,6 +93,15 @@ struct StepBuffers {
     const int32_t* split_tiles;
     int64_t n_split;
     int ucap;
+    // The same for the PML divergence, whose unions hold the layer
+    // neighbours and one row outside the layer (Gf is zero there): its
+    // packed records, lists, offsets and split tiles.
+    const float* p2slab;
+    const uint32_t* p2ulist;
+    const int64_t* p2uoff;
+    const int32_t* p2usplit;
+    const int32_t* p2split_tiles;
+    int64_t p2n_split;
 };
 
 // Batched launches: one launch of a pass covers the scenes of a batch; the
