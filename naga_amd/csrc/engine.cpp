@@ -214,6 +214,8 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     p2slab_ = DeviceBuffer<float>();
     p2split_of_.clear();
     p2n_split_ = 0;
+    bulist_ = DeviceBuffer<uint32_t>();
+    buoff_ = DeviceBuffer<int64_t>();
     // Tile unions: the gather kernels copy each tile's distinct stencil
     // neighbours into LDS by LDS DMA and read them from there (fewer bytes
     // through the vector L1 than direct gathers: 0.2 against 0.27 ms per
@@ -275,8 +277,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
             launch_layer_only(nfluid * k, idx_.get(), uint32_t(nbulk), uint32_t(nfluid), zero_row,
                               idx2.get(), st);
             slab2.allocate(nfluid * k * (1 + D));
-            NAGA_HIP(hipMemcpyAsync(slab2.get(), slab_.get(), slab2.size() * sizeof(float),
-                                    hipMemcpyDeviceToDevice, st));
+            launch_make_slab(nfluid, k, D, idx_.get(), W_.get(), slab2.get(), st);
             build(idx2.get(), slab2.get(), D, nullptr, 0, p2ulist_, p2uoff_, p2usplit_,
                   p2split_tiles_, p2split_of_, p2n_split_);
             p2slab_.allocate(nfluid * packed_div_units(k) * 4);
@@ -331,6 +332,24 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         bsrc_ = to_device(src, st);
         bslab_.allocate(nbnd * kb * 2);
         launch_make_slab(nbnd, kb, 1, bidx_.get(), bw_.get(), bslab_.get(), st);
+        if (tile_nodes_ > 0) {
+            // boundary tiles of 32 nodes: their distinct source nodes are
+            // staged in LDS by the interpolation kernels; bslab index fields
+            // become positions in that list
+            const int64_t nbt = (nbnd + kBTileNodes - 1) / kBTileNodes;
+            DeviceBuffer<int32_t> counts(nbt);
+            launch_tile_unions(nbnd, kb, kBTileNodes, bidx_.get(), 0, counts.get(), nullptr,
+                               nullptr, nullptr, nullptr, 0, nullptr, 0, st);
+            std::vector<int32_t> hc(nbt);
+            counts.download(hc.data(), nbt, st);
+            NAGA_HIP(hipStreamSynchronize(st));
+            std::vector<int64_t> off(nbt + 1, 0);
+            for (int64_t t = 0; t < nbt; ++t) off[t + 1] = off[t] + hc[t];
+            buoff_ = to_device(off, st);
+            bulist_.allocate(std::max<int64_t>(off[nbt], 1));
+            launch_tile_unions(nbnd, kb, kBTileNodes, bidx_.get(), 0, nullptr, nullptr, buoff_.get(),
+                               bulist_.get(), bslab_.get(), 1, nullptr, 0, st);
+        }
         normals_.allocate(nbnd * D);
         normals_.upload(normals, nbnd * D, st);
     }
@@ -462,6 +481,8 @@ StepBuffers Engine::buffers() {
     S.p2usplit = p2usplit_.get();
     S.p2split_tiles = p2split_tiles_.get();
     S.p2n_split = p2n_split_;
+    S.bulist = bulist_.get();
+    S.buoff = buoff_.get();
     S.ucap = ucap_;
     return S;
 }

@@ -153,6 +153,9 @@ class Engine {
     std::vector<int32_t> p2split_of_;
     int64_t p2n_split_ = 0;
     DeviceBuffer<float> p2slab_;
+    // boundary-interpolation tiles: their distinct source nodes
+    DeviceBuffer<uint32_t> bulist_;
+    DeviceBuffer<int64_t> buoff_;
     int ucap_ = 0;
 
     struct ProbeSet {

@@ -1336,6 +1336,94 @@ __global__ __launch_bounds__(kBlock) void k_binterp_v4(StepGeom G, StepBuffers S
     st4(field, own, out);
 }
 
+// Boundary interpolation for D3Q27 with the sources of each 32-node tile
+// staged in LDS (the field rows, and for the first pass the collision
+// increments too) by LDS DMA: about 6 rows per node instead of 32 (64)
+// gathers.  Same records, products and order as k_binterp_v4.
+static_assert(kBTileNodes == kV4Nodes, "one boundary tile per block");
+// LDS of k_binterp_u: the tile's records (kb <= 63: 32 x 63 {position, w})
+// after the field rows and, for the first pass, the increments.
+constexpr int kBRecUnits = (kBTileNodes * 63 * 8 + 15) / 16;
+template <bool ACCUMULATE>
+constexpr size_t binterp_lds() {
+    return size_t((ACCUMULATE ? 2 : 1) * kBUnionCap * kV4Lanes + kBRecUnits) * 16u;
+}
+
+template <bool ACCUMULATE>
+__global__ __launch_bounds__(kBlock) void k_binterp_u(StepGeom G, StepBuffers S, BatchArgs B) {
+    extern __shared__ float4 urows[];  // field rows, (ACCUMULATE) increments, records
+    float4* recs = urows + (ACCUMULATE ? 2 : 1) * kBUnionCap * kV4Lanes;
+    const BlockCtx bc = locate(B, G, S);
+    const int slot = threadIdx.x / kV4Lanes, q = threadIdx.x % kV4Lanes;
+    const int64_t nbnd = G.N - G.nfluid;
+    const int64_t tile = xcd_tile(bc.b, bc.nb);
+    const int64_t m0 = tile * kV4Nodes, m = m0 + slot;
+    float* field = ACCUMULATE ? S.A : S.g;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const __amdgpu_buffer_rsrc_t f = rsrc(field, rows), cl = rsrc(S.coll, rows);
+    int64_t u0;
+    int U;
+    union_range(S.buoff, tile, u0, U);
+    const uint32_t* ul = S.bulist + u0;
+    const bool lds = U <= kBUnionCap;
+    const UnionIdx<kBlock> ux = union_indices<kBlock>(ul, lds ? U : 0);
+    __builtin_amdgcn_sched_barrier(0);
+    // the tile's records: cnt * kb pairs from the tile's start (the last
+    // tile's piece past the slab reads as zero)
+    const int64_t left = nbnd - m0;
+    const int cnt = left < kV4Nodes ? int(left) : kV4Nodes;
+    dma_contiguous<kBRecUnits, kBlock>(
+        rsrc(S.bslab + m0 * G.kb * 2, uint64_t(nbnd - m0) * G.kb * 8u), recs,
+        (cnt * G.kb * 8 + 15) / 16);
+    __builtin_amdgcn_sched_barrier(0);
+    if (lds) {
+        union_copy<kBlock>(urows, ux, U, f);
+        if (ACCUMULATE) union_copy<kBlock>(urows + kBUnionCap * kV4Lanes, ux, U, cl);
+    }
+    dma_fence();
+    if (m >= nbnd) return;
+    const float2* rp = reinterpret_cast<const float2*>(recs) + slot * G.kb;
+    const uint32_t lane_off = uint32_t(q) * 16u;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    constexpr int BT = 8;
+    for (int s0 = 0; s0 < G.kb; s0 += BT) {
+        float2 rec[BT];
+        float4 v[BT];
+#pragma unroll
+        for (int u = 0; u < BT; ++u)
+            rec[u] = s0 + u < G.kb ? rp[s0 + u] : make_float2(__uint_as_float(0u), 0.0f);
+#pragma unroll
+        for (int u = 0; u < BT; ++u) {
+            const uint32_t j = __float_as_uint(rec[u].x);
+            if (lds) {
+                v[u] = urows[j * kV4Lanes + q];
+                if (ACCUMULATE) {
+                    const float4 c = urows[(kBUnionCap + j) * kV4Lanes + q];
+                    v[u] = make_float4(v[u].x - c.x, v[u].y - c.y, v[u].z - c.z, v[u].w - c.w);
+                }
+            } else {  // sources over the cap: rows read from HBM through the list
+                const uint32_t off = ul[j] * 128u + lane_off;
+                v[u] = ld4(f, off);
+                if (ACCUMULATE) {
+                    const float4 c = ld4(cl, off);
+                    v[u] = make_float4(v[u].x - c.x, v[u].y - c.y, v[u].z - c.z, v[u].w - c.w);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BT; ++u)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[e] += rec[u].y * get(v[u], e);
+    }
+    const uint32_t own = uint32_t(G.nfluid + m) * 128u + lane_off;
+    float4 out = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    if (ACCUMULATE) {
+        const float4 b = ld4(f, own);
+        out = make_float4(b.x + out.x, b.y + out.y, b.z + out.z, b.w + out.w);
+    }
+    st4(field, own, out);
+}
+
 __global__ __launch_bounds__(kBlock) void k_source_field(const int32_t* nodes,
                                                        const int32_t* off, const int32_t* ids,
                                                        const float* prof, int64_t n,
@@ -1569,7 +1657,9 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
             else k_p2<Q, 0><<<nb, kBlock, 0, st>>>(G, S, c.B);
             break;
         case kP3:
-            if (v4) k_binterp_v4<true><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            if (v4 && c.tiles)
+                k_binterp_u<true><<<nb, kBlock, binterp_lds<true>(), st>>>(G, S, c.B);
+            else if (v4) k_binterp_v4<true><<<nb, kBlock, 0, st>>>(G, S, c.B);
             else k_binterp<Q, true><<<nb, kBlock, 0, st>>>(G, S, c.B);
             break;
         case kP4:
@@ -1582,7 +1672,9 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         case kS3: launch_stage_pass<Q, 3>(nb, G, S, c, st); break;
         case kS4: launch_stage_pass<Q, 4>(nb, G, S, c, st); break;
         default:
-            if (Q == 27 && c.unions) k_binterp_v4<false><<<nb, kBlock, 0, st>>>(G, S, c.B);
+            if (Q == 27 && c.unions && c.tiles)
+                k_binterp_u<false><<<nb, kBlock, binterp_lds<false>(), st>>>(G, S, c.B);
+            else if (Q == 27 && c.unions) k_binterp_v4<false><<<nb, kBlock, 0, st>>>(G, S, c.B);
             else k_binterp<Q, false><<<nb, kBlock, 0, st>>>(G, S, c.B);
             break;
     }

@@ -102,6 +102,12 @@ struct StepBuffers {
     const int32_t* p2usplit;
     const int32_t* p2split_tiles;
     int64_t p2n_split;
+    // Boundary interpolation tiles (32 boundary nodes): the distinct source
+    // nodes of each tile, bulist[buoff[t] .. buoff[t+1]), and bslab index
+    // fields holding positions in that list (null: no boundary unions).
+    // Tiles of at most kBUnionCap sources are staged in LDS.
+    const uint32_t* bulist;
+    const int64_t* buoff;
 };
 
 // Batched launches: one launch of a pass covers the scenes of a batch; the
@@ -174,6 +180,8 @@ int union_tile_nodes(int k);
 // k = 48 (two blocks per CU, the records taking 22 KB).  A tile with a
 // larger union reads its rows from HBM through the union list.
 int union_cap_default(int k);
+constexpr int kBUnionCap = 256;   // LDS rows per boundary-interpolation tile
+constexpr int kBTileNodes = 32;   // boundary nodes per boundary-interpolation tile
 constexpr int kUnionRowsMax = 512;  // LDS union rows per block, upper bound
 
 // Per-kernel entry points (used by the profiler and the standalone C-ABI).

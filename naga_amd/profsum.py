@@ -10,7 +10,8 @@
         over the same program.  Each counter is divided by what it reads,
         per byte, for the runtime's device-to-device copy kernel
         (__amd_rocclr_copyBuffer) of known size in the same directory
-        (``python bench.py --copy-calibration BYTES`` under the same pass).
+        (``python bench.py --copy-calibration BYTES`` under the same pass,
+        written with ``-o copy``: only copy_* files calibrate).
 
 Output is Markdown on stdout; the traffic summary carries one line
 ``traffic_bytes_per_launch = <int>`` that bench.py reads.
@@ -62,15 +63,22 @@ def stats(args) -> None:
 
 
 def _per_dispatch(dirname: str, counter: str, match: str):
+    """Counter bytes of the dispatches of `match`, and of the copy kernel in
+    the calibration run's output (files named copy_*; the program measured
+    may run copies of its own)."""
     vals, copies = [], []
-    for r in _rows(os.path.join(dirname, "*counter_collection.csv")):
-        if r["Counter_Name"] != counter:
-            continue
-        v = float(r["Counter_Value"]) * 1024.0  # KiB -> B
-        if COPY_KERNEL in r["Kernel_Name"]:
-            copies.append(v)
-        elif match in r["Kernel_Name"]:
-            vals.append(v)
+    for path in sorted(glob.glob(os.path.join(dirname, "*counter_collection.csv"))):
+        calib = os.path.basename(path).startswith("copy")
+        with open(path, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != counter:
+                    continue
+                v = float(r["Counter_Value"]) * 1024.0  # KiB -> B
+                if COPY_KERNEL in r["Kernel_Name"]:
+                    if calib:
+                        copies.append(v)
+                elif match in r["Kernel_Name"]:
+                    vals.append(v)
     return vals, copies
 
 
