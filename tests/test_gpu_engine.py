@@ -295,6 +295,27 @@ def test_tile_unions_bit_identical(oracle64, monkeypatch):
     assert np.array_equal(runs[0], runs[1])
 
 
+def test_3d_k16_unions(oracle64, monkeypatch):
+    """D3Q27 with k = 16 (64-node union tiles, packed rows of 15 units):
+    10 steps match the FP64 oracle, and the tile-union and direct-gather
+    paths agree bit for bit."""
+    h = 0.05
+    nodes = grid_nodes(oracle64, 3, h, layer=3 * h)
+    runs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("NAGA_UNIONS", flag)
+        eng = gpu_engine(nodes, 27, 16, h)
+        eng.set_initial_pulse([0.4, 0.5, 0.5], 1e-3, 0.15)
+        eng.step_forward(10)
+        runs.append((eng.distributions()["f"], eng.macroscopic(fresh=True)))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    ora = oracle_engine(oracle64, nodes, 27, 16, h)
+    ora.set_pulse([0.4, 0.5, 0.5], 1e-3, 0.15)
+    ora.step(10)
+    g, o = runs[0][1], ora.get(fresh=True)
+    assert rel_l2(g["drho"], o["drho"]) < TOL and rel_l2(g["u"], o["u"]) < TOL
+
+
 def test_split_tiles_bit_identical(oracle64, monkeypatch):
     """With a union cap below most 64-node tiles' unions, tiles whose two
     half-tile unions fit are split (a second block per tile serves the
