@@ -425,7 +425,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     nparts_ = int(std::max<int64_t>(pml_parts(Q, k, N, n_rq_), 1));
     partial_.allocate(size_t(nparts_) * 2 * Q);
     partial_.zero(st);
-    means_.allocate(2 * Q);
+    means_.allocate(2 * means_stride(Q));
     means_.zero(st);
 
     geo_.N = N;

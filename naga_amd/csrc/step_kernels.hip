@@ -251,7 +251,9 @@ __global__ __launch_bounds__(256) void k_means(StepGeom G, StepBuffers S, BatchA
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) S.means[col] = G.nlayer > 0 ? float(red[0] / double(G.nlayer)) : 0.0f;
+    if (threadIdx.x == 0)
+        S.means[(col / Q) * means_stride(Q) + col % Q] =
+            G.nlayer > 0 ? float(red[0] / double(G.nlayer)) : 0.0f;
 }
 
 // PML auxiliary fields on layer rows (pml_absorption.inl:440-453, :500-513):
@@ -270,7 +272,8 @@ __global__ __launch_bounds__(kBlock) void k_pml_field(StepGeom G, StepBuffers S,
     const int a = m.a;
     const float sig = S.sigma[m.node];
     const float q1 = S.Qp[p * RS + a], q2 = S.Q2v[p * RS + a];
-    S.Gf[j * RS + a] = -2.0f * sig * (q1 - S.means[a]) + sig * sig * (q2 - S.means[Q + a]);
+    S.Gf[j * RS + a] =
+        -2.0f * sig * (q1 - S.means[a]) + sig * sig * (q2 - S.means[means_stride(Q) + a]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1211,12 +1214,14 @@ __global__ __launch_bounds__(kBlock) void k_pml_field_v4(StepGeom G, StepBuffers
     const float sig = S.sigma[m];
     const float4 q1 = ld4(rsrc(S.Qp, rows), uint32_t(p) * 128u + lane_off);
     const float4 q2 = ld4(rsrc(S.Q2v, rows), uint32_t(p) * 128u + lane_off);
+    const float4 m1 = reinterpret_cast<const float4*>(S.means)[q];
+    const float4 m2 = reinterpret_cast<const float4*>(S.means + means_stride(27))[q];
     float4 out;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int a = 4 * q + e;
-        const float v = a < 27 ? -2.0f * sig * (get(q1, e) - S.means[a]) +
-                                     sig * sig * (get(q2, e) - S.means[27 + a])
+        const float v = a < 27 ? -2.0f * sig * (get(q1, e) - get(m1, e)) +
+                                     sig * sig * (get(q2, e) - get(m2, e))
                                : 0.0f;
         set(out, e, v);
     }

@@ -74,7 +74,7 @@ struct StepBuffers {
     float* rho;      // [N] density (reference semantics, FP32)
     float* u;        // [D][N] velocity
     double* partial; // [nparts][2Q] layer column sums (one row per pl block)
-    float* means;    // [2Q] layer means of Q1, Q2
+    float* means;    // layer means: Q1 at [a], Q2 at [means_stride(Q) + a]
     int nparts;
     const float* src;  // [N] density source of this step (physical units), or null
     // Tile unions (D3Q27 four-per-lane kernels): the fluid nodes form tiles
@@ -180,6 +180,9 @@ int union_tile_nodes(int k);
 // k = 48 (two blocks per CU, the records taking 22 KB).  A tile with a
 // larger union reads its rows from HBM through the union list.
 int union_cap_default(int k);
+// Stride of the two rows of layer means (D3Q27: 32, so each row's
+// quarters load as 16-byte vectors; padding entries stay zero).
+constexpr int means_stride(int Q) { return Q == 27 ? 32 : Q; }
 constexpr int kBUnionCap = 256;   // LDS rows per boundary-interpolation tile
 constexpr int kBTileNodes = 32;   // boundary nodes per boundary-interpolation tile
 constexpr int kUnionRowsMax = 512;  // LDS union rows per block, upper bound
