@@ -102,31 +102,18 @@ def cpu_baseline(threads: int) -> dict:
                       f"after 1 warm-up step, FP32 oracle port, OpenMP {threads} threads"}
 
 
-def batched_workload(steps: int, n_scenes: int = 256, seed: int = 0x4E414741) -> dict:
-    """Many small D2Q9 scenes: node counts log-uniform in [2k, 50k], 0-5
-    circular obstacles of radius 0.03-0.1, k in [12, 24], random pulse.
-    The same scenes are stepped one scene at a time (each engine's own
-    launches, scene after scene) and together in one batch (one launch per
-    pass over all scenes, graph replay, autotuned gather batch)."""
+def batched_workload(steps: int, n_scenes: int = 256) -> dict:
+    """Many small D2Q9 scenes (naga_amd.workloads.batched_scenes: node counts
+    log-uniform in [2k, 50k], 0-5 circular obstacles of radius 0.03-0.1, k in
+    [12, 24], random pulse).  The same scenes are stepped one scene at a time
+    (each engine's own launches, scene after scene) and together in one
+    batch (one launch per pass over all scenes, graph replay, autotuned)."""
     import naga_amd
-    rng = np.random.default_rng(seed)
+    from naga_amd import workloads
     engines, total = [], 0
     t0 = time.perf_counter()
-    for _ in range(n_scenes):
-        n = float(np.exp(rng.uniform(np.log(2e3), np.log(5e4))))
-        h = 1.0 / (np.sqrt(n) - 1.0)
-        obst = [([float(v) for v in rng.uniform(0.15, 0.85, 2)], float(rng.uniform(0.03, 0.1)))
-                for _ in range(int(rng.integers(0, 6)))]
-        k = int(rng.integers(12, 25))
-        nodes = naga_amd.ObstacleGridProvider([0, 0], [1, 1], h, obst, 8 * h, 0.5, jitter=0.25,
-                                              seed=int(rng.integers(1, 1 << 62))).get()
-        eng = naga_amd.SimulationEngine(lattice=9, k=k, k_boundary=k,
-                                        rbf=naga_amd.RBF_POLYHARMONIC3,
-                                        rbf_boundary=naga_amd.RBF_POLYHARMONIC3)
-        p = naga_amd.problem_parameters(9, 0.6, 0.4 * h)
-        eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
-        eng.init_domain(nodes)
-        eng.set_initial_pulse([float(v) for v in rng.uniform(0.2, 0.8, 2)], 1e-3, 0.05)
+    for scene in workloads.batched_scenes(n_scenes):
+        eng, _ = workloads.batched_engine(scene)
         engines.append(eng)
         total += eng.num_nodes
     setup_s = time.perf_counter() - t0
@@ -153,39 +140,22 @@ def batched_workload(steps: int, n_scenes: int = 256, seed: int = 0x4E414741) ->
                         "k 12-24, PML 8h, tau 0.6, dt 0.4h"}
 
 
-def served_workload(n_requests: int = 32, rate: float = 50.0, seed: int = 0x4E414742) -> dict:
-    """Poisson arrivals at `rate` requests/s, D2Q9/D3Q27 50/50, node counts
-    log-uniform in [5k, 2M], steps uniform in [100, 2000], random pulse
-    position.  D3Q27 requests are the irregular domain: the unit cube minus
-    a central sphere of radius 0.15, 2x node density within 0.05 of it, k
-    from 24 to 48 by local density; D2Q9 requests the unit square with 0-5
-    circles of radius 0.03-0.1, k = 16.  Served with continuous batching,
-    then the same requests solved one at a time with their own launches
-    (setup included in both)."""
+def served_workload(n_requests: int = 32, rate: float = 50.0) -> dict:
+    """Poisson arrivals at `rate` requests/s (naga_amd.workloads.served_requests:
+    D2Q9/D3Q27 50/50, node counts log-uniform in [5k, 2M], steps uniform in
+    [100, 2000], random pulse position; D3Q27 requests the unit cube minus a
+    central sphere of radius 0.15, 2x node density within 0.05 of it, k from
+    24 to 48 by local density; D2Q9 requests the unit square with 0-5 circles
+    of radius 0.03-0.1, k = 16).  Served with continuous batching, then the
+    same requests solved one at a time with their own launches (setup
+    included in both)."""
     import naga_amd
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for _ in range(n_requests):
-        lat = 9 if rng.random() < 0.5 else 27
-        D = 2 if lat == 9 else 3
-        n = float(np.exp(rng.uniform(np.log(5e3), np.log(2e6))))
-        h = 1.0 / (n ** (1.0 / D) - 1.0)
-        pc = [float(v) for v in rng.uniform(0.2, 0.3, 1)] + \
-             [float(v) for v in rng.uniform(0.35, 0.65, D - 1)]
-        if D == 2:
-            obst = [([float(v) for v in rng.uniform(0.15, 0.85, 2)], float(rng.uniform(0.03, 0.1)))
-                    for _ in range(int(rng.integers(0, 6)))]
-            kw = dict(k=16)
-        else:
-            obst = [([0.5, 0.5, 0.5], 0.15)]
-            kw = dict(k=48, k_min=24, refine_width=0.05)
-        reqs.append(naga_amd.make_request(
-            lat, h, int(rng.integers(100, 2001)), obst, seed=int(rng.integers(1, 1 << 62)),
-            pulse_center=pc, pulse_width=0.05, **kw))
-    gaps = rng.exponential(1.0 / rate, n_requests)
+    from naga_amd import workloads
+    trace = workloads.served_requests(n_requests, rate)
+    reqs = trace.requests
     srv = naga_amd.SolveServer(hbm_budget_bytes=96 << 30, max_active=64, chunk_steps=50)
     t0 = time.perf_counter()
-    for r, g in zip(reqs, gaps):
+    for r, g in zip(reqs, trace.gaps):
         time.sleep(float(g))
         srv.submit(r)
     srv.drain()

@@ -57,6 +57,12 @@ typedef struct naga_scene_config {
     double filter_strength; /* selective filter strength per step: 0 gives the
                                reference's 0.75 (detail/simulation_engine.cuh:
                                1242), > 0 that strength, < 0 no filter */
+    int32_t solve_precision; /* bits of the (k + D + 1)^2 RBF solves of the
+                               divergence and boundary weights: 64 (or 0) FP64;
+                               32 FP32, the reference's precision
+                               (linalg/batched_matrix_inverse.cuh:229-250) --
+                               with k = 32 and multiquadric RBFs the
+                               reference-parity mode */
 } naga_scene_config;
 
 const char* naga_last_error(void);

@@ -43,6 +43,7 @@ class SceneConfig(C.Structure):
         ("pml_ref_index", C.c_int32), ("k_min", C.c_int32),
         ("nu", C.c_double), ("rho0", C.c_double), ("dt", C.c_double),
         ("c", C.c_double), ("h", C.c_double), ("filter_strength", C.c_double),
+        ("solve_precision", C.c_int32),
     ]
 
 
@@ -363,14 +364,17 @@ class SimulationEngine:
     def __init__(self, lattice: int = 27, k: int | None = None,
                  k_boundary: int | None = None, rbf: int = RBF_POLYHARMONIC3,
                  rbf_boundary: int | None = None, pml_ref_index: bool = True,
-                 k_min: int = 0, filter_strength: float = 0.0):
+                 k_min: int = 0, filter_strength: float = 0.0, solve_precision: int = 64):
         """filter_strength: selective filter strength per step; 0 gives the
         reference's 0.75 (detail/simulation_engine.cuh:1242), a negative
-        value no filter."""
+        value no filter.  solve_precision: 64 (FP64 RBF weight solves) or 32
+        (FP32, the reference's; with k = 32 and multiquadric RBFs the
+        reference-parity mode)."""
         if lattice not in (9, 27):
             raise ValueError(_ENGINE_PREFIX + "lattice must be D2Q9 or D3Q27")
         self.k_min = int(k_min)
         self.filter_strength = float(filter_strength)
+        self.solve_precision = int(solve_precision)
         self.lattice = lattice
         self.dimensions = 2 if lattice == 9 else 3
         self.k = 32 if k is None else int(k)
@@ -420,7 +424,8 @@ class SimulationEngine:
                           pml_ref_index=int(self.pml_ref_index), k_min=self.k_min,
                           nu=self._params["nu"], rho0=self._params["rho0"],
                           dt=self._params["dt"], c=self._params["c"],
-                          h=float(nodes.nodal_spacing), filter_strength=self.filter_strength)
+                          h=float(nodes.nodal_spacing), filter_strength=self.filter_strength,
+                          solve_precision=self.solve_precision)
         h = C.c_void_p()
         _check(lib.naga_engine_create(C.byref(cfg), C.byref(h)), _ENGINE_PREFIX)
         self._h = h

@@ -20,8 +20,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnaga_amd.so")
 ARCH = os.environ.get("NAGA_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["step_kernels.hip", "setup_kernels.hip", "calculus.hip", "engine.cpp", "server.cpp",
-           "capi.cpp"]
+SOURCES = ["step_kernels.hip", "setup_kernels.hip", "calculus.hip", "engine.cpp",
+           "server.cpp", "capi.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -73,16 +73,10 @@ Advection::Advection(const float* pts, int64_t n, int D, int k, int rbf) : n_(n)
     try {
         DeviceBuffer<float> dp(n * D);
         dp.upload(pts, n * D, st_);
-        CellGrid grid{};
-        std::vector<uint32_t> sorted;
-        std::vector<int64_t> start;
-        bin_points(D, pts, n, grid, sorted, start);
-        auto dsorted = to_device(sorted, st_);
-        auto dstart = to_device(start, st_);
+        const DeviceCells cells = bin_points_device(D, dp.get(), n, st_);
         DeviceBuffer<float> d2(n * k);
         idx_.allocate(n * k);
-        launch_knn(grid, dp.get(), dsorted.get(), dstart.get(), dp.get(), n, k, d2.get(),
-                   idx_.get(), st_);
+        launch_knn(cells.bins, dp.get(), dp.get(), n, k, d2.get(), idx_.get(), st_);
         // multiquadric shape parameter per 16384-query batch (divergence.cuh:118-160)
         const int64_t nbatch = (n + 16383) / 16384;
         std::vector<double> hbar(nbatch, 1.0);

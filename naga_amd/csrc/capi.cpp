@@ -151,6 +151,10 @@ int naga_engine_create(const naga_scene_config* cfg, naga_engine** out) {
         c.c = cfg->c;
         c.h = cfg->h;
         c.filter = cfg->filter_strength;
+        require(cfg->solve_precision == 0 || cfg->solve_precision == 32 ||
+                    cfg->solve_precision == 64,
+                "solve_precision must be 0, 32 or 64");
+        c.solve_bits = cfg->solve_precision == 32 ? 32 : 64;
         auto* e = new naga_engine{};
         try {
             e->impl = std::make_unique<Engine>(c);

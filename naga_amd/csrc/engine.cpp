@@ -20,6 +20,22 @@ double lat_w(int a) {
 
 double weight(int Q, int a) { return Q == 27 ? lat_w<27>(a) : lat_w<9>(a); }
 
+// Runs the launches of a stream capture; if one of them throws, the
+// capture is ended and its partial graph dropped before the error
+// propagates, so the stream is usable again.
+template <class F>
+void capture_guarded(hipStream_t st, F&& body) {
+    try {
+        body();
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        throw;
+    }
+}
+
 struct Timer {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double ms() const {
@@ -35,18 +51,12 @@ void knn_host(int D, const float* ref, int64_t n_ref, const float* query, int64_
     require(D == 2 || D == 3, "dimension must be 2 or 3");
     require(k >= 1 && k <= 64, "k must be in [1, 64]");
     require(n_ref >= k, "fewer reference points than neighbours requested");
-    CellGrid grid{};
-    std::vector<uint32_t> sorted;
-    std::vector<int64_t> start;
-    bin_points(D, ref, n_ref, grid, sorted, start);
     DeviceBuffer<float> dref(n_ref * D), dq(n_q * D), dd2(n_q * k);
     DeviceBuffer<uint32_t> didx(n_q * k);
     dref.upload(ref, n_ref * D, st);
     dq.upload(query, n_q * D, st);
-    auto dsorted = to_device(sorted, st);
-    auto dstart = to_device(start, st);
-    launch_knn(grid, dref.get(), dsorted.get(), dstart.get(), dq.get(), n_q, k, dd2.get(),
-               didx.get(), st);
+    const DeviceCells cells = bin_points_device(D, dref.get(), n_ref, st);
+    launch_knn(cells.bins, dref.get(), dq.get(), n_q, k, dd2.get(), didx.get(), st);
     dd2.download(d2, n_q * k, st);
     didx.download(idx, n_q * k, st);
     NAGA_HIP(hipStreamSynchronize(st));
@@ -129,16 +139,10 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
 
     // divergence operator over all nodes (init_domain :169-181)
     {
-        CellGrid grid{};
-        std::vector<uint32_t> sorted;
-        std::vector<int64_t> start;
-        bin_points(D, pts, N, grid, sorted, start);
-        auto dsorted = to_device(sorted, st);
-        auto dstart = to_device(start, st);
+        const DeviceCells cells = bin_points_device(D, pts_.get(), N, st);
         DeviceBuffer<float> d2(N * k);
         idx_.allocate(N * k);
-        launch_knn(grid, pts_.get(), dsorted.get(), dstart.get(), pts_.get(), N, k, d2.get(),
-                   idx_.get(), st);
+        launch_knn(cells.bins, pts_.get(), pts_.get(), N, k, d2.get(), idx_.get(), st);
         // multiquadric shape parameter: mean nearest distance per 16384-query
         // batch (nonlocal_calculus/divergence.cuh:118-160, operators.cuh:96-117)
         const int64_t nbatch = (N + 16383) / 16384;
@@ -185,7 +189,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         DeviceBuffer<int> info(N);
         launch_divergence_weights(D, pts_.get(), pts_.get(), N, idx_.get(), d2.get(), k,
                                   cfg_.rbf, dhbar.get(), W_.get(), nullptr, info.get(), st,
-                                  kvar_.get());
+                                  kvar_.get(), cfg_.solve_bits);
         std::vector<int> hinfo(N);
         info.download(hinfo.data(), N, st);
         NAGA_HIP(hipStreamSynchronize(st));
@@ -302,20 +306,16 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     // boundary interpolator (init_domain :223-258): kNN of boundary nodes
     // among the bulk and layer nodes, RPIM weights at spacing h
     if (nbnd > 0) {
-        CellGrid grid{};
-        std::vector<uint32_t> sorted;
-        std::vector<int64_t> start;
-        bin_points(D, pts, nfluid, grid, sorted, start);
-        auto dsorted = to_device(sorted, st);
-        auto dstart = to_device(start, st);
+        const DeviceCells cells = bin_points_device(D, pts_.get(), nfluid, st);
         DeviceBuffer<float> bd2(nbnd * kb);
         bidx_.allocate(nbnd * kb);
-        launch_knn(grid, pts_.get(), dsorted.get(), dstart.get(), pts_.get() + nfluid * D, nbnd,
-                   kb, bd2.get(), bidx_.get(), st);
+        launch_knn(cells.bins, pts_.get(), pts_.get() + nfluid * D, nbnd, kb, bd2.get(),
+                   bidx_.get(), st);
         bw_.allocate(nbnd * kb);
         DeviceBuffer<int> info(nbnd);
         launch_rpim_weights(D, pts_.get(), pts_.get() + nfluid * D, nbnd, bidx_.get(), kb,
-                            cfg_.rbf_boundary, cfg_.h, bw_.get(), nullptr, info.get(), st);
+                            cfg_.rbf_boundary, cfg_.h, bw_.get(), nullptr, info.get(), st,
+                            cfg_.solve_bits);
         std::vector<int> hinfo(nbnd);
         info.download(hinfo.data(), nbnd, st);
         NAGA_HIP(hipStreamSynchronize(st));
@@ -493,7 +493,6 @@ StepBuffers Engine::buffers() {
 // step's source value of every listed node into src_field_, read by p1.
 void Engine::add_density_source(const DensitySource& src) {
     require_domain();
-    require(batch == nullptr, "remove the engine from its batch before adding sources");
     require(int(sources_.size()) < kMaxSources, "at most 16 density sources per engine");
     require(src.radius > 0.0, "density source radius must be positive");
     sources_.push_back(src);
@@ -530,12 +529,17 @@ void Engine::add_density_source(const DensitySource& src) {
     if (src_field_.size() != size_t(N_)) src_field_.allocate(N_);
     src_field_.zero(stream_);
     NAGA_HIP(hipStreamSynchronize(stream_));
+    // the batch's scene table carries the source field pointer
+    if (batch) batch->invalidate();
 }
 
 void Engine::clear_density_sources() {
     require_domain();
     sources_.clear();
     n_snodes_ = 0;
+    // the batch's scene table still points at the source field: rebuild it
+    // with none (else the last step's source values would be added again)
+    if (batch) batch->invalidate();
 }
 
 void Engine::refresh_macroscopic_state() {
@@ -757,11 +761,13 @@ void Engine::launch_steps(int nsteps) {
         // geometry, rates); the clock is advanced on the host per replay
         bool ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed) == hipSuccess;
         if (ok) {
-            const StepBuffers S = buffers();
-            if (cfg_.Q == 27)
-                launch_step<27>(geo_, S, cp_, stream_, nullptr);
-            else
-                launch_step<9>(geo_, S, cp_, stream_, nullptr);
+            capture_guarded(stream_, [&] {
+                const StepBuffers S = buffers();
+                if (cfg_.Q == 27)
+                    launch_step<27>(geo_, S, cp_, stream_, nullptr);
+                else
+                    launch_step<9>(geo_, S, cp_, stream_, nullptr);
+            });
             ok = hipStreamEndCapture(stream_, &step_graph_) == hipSuccess && step_graph_;
         }
         ok = ok && hipGraphInstantiate(&step_exec_, step_graph_, nullptr, nullptr, 0) ==
@@ -981,7 +987,7 @@ void Batch::step(int nsteps) {
         // that fails anyway falls back to direct launches for this call.
         bool ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed) == hipSuccess;
         if (ok) {
-            launch_one_step();
+            capture_guarded(stream_, [&] { launch_one_step(); });
             ok = hipStreamEndCapture(stream_, &graph_) == hipSuccess && graph_ != nullptr;
         }
         ok = ok && hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0) == hipSuccess;

@@ -21,6 +21,7 @@ struct SceneConfig {
     int k_min = 0;  // > 0: per-node stencil widths in [k_min, k] by local density
     double nu = 0.0, rho0 = 1.0, dt = 0.0, c = kCsLat, h = 0.01;
     double filter = 0.0;  // 0: the reference's 0.75; > 0: that strength; < 0: off
+    int solve_bits = 64;  // precision of the RBF weight solves: 64 or 32
 };
 
 // A declarative density source: amplitude * cos(2 pi frequency t + phase)
@@ -196,6 +197,10 @@ class Batch {
     int autotune(int trial_steps);
     int size() const { return int(scenes_.size()); }
     hipStream_t stream() const { return stream_; }
+    // A member's scene description changed (density sources added or
+    // cleared): the scene tables and the graph are rebuilt before the next
+    // step.
+    void invalidate() { dirty_ = true; }
 
   private:
     struct Group {

@@ -31,7 +31,13 @@ __device__ __forceinline__ double shfl_xor_d(double v, int m) {
         (long long)shfl_xor64(uint64_t(__double_as_longlong(v)), m));
 }
 
+__device__ __forceinline__ float shfl_xor_d(float v, int m) { return __shfl_xor(v, m); }
+
 // value of lane `src` (wave-uniform) for every lane
+__device__ __forceinline__ float bcast_d(float v, int src) {
+    const int s = __builtin_amdgcn_readfirstlane(src);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), s));
+}
 __device__ __forceinline__ double bcast_d(double v, int src) {
     const int s = __builtin_amdgcn_readfirstlane(src);
     const long long b = __double_as_longlong(v);
@@ -88,14 +94,15 @@ __device__ __forceinline__ void knn_merge(uint64_t key, uint64_t& top, uint64_t&
     kth = shfl64(top, k - 1);
 }
 
-__global__ __launch_bounds__(256) void k_knn(CellGrid G, const float* ref,
+__global__ __launch_bounds__(256) void k_knn(const CellGrid* gp, const float* ref,
                                              const uint32_t* sorted,
-                                             const int64_t* start,
+                                             const uint32_t* start,
                                              const float* query, int64_t n_q,
                                              int k, float* d2o, uint32_t* idxo) {
     const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     if (q >= n_q) return;
+    const CellGrid G = *gp;
     const int D = G.D;
     float x[3] = {0.f, 0.f, 0.f};
     for (int d = 0; d < D; ++d) x[d] = query[q * D + d];
@@ -135,7 +142,7 @@ __global__ __launch_bounds__(256) void k_knn(CellGrid G, const float* ref,
                 }
                 for (int t = 0; t < nseg; ++t) {
                     const int64_t p0 = start[row + seg[t][0]];
-                    const int64_t p1 = start[row + seg[t][1] + 1];
+                    const int64_t p1 = start[row + seg[t][1] + 1];  // cells are row-major in x
                     for (int64_t base = p0; base < p1; base += 64) {
                         const int64_t p = base + lane;
                         uint64_t key = kMaxKey;
@@ -183,8 +190,8 @@ __global__ __launch_bounds__(256) void k_knn(CellGrid G, const float* ref,
 // On return the lane at position p holds unknown p in rhs[].
 // Returns the lane's position, or -1 for a zero pivot (wave-uniform).
 // ---------------------------------------------------------------------------
-template <int NMAX, int NR>
-__device__ int wave_lu_solve(double (&row)[NMAX], double (&rhs)[NR], int n) {
+template <int NMAX, int NR, class T>
+__device__ int wave_lu_solve(T (&row)[NMAX], T (&rhs)[NR], int n) {
     const int lane = threadIdx.x & 63;
     int pos = lane;
     bool done = lane >= n;
@@ -192,11 +199,11 @@ __device__ int wave_lu_solve(double (&row)[NMAX], double (&rhs)[NR], int n) {
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) {
         if (j >= n) break;
-        double v = done ? -1.0 : fabs(row[j]);
+        T v = done ? T(-1) : fabs(row[j]);
         int bp = done ? 1 << 20 : pos, bl = lane;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
-            const double ov = shfl_xor_d(v, o);
+            const T ov = shfl_xor_d(v, o);
             const int op = __shfl_xor(bp, o), ol = __shfl_xor(bl, o);
             if (ov > v || (ov == v && op < bp)) {
                 v = ov;
@@ -205,19 +212,19 @@ __device__ int wave_lu_solve(double (&row)[NMAX], double (&rhs)[NR], int n) {
             }
         }
         const int p = __builtin_amdgcn_readfirstlane(bl);
-        if (!(v > 0.0)) singular = true;
+        if (!(v > T(0))) singular = true;
         const int pos_p = __builtin_amdgcn_readlane(pos, p);
         if (pos == j) pos = pos_p;
         if (lane == p) pos = j;
-        const double piv = bcast_d(row[j], p);
-        double prhs[NR];
+        const T piv = bcast_d(row[j], p);
+        T prhs[NR];
 #pragma unroll
         for (int m = 0; m < NR; ++m) prhs[m] = bcast_d(rhs[m], p);
         const bool upd = !done && lane != p;
-        const double l = upd ? row[j] / piv : 0.0;
+        const T l = upd ? row[j] / piv : T(0);
 #pragma unroll
         for (int c = j + 1; c < NMAX; ++c) {
-            const double pc = bcast_d(row[c], p);
+            const T pc = bcast_d(row[c], p);
             if (upd) row[c] -= l * pc;
         }
         if (upd) {
@@ -229,7 +236,7 @@ __device__ int wave_lu_solve(double (&row)[NMAX], double (&rhs)[NR], int n) {
     }
     if (singular) return -1;
     // back substitution, unknown j lives in the lane at position j
-    double s[NR];
+    T s[NR];
 #pragma unroll
     for (int m = 0; m < NR; ++m) s[m] = rhs[m];
 #pragma unroll
@@ -237,10 +244,10 @@ __device__ int wave_lu_solve(double (&row)[NMAX], double (&rhs)[NR], int n) {
         if (j >= n) continue;
         const unsigned long long mask = __ballot(pos == j && lane < n);
         const int lj = __builtin_amdgcn_readfirstlane(int(__ffsll((long long)mask) - 1));
-        const double ujj = bcast_d(row[j], lj);
+        const T ujj = bcast_d(row[j], lj);
 #pragma unroll
         for (int m = 0; m < NR; ++m) {
-            const double xj = bcast_d(s[m], lj) / ujj;
+            const T xj = bcast_d(s[m], lj) / ujj;
             if (pos == j) rhs[m] = xj;
             if (pos < j && lane < n) s[m] -= row[j] * xj;
         }
@@ -336,7 +343,7 @@ __device__ void quad_term(int D, double delta, const double* xi, const double* x
 // write W(d, s) = y_d[s] (+ sum_q alpha_d w for s = 0).  By linearity this
 // equals the reference's per-quadrature-point RPIM weights summed against
 // the kernel (detail/divergence.inl:203-215) with D solves instead of Nq.
-template <int NMAX, int D>
+template <int NMAX, int D, class T>
 __global__ __launch_bounds__(64) void k_divw(const float* src, const float* query,
                                              int64_t n_q, const uint32_t* idx,
                                              const float* d2, int kmax, int rbf,
@@ -363,10 +370,13 @@ __global__ __launch_bounds__(64) void k_divw(const float* src, const float* quer
     for (int d = 0; d < D; ++d) xi[d] = double(query[i * D + d]);
     const double delta = 0.05 * sqrt(double(d2[i * kmax + 1]));
     const double spacing = hbar[i / 16384];
-    double row[NMAX];
-    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row);
-    double b[D], S[D];
-    for (int d = 0; d < D; ++d) b[d] = S[d] = 0.0;
+    double row64[NMAX];
+    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row64);
+    T row[NMAX];
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) row[c] = T(row64[c]);
+    double b64[D], S[D];
+    for (int d = 0; d < D; ++d) b64[d] = S[d] = 0.0;
     const int Nq = D == 2 ? 16 : 64;
     for (int q = 0; q < Nq; ++q) {
         double xq[3] = {0.0, 0.0, 0.0};
@@ -375,11 +385,13 @@ __global__ __launch_bounds__(64) void k_divw(const float* src, const float* quer
         quad_term(D, delta, xi, xq, q, aw);
         const double gx = lane < n ? gx_entry(D, xs, k, rbf, spacing, lane, xq) : 0.0;
         for (int d = 0; d < D; ++d) {
-            b[d] += aw[d] * gx;
+            b64[d] += aw[d] * gx;
             S[d] += aw[d];
         }
     }
-    const int pos = wave_lu_solve<NMAX, D>(row, b, n);
+    T b[D];
+    for (int d = 0; d < D; ++d) b[d] = T(b64[d]);
+    const int pos = wave_lu_solve<NMAX, D, T>(row, b, n);
     if (pos < 0) {
         if (lane == 0) info[i] = 1;
         return;
@@ -387,14 +399,14 @@ __global__ __launch_bounds__(64) void k_divw(const float* src, const float* quer
     if (lane == 0) info[i] = 0;
     if (pos < k) {
         for (int d = 0; d < D; ++d) {
-            const double v = b[d] + (pos == 0 ? S[d] : 0.0);
+            const double v = double(b[d]) + (pos == 0 ? S[d] : 0.0);
             W[(i * kmax + pos) * D + d] = float(v);
             if (W64) W64[(i * kmax + pos) * D + d] = v;
         }
     }
 }
 
-template <int NMAX, int D>
+template <int NMAX, int D, class T>
 __global__ __launch_bounds__(64) void k_rpim(const float* src, const float* query,
                                              int64_t n_q, const uint32_t* idx,
                                              int k, int rbf, double spacing,
@@ -411,10 +423,13 @@ __global__ __launch_bounds__(64) void k_rpim(const float* src, const float* quer
     __syncthreads();
     double xq[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < D; ++d) xq[d] = double(query[i * D + d]);
-    double row[NMAX];
-    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row);
-    double b[1] = {lane < n ? gx_entry(D, xs, k, rbf, spacing, lane, xq) : 0.0};
-    const int pos = wave_lu_solve<NMAX, 1>(row, b, n);
+    double row64[NMAX];
+    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row64);
+    T row[NMAX];
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) row[c] = T(row64[c]);
+    T b[1] = {T(lane < n ? gx_entry(D, xs, k, rbf, spacing, lane, xq) : 0.0)};
+    const int pos = wave_lu_solve<NMAX, 1, T>(row, b, n);
     if (pos < 0) {
         if (lane == 0) info[i] = 1;
         return;
@@ -422,7 +437,7 @@ __global__ __launch_bounds__(64) void k_rpim(const float* src, const float* quer
     if (lane == 0) info[i] = 0;
     if (pos < k) {
         w[i * k + pos] = float(b[0]);
-        if (w64) w64[i * k + pos] = b[0];
+        if (w64) w64[i * k + pos] = double(b[0]);
     }
 }
 
@@ -445,7 +460,7 @@ __global__ __launch_bounds__(64) void k_batched_lu(int n, int nrhs, int64_t batc
         double rhs[4];
         for (int m = 0; m < 4; ++m)
             rhs[m] = (lane < n && m0 + m < nrhs) ? Bb[lane + int64_t(m0 + m) * n] : 0.0;
-        const int pos = wave_lu_solve<NMAX, 4>(r2, rhs, n);
+        const int pos = wave_lu_solve<NMAX, 4, double>(r2, rhs, n);
         if (pos < 0) {
             if (lane == 0) info[b] = 1;
             return;
@@ -716,6 +731,190 @@ __global__ void k_div_apply(int D, int64_t n, int k, const uint32_t* idx,
     out[i] = acc;
 }
 
+// ---------------------------------------------------------------------------
+// Cell binning on the device (segmentation/detail/nd_cubic_segmentation.inl:
+// 38-273: bounds, partition sizes, offsets by a scan, indices by a scatter).
+// The order of indices within a cell follows the scatter's atomics, as in
+// the reference; the kNN over the cells is exact whatever that order.
+// ---------------------------------------------------------------------------
+// Per-block bounds of the points (grid-stride), partial[b][0..3) minima and
+// [3..6) maxima.
+__global__ __launch_bounds__(256) void k_bounds(const float* pts, int64_t n, int D,
+                                                double* partial) {
+    __shared__ float lo[3][256], hi[3][256];
+    float l[3] = {3.0e38f, 3.0e38f, 3.0e38f}, h[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x; j < n;
+         j += int64_t(gridDim.x) * 256)
+        for (int d = 0; d < D; ++d) {
+            const float v = pts[j * D + d];
+            l[d] = fminf(l[d], v);
+            h[d] = fmaxf(h[d], v);
+        }
+    for (int d = 0; d < 3; ++d) {
+        lo[d][threadIdx.x] = l[d];
+        hi[d][threadIdx.x] = h[d];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int d = 0; d < 3; ++d) {
+                lo[d][threadIdx.x] = fminf(lo[d][threadIdx.x], lo[d][threadIdx.x + w]);
+                hi[d][threadIdx.x] = fmaxf(hi[d][threadIdx.x], hi[d][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        partial[blockIdx.x * 6 + threadIdx.x] = double(lo[threadIdx.x][0]);
+        partial[blockIdx.x * 6 + 3 + threadIdx.x] = double(hi[threadIdx.x][0]);
+    }
+}
+
+// The grid from the per-block bounds (nd_cubic_segmentation.cuh:241-279).
+__global__ __launch_bounds__(256) void k_grid(const double* partial, int D, int64_t n,
+                                              double per_cell, double width, int64_t cap,
+                                              CellGrid* g) {
+    __shared__ double lo[3][kBoundBlocks], hi[3][kBoundBlocks];
+    const int t = threadIdx.x;
+    for (int d = 0; d < 3; ++d) {
+        lo[d][t] = partial[t * 6 + d];
+        hi[d][t] = partial[t * 6 + 3 + d];
+    }
+    __syncthreads();
+    for (int w = kBoundBlocks / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int d = 0; d < 3; ++d) {
+                lo[d][t] = fmin(lo[d][t], lo[d][t + w]);
+                hi[d][t] = fmax(hi[d][t], hi[d][t + w]);
+            }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    CellGrid G{};
+    G.D = D;
+    double ext[3] = {1.0, 1.0, 1.0}, vol = 1.0;
+    for (int d = 0; d < 3; ++d) {
+        G.lo[d] = d < D ? lo[d][0] : 0.0;
+        if (d < D) {
+            ext[d] = hi[d][0] - lo[d][0];
+            if (ext[d] < 1e-3) ext[d] = 1e-3;  // the reference's tolerance
+            vol *= ext[d];
+        }
+    }
+    double w = width > 0.0 ? width : pow(per_cell * vol / double(n), 1.0 / double(D));
+    int64_t nc = 1;
+    for (int it = 0; it < 4096; ++it) {
+        nc = 1;
+        for (int d = 0; d < 3; ++d) {
+            double m = d < D ? ceil(ext[d] / w) : 1.0;
+            m = m < 1.0 ? 1.0 : (m > 1048576.0 ? 1048576.0 : m);
+            G.dims[d] = int64_t(m);
+            nc *= G.dims[d];
+        }
+        if (nc <= cap) break;
+        w *= 1.25;
+    }
+    if (nc > cap) {  // unreachable for finite input: one cell then
+        G.dims[0] = G.dims[1] = G.dims[2] = 1;
+        nc = 1;
+        w = 1e300;
+    }
+    G.w = w;
+    G.ncell = nc;
+    *g = G;
+}
+
+// Cell of a point: floor((x - lo) / w) per axis, clamped to the grid;
+// linear index with x fastest (the kNN's cell rows).
+__device__ __forceinline__ int64_t cell_of(const CellGrid& G, const float* p) {
+    int64_t c = 0, mul = 1;
+    for (int d = 0; d < G.D; ++d) {
+        int64_t ci = int64_t(floor((double(p[d]) - G.lo[d]) / G.w));
+        ci = ci < 0 ? 0 : ci >= G.dims[d] ? G.dims[d] - 1 : ci;
+        c += ci * mul;
+        mul *= G.dims[d];
+    }
+    return c;
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_count_cells(const float* pts, int64_t n,
+                                                     const CellGrid* gp, uint32_t* counts) {
+    const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (j >= n) return;
+    const CellGrid G = *gp;
+    atomicAdd(&counts[cell_of(G, pts + j * G.D)], 1u);
+}
+
+// Exclusive scan of in[0..M) into out, block b covering [b * 2048, ..); its
+// total goes to bsum[b].
+__device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t* tmp, uint32_t& total) {
+    const int t = threadIdx.x;
+    tmp[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t a = t >= off ? tmp[t - off] : 0u;
+        __syncthreads();
+        tmp[t] += a;
+        __syncthreads();
+    }
+    total = tmp[255];
+    const uint32_t incl = tmp[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_local(const uint32_t* in, int64_t M, uint32_t* out,
+                                                    uint32_t* bsum) {
+    __shared__ uint32_t tmp[256];
+    const int64_t base = int64_t(blockIdx.x) * kScanChunk + threadIdx.x * 8;
+    uint32_t v[8], s = 0;
+    for (int e = 0; e < 8; ++e) {
+        v[e] = base + e < M ? in[base + e] : 0u;
+        s += v[e];
+    }
+    uint32_t total;
+    uint32_t run = block_scan_256(s, tmp, total);
+    for (int e = 0; e < 8; ++e) {
+        if (base + e < M) out[base + e] = run;
+        run += v[e];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(uint32_t* bsum, int64_t nb) {
+    __shared__ uint32_t tmp[256];
+    uint32_t carry = 0;
+    for (int64_t c0 = 0; c0 < nb; c0 += 256) {
+        const int64_t i = c0 + threadIdx.x;
+        const uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_scan_256(v, tmp, total);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t* out, int64_t M, const uint32_t* bsum) {
+    const int64_t base = int64_t(blockIdx.x) * kScanChunk;
+    const uint32_t add = bsum[blockIdx.x];
+    for (int e = threadIdx.x; e < kScanChunk; e += 256)
+        if (base + e < M) out[base + e] += add;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_cells(const float* pts, int64_t n,
+                                                       const CellGrid* gp, const uint32_t* start,
+                                                       uint32_t* fill, uint32_t* sorted) {
+    const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (j >= n) return;
+    const CellGrid G = *gp;
+    const int64_t c = cell_of(G, pts + j * G.D);
+    sorted[start[c] + atomicAdd(&fill[c], 1u)] = uint32_t(j);
+}
+
 int pick_nmax(int n) {
     const int opts[] = {20, 28, 36, 44, 52, 64};
     for (int o : opts)
@@ -763,15 +962,15 @@ void upload_quadrature(int /*D*/) {
     done.push_back(dev);
 }
 
-template <int D>
+template <int D, class T>
 void launch_divw_d(int nmax, const float* src, const float* query, int64_t n_q,
                    const uint32_t* idx, const float* d2, int k, int rbf,
                    const double* hbar, const uint8_t* kvar, float* W, double* W64, int* info,
                    hipStream_t st) {
-#define NAGA_DIVW(NM)                                                              \
-    case NM:                                                                       \
-        k_divw<NM, D><<<n_q, 64, 0, st>>>(src, query, n_q, idx, d2, k, rbf, hbar, \
-                                           kvar, W, W64, info);                    \
+#define NAGA_DIVW(NM)                                                                 \
+    case NM:                                                                          \
+        k_divw<NM, D, T><<<n_q, 64, 0, st>>>(src, query, n_q, idx, d2, k, rbf, hbar, \
+                                              kvar, W, W64, info);                    \
         break;
     switch (nmax) {
         NAGA_DIVW(20) NAGA_DIVW(28) NAGA_DIVW(36) NAGA_DIVW(44) NAGA_DIVW(52) NAGA_DIVW(64)
@@ -780,14 +979,14 @@ void launch_divw_d(int nmax, const float* src, const float* query, int64_t n_q,
 #undef NAGA_DIVW
 }
 
-template <int D>
+template <int D, class T>
 void launch_rpim_d(int nmax, const float* src, const float* query, int64_t n_q,
                    const uint32_t* idx, int k, int rbf, double spacing, float* w,
                    double* w64, int* info, hipStream_t st) {
-#define NAGA_RPIM(NM)                                                               \
-    case NM:                                                                        \
-        k_rpim<NM, D><<<n_q, 64, 0, st>>>(src, query, n_q, idx, k, rbf, spacing, w, \
-                                           w64, info);                              \
+#define NAGA_RPIM(NM)                                                                  \
+    case NM:                                                                           \
+        k_rpim<NM, D, T><<<n_q, 64, 0, st>>>(src, query, n_q, idx, k, rbf, spacing, w, \
+                                              w64, info);                              \
         break;
     switch (nmax) {
         NAGA_RPIM(20) NAGA_RPIM(28) NAGA_RPIM(36) NAGA_RPIM(44) NAGA_RPIM(52) NAGA_RPIM(64)
@@ -798,60 +997,60 @@ void launch_rpim_d(int nmax, const float* src, const float* query, int64_t n_q,
 
 }  // namespace
 
-void bin_points(int D, const float* pts, int64_t n, CellGrid& g,
-                std::vector<uint32_t>& sorted, std::vector<int64_t>& start) {
+size_t cell_bins_bytes(int64_t n) { return bin_layout(n).total; }
+
+CellBins launch_bin_points(int D, const float* pts, int64_t n, double per_cell, double width,
+                           void* scratch_mem, size_t bytes, hipStream_t st) {
+    require(D >= 1 && D <= 3, "dimension must be 1, 2 or 3");
     require(n > 0, "no reference points");
-    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, vol = 1.0;
-    for (int d = 0; d < D; ++d) {
-        lo[d] = 1e300;
-        hi[d] = -1e300;
-    }
-    for (int64_t j = 0; j < n; ++j)
-        for (int d = 0; d < D; ++d) {
-            const double x = pts[j * D + d];
-            lo[d] = x < lo[d] ? x : lo[d];
-            hi[d] = x > hi[d] ? x : hi[d];
-        }
-    for (int d = 0; d < D; ++d) {
-        if (hi[d] - lo[d] < 1e-6) hi[d] = lo[d] + 1e-6;
-        vol *= hi[d] - lo[d];
-    }
-    g.D = D;
-    g.w = std::pow(2.0 * vol / double(n), 1.0 / D);
-    int64_t ncell = 1;
-    for (int d = 0; d < 3; ++d) {
-        g.lo[d] = d < D ? lo[d] : 0.0;
-        int64_t m = d < D ? int64_t(std::ceil((hi[d] - lo[d]) / g.w)) : 1;
-        m = m < 1 ? 1 : m > 2048 ? 2048 : m;
-        g.dims[d] = m;
-        ncell *= m;
-    }
-    std::vector<int64_t> cell(n);
-    start.assign(ncell + 1, 0);
-    for (int64_t j = 0; j < n; ++j) {
-        int64_t c = 0, mul = 1;
-        for (int d = 0; d < D; ++d) {
-            int64_t ci = int64_t(std::floor((double(pts[j * D + d]) - g.lo[d]) / g.w));
-            ci = ci < 0 ? 0 : ci >= g.dims[d] ? g.dims[d] - 1 : ci;
-            c += ci * mul;
-            mul *= g.dims[d];
-        }
-        cell[j] = c;
-        start[c + 1]++;
-    }
-    for (int64_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
-    sorted.resize(n);
-    std::vector<int64_t> fill(start.begin(), start.end() - 1);
-    for (int64_t j = 0; j < n; ++j) sorted[fill[cell[j]]++] = uint32_t(j);
+    require(n < (int64_t(1) << 31), "at most 2^31 - 1 points");
+    require(per_cell > 0.0 || width > 0.0, "cell size must be positive");
+    const BinLayout L = bin_layout(n);
+    require(scratch_mem != nullptr && bytes >= L.total, "cell binning scratch too small");
+    char* base = static_cast<char*>(scratch_mem);
+    CellBins b{};
+    b.grid = reinterpret_cast<CellGrid*>(base + L.grid);
+    b.start = reinterpret_cast<uint32_t*>(base + L.start);
+    b.counts = reinterpret_cast<uint32_t*>(base + L.counts);
+    b.sorted = reinterpret_cast<uint32_t*>(base + L.sorted);
+    b.cap = L.cap;
+    b.n = n;
+    double* partial = reinterpret_cast<double*>(base + L.partial);
+    uint32_t* bsum = reinterpret_cast<uint32_t*>(base + L.bsum);
+    const int64_t M = L.cap + 1;
+    const int64_t nbM = (M + 255) / 256, nbn = (n + 255) / 256;
+    k_bounds<<<kBoundBlocks, 256, 0, st>>>(pts, n, D, partial);
+    k_grid<<<1, kBoundBlocks, 0, st>>>(partial, D, n, per_cell > 0.0 ? per_cell : 0.0,
+                                       per_cell > 0.0 ? 0.0 : width, L.cap, b.grid);
+    k_fill_u32<<<nbM, 256, 0, st>>>(b.counts, M, 0u);
+    k_count_cells<<<nbn, 256, 0, st>>>(pts, n, b.grid, b.counts);
+    k_scan_local<<<L.nblocks, 256, 0, st>>>(b.counts, M, b.start, bsum);
+    k_scan_sums<<<1, 256, 0, st>>>(bsum, L.nblocks);
+    k_scan_add<<<L.nblocks, 256, 0, st>>>(b.start, M, bsum);
+    // the scatter counts each cell up from zero again, leaving the sizes
+    k_fill_u32<<<nbM, 256, 0, st>>>(b.counts, M, 0u);
+    k_scatter_cells<<<nbn, 256, 0, st>>>(pts, n, b.grid, b.start, b.counts, b.sorted);
+    hip_check(hipGetLastError(), "cell binning");
+    return b;
 }
 
-void launch_knn(const CellGrid& grid, const float* ref, const uint32_t* sorted,
-                const int64_t* start, const float* query, int64_t n_q, int k,
+DeviceCells bin_points_device(int D, const float* pts, int64_t n, hipStream_t st,
+                              double per_cell, double width) {
+    DeviceCells c;
+    const size_t bytes = cell_bins_bytes(n);
+    c.mem.allocate(bytes);
+    c.bins = launch_bin_points(D, pts, n, per_cell, width, c.mem.get(), bytes, st);
+    return c;
+}
+
+void launch_knn(const CellBins& bins, const float* ref, const float* query, int64_t n_q, int k,
                 float* d2_out, uint32_t* idx_out, hipStream_t st) {
     require(k >= 1 && k <= 64, "k must be in [1, 64]");
+    require(bins.n >= k, "fewer reference points than neighbours requested");
     if (n_q == 0) return;
     const int64_t blocks = (n_q * 64 + 255) / 256;
-    k_knn<<<blocks, 256, 0, st>>>(grid, ref, sorted, start, query, n_q, k, d2_out, idx_out);
+    k_knn<<<blocks, 256, 0, st>>>(bins.grid, ref, bins.sorted, bins.start, query, n_q, k, d2_out,
+                                  idx_out);
     hip_check(hipGetLastError(), "knn");
 }
 
@@ -871,29 +1070,46 @@ void launch_batched_lu_solve(int n, int nrhs, int64_t batch, double* A, double* 
 void launch_divergence_weights(int D, const float* src, const float* query,
                                int64_t n_q, const uint32_t* idx, const float* d2,
                                int k, int rbf, const double* hbar, float* W,
-                               double* W64, int* info, hipStream_t st, const uint8_t* kvar) {
+                               double* W64, int* info, hipStream_t st, const uint8_t* kvar,
+                               int solve_bits) {
     const int nmax = pick_nmax(k + D + 1);
     require(nmax > 0, "stencil too wide (k + D + 1 must be <= 64)");
     require(k >= 2, "k must be >= 2");
+    require(solve_bits == 64 || solve_bits == 32, "solve precision must be 64 or 32 bits");
     if (n_q == 0) return;
     upload_quadrature(D);
-    if (D == 2)
-        launch_divw_d<2>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, kvar, W, W64, info, st);
+    const bool f32 = solve_bits == 32;
+    if (D == 2 && f32)
+        launch_divw_d<2, float>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, kvar, W, W64, info,
+                                st);
+    else if (D == 2)
+        launch_divw_d<2, double>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, kvar, W, W64,
+                                 info, st);
+    else if (f32)
+        launch_divw_d<3, float>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, kvar, W, W64, info,
+                                st);
     else
-        launch_divw_d<3>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, kvar, W, W64, info, st);
+        launch_divw_d<3, double>(nmax, src, query, n_q, idx, d2, k, rbf, hbar, kvar, W, W64,
+                                 info, st);
     hip_check(hipGetLastError(), "divergence weights");
 }
 
 void launch_rpim_weights(int D, const float* src, const float* query, int64_t n_q,
                          const uint32_t* idx, int k, int rbf, double spacing,
-                         float* w, double* w64, int* info, hipStream_t st) {
+                         float* w, double* w64, int* info, hipStream_t st, int solve_bits) {
     const int nmax = pick_nmax(k + D + 1);
     require(nmax > 0, "stencil too wide (k + D + 1 must be <= 64)");
+    require(solve_bits == 64 || solve_bits == 32, "solve precision must be 64 or 32 bits");
     if (n_q == 0) return;
-    if (D == 2)
-        launch_rpim_d<2>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+    const bool f32 = solve_bits == 32;
+    if (D == 2 && f32)
+        launch_rpim_d<2, float>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+    else if (D == 2)
+        launch_rpim_d<2, double>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+    else if (f32)
+        launch_rpim_d<3, float>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
     else
-        launch_rpim_d<3>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+        launch_rpim_d<3, double>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
     hip_check(hipGetLastError(), "rpim weights");
 }
 

@@ -4,29 +4,89 @@
 
 #include <vector>
 
+#include "device_buffer.hpp"
 #include "naga_common.hpp"
 
 namespace naga_amd {
 
-// Uniform cell binning of the reference points (built on the host):
-// points sorted by cell, cell c holds sorted[start[c] .. start[c+1]).
+// Uniform cell binning of points on the device, the nd_cubic_segmentation
+// of segmentation/nd_cubic_segmentation.cuh:227-303: bounds by reduction
+// (extents below 1e-3 widened to 1e-3), cells of width
+// w = (per_cell * V / n)^(1/D) (V the bounding volume) or a given width,
+// dims[d] = ceil(extent_d / w), cell of a point floor((x - lo) / w) clamped
+// to the grid; points listed by cell with ascending indices within a cell.
+// If the cells would exceed the capacity (2 n + 64, reached only by
+// degenerate point sets) w grows by 5/4 until they fit.
 struct CellGrid {
     double lo[3];
     double w;          // cell width
     int64_t dims[3];
+    int64_t ncell;
     int D;
 };
 
+// Views of one binning in a scratch block of device memory: cell c holds
+// sorted[start[c] .. start[c + 1]); counts[c] = start[c + 1] - start[c].
+struct CellBins {
+    CellGrid* grid;     // device
+    uint32_t* start;    // cap + 1
+    uint32_t* counts;   // cap + 1
+    uint32_t* sorted;   // n
+    int64_t cap;        // cell capacity
+    int64_t n;
+};
+// Scratch layout of one binning of n points (offsets 256-byte aligned):
+// the grid, per-block bounds, cell counts and starts, scan block sums, the
+// sorted indices.
+constexpr int kBoundBlocks = 256;
+constexpr int kScanChunk = 2048;  // elements per scan block: 256 threads x 8
+struct BinLayout {
+    size_t grid, partial, counts, start, bsum, sorted, total;
+    int64_t cap, nblocks;
+};
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr BinLayout bin_layout(int64_t n) {
+    BinLayout L{};
+    L.cap = 2 * n + 64;
+    const int64_t M = L.cap + 1;
+    L.nblocks = (M + kScanChunk - 1) / kScanChunk;
+    size_t off = 0;
+    L.grid = off;
+    off = align256(off + sizeof(CellGrid));
+    L.partial = off;
+    off = align256(off + size_t(kBoundBlocks) * 6 * sizeof(double));
+    L.counts = off;
+    off = align256(off + size_t(M) * 4);
+    L.start = off;
+    off = align256(off + size_t(M) * 4);
+    L.bsum = off;
+    off = align256(off + size_t(L.nblocks) * 4);
+    L.sorted = off;
+    off = align256(off + size_t(n) * 4);
+    L.total = off;
+    return L;
+}
+
+// Scratch bytes for binning n points.
+size_t cell_bins_bytes(int64_t n);
+// Bins pts[n][D] (device) into scratch (device, at least cell_bins_bytes(n)
+// bytes, 256-byte aligned) on `stream`: per_cell > 0 sets the width from the
+// points per cell, else width > 0 is the cell width.  Allocates nothing.
+CellBins launch_bin_points(int D, const float* pts, int64_t n, double per_cell, double width,
+                           void* scratch, size_t bytes, hipStream_t stream);
+// Binning with its own device memory.
+struct DeviceCells {
+    DeviceBuffer<uint8_t> mem;
+    CellBins bins{};
+};
+DeviceCells bin_points_device(int D, const float* pts, int64_t n, hipStream_t stream,
+                              double per_cell = 2.0, double width = 0.0);
+
 // Exact k-nearest neighbours of each query among ref, key (d2, index)
 // ascending, d2 in FP32 as dz^2 + (dy^2 + dx^2) with no contraction.
-// All pointers are device pointers; start has ncell+1 entries.
-void launch_knn(const CellGrid& grid, const float* ref, const uint32_t* sorted,
-                const int64_t* start, const float* query, int64_t n_q, int k,
+// All pointers are device pointers; bins are those of ref.
+void launch_knn(const CellBins& bins, const float* ref, const float* query, int64_t n_q, int k,
                 float* d2_out, uint32_t* idx_out, hipStream_t stream);
-
-// Host-side binning helper: returns sorted point indices and cell starts.
-void bin_points(int D, const float* pts, int64_t n, CellGrid& grid,
-                std::vector<uint32_t>& sorted, std::vector<int64_t>& start);
 
 // Batched dense LU with partial pivoting + solve, FP64, one wavefront per
 // system (n <= 64).  A: batch x n x n column-major; B: batch x n x nrhs
@@ -45,13 +105,16 @@ void launch_divergence_weights(int D, const float* src, const float* query,
                                const float* d2, int k, int rbf,
                                const double* hbar, float* W, double* W64,
                                int* info, hipStream_t stream,
-                               const uint8_t* kvar = nullptr);
+                               const uint8_t* kvar = nullptr, int solve_bits = 64);
 
 // RPIM interpolation weights, group size 1: w[q*k + s].
+// solve_bits (both weight kernels): 64 solves the (k + D + 1)^2 systems in
+// FP64; 32 rounds them to FP32 and solves in FP32, the reference's solve
+// precision (linalg/batched_matrix_inverse.cuh:229-250).
 void launch_rpim_weights(int D, const float* src, const float* query,
                          int64_t n_q, const uint32_t* idx, int k, int rbf,
                          double spacing, float* w, double* w64, int* info,
-                         hipStream_t stream);
+                         hipStream_t stream, int solve_bits = 64);
 
 // Normalised Gaussian filter weights over the stencil (sigma_g = h/2.5).
 void launch_filter_weights(int D, const float* pts, int64_t n,

@@ -164,7 +164,20 @@ struct stencil_config {
     bool reference_pml_indexing = true;
     int min_support_size = 0;  // > 0: per-node widths in [min, support_size] by density
     double filter_strength = 0.0;  // 0: the reference's 0.75 per step; < 0: no filter
+    int solve_precision = 64;      // 32: FP32 weight solves, as the reference
 };
+
+// The reference's own stencil settings (reference-parity mode): k = 32
+// multiquadric RBFs for the divergence and the boundary interpolator
+// (interpolation/radial_point_method.cuh:44-66, 88-90) with FP32 solves
+// (linalg/batched_matrix_inverse.cuh:229-250).
+inline stencil_config reference_stencil_config() {
+    stencil_config s;
+    s.rbf = NAGA_RBF_MULTIQUADRIC;
+    s.boundary_rbf = NAGA_RBF_MULTIQUADRIC;
+    s.solve_precision = 32;
+    return s;
+}
 
 // Declarative density source (naga_engine_add_density_source): while
 // t_start <= t < t_end, nodes within radius of center receive
@@ -222,6 +235,7 @@ class simulation_engine {
         cfg.c = double(params_.speed_of_sound);
         cfg.h = double(domain.nodal_spacing);
         cfg.filter_strength = stencil_.filter_strength;
+        cfg.solve_precision = stencil_.solve_precision;
         naga_engine* e = nullptr;
         abi::check(naga_engine_create(&cfg, &e), where);
         handle_.reset(e, [](naga_engine* p) { naga_engine_destroy(p); });

@@ -28,7 +28,7 @@ class Config(C.Structure):
         ("pml_ref_index", C.c_int), ("threads", C.c_int),
         ("nu", C.c_double), ("rho0", C.c_double), ("dt", C.c_double),
         ("c", C.c_double), ("h", C.c_double), ("filter", C.c_double),
-        ("kvar", C.c_void_p),
+        ("kvar", C.c_void_p), ("solve_bits", C.c_int),
     ]
 
 
@@ -77,12 +77,12 @@ class Oracle:
         self._knn = fn("knn", C.c_int, knn_args)
         self._knn_brute = fn("knn_brute", C.c_int, knn_args)
         self._lu = fn("lu_solve", C.c_int, [C.c_int, C.c_int, _f64p, _f64p])
-        self._rpim = fn("rpim_weights", C.c_int,
+        self._rpim = fn("rpim_weights_ex", C.c_int,
                         [C.c_int, _f32p, _u32p, C.c_int, _f32p, C.c_int64,
-                         C.c_int, C.c_double, _f64p])
-        self._divw = fn("divergence_weights", C.c_int,
+                         C.c_int, C.c_double, C.c_int, _f64p])
+        self._divw = fn("divergence_weights_ex", C.c_int,
                         [C.c_int, _f32p, C.c_int64, _f32p, C.c_int64, _u32p,
-                         _f32p, C.c_int, C.c_int, C.c_void_p, _f64p, _f64p])
+                         _f32p, C.c_int, C.c_int, C.c_void_p, C.c_int, _f64p, _f64p])
         self._diva = fn("divergence_apply", C.c_int,
                         [C.c_int, C.c_int64, C.c_int, _u32p, _f64p, _f64p,
                          C.c_double, _f64p])
@@ -103,6 +103,8 @@ class Oracle:
         self._get_setup = fn("engine_get_setup", C.c_int,
                              [C.c_void_p, opt64, opt64, opt64, opt64, opt64])
         self._get_pml = fn("engine_get_pml", C.c_int, [C.c_void_p, opt64, opt64])
+        self._set_pml = fn("engine_set_pml", C.c_int, [C.c_void_p, opt64, opt64])
+        self._pml_stage = fn("engine_pml_stage", C.c_int, [C.c_void_p, _f64p, _f64p])
         self._time = fn("engine_time", C.c_double, [C.c_void_p])
 
     def _ct(self):
@@ -175,17 +177,17 @@ class Oracle:
         X = Bc.reshape(nrhs, n).T
         return (X[:, 0] if B2.ndim == 1 else X), info
 
-    def rpim_weights(self, src, idx, query, rbf, spacing):
+    def rpim_weights(self, src, idx, query, rbf, spacing, solve_bits=64):
         src = np.ascontiguousarray(src, np.float32)
         query = np.ascontiguousarray(query, np.float32)
         idx = np.ascontiguousarray(idx, np.uint32)
         nq, k = idx.shape
         w = np.zeros(nq * k)
-        if self._rpim(src.shape[1], src, idx, k, query, nq, rbf, spacing, w):
+        if self._rpim(src.shape[1], src, idx, k, query, nq, rbf, spacing, solve_bits, w):
             raise ValueError("singular RPIM system")
         return w.reshape(nq, k)
 
-    def divergence_weights(self, src, query, idx, d2, rbf, kvar=None):
+    def divergence_weights(self, src, query, idx, d2, rbf, kvar=None, solve_bits=64):
         src = np.ascontiguousarray(src, np.float32)
         query = np.ascontiguousarray(query, np.float32)
         idx = np.ascontiguousarray(idx, np.uint32)
@@ -196,7 +198,7 @@ class Oracle:
         delta = np.zeros(nq)
         kv = None if kvar is None else np.ascontiguousarray(kvar, np.uint8)
         if self._divw(D, src, src.shape[0], query, nq, idx, d2, k, rbf,
-                      None if kv is None else kv.ctypes.data, W, delta):
+                      None if kv is None else kv.ctypes.data, solve_bits, W, delta):
             raise ValueError("singular divergence system")
         return W.reshape(nq, k, D), delta
 
@@ -292,7 +294,7 @@ class OracleEngine:
                  rbf_boundary: int | None = None, tau: float = 0.6,
                  dt: float | None = None, h: float = 0.01, rho0: float = 1.0,
                  pml_ref_index: int = 1, threads: int = 0, c=None,
-                 filter_strength: float = 0.75, kvar=None):
+                 filter_strength: float = 0.75, kvar=None, solve_bits: int = 64):
         self.o = oracle
         self.Q = Q
         self.D = 2 if Q == 9 else 3
@@ -306,7 +308,7 @@ class OracleEngine:
                      rbf_boundary=rbf if rbf_boundary is None else rbf_boundary,
                      pml_ref_index=pml_ref_index, threads=threads,
                      nu=ph["nu"], rho0=rho0, dt=ph["dt"], c=ph["c"], h=h,
-                     filter=filter_strength)
+                     filter=filter_strength, solve_bits=solve_bits)
         # per-node stencil widths; kept alive for the engine's setup
         self._kvar = None if kvar is None else np.ascontiguousarray(kvar, np.uint8)
         cfg.kvar = None if self._kvar is None else self._kvar.ctypes.data
@@ -378,6 +380,22 @@ class OracleEngine:
         q2 = np.zeros(Q * N)
         self.o._get_pml(self.h, q1.ctypes.data, q2.ctypes.data)
         return q1.reshape(Q, N), q2.reshape(Q, N)
+
+    def set_pml(self, q1=None, q2=None):
+        """Overwrite the stored PML history (Q1: previous equilibrium, Q2:
+        previous trapezoid value), each Q x N."""
+        a = None if q1 is None else np.ascontiguousarray(q1, np.float64).reshape(-1)
+        b = None if q2 is None else np.ascontiguousarray(q2, np.float64).reshape(-1)
+        self.o._set_pml(self.h, None if a is None else a.ctypes.data,
+                        None if b is None else b.ctypes.data)
+
+    def pml_stage(self, s1):
+        """The PML terms of one step applied to the stage input s1 (Q x N),
+        with the macroscopic values of the current distributions."""
+        s1 = np.ascontiguousarray(s1, np.float64).reshape(-1)
+        out = np.zeros_like(s1)
+        self.o._pml_stage(self.h, s1, out)
+        return out.reshape(self.Q, self.N)
 
     @property
     def time(self):

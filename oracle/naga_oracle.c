@@ -794,6 +794,69 @@ int NO_FN(lu_solve)(int n, int nrhs, double* A, double* B) {
     return 0;
 }
 
+/* The same elimination in FP32: the solve precision of the reference,
+ * whose batched_matrix_inverse runs cuBLAS getrf/getri on T = float
+ * (linalg/batched_matrix_inverse.cuh:229-250).  A and B are rounded to
+ * FP32, solved, and the FP32 solution is returned in B. */
+static int lu_solve_f32(int n, int nrhs, double* A, double* B) {
+    float* a = (float*)malloc(sizeof(float) * (size_t)n * n);
+    float* b = (float*)malloc(sizeof(float) * (size_t)n * nrhs);
+    for (size_t t = 0; t < (size_t)n * n; ++t) a[t] = (float)A[t];
+    for (size_t t = 0; t < (size_t)n * nrhs; ++t) b[t] = (float)B[t];
+    int info = 0;
+    for (int j = 0; j < n && !info; ++j) {
+        int p = j;
+        float best = fabsf(a[j + (size_t)j * n]);
+        for (int i = j + 1; i < n; ++i) {
+            float v = fabsf(a[i + (size_t)j * n]);
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        if (best == 0.0f) {
+            info = j + 1;
+            break;
+        }
+        if (p != j) {
+            for (int m = 0; m < n; ++m) {
+                float t = a[j + (size_t)m * n];
+                a[j + (size_t)m * n] = a[p + (size_t)m * n];
+                a[p + (size_t)m * n] = t;
+            }
+            for (int r = 0; r < nrhs; ++r) {
+                float t = b[j + (size_t)r * n];
+                b[j + (size_t)r * n] = b[p + (size_t)r * n];
+                b[p + (size_t)r * n] = t;
+            }
+        }
+        float piv = a[j + (size_t)j * n];
+        for (int i = j + 1; i < n; ++i) {
+            float l = a[i + (size_t)j * n] / piv;
+            a[i + (size_t)j * n] = l;
+            for (int m = j + 1; m < n; ++m) a[i + (size_t)m * n] -= l * a[j + (size_t)m * n];
+            for (int r = 0; r < nrhs; ++r) b[i + (size_t)r * n] -= l * b[j + (size_t)r * n];
+        }
+    }
+    if (!info)
+        for (int r = 0; r < nrhs; ++r) {
+            float* x = b + (size_t)r * n;
+            for (int j = n - 1; j >= 0; --j) {
+                float sum = x[j];
+                for (int m = j + 1; m < n; ++m) sum -= a[j + (size_t)m * n] * x[m];
+                x[j] = sum / a[j + (size_t)j * n];
+            }
+        }
+    for (size_t t = 0; t < (size_t)n * nrhs; ++t) B[t] = (double)b[t];
+    free(a);
+    free(b);
+    return info;
+}
+
+static int lu_solve_bits(int bits, int n, int nrhs, double* A, double* B) {
+    return bits == 32 ? lu_solve_f32(n, nrhs, A, B) : NO_FN(lu_solve)(n, nrhs, A, B);
+}
+
 /* ======================================================================
  * RPIM (augmented RBF) weights.
  * ====================================================================== */
@@ -843,6 +906,12 @@ static void assemble_Gx(int D, const double* xs, int k, const double* xq,
 int NO_FN(rpim_weights)(int D, const float* src, const uint32_t* idx, int k,
                         const float* query, int64_t n_q, int rbf,
                         double spacing, double* w_out) {
+    return NO_FN(rpim_weights_ex)(D, src, idx, k, query, n_q, rbf, spacing, 64, w_out);
+}
+
+int NO_FN(rpim_weights_ex)(int D, const float* src, const uint32_t* idx, int k,
+                           const float* query, int64_t n_q, int rbf,
+                           double spacing, int solve_bits, double* w_out) {
     int n = k + D + 1;
     int err = 0;
 #pragma omp parallel
@@ -865,7 +934,7 @@ int NO_FN(rpim_weights)(int D, const float* src, const uint32_t* idx, int k,
                     G0[i + (size_t)j * n] = G0[j + (size_t)i * n];
                     G0[j + (size_t)i * n] = t;
                 }
-            if (NO_FN(lu_solve)(n, 1, G0, g)) {
+            if (lu_solve_bits(solve_bits, n, 1, G0, g)) {
 #pragma omp atomic write
                 err = 2;
             }
@@ -949,6 +1018,15 @@ int NO_FN(divergence_weights)(int D, const float* src, int64_t n_src,
                               const uint32_t* idx, const float* d2, int k,
                               int rbf, const unsigned char* kvar, double* W_out,
                               double* delta_out) {
+    return NO_FN(divergence_weights_ex)(D, src, n_src, query, n_q, idx, d2, k, rbf, kvar, 64,
+                                        W_out, delta_out);
+}
+
+int NO_FN(divergence_weights_ex)(int D, const float* src, int64_t n_src,
+                                 const float* query, int64_t n_q,
+                                 const uint32_t* idx, const float* d2, int k,
+                                 int rbf, const unsigned char* kvar, int solve_bits,
+                                 double* W_out, double* delta_out) {
     (void)n_src;
     const int kmax = k;
     const int nmax = kmax + D + 1;
@@ -1000,7 +1078,7 @@ int NO_FN(divergence_weights)(int D, const float* src, int64_t n_src,
                     G0[r + (size_t)j * n] = G0[j + (size_t)r * n];
                     G0[j + (size_t)r * n] = t;
                 }
-            if (NO_FN(lu_solve)(n, Nq, G0, B)) {
+            if (lu_solve_bits(solve_bits, n, Nq, G0, B)) {
 #pragma omp atomic write
                 err = 2;
             }
@@ -1417,8 +1495,9 @@ no_engine* NO_FN(engine_create)(const no_config* cfg, const float* pts,
     float* d2 = (float*)malloc(sizeof(float) * N * k);
     NO_FN(knn)(D, pts, N, pts, N, k, d2, e->idx);
     e->W64 = (double*)malloc(sizeof(double) * N * k * D);
-    int err = NO_FN(divergence_weights)(D, pts, N, pts, N, e->idx, d2, k,
-                                        cfg->rbf, cfg->kvar, e->W64, NULL);
+    const int bits = cfg->solve_bits == 32 ? 32 : 64;
+    int err = NO_FN(divergence_weights_ex)(D, pts, N, pts, N, e->idx, d2, k,
+                                           cfg->rbf, cfg->kvar, bits, e->W64, NULL);
     e->W = (no_real*)malloc(sizeof(no_real) * N * k * D);
     for (int64_t t = 0; t < N * k * D; ++t) e->W[t] = (no_real)e->W64[t];
     /* filter weights: Gaussian of sigma_g = h/2.5 over the divergence
@@ -1451,8 +1530,8 @@ no_engine* NO_FN(engine_create)(const no_config* cfg, const float* pts,
         float* bd2 = (float*)malloc(sizeof(float) * n_boundary * kb);
         double* bw64 = (double*)malloc(sizeof(double) * n_boundary * kb);
         NO_FN(knn)(D, pts, nsrc, pts + nsrc * D, n_boundary, kb, bd2, e->bidx);
-        err |= NO_FN(rpim_weights)(D, pts, e->bidx, kb, pts + nsrc * D,
-                                   n_boundary, cfg->rbf_boundary, cfg->h, bw64);
+        err |= NO_FN(rpim_weights_ex)(D, pts, e->bidx, kb, pts + nsrc * D, n_boundary,
+                                      cfg->rbf_boundary, cfg->h, bits, bw64);
         for (int64_t t = 0; t < n_boundary * kb; ++t) e->bw[t] = (no_real)bw64[t];
         free(bd2);
         free(bw64);
@@ -1581,6 +1660,23 @@ int NO_FN(engine_get_pml)(no_engine* e, double* Q1, double* Q2) {
         if (Q1) Q1[t] = (double)e->Q1[t];
         if (Q2) Q2[t] = (double)e->Q2[t];
     }
+    return 0;
+}
+
+int NO_FN(engine_set_pml)(no_engine* e, const double* Q1, const double* Q2) {
+    for (int64_t t = 0; t < (int64_t)e->Q * e->N; ++t) {
+        if (Q1) e->Q1[t] = (no_real)Q1[t];
+        if (Q2) e->Q2[t] = (no_real)Q2[t];
+    }
+    return 0;
+}
+
+int NO_FN(engine_pml_stage)(no_engine* e, const double* s1_in, double* s1_out) {
+    const int64_t QN = (int64_t)e->Q * e->N;
+    eng_macroscopic(e, e->f);
+    for (int64_t t = 0; t < QN; ++t) e->s1[t] = (no_real)s1_in[t];
+    eng_pml(e);
+    for (int64_t t = 0; t < QN; ++t) s1_out[t] = (double)e->s1[t];
     return 0;
 }
 

@@ -9,8 +9,9 @@
  * One source file is compiled twice:
  *   -DNO_REAL=double -DNO_PFX=o64_   -> libnaga_oracle_f64.so (the FP64 oracle)
  *   -DNO_REAL=float  -DNO_PFX=o32_   -> libnaga_oracle_f32.so (FP32 CPU port)
- * Setup arithmetic (RBF assembly, LU solves, quadrature, weights) is always
- * FP64; the stepping state and arithmetic use NO_REAL.
+ * Setup arithmetic (RBF assembly, LU solves, quadrature, weights) is FP64
+ * (the RBF solves optionally FP32, as the reference's); the stepping state
+ * and arithmetic use NO_REAL.
  *
  * Every function cites the reference file:line it restates (paths relative
  * to the NAGA source tree).
@@ -97,6 +98,11 @@ int NO_FN(lu_solve)(int n, int nrhs, double* A, double* B);
 int NO_FN(rpim_weights)(int D, const float* src, const uint32_t* idx, int k,
                         const float* query, int64_t n_q, int rbf,
                         double spacing, double* w_out);
+/* The same with the systems solved in FP32 (solve_bits = 32, the reference's
+ * precision) or FP64 (64). */
+int NO_FN(rpim_weights_ex)(int D, const float* src, const uint32_t* idx, int k,
+                           const float* query, int64_t n_q, int rbf,
+                           double spacing, int solve_bits, double* w_out);
 
 /* ---- nonlocal divergence weights ---------------------------------------
  * W[(i*k + s)*D + d] for query i, restating operator_builder + RPIM at the
@@ -109,6 +115,11 @@ int NO_FN(divergence_weights)(int D, const float* src, int64_t n_src,
                               const uint32_t* idx, const float* d2, int k,
                               int rbf, const unsigned char* kvar, double* W_out,
                               double* delta_out);
+int NO_FN(divergence_weights_ex)(int D, const float* src, int64_t n_src,
+                                 const float* query, int64_t n_q,
+                                 const uint32_t* idx, const float* d2, int k,
+                                 int rbf, const unsigned char* kvar, int solve_bits,
+                                 double* W_out, double* delta_out);
 
 /* Apply a divergence operator to a vector field F[j*D + d]:
  * out[i] = tree-sum over s of sum_d W(d,s,i) (F_d(idx(s,i)) - off),
@@ -140,6 +151,10 @@ typedef struct no_config {
                            (simulation_engine.cuh:1242); other values exist
                            only for tests of the scheme without it */
     const unsigned char* kvar; /* optional per-node stencil widths <= k */
+    int solve_bits;     /* 64 (default, also for 0) or 32: the RBF systems of the
+                           divergence and boundary weights solved in FP32, as
+                           the reference does (batched_matrix_inverse.cuh:
+                           229-250) */
 } no_config;
 
 no_engine* NO_FN(engine_create)(const no_config* cfg, const float* pts,
@@ -162,6 +177,15 @@ int NO_FN(engine_get)(no_engine* e, double* f, double* rho, double* u,
 int NO_FN(engine_get_setup)(no_engine* e, uint32_t* idx, double* W,
                             double* filt, uint32_t* bidx, double* bw);
 int NO_FN(engine_get_pml)(no_engine* e, double* Q1, double* Q2);
+/* Test hooks of the PML stage alone: engine_set_pml overwrites the stored
+ * PML history (Q1: the previous equilibrium, Q2: the previous trapezoid
+ * value, both Q*N alpha-major); engine_pml_stage takes the macroscopic
+ * values of the current f, sets the stage input s1 = s1_in (Q*N), runs the
+ * PML terms of one step (compute_Qvalues, the layer's local damping, the
+ * divergences of both auxiliary fields, post_collision) and returns the
+ * stage output in s1_out. */
+int NO_FN(engine_set_pml)(no_engine* e, const double* Q1, const double* Q2);
+int NO_FN(engine_pml_stage)(no_engine* e, const double* s1_in, double* s1_out);
 double NO_FN(engine_time)(no_engine* e);
 /* density sources (see eng_sources in naga_oracle.c); at most
  * NO_MAX_SOURCES; returns non-zero when full or radius <= 0 */

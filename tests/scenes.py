@@ -83,3 +83,19 @@ def mirror_nodes(D, h=0.05, m=8, n=17, layer=0.15, sigma_max=0.5, seed=3):
     pos = {tuple(p): i for i, p in enumerate(nodes["points"].tolist())}
     mirror = np.array([pos[(-p[0],) + tuple(p[1:])] for p in nodes["points"].tolist()])
     return nodes, mirror
+
+
+def kvar_like_engine(oracle, points, k, k_min):
+    """Per-node stencil widths by local density, as the engine forms them
+    (k_min > 0): d_i the mean distance to the 4 nearest other nodes, d_ref
+    its 90th percentile (element floor(0.9 (N - 1)) in ascending order),
+    k_i = round(k_min d_ref / d_i) clamped to [k_min, k] (halves away from
+    zero); computed here from the oracle's kNN."""
+    pts = np.ascontiguousarray(points, np.float32)
+    d2, _ = oracle.knn(pts, pts, k)
+    r = np.sqrt(d2[:, 1:5].astype(np.float64))
+    dm = (((r[:, 0] + r[:, 1]) + r[:, 2]) + r[:, 3]) / 4.0
+    N = dm.size
+    dref = np.partition(dm, int(0.9 * (N - 1)))[int(0.9 * (N - 1))]
+    want = float(k_min) * dref / np.maximum(dm, 1e-30)
+    return np.clip(np.floor(want + 0.5), k_min, k).astype(np.uint8)

@@ -125,3 +125,32 @@ def test_batch_mixes_union_and_direct_scenes(oracle64, monkeypatch):
         ma, mb = a.macroscopic(fresh=True), b.macroscopic(fresh=True)
         assert rel_l2(mb["drho"], ma["drho"]) < 1e-6
     batch.close()
+
+
+def test_batch_sources_cleared_and_added_in_a_batch(oracle64):
+    """Density sources added or cleared while the engine is in a batch take
+    effect at the next batched step (the batch rebuilds its scene table):
+    a scene whose source is cleared after 5 steps ends as the same scene
+    run alone with the same history, and differs from one that keeps it."""
+    spec = _specs()[0]
+    src = naga_amd.DensitySource([0.5, 0.5], 0.1, 1e-3, 5.0)
+    alone, kept = _scene(oracle64, *spec), _scene(oracle64, *spec)
+    mixed = _scene(oracle64, *spec)
+    other = _scene(oracle64, *_specs()[1])
+    batch = naga_amd.SceneBatch([mixed, other])
+    batch.step_forward(2)                  # no source yet: graph replay
+    mixed.register_density_source(src)     # added while in the batch
+    batch.step_forward(5)
+    mixed.clear_density_sources()          # cleared while in the batch
+    batch.step_forward(6)
+    for e in (alone, kept):
+        e.step_forward(2)
+        e.register_density_source(src)
+        e.step_forward(5)
+    alone.clear_density_sources()
+    alone.step_forward(6)
+    kept.step_forward(6)
+    got = mixed.macroscopic(fresh=True)["drho"]
+    assert rel_l2(got, alone.macroscopic(fresh=True)["drho"]) < 1e-6
+    assert rel_l2(got, kept.macroscopic(fresh=True)["drho"]) > 1e-3
+    batch.close()
