@@ -29,23 +29,51 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
-ROOFLINE_KERNEL = "k_stage_u<2, 32, 8, true, 64>"
-# HBM bytes per launch of ROOFLINE_KERNEL on this workload, from two
-# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) summarised by
-# naga_amd.profsum; the file records the commands.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_stage2_d3q27_h0.01_traffic.md")
+# The RK4 stage kernels of the 1M-node workload (tile unions, 64-node tiles);
+# the dominant one, stage 4, is the roofline line.
+STAGE_KERNELS = {s: f"k_stage_u<{s}, 32, 8, true, 64>" for s in (1, 2, 3, 4)}
+ROOFLINE_STAGE = 4
 
 
-def measured_traffic(h: float):
-    if h != 0.01 or not os.path.exists(TRAFFIC_PROFILE):
+def traffic_profile(stage: int) -> str:
+    """HBM bytes per launch of a stage kernel on this workload, from two
+    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) summarised by
+    naga_amd.profsum; the file records the commands."""
+    return os.path.join(ROOT, "profiles", f"k_stage{stage}_d3q27_h0.01_traffic.md")
+
+
+def measured_traffic(h: float, stage: int):
+    path = traffic_profile(stage)
+    if h != 0.01 or not os.path.exists(path):
         return None
-    text = open(TRAFFIC_PROFILE).read()
-    if f"`{ROOFLINE_KERNEL}`" not in text.splitlines()[0]:
+    text = open(path).read()
+    if f"`{STAGE_KERNELS[stage]}`" not in text.splitlines()[0]:
         return None  # the profile is of another kernel build
     for line in text.splitlines():
         if line.startswith("traffic_bytes_per_launch"):
             return float(line.split("=")[1])
     return None
+
+
+def copy_bandwidth(nbytes: int = 2 << 30, reps: int = 10) -> float:
+    """Measured device-to-device copy rate (bytes read + written per second)
+    of the runtime's copy over nbytes, the practical HBM ceiling beside the
+    8 TB/s nominal peak."""
+    import torch
+    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    b.synchronize()
+    rate = 2.0 * nbytes * reps / (a.elapsed_time(b) * 1e-3)
+    del x, y
+    torch.cuda.empty_cache()
+    return rate
 
 
 def copy_calibration(nbytes: int) -> int:
@@ -82,13 +110,26 @@ def step_model_bytes(nodes, Q, D, k):
 
 
 def cpu_baseline(threads: int) -> dict:
+    import naga_amd
     from oracle import Oracle
 
     h = 0.02
     o = Oracle(32)
     nodes = o.uniform_grid(3, [0, 0, 0], [1, 1, 1], h, jitter=0.25, layer=8 * h,
                            sigma_max=0.5)
+    t0 = time.perf_counter()
     eng = o.engine(nodes, Q=27, k=32, h=h, tau=0.6, threads=threads)
+    cpu_setup = time.perf_counter() - t0
+    # the GPU setup of the same sample, for the setup comparison
+    g = naga_amd.SimulationEngine(lattice=27, k=32, rbf=naga_amd.RBF_POLYHARMONIC3)
+    p = naga_amd.problem_parameters(27, 0.6, 0.4 * h)
+    g.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+    sim = naga_amd.SimulationNodes(nodes["points"], nodes["normals"], nodes["sigma"],
+                                   nodes["n_bulk"], nodes["n_layer"], nodes["n_boundary"], h)
+    t0 = time.perf_counter()
+    g.init_domain(sim)
+    gpu_setup = time.perf_counter() - t0
+    g.close()
     eng.set_pulse([0.3, 0.5, 0.5], 1e-3, 0.05)
     eng.step(1)  # warm-up
     n = 80  # about 10 s of CPU work on 16 threads
@@ -99,7 +140,8 @@ def cpu_baseline(threads: int) -> dict:
     return {"value": N * n / dt, "unit": "node-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"D3Q27 unit cube h=0.02 ({N} nodes, k=32, PML 8h), {n} steps "
-                      f"after 1 warm-up step, FP32 oracle port, OpenMP {threads} threads"}
+                      f"after 1 warm-up step, FP32 oracle port, OpenMP {threads} threads",
+            "setup_s": cpu_setup, "gpu_setup_s_same_sample": gpu_setup}
 
 
 def batched_workload(steps: int, n_scenes: int = 256) -> dict:
@@ -291,14 +333,20 @@ def main() -> int:
         for kname, ms in eng.profile_step().items():
             prof[kname] = prof.get(kname, 0.0) + ms / args.profile_reps
     nfluid = nodes.num_bulk_points + nodes.num_layer_points
-    stage_ms = prof["s2_rk4_stage"]
-    sbytes = nfluid * stage_bytes_per_node(Q, D, k, 2)
-    achieved = sbytes / (stage_ms * 1e-3)
+    stages = {}
+    for s in (1, 2, 3, 4):
+        ms = prof[f"s{s}_rk4_stage"]
+        b = nfluid * stage_bytes_per_node(Q, D, k, s)
+        stages[f"s{s}"] = {"kernel": STAGE_KERNELS[s], "bytes_per_launch": b, "avg_ms": ms,
+                           "frac": b / (ms * 1e-3) / HBM_PEAK,
+                           "traffic": measured_traffic(h, s)}
+    top = stages[f"s{ROOFLINE_STAGE}"]
+    achieved = top["bytes_per_launch"] / (top["avg_ms"] * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                "traffic": measured_traffic(h),
-                "kernel": ROOFLINE_KERNEL + " (RK4 stage 2)",
-                "bytes_per_launch": sbytes, "avg_ms": stage_ms}
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": top["traffic"],
+                "kernel": top["kernel"] + f" (RK4 stage {ROOFLINE_STAGE}, the longest kernel)",
+                "bytes_per_launch": top["bytes_per_launch"], "avg_ms": top["avg_ms"],
+                "stages": stages}
     model = step_model_bytes(nodes, Q, D, k)
     out = {
         "metric": "D3Q27 nonlocal-LBM node-updates/s @1M nodes",
@@ -313,11 +361,12 @@ def main() -> int:
                    "nodes": N, "bulk": nodes.num_bulk_points, "layer": nodes.num_layer_points,
                    "boundary": nodes.num_boundary_points},
         "gpu_event_ms_per_step": gpu_ms / args.steps,
-        "setup_s": t_setup, "nodes_s": t_nodes,
+        "setup_s": t_setup, "nodes_s": t_nodes, "setup_phases_ms": eng.setup_phases(),
         "step_model_bytes_per_node_update": model,
         "step_model_hbm_frac": value * model / HBM_PEAK,
         "kernel_ms": prof,
         "roofline": roofline,
+        "copy_gbs": copy_bandwidth() / 1e9,
     }
     if not args.no_batched:
         eng.close()

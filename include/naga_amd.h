@@ -164,6 +164,22 @@ int naga_engine_get_setup(naga_engine* e, uint32_t* idx, float* W, float* filt,
 int naga_engine_get_pml(naga_engine* e, double* Q1_minus_w, double* Q2);
 int naga_engine_info(naga_engine* e, int64_t* n_nodes, double* time,
                      int64_t* frame, double* setup_ms);
+/* The output of vtk_observer::update_all (simulation_observers/
+ * vtk_observer.cuh:86-247) for the engine's scene: one VTK XML PolyData
+ * file (.vtp; inline binary, UInt32 headers, uncompressed) with the nodes as
+ * vertices and the point arrays f (f_alpha / w_alpha, Q components),
+ * density, velocity (3 components), absorption, normals (3) and type
+ * (0 bulk, 1 layer, 2 boundary), all FP32 but type.  fresh = 0 writes the
+ * macroscopic values held from the start of the last step (what an observer
+ * sees in update()), fresh = 1 the current state.  The file name (the
+ * reference's result.<frame>.vtp) is the caller's. */
+int naga_engine_write_vtp(naga_engine* e, const char* path, int32_t fresh);
+/* The last init_domain's phases, ms[5] of wall time on the engine's stream:
+ * cell binning and kNN of the divergence stencils, divergence weights (RBF
+ * solves), filter weights + stencil slabs + tile unions, boundary
+ * interpolator (kNN and RPIM solves), PML lists and state.  No counterpart
+ * in the reference (for measurement). */
+int naga_engine_setup_phases(naga_engine* e, double* ms);
 
 /* ---- batch: many scenes stepped together.  No counterpart in the
  *      reference (which steps one engine at a time): every pass of a step
@@ -276,12 +292,82 @@ int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double 
  * exact: the k smallest (d2, index), d2 FP32 = dz^2 + (dy^2 + dx^2). */
 int naga_knn(const float* ref, int64_t n_ref, const float* query, int64_t n_q,
              int32_t dims, int32_t k, float* d2_out, uint32_t* idx_out);
-/* batched_matrix_inverse replacement (linalg/batched_matrix_inverse.cuh:
- * 325-397): solves A X = B per item in FP64, A column-major n x n, B
- * column-major n x nrhs; info[b] != 0 flags a zero pivot. */
-int naga_batched_lu_solve_f64(const double* A, int32_t n, int64_t batch,
-                              const double* B, int32_t nrhs, double* X,
-                              int32_t* info);
+/* batched_matrix_inverse(A, A_inv) (linalg/batched_matrix_inverse.cuh:
+ * 325-397): A_inv[b] = A[b]^-1 for column-major n x n matrices (FP64 LU with
+ * partial pivoting against the identity); info[b] = 0 or the first zero pivot
+ * (1-based), which the reference leaves unchecked.  batch must be below
+ * INT_MAX as in the reference. */
+int naga_batched_matrix_inverse_f64(const double* A, int32_t n, int64_t batch, double* A_inv,
+                                    int32_t* info);
+
+/* ---- interpolation::radial_point_method<T>
+ *      (interpolation/radial_point_method.cuh:92-280) ---------------------- */
+typedef struct naga_rpim naga_rpim;
+/* get_scratchpad_size(query_size, support_size, dimensions, group_size)
+ * (:97-118) for T of bytes_per_value (4 or 8) bytes; -1 (and the reference's
+ * message in naga_last_error) when query_size is not a multiple of
+ * group_size. */
+int64_t naga_rpim_scratchpad_size(int64_t query_size, int32_t support_size, int32_t dims,
+                                  int32_t group_size, int32_t bytes_per_value);
+/* create_interpolator(source_points, interpolating_indices, query_points,
+ * approx_particle_spacing, group_size, shape_function) (:143-169): src
+ * [n_src*dims], idx [(n_q / group_size)*k] (the stencil of each group of
+ * group_size consecutive queries), query [n_q*dims]; rbf 0 multiquadric
+ * (the reference's default shape function) or 1 r^3.  Weights in FP64
+ * solves, kept on the device as FP32. */
+int naga_rpim_create(const float* src, int64_t n_src, const uint32_t* idx, int32_t k,
+                     const float* query, int64_t n_q, int32_t dims, int32_t group_size,
+                     int32_t rbf, double spacing, naga_rpim** out);
+int naga_rpim_destroy(naga_rpim* r);
+/* interpolate(field, destination, centering_offset) (:171-251): field
+ * [n_field][m] (m components per source point; n_field must equal the
+ * source count), dest [n_q][m] = sum_s w (field - offset) + offset. */
+int naga_rpim_interpolate(naga_rpim* r, const float* field, int64_t n_field, int32_t m,
+                          float* dest, float centering_offset);
+/* the weights [n_q][k] (compute_weights' output) */
+int naga_rpim_get_weights(naga_rpim* r, float* w);
+
+/* ---- segmentation::nd_cubic_segmentation<T, D>
+ *      (segmentation/nd_cubic_segmentation.cuh:219-461) ------------------- */
+typedef struct naga_segmentation naga_segmentation;
+/* nd_cubic_segmentation(points, approx_partition_size) (:227-303) when
+ * approx_partition_size > 0, else nd_cubic_segmentation(points,
+ * partition_width) (:305-379); points [n*dims] (dims 1..3), binned on the
+ * device. */
+int naga_segmentation_create(const float* points, int64_t n, int32_t dims,
+                             int32_t approx_partition_size, double partition_width,
+                             naga_segmentation** out);
+int naga_segmentation_destroy(naga_segmentation* s);
+/* shape()[3] (cells per axis, 1 past dims), lower bounds[3], partition width,
+ * partition_count() = product of the shape.  Any output may be NULL. */
+int naga_segmentation_shape(naga_segmentation* s, int64_t* shape, double* lower_bounds,
+                            double* partition_width, int64_t* partition_count);
+/* partition sizes [count], index offsets [count] and indices [n]: the points
+ * of cell c (x fastest) are indices[offsets[c] .. offsets[c] + sizes[c]),
+ * in no particular order within a cell (as the reference's atomics). */
+int naga_segmentation_partitions(naga_segmentation* s, uint32_t* sizes, uint32_t* offsets,
+                                 uint32_t* indices);
+/* batched_nearest_neighbors(k, query_points, segmentation)
+ * (segmentation/nearest_neighbors.cuh:454-477), exact as naga_knn. */
+int naga_segmentation_knn(naga_segmentation* s, const float* query, int64_t n_q, int32_t k,
+                          float* d2_out, uint32_t* idx_out);
+
+/* One stage of the classic RK4 of the streaming equation
+ * (detail/rk4_solver.cuh:48-171 with detail/pde_right_hand_side.cuh:82-176):
+ * rate_alpha(i) = -sum_s sum_d W(d,s,i) c_{alpha,d} in_alpha(idx(s,i)) for
+ * nodes i < n_fluid, zero past them.  Arrays are deviation rows g = f - w,
+ * [n][q] FP32: y the state at the step start, in the stage input.
+ *   stage 1: out = y + dt/2 rate(y)           (in unused)
+ *   stage 2: out = y + dt/2 rate(in)          (in = stage 1's output)
+ *   stage 3: out = y + dt rate(in)            (in = stage 2's output)
+ *   stage 4: out = y + ((y1 - y) + 2 (y2 - y) + (in - y)) / 3 + dt/6 rate(in)
+ *            (y1, y2 the outputs of stages 1 and 2, in that of stage 3),
+ * which chains to y + dt/6 (k1 + 2 k2 + 2 k3 + k4).  idx [n_fluid*k]
+ * (indices < n), W [n_fluid*k*D]; dt the lattice time step. */
+int naga_rk4_stage(int32_t q, int64_t n, int64_t n_fluid, int32_t k, const uint32_t* idx,
+                   const float* W, double dt, int32_t stage, const float* y, const float* in,
+                   const float* y1, const float* y2, float* out);
+
 /* radial_point_method::compute_weights, group size 1
  * (interpolation/radial_point_method.cuh:118-139): w[n_q*k]. */
 int naga_rpim_weights(const float* src, int64_t n_src, const uint32_t* idx,

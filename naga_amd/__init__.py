@@ -105,6 +105,21 @@ def load() -> C.CDLL:
         "naga_engine_get_setup": (C.c_int, [vp, vp, vp, vp, vp, vp]),
         "naga_engine_get_pml": (C.c_int, [vp, vp, vp]),
         "naga_engine_info": (C.c_int, [vp, vp, vp, vp, vp]),
+        "naga_engine_setup_phases": (C.c_int, [vp, vp]),
+        "naga_engine_write_vtp": (C.c_int, [vp, C.c_char_p, i32]),
+        "naga_batched_matrix_inverse_f64": (C.c_int, [vp, i32, i64, vp, vp]),
+        "naga_rpim_scratchpad_size": (i64, [i64, i32, i32, i32, i32]),
+        "naga_rpim_create": (C.c_int, [vp, i64, vp, i32, vp, i64, i32, i32, i32, f64,
+                                       C.POINTER(vp)]),
+        "naga_rpim_destroy": (C.c_int, [vp]),
+        "naga_rpim_interpolate": (C.c_int, [vp, vp, i64, i32, vp, C.c_float]),
+        "naga_rpim_get_weights": (C.c_int, [vp, vp]),
+        "naga_segmentation_create": (C.c_int, [vp, i64, i32, i32, f64, C.POINTER(vp)]),
+        "naga_segmentation_destroy": (C.c_int, [vp]),
+        "naga_segmentation_shape": (C.c_int, [vp, vp, vp, vp, vp]),
+        "naga_segmentation_partitions": (C.c_int, [vp, vp, vp, vp]),
+        "naga_segmentation_knn": (C.c_int, [vp, vp, i64, i32, vp, vp]),
+        "naga_rk4_stage": (C.c_int, [i32, i64, i64, i32, vp, vp, f64, i32, vp, vp, vp, vp, vp]),
         "naga_uniform_grid_size": (i64, [i32, vp, vp, f64]),
         "naga_uniform_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, f64, f64, vp, vp, vp, vp]),
         "naga_knn": (C.c_int, [vp, i64, vp, i64, i32, i32, vp, vp]),
@@ -163,6 +178,11 @@ EXPORTED = [
     "naga_engine_step_timed", "naga_engine_profile_step",
     "naga_engine_get_macroscopic", "naga_engine_get_distributions",
     "naga_engine_get_setup", "naga_engine_get_pml", "naga_engine_info",
+    "naga_engine_setup_phases", "naga_engine_write_vtp", "naga_batched_matrix_inverse_f64",
+    "naga_rpim_scratchpad_size", "naga_rpim_create", "naga_rpim_destroy",
+    "naga_rpim_interpolate", "naga_rpim_get_weights", "naga_segmentation_create",
+    "naga_segmentation_destroy", "naga_segmentation_shape", "naga_segmentation_partitions",
+    "naga_segmentation_knn", "naga_rk4_stage",
     "naga_uniform_grid_size", "naga_uniform_grid", "naga_knn",
     "naga_batched_lu_solve_f64", "naga_rpim_weights", "naga_divergence_weights",
     "naga_divergence_apply", "naga_engine_add_density_source",
@@ -519,7 +539,35 @@ class SimulationEngine:
                _ENGINE_PREFIX)
 
     def step_forward(self, nsteps: int = 1):
-        _check(load().naga_engine_step(self._live(), int(nsteps)), _ENGINE_PREFIX)
+        """step_forward() x nsteps (detail/simulation_engine.cuh:1325-1375).
+        With observers attached each step first hands them the state at its
+        start (update_observers, :1332), then advances one step."""
+        obs = getattr(self, "_observers", None)
+        if not obs:
+            _check(load().naga_engine_step(self._live(), int(nsteps)), _ENGINE_PREFIX)
+            return
+        for _ in range(int(nsteps)):
+            sol = self.solution(fresh=True)
+            t = self.time()
+            for o in list(obs):
+                o.update(t, self._nodes, self._params, sol)
+            _check(load().naga_engine_step(self._live(), 1), _ENGINE_PREFIX)
+
+    def attach_observer(self, observer):
+        """attach_observer (simulation_engine.cuh:90-92): an object with
+        update(time, domain, parameters, solution), called at the start of
+        every step with the macroscopic values of that moment."""
+        self._observers = getattr(self, "_observers", [])
+        self._observers.append(observer)
+
+    def detach_observer(self, observer):
+        obs = getattr(self, "_observers", [])
+        self._observers = [o for o in obs if o is not observer]
+
+    def write_vtp(self, path, fresh: bool = False):
+        """vtk_observer's output format for this scene (see naga_engine_write_vtp)."""
+        _check(load().naga_engine_write_vtp(self._live(), os.fsencode(str(path)), int(fresh)),
+               _ENGINE_PREFIX)
 
     def step_timed(self, nsteps: int) -> float:
         ms = np.zeros(1)
@@ -531,6 +579,16 @@ class SimulationEngine:
         ms = np.zeros(len(STEP_KERNELS))
         _check(load().naga_engine_profile_step(self._live(), _ptr(ms)), _ENGINE_PREFIX)
         return dict(zip(STEP_KERNELS, ms.tolist()))
+
+    SETUP_PHASES = ("knn", "divergence_weights", "filter_slabs_unions", "boundary_interpolator",
+                    "pml_and_state")
+
+    def setup_phases(self) -> dict:
+        """Wall time (ms) of the last init_domain's phases on the engine's
+        stream."""
+        ms = np.zeros(len(self.SETUP_PHASES))
+        _check(load().naga_engine_setup_phases(self._live(), _ptr(ms)), _ENGINE_PREFIX)
+        return dict(zip(self.SETUP_PHASES, ms.tolist()))
 
     def info(self) -> dict:
         n = np.zeros(1, np.int64)
@@ -560,12 +618,12 @@ class SimulationEngine:
                _ENGINE_PREFIX)
         return dict(f=f.reshape(Q, N), g=g.reshape(Q, N))
 
-    def solution(self) -> dict:
+    def solution(self, fresh: bool = False) -> dict:
         """state_variables: distributions plus macroscopic values as held by
         the engine (reference semantics, computed at the start of the last
-        step)."""
+        step), or fresh ones of the current state."""
         out = self.distributions()
-        out.update(self.macroscopic(fresh=False))
+        out.update(self.macroscopic(fresh=fresh))
         return out
 
     def setup_products(self) -> dict:
@@ -638,6 +696,176 @@ def rpim_weights(src, idx, query, rbf=RBF_POLYHARMONIC3, spacing=1.0):
     return w.reshape(nq, k)
 
 
+def batched_matrix_inverse(A, return_info: bool = False):
+    """linalg::batched_matrix_inverse(A) (linalg/batched_matrix_inverse.cuh:
+    325-397): A [batch, n, n] -> A^-1 per item (FP64).  The reference leaves
+    singular items unchecked; return_info gives the per-item pivot info."""
+    A = np.asarray(A, np.float64)
+    if A.ndim != 3 or A.shape[1] != A.shape[2]:
+        raise ValueError("naga::linalg::Matrices must be square")
+    b, n, _ = A.shape
+    Ac = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))  # column-major items
+    out = np.zeros_like(Ac)
+    info = np.zeros(b, np.int32)
+    _check(load().naga_batched_matrix_inverse_f64(_ptr(Ac), n, b, _ptr(out), _ptr(info)),
+           "naga::linalg::")
+    inv = np.transpose(out, (0, 2, 1))
+    return (inv, info) if return_info else inv
+
+
+class RadialPointMethod:
+    """interpolation::radial_point_method<T> (interpolation/
+    radial_point_method.cuh:92-280): weights of each query over its group's
+    stencil (group_size consecutive queries share one stencil and one RBF
+    system), and interpolation with them."""
+
+    def __init__(self, handle, n_src, n_q, k):
+        self._h, self.n_src, self.n_q, self.k = handle, n_src, n_q, k
+
+    @staticmethod
+    def get_scratchpad_size(query_size, support_size, dimensions, group_size=1,
+                            dtype=np.float32) -> int:
+        v = load().naga_rpim_scratchpad_size(int(query_size), int(support_size),
+                                             int(dimensions), int(group_size),
+                                             int(np.dtype(dtype).itemsize))
+        if v < 0:
+            raise ValueError("naga::interpolation::radial_point_method::query_size must be a "
+                             "multiple of group_size")
+        return int(v)
+
+    @classmethod
+    def create_interpolator(cls, source_points, interpolating_indices, query_points,
+                            approx_particle_spacing, group_size=1, rbf=RBF_MULTIQUADRIC):
+        src = _f32(source_points)
+        q = _f32(query_points)
+        idx = np.ascontiguousarray(interpolating_indices, np.uint32)
+        k = idx.shape[1]
+        h = C.c_void_p()
+        _check(load().naga_rpim_create(_ptr(src), src.shape[0], _ptr(idx), k, _ptr(q),
+                                       q.shape[0], src.shape[1], int(group_size), int(rbf),
+                                       float(approx_particle_spacing), C.byref(h)),
+               "naga::interpolation::radial_point_method::")
+        return cls(h, src.shape[0], q.shape[0], k)
+
+    @classmethod
+    def compute_weights(cls, source_points, interpolating_indices, query_points,
+                        approx_particle_spacing, group_size=1, rbf=RBF_MULTIQUADRIC):
+        r = cls.create_interpolator(source_points, interpolating_indices, query_points,
+                                    approx_particle_spacing, group_size, rbf)
+        w = r.weights()
+        r.close()
+        return w
+
+    def weights(self):
+        w = np.zeros(self.n_q * self.k, np.float32)
+        _check(load().naga_rpim_get_weights(self._h, _ptr(w)),
+               "naga::interpolation::radial_point_method::")
+        return w.reshape(self.n_q, self.k)
+
+    def interpolate(self, field, centering_offset=0.0):
+        """field [n_src] or [n_src, m] -> [n_q] or [n_q, m]."""
+        f = _f32(field)
+        m = 1 if f.ndim == 1 else f.shape[1]
+        out = np.zeros((self.n_q, m), np.float32)
+        _check(load().naga_rpim_interpolate(self._h, _ptr(f), f.shape[0], m, _ptr(out),
+                                            float(centering_offset)),
+               "naga::interpolation::radial_point_method::")
+        return out[:, 0] if f.ndim == 1 else out
+
+    def close(self):
+        if self._h is not None:
+            load().naga_rpim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NdCubicSegmentation:
+    """segmentation::nd_cubic_segmentation<T, D>(points, approx_partition_size
+    | partition_width) (segmentation/nd_cubic_segmentation.cuh:219-461),
+    binned on the device."""
+
+    def __init__(self, points, approx_partition_size: int = 32, partition_width: float = 0.0):
+        pts = _f32(points)
+        if pts.ndim != 2:
+            raise ValueError("naga::The number of dimensions in the points array does not "
+                             "match the number of dimensions in the segmentation")
+        self.points = pts
+        h = C.c_void_p()
+        approx = 0 if partition_width > 0 else int(approx_partition_size)
+        _check(load().naga_segmentation_create(_ptr(pts), pts.shape[0], pts.shape[1], approx,
+                                               float(partition_width), C.byref(h)),
+               "naga::segmentation::")
+        self._h = h
+        shape = np.zeros(3, np.int64)
+        lo = np.zeros(3)
+        w = C.c_double()
+        cnt = C.c_int64()
+        _check(load().naga_segmentation_shape(h, _ptr(shape), _ptr(lo), C.byref(w), C.byref(cnt)),
+               "naga::segmentation::")
+        D = pts.shape[1]
+        self.shape = tuple(int(v) for v in shape[:D])
+        self.lower_bounds = lo[:D].copy()
+        self.partition_width = w.value
+        self.upper_bounds = self.lower_bounds + np.array(self.shape) * w.value
+        self.partition_count = int(cnt.value)
+
+    def partitions(self):
+        """(sizes, index offsets, indices), cells with x fastest."""
+        sizes = np.zeros(self.partition_count, np.uint32)
+        offs = np.zeros(self.partition_count, np.uint32)
+        idx = np.zeros(self.points.shape[0], np.uint32)
+        _check(load().naga_segmentation_partitions(self._h, _ptr(sizes), _ptr(offs), _ptr(idx)),
+               "naga::segmentation::")
+        return sizes, offs, idx
+
+    def batched_nearest_neighbors(self, k: int, query):
+        q = _f32(query)
+        d2 = np.zeros(q.shape[0] * k, np.float32)
+        idx = np.zeros(q.shape[0] * k, np.uint32)
+        _check(load().naga_segmentation_knn(self._h, _ptr(q), q.shape[0], int(k), _ptr(d2),
+                                            _ptr(idx)), "naga::segmentation::")
+        return d2.reshape(-1, k), idx.reshape(-1, k)
+
+    def close(self):
+        if self._h is not None:
+            load().naga_segmentation_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rk4_stage(stage, y, stencil_idx, weights, dt, n_fluid=None, stage_input=None, y1=None,
+              y2=None):
+    """One RK4 stage of the streaming equation (naga_rk4_stage): y and the
+    stage input are [n, Q] deviation rows; stencil_idx [n_fluid, k],
+    weights [n_fluid, k, D]."""
+    y = _f32(y)
+    n, q = y.shape
+    idx = np.ascontiguousarray(stencil_idx, np.uint32)
+    W = _f32(weights)
+    nf = idx.shape[0] if n_fluid is None else int(n_fluid)
+    k = idx.shape[1]
+    out = np.zeros_like(y)
+    ins = None if stage_input is None else _f32(stage_input)
+    a1 = None if y1 is None else _f32(y1)
+    a2 = None if y2 is None else _f32(y2)
+    _check(load().naga_rk4_stage(q, n, nf, k, _ptr(idx), _ptr(W), float(dt), int(stage),
+                                 _ptr(y), None if ins is None else _ptr(ins),
+                                 None if a1 is None else _ptr(a1),
+                                 None if a2 is None else _ptr(a2), _ptr(out)),
+           "naga::nonlocal_lbm::rk4_solver::")
+    return out
+
+
 class DivergenceOperator:
     """nonlocal_calculus::divergence_operator<T, D> (nonlocal_calculus/
     divergence.cuh:44-612)."""
@@ -647,17 +875,26 @@ class DivergenceOperator:
         self._idx = np.ascontiguousarray(support_indices, np.uint32)
 
     @classmethod
-    def create(cls, domain, query=None, k: int = 32, rbf: int = RBF_POLYHARMONIC3):
+    def create(cls, domain, query=None, k: int = 32, rbf: int = RBF_POLYHARMONIC3,
+               override_query_size: int = 0):
+        """create(domain[, query_points, override_query_size])
+        (nonlocal_calculus/divergence.cuh:53-166): stencils over domain for
+        each query point; override_query_size > 0 sizes the operator for that
+        many queries, the rows past the query count zero."""
         domain = _f32(domain)
         query = domain if query is None else _f32(query)
         D = domain.shape[1]
         nq = query.shape[0]
-        idx = np.zeros(nq * k, np.uint32)
-        W = np.zeros(nq * k * D)
+        size = nq if int(override_query_size) == 0 else int(override_query_size)
+        if size < nq:
+            raise ValueError("naga::nonlocal_calculus::divergence_operator::override_query_size "
+                             "must be at least as large as the number of query points.")
+        idx = np.zeros(size * k, np.uint32)
+        W = np.zeros(size * k * D)
         _check(load().naga_divergence_weights(_ptr(domain), domain.shape[0], _ptr(query), nq,
                                               D, int(k), int(rbf), _ptr(idx), _ptr(W)),
                "naga::nonlocal_calculus::divergence_operator::")
-        return cls(W.reshape(nq, k, D), idx.reshape(nq, k))
+        return cls(W.reshape(size, k, D), idx.reshape(size, k))
 
     def weights(self):
         return self._W
@@ -1015,3 +1252,57 @@ class DensityObserver:
             self.engine.step_forward(1)
             self.density_values.append(float(self.engine.sample_probes(self.set)[0]))
             self.time_values.append(t * self.time_multiplier)
+
+    def update(self, time, domain=None, parameters=None, solution=None):
+        """simulation_observer::update (simulation_observer.cuh:41-55), as an
+        attached observer: the density of the state at the step's start."""
+        self.density_values.append(float(self.engine.sample_probes(self.set, fresh=True)[0]))
+        self.time_values.append(float(time) * self.time_multiplier)
+
+
+class VtkObserver:
+    """vtk_observer<Lattice> (simulation_observers/vtk_observer.cuh:45-431):
+    as an attached observer, writes output_directory/result.<frame>.vtp with
+    the state at each step's start.  frame = floor(time_multiplier * t *
+    frame_rate) when frame_rate != 0 (a frame already written is skipped),
+    else ceil(t / dt)."""
+
+    def __init__(self, engine: "SimulationEngine", output_directory, frame_rate: float = 0.0,
+                 time_multiplier: float = 1.0):
+        self.engine = engine
+        self.output_directory = str(output_directory)
+        self.frame_rate = float(frame_rate)
+        self.time_multiplier = float(time_multiplier)
+        self.current_frame = 0
+        self.files = []
+        os.makedirs(self.output_directory, exist_ok=True)
+
+    def update(self, time, domain=None, parameters=None, solution=None):
+        if self.frame_rate != 0.0:
+            frame = int(self.time_multiplier * time * self.frame_rate)
+            if frame < self.current_frame:
+                return
+        else:
+            dt = self.engine._params["dt"]
+            frame = int(np.ceil(time / dt - 1e-9))
+        self.current_frame = frame + 1
+        path = os.path.join(self.output_directory, f"result.{frame}.vtp")
+        self.engine.write_vtp(path, fresh=True)
+        self.files.append(path)
+
+
+def read_vtp(path) -> dict:
+    """Point arrays of a .vtp file written by write_vtp (inline binary,
+    UInt32 headers): name -> array [n] or [n, components]."""
+    import base64
+    import xml.etree.ElementTree as ET
+    root = ET.parse(path).getroot()
+    out = {}
+    types = {"Float32": np.float32, "Int32": np.int32}
+    for da in root.iter("DataArray"):
+        raw = base64.b64decode("".join(da.text.split()))
+        nbytes = int(np.frombuffer(raw[:4], np.uint32)[0])
+        v = np.frombuffer(raw[4:4 + nbytes], types[da.get("type")])
+        c = int(da.get("NumberOfComponents", "1"))
+        out[da.get("Name")] = v.reshape(-1, c) if c > 1 else v
+    return out

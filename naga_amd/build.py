@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnaga_amd.so")
 ARCH = os.environ.get("NAGA_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["step_kernels.hip", "setup_kernels.hip", "calculus.hip", "engine.cpp",
+SOURCES = ["step_kernels.hip", "setup_kernels.hip", "calculus.hip", "operators.hip", "engine.cpp",
            "server.cpp", "capi.cpp"]
 
 COMMON = [

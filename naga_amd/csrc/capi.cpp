@@ -10,6 +10,7 @@
 #include "../../include/naga_amd.h"
 #include "calculus.hpp"
 #include "engine.hpp"
+#include "operators.hpp"
 #include "server.hpp"
 #include "setup_kernels.hpp"
 
@@ -33,6 +34,14 @@ struct naga_advection {
 
 struct naga_csr {
     std::unique_ptr<Csr> impl;
+};
+
+struct naga_segmentation {
+    std::unique_ptr<Segmentation> impl;
+};
+
+struct naga_rpim {
+    std::unique_ptr<Rpim> impl;
 };
 
 namespace {
@@ -408,6 +417,18 @@ int naga_engine_info(naga_engine* e, int64_t* n_nodes, double* time, int64_t* fr
     });
 }
 
+int naga_engine_write_vtp(naga_engine* e, const char* path, int32_t fresh) {
+    return guarded([&] { engine_of(e).write_vtp(path, fresh != 0); });
+}
+
+int naga_engine_setup_phases(naga_engine* e, double* ms) {
+    return guarded([&] {
+        require(ms != nullptr, "null argument");
+        const double* p = engine_of(e).setup_phase_ms();
+        for (int i = 0; i < Engine::kSetupPhases; ++i) ms[i] = p[i];
+    });
+}
+
 int64_t naga_uniform_grid_size(int32_t dims, const double* lo, const double* hi, double h) {
     if ((dims != 2 && dims != 3) || !lo || !hi || !(h > 0)) return -1;
     int64_t d[3];
@@ -759,26 +780,213 @@ int naga_knn(const float* ref, int64_t n_ref, const float* query, int64_t n_q, i
     });
 }
 
-int naga_batched_lu_solve_f64(const double* A, int32_t n, int64_t batch, const double* B,
-                              int32_t nrhs, double* X, int32_t* info) {
+int naga_batched_matrix_inverse_f64(const double* A, int32_t n, int64_t batch, double* A_inv,
+                                    int32_t* info) {
     return guarded([&] {
-        require(A && B && X, "null argument");
+        require(A && A_inv, "null argument");
         require(n >= 1 && n <= 64, "n must be in [1, 64]");
-        require(nrhs >= 1 && batch >= 0, "bad nrhs or batch");
+        require(batch >= 0 && batch < int64_t(INT32_MAX),
+                "Batch size must be less than std::numeric_limits<int>::max()");
         hipStream_t s;
         NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        DeviceBuffer<double> dA(size_t(batch) * n * n), dB(size_t(batch) * n * nrhs);
-        DeviceBuffer<int> dinfo(batch);
-        dA.upload(A, dA.size(), s);
-        dB.upload(B, dB.size(), s);
-        launch_batched_lu_solve(n, nrhs, batch, dA.get(), dB.get(), dinfo.get(), s);
-        dB.download(X, dB.size(), s);
-        std::vector<int> hi(batch);
-        dinfo.download(hi.data(), batch, s);
-        NAGA_HIP(hipStreamSynchronize(s));
+        try {
+            // A X = I per item, by the batched LU solve
+            std::vector<double> eye(size_t(batch) * n * n, 0.0);
+            for (int64_t b = 0; b < batch; ++b)
+                for (int i = 0; i < n; ++i) eye[size_t(b) * n * n + size_t(i) * n + i] = 1.0;
+            DeviceBuffer<double> dA(size_t(batch) * n * n), dB(size_t(batch) * n * n);
+            DeviceBuffer<int> dinfo(std::max<int64_t>(batch, 1));
+            dA.upload(A, dA.size(), s);
+            dB.upload(eye.data(), dB.size(), s);
+            launch_batched_lu_solve(n, n, batch, dA.get(), dB.get(), dinfo.get(), s);
+            dB.download(A_inv, dB.size(), s);
+            std::vector<int> hi(batch);
+            if (batch) dinfo.download(hi.data(), batch, s);
+            NAGA_HIP(hipStreamSynchronize(s));
+            for (int64_t b = 0; b < batch; ++b)
+                if (info) info[b] = hi[b];
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
         (void)hipStreamDestroy(s);
-        for (int64_t b = 0; b < batch; ++b)
-            if (info) info[b] = hi[b];
+    });
+}
+
+int64_t naga_rpim_scratchpad_size(int64_t query_size, int32_t support_size, int32_t dims,
+                                  int32_t group_size, int32_t bytes_per_value) {
+    if (query_size < 0 || support_size < 1 || dims < 1 || group_size < 1 ||
+        (bytes_per_value != 4 && bytes_per_value != 8) || query_size % group_size != 0) {
+        g_last_error = "query_size must be a multiple of group_size";
+        return -1;
+    }
+    // the per-group memory of interpolation/detail/radial_point_method.cuh:
+    // 38-90: G0 and its inverse, G_x per query, info and pivots
+    const int64_t n = int64_t(support_size) + dims + 1;
+    const int64_t per_group = 2 * n * n * bytes_per_value + n * bytes_per_value * group_size +
+                              4 * int64_t(group_size) + 4 * n * group_size;
+    g_last_error.clear();
+    return query_size / group_size * per_group;
+}
+
+int naga_rpim_create(const float* src, int64_t n_src, const uint32_t* idx, int32_t k,
+                     const float* query, int64_t n_q, int32_t dims, int32_t group_size,
+                     int32_t rbf, double spacing, naga_rpim** out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        *out = nullptr;
+        auto r = std::make_unique<naga_rpim>();
+        r->impl = std::make_unique<Rpim>(src, n_src, idx, k, query, n_q, dims, group_size, rbf,
+                                         spacing, 64);
+        *out = r.release();
+    });
+}
+
+int naga_rpim_destroy(naga_rpim* r) {
+    return guarded([&] { delete r; });
+}
+
+int naga_rpim_interpolate(naga_rpim* r, const float* field, int64_t n_field, int32_t m,
+                          float* dest, float centering_offset) {
+    return guarded([&] {
+        require(r != nullptr && r->impl, "null interpolator");
+        require(n_field == r->impl->sources(),
+                "field array must have the same number of rows as the number of source "
+                "points used to construct the interpolator");
+        r->impl->interpolate(field, m, dest, centering_offset);
+    });
+}
+
+int naga_rpim_get_weights(naga_rpim* r, float* w) {
+    return guarded([&] {
+        require(r != nullptr && r->impl, "null interpolator");
+        r->impl->weights(w);
+    });
+}
+
+int naga_segmentation_create(const float* points, int64_t n, int32_t dims,
+                             int32_t approx_partition_size, double partition_width,
+                             naga_segmentation** out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        *out = nullptr;
+        auto s = std::make_unique<naga_segmentation>();
+        s->impl = std::make_unique<Segmentation>(points, n, dims, approx_partition_size,
+                                                 partition_width);
+        *out = s.release();
+    });
+}
+
+int naga_segmentation_destroy(naga_segmentation* s) {
+    return guarded([&] { delete s; });
+}
+
+int naga_segmentation_shape(naga_segmentation* s, int64_t* shape, double* lower_bounds,
+                            double* partition_width, int64_t* partition_count) {
+    return guarded([&] {
+        require(s != nullptr && s->impl, "null segmentation");
+        const CellGrid& g = s->impl->grid();
+        for (int d = 0; d < 3; ++d) {
+            if (shape) shape[d] = g.dims[d];
+            if (lower_bounds) lower_bounds[d] = g.lo[d];
+        }
+        if (partition_width) *partition_width = g.w;
+        if (partition_count) *partition_count = g.ncell;
+    });
+}
+
+int naga_segmentation_partitions(naga_segmentation* s, uint32_t* sizes, uint32_t* offsets,
+                                 uint32_t* indices) {
+    return guarded([&] {
+        require(s != nullptr && s->impl, "null segmentation");
+        s->impl->partitions(sizes, offsets, indices);
+    });
+}
+
+int naga_segmentation_knn(naga_segmentation* s, const float* query, int64_t n_q, int32_t k,
+                          float* d2_out, uint32_t* idx_out) {
+    return guarded([&] {
+        require(s != nullptr && s->impl, "null segmentation");
+        s->impl->knn(query, n_q, k, d2_out, idx_out);
+    });
+}
+
+int naga_rk4_stage(int32_t q, int64_t n, int64_t n_fluid, int32_t k, const uint32_t* idx,
+                   const float* W, double dt, int32_t stage, const float* y, const float* in,
+                   const float* y1, const float* y2, float* out) {
+    return guarded([&] {
+        require(q == 9 || q == 27, "lattice must be D2Q9 (9) or D3Q27 (27)");
+        require(stage >= 1 && stage <= 4, "stage must be 1, 2, 3 or 4");
+        require(n >= 1 && n_fluid >= 0 && n_fluid <= n && n < (int64_t(1) << 31),
+                "need 0 <= n_fluid <= n < 2^31");
+        require(k >= 1 && k <= 64, "k must be in [1, 64]");
+        require(y && out && (stage == 1 || in) && (stage != 4 || (y1 && y2)),
+                "null argument");
+        require(n_fluid == 0 || (idx && W), "null stencil");
+        for (int64_t t = 0; t < n_fluid * k; ++t) require(idx[t] < n, "index out of range");
+        const int D = q == 9 ? 2 : 3, RS = q == 9 ? 9 : 32;
+        hipStream_t s;
+        NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            auto rows = [&](const float* src) {  // [n][q] -> [n][RS]
+                std::vector<float> r(size_t(n) * RS, 0.0f);
+                for (int64_t i = 0; i < n; ++i)
+                    std::memcpy(&r[size_t(i) * RS], src + size_t(i) * q, sizeof(float) * q);
+                return r;
+            };
+            DeviceBuffer<float> g(size_t(n) * RS), A(size_t(n) * RS), B(size_t(n) * RS),
+                C(size_t(n) * RS);
+            const std::vector<float> hy = rows(y);
+            // every buffer starts as y: rows past n_fluid keep it (zero rate)
+            for (DeviceBuffer<float>* b : {&g, &A, &B, &C}) b->upload(hy.data(), hy.size(), s);
+            if (stage >= 2) {
+                const std::vector<float> hin = rows(in);
+                (stage == 2 ? A : stage == 3 ? B : C).upload(hin.data(), hin.size(), s);
+            }
+            if (stage == 4) {
+                const std::vector<float> h1 = rows(y1), h2 = rows(y2);
+                A.upload(h1.data(), h1.size(), s);
+                B.upload(h2.data(), h2.size(), s);
+            }
+            DeviceBuffer<uint32_t> didx(std::max<int64_t>(n_fluid * k, 1));
+            DeviceBuffer<float> dW(std::max<int64_t>(n_fluid * k * D, 1));
+            DeviceBuffer<float> slab(std::max<int64_t>(n_fluid * k * (1 + D), 1));
+            if (n_fluid) {
+                didx.upload(idx, size_t(n_fluid) * k, s);
+                dW.upload(W, size_t(n_fluid) * k * D, s);
+                launch_make_slab(n_fluid, k, D, didx.get(), dW.get(), slab.get(), s);
+            }
+            StepGeom G{};
+            G.N = n;
+            G.nbulk = n_fluid;
+            G.nlayer = 0;
+            G.nfluid = n_fluid;
+            G.k = k;
+            G.kb = 1;
+            G.dt = float(dt);
+            G.rho0 = 1.0f;
+            G.vel_scale = 1.0f;
+            StepBuffers S{};
+            S.g = g.get();
+            S.A = A.get();
+            S.B = B.get();
+            S.acc = C.get();
+            S.slab = slab.get();
+            if (q == 27)
+                launch_rk4_stage<27>(stage, G, S, s);
+            else
+                launch_rk4_stage<9>(stage, G, S, s);
+            std::vector<float> res(size_t(n) * RS);
+            (stage == 1 ? A : stage == 2 ? B : stage == 3 ? C : g).download(res.data(), res.size(),
+                                                                          s);
+            NAGA_HIP(hipStreamSynchronize(s));
+            for (int64_t i = 0; i < n; ++i)
+                std::memcpy(out + size_t(i) * q, &res[size_t(i) * RS], sizeof(float) * q);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
     });
 }
 

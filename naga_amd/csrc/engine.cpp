@@ -36,6 +36,30 @@ void capture_guarded(hipStream_t st, F&& body) {
     }
 }
 
+// Events recorded on a stream at phase boundaries; read() gives the elapsed
+// milliseconds between consecutive marks once the stream has reached them.
+struct PhaseMarks {
+    hipStream_t st;
+    std::vector<hipEvent_t> ev;
+    explicit PhaseMarks(hipStream_t s) : st(s) { mark(); }
+    ~PhaseMarks() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    void mark() {
+        hipEvent_t e = nullptr;
+        NAGA_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+        NAGA_HIP(hipEventRecord(e, st));
+    }
+    void read(double* ms, int n) const {
+        for (int i = 0; i < n; ++i) {
+            float v = 0.0f;
+            if (i + 1 < int(ev.size())) NAGA_HIP(hipEventElapsedTime(&v, ev[i], ev[i + 1]));
+            ms[i] = double(v);
+        }
+    }
+};
+
 struct Timer {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double ms() const {
@@ -130,9 +154,14 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     ready_ = false;
     hipStream_t st = stream_;
 
+    // phase marks on the stream: setup_phase_ms_ reports the wall time
+    // between consecutive marks
+    PhaseMarks marks(st);
     pts_.allocate(N * D);
     pts_.upload(pts, N * D, st);
     hpts_.assign(pts, pts + N * D);
+    hnormals_.assign(normals ? normals : pts, normals ? normals + nbnd * D : pts);
+    hsigma_.assign(sigma ? sigma : pts, sigma ? sigma + nlayer : pts);
     probes_.clear();
     sources_.clear();
     n_snodes_ = 0;
@@ -143,6 +172,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         DeviceBuffer<float> d2(N * k);
         idx_.allocate(N * k);
         launch_knn(cells.bins, pts_.get(), pts_.get(), N, k, d2.get(), idx_.get(), st);
+        marks.mark();  // binning and kNN done
         // multiquadric shape parameter: mean nearest distance per 16384-query
         // batch (nonlocal_calculus/divergence.cuh:118-160, operators.cuh:96-117)
         const int64_t nbatch = (N + 16383) / 16384;
@@ -198,6 +228,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
                 fail(kNumericalError, "singular RBF system while building the divergence "
                                       "operator at node " + std::to_string(i));
     }
+    marks.mark();  // divergence weights done
     filt_.allocate(N * k);
     launch_filter_weights(D, pts_.get(), N, idx_.get(), k, cfg_.h, filt_.get(), st, kvar_.get());
     slab_.allocate(nfluid * k * (1 + D));
@@ -303,6 +334,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         ucap_ = std::min(maxu, cap);
     }
 
+    marks.mark();  // filter weights, slabs and tile unions done
     // boundary interpolator (init_domain :223-258): kNN of boundary nodes
     // among the bulk and layer nodes, RPIM weights at spacing h
     if (nbnd > 0) {
@@ -353,6 +385,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         normals_.allocate(nbnd * D);
         normals_.upload(normals, nbnd * D, st);
     }
+    marks.mark();  // boundary interpolator done
     // fluid nodes whose stencil reaches the absorbing layer
     n_pml_ = 0;
     n_ptiles_ = 0;
@@ -436,8 +469,10 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     time_ = 0.0;
     frame_ = 0;
     refresh_macroscopic_state();
+    marks.mark();  // PML lists and state done
     NAGA_HIP(hipStreamSynchronize(st));
     setup_ms_ = timer.ms();
+    marks.read(setup_phase_ms_, kSetupPhases);
 }
 
 StepBuffers Engine::buffers() {
@@ -1124,6 +1159,108 @@ void Engine::get_setup(uint32_t* idx, float* W, float* filt, uint32_t* bidx, flo
     if (bidx && nbnd_) bidx_.download(bidx, nbnd_ * kb, stream_);
     if (bw && nbnd_) bw_.download(bw, nbnd_ * kb, stream_);
     NAGA_HIP(hipStreamSynchronize(stream_));
+}
+
+namespace {
+// RFC 4648 base64 of n bytes.
+void base64_append(std::string& out, const unsigned char* p, size_t n) {
+    static const char tab[] = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+    size_t i = 0;
+    for (; i + 2 < n; i += 3) {
+        const uint32_t v = (uint32_t(p[i]) << 16) | (uint32_t(p[i + 1]) << 8) | p[i + 2];
+        out += tab[v >> 18];
+        out += tab[(v >> 12) & 63];
+        out += tab[(v >> 6) & 63];
+        out += tab[v & 63];
+    }
+    if (i < n) {
+        uint32_t v = uint32_t(p[i]) << 16;
+        if (i + 1 < n) v |= uint32_t(p[i + 1]) << 8;
+        out += tab[v >> 18];
+        out += tab[(v >> 12) & 63];
+        out += i + 1 < n ? tab[(v >> 6) & 63] : '=';
+        out += '=';
+    }
+}
+
+// One inline binary DataArray: a UInt32 byte count, then the values, both
+// base64-encoded in one block.
+template <class T>
+void vtp_array(std::string& out, const char* type, const char* name, int comps,
+               const std::vector<T>& v) {
+    out += "        <DataArray type=\"";
+    out += type;
+    out += "\" Name=\"";
+    out += name;
+    out += "\" NumberOfComponents=\"" + std::to_string(comps) + "\" format=\"binary\">\n          ";
+    std::vector<unsigned char> buf(4 + v.size() * sizeof(T));
+    const uint32_t bytes = uint32_t(v.size() * sizeof(T));
+    std::memcpy(buf.data(), &bytes, 4);
+    if (!v.empty()) std::memcpy(buf.data() + 4, v.data(), v.size() * sizeof(T));
+    base64_append(out, buf.data(), buf.size());
+    out += "\n        </DataArray>\n";
+}
+}  // namespace
+
+// The output of vtk_observer::update_all (simulation_observers/
+// vtk_observer.cuh:99-247) for this scene: one VTK XML PolyData file, the
+// nodes as vertices, point arrays f (f_alpha / w_alpha), density, velocity
+// (3 components), absorption, normals (3), type (0 bulk, 1 layer, 2
+// boundary); FP32 values, inline binary, uncompressed.
+void Engine::write_vtp(const char* path, bool fresh) {
+    require_domain();
+    require(path != nullptr && path[0] != 0, "empty output path");
+    const int Q = cfg_.Q, D = D_;
+    const int64_t N = N_;
+    std::vector<float> rho(N), u(size_t(N) * D);
+    std::vector<double> f(size_t(N) * Q);
+    get_macroscopic(rho.data(), u.data(), nullptr, fresh);
+    get_distributions(f.data(), nullptr);
+    std::vector<float> pts(size_t(N) * 3, 0.0f), fw(size_t(N) * Q), vel(size_t(N) * 3, 0.0f),
+        nrm(size_t(N) * 3, 0.0f), absorb(N, 0.0f);
+    std::vector<int32_t> type(N, 0), conn(N), offs(N);
+    for (int64_t i = 0; i < N; ++i) {
+        for (int d = 0; d < D; ++d) {
+            pts[size_t(i) * 3 + d] = hpts_[size_t(i) * D + d];
+            vel[size_t(i) * 3 + d] = u[size_t(d) * N + i];
+        }
+        for (int a = 0; a < Q; ++a) fw[size_t(i) * Q + a] = float(f[size_t(a) * N + i] / weight(Q, a));
+        if (i >= nfluid_) {
+            type[i] = 2;
+            for (int d = 0; d < D; ++d)
+                nrm[size_t(i) * 3 + d] = hnormals_[size_t(i - nfluid_) * D + d];
+        } else if (i >= nbulk_) {
+            type[i] = 1;
+            absorb[i] = hsigma_[size_t(i - nbulk_)];
+        }
+        conn[i] = int32_t(i);
+        offs[i] = int32_t(i + 1);
+    }
+    std::string x;
+    x.reserve(size_t(N) * (Q + 12) * 6 + 4096);
+    x += "<?xml version=\"1.0\"?>\n";
+    x += "<VTKFile type=\"PolyData\" version=\"0.1\" byte_order=\"LittleEndian\" "
+         "header_type=\"UInt32\">\n  <PolyData>\n";
+    x += "    <Piece NumberOfPoints=\"" + std::to_string(N) + "\" NumberOfVerts=\"" +
+         std::to_string(N) + "\" NumberOfLines=\"0\" NumberOfStrips=\"0\" NumberOfPolys=\"0\">\n";
+    x += "      <PointData>\n";
+    vtp_array(x, "Float32", "f", Q, fw);
+    vtp_array(x, "Float32", "density", 1, rho);
+    vtp_array(x, "Float32", "velocity", 3, vel);
+    vtp_array(x, "Float32", "absorption", 1, absorb);
+    vtp_array(x, "Float32", "normals", 3, nrm);
+    vtp_array(x, "Int32", "type", 1, type);
+    x += "      </PointData>\n      <Points>\n";
+    vtp_array(x, "Float32", "Points", 3, pts);
+    x += "      </Points>\n      <Verts>\n";
+    vtp_array(x, "Int32", "connectivity", 1, conn);
+    vtp_array(x, "Int32", "offsets", 1, offs);
+    x += "      </Verts>\n    </Piece>\n  </PolyData>\n</VTKFile>\n";
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) fail(kInvalidArgument, std::string("cannot open ") + path + " for writing");
+    const size_t wrote = std::fwrite(x.data(), 1, x.size(), fp);
+    const int closed = std::fclose(fp);
+    if (wrote != x.size() || closed != 0) fail(kDeviceError, std::string("short write to ") + path);
 }
 
 // feq_a - w_a at ru = (rho - 1, u), as the PML kernels form it.

@@ -76,6 +76,9 @@ class Engine {
     void save_state(void* buf, int64_t capacity);
     void load_state(const void* buf, int64_t bytes);
     void clear_density_sources();
+    // The vtk_observer output of the current scene state (fresh: the state
+    // now; else the macroscopic values held from the last step's start).
+    void write_vtp(const char* path, bool fresh);
 
     // Descriptor of this scene for batched launches.
     SceneDesc scene_desc();
@@ -92,6 +95,11 @@ class Engine {
     double time() const { return time_; }
     int64_t frame() const { return frame_; }
     double setup_ms() const { return setup_ms_; }
+    // Setup phases (ms on the engine's stream): binning and kNN, divergence
+    // weights, filter weights + slabs + tile unions, boundary interpolator,
+    // PML lists and state.
+    static constexpr int kSetupPhases = 5;
+    const double* setup_phase_ms() const { return setup_phase_ms_; }
     int lattice() const { return cfg_.Q; }
     int dims() const { return D_; }
     hipStream_t stream() const { return stream_; }
@@ -116,6 +124,7 @@ class Engine {
     StepGeom geo_{};
     CollisionParams cp_{};
     double time_ = 0.0, setup_ms_ = 0.0;
+    double setup_phase_ms_[kSetupPhases] = {};
     int64_t frame_ = 0;
     bool ready_ = false;
 
@@ -169,6 +178,7 @@ class Engine {
     std::vector<std::unique_ptr<ProbeSet>> probes_;
 
     std::vector<float> hpts_;  // host copy of the node coordinates
+    std::vector<float> hnormals_, hsigma_;  // host copies of the normals and absorption
     std::vector<DensitySource> sources_;
     DeviceBuffer<int32_t> snodes_, soff_, sids_;
     DeviceBuffer<float> sprof_, src_field_;

@@ -410,14 +410,14 @@ template <int NMAX, int D, class T>
 __global__ __launch_bounds__(64) void k_rpim(const float* src, const float* query,
                                              int64_t n_q, const uint32_t* idx,
                                              int k, int rbf, double spacing,
-                                             float* w, double* w64, int* info) {
+                                             float* w, double* w64, int* info, int group) {
     __shared__ double xs[64][3];
     const int lane = threadIdx.x;
     const int64_t i = blockIdx.x;
     if (i >= n_q) return;
     const int n = k + D + 1;
-    if (lane < k) {
-        const int64_t j = idx[i * k + lane];
+    if (lane < k) {  // the queries of a group share their group's stencil
+        const int64_t j = idx[(i / group) * k + lane];
         for (int d = 0; d < D; ++d) xs[lane][d] = double(src[j * D + d]);
     }
     __syncthreads();
@@ -982,11 +982,11 @@ void launch_divw_d(int nmax, const float* src, const float* query, int64_t n_q,
 template <int D, class T>
 void launch_rpim_d(int nmax, const float* src, const float* query, int64_t n_q,
                    const uint32_t* idx, int k, int rbf, double spacing, float* w,
-                   double* w64, int* info, hipStream_t st) {
+                   double* w64, int* info, hipStream_t st, int group) {
 #define NAGA_RPIM(NM)                                                                  \
     case NM:                                                                           \
         k_rpim<NM, D, T><<<n_q, 64, 0, st>>>(src, query, n_q, idx, k, rbf, spacing, w, \
-                                              w64, info);                              \
+                                              w64, info, group);                       \
         break;
     switch (nmax) {
         NAGA_RPIM(20) NAGA_RPIM(28) NAGA_RPIM(36) NAGA_RPIM(44) NAGA_RPIM(52) NAGA_RPIM(64)
@@ -1096,20 +1096,22 @@ void launch_divergence_weights(int D, const float* src, const float* query,
 
 void launch_rpim_weights(int D, const float* src, const float* query, int64_t n_q,
                          const uint32_t* idx, int k, int rbf, double spacing,
-                         float* w, double* w64, int* info, hipStream_t st, int solve_bits) {
+                         float* w, double* w64, int* info, hipStream_t st, int solve_bits,
+                         int group) {
     const int nmax = pick_nmax(k + D + 1);
     require(nmax > 0, "stencil too wide (k + D + 1 must be <= 64)");
+    require(group >= 1, "group size must be >= 1");
     require(solve_bits == 64 || solve_bits == 32, "solve precision must be 64 or 32 bits");
     if (n_q == 0) return;
     const bool f32 = solve_bits == 32;
     if (D == 2 && f32)
-        launch_rpim_d<2, float>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+        launch_rpim_d<2, float>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st, group);
     else if (D == 2)
-        launch_rpim_d<2, double>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+        launch_rpim_d<2, double>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st, group);
     else if (f32)
-        launch_rpim_d<3, float>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+        launch_rpim_d<3, float>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st, group);
     else
-        launch_rpim_d<3, double>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st);
+        launch_rpim_d<3, double>(nmax, src, query, n_q, idx, k, rbf, spacing, w, w64, info, st, group);
     hip_check(hipGetLastError(), "rpim weights");
 }
 

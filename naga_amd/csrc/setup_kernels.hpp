@@ -107,14 +107,16 @@ void launch_divergence_weights(int D, const float* src, const float* query,
                                int* info, hipStream_t stream,
                                const uint8_t* kvar = nullptr, int solve_bits = 64);
 
-// RPIM interpolation weights, group size 1: w[q*k + s].
+// RPIM interpolation weights: w[q*k + s]; the queries of a group of `group`
+// consecutive queries share the stencil idx[(q / group)*k ..]
+// (radial_point_method::compute_weights' group_size).
 // solve_bits (both weight kernels): 64 solves the (k + D + 1)^2 systems in
 // FP64; 32 rounds them to FP32 and solves in FP32, the reference's solve
 // precision (linalg/batched_matrix_inverse.cuh:229-250).
 void launch_rpim_weights(int D, const float* src, const float* query,
                          int64_t n_q, const uint32_t* idx, int k, int rbf,
                          double spacing, float* w, double* w64, int* info,
-                         hipStream_t stream, int solve_bits = 64);
+                         hipStream_t stream, int solve_bits = 64, int group = 1);
 
 // Normalised Gaussian filter weights over the stencil (sigma_g = h/2.5).
 void launch_filter_weights(int D, const float* pts, int64_t n,

@@ -21,7 +21,9 @@
 // reference's "naga::" prefixes.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <future>
 #include <memory>
@@ -188,6 +190,21 @@ struct density_source_spec {
     double t_start = 0.0, t_end = 1e300;
 };
 
+// simulation_observer<Lattice> (fluids/nonlocal_lattice_boltzmann/
+// simulation_observer.cuh:41-55): update() is called at the start of every
+// step with the time and the state of that moment.
+template <class Lattice>
+class simulation_observer {
+  public:
+    using value_type = typename lattice_traits<Lattice>::value_type;
+    using simulation_domain_t = simulation_nodes<value_type>;
+    using problem_parameters_t = ::naga::fluids::nonlocal_lbm::problem_parameters<value_type>;
+    using solution_t = state_variables<Lattice>;
+    virtual ~simulation_observer() = default;
+    virtual void update(const value_type& time, const simulation_domain_t& domain,
+                        const problem_parameters_t& parameters, const solution_t& solution) = 0;
+};
+
 template <class Lattice>
 class simulation_engine {
   public:
@@ -272,10 +289,33 @@ class simulation_engine {
     }
 
     void step_forward() { step_forward(1); }
+    // With observers attached every step first hands them the state at its
+    // start (update_observers, detail/simulation_engine.cuh:1332, 1444-1448).
     void step_forward(int nsteps) {
-        abi::check(naga_engine_step(live(), nsteps),
-                   "naga::nonlocal_lbm::simulation_engine::step_forward: ");
-        stale_ = true;
+        if (observers_.empty()) {
+            abi::check(naga_engine_step(live(), nsteps),
+                       "naga::nonlocal_lbm::simulation_engine::step_forward: ");
+            stale_ = true;
+            return;
+        }
+        for (int n = 0; n < nsteps; ++n) {
+            const solution_t& sol = fresh_solution();
+            const value_type t = time();
+            for (simulation_observer<Lattice>* o : observers_)
+                o->update(t, domain_, params_, sol);
+            abi::check(naga_engine_step(live(), 1),
+                       "naga::nonlocal_lbm::simulation_engine::step_forward: ");
+            stale_ = true;
+        }
+    }
+
+    // attach_observer / detach_observer (simulation_engine.cuh:90-96)
+    void attach_observer(simulation_observer<Lattice>& observer) {
+        observers_.push_back(&observer);
+    }
+    void detach_observer(simulation_observer<Lattice>& observer) {
+        observers_.erase(std::remove(observers_.begin(), observers_.end(), &observer),
+                         observers_.end());
     }
 
     // f = w everywhere, time 0 (detail/simulation_engine.cuh:1377-1386)
@@ -299,12 +339,20 @@ class simulation_engine {
 
     // Copies the state out of the engine (the values the reference's
     // solution() holds: macroscopic values from the start of the last step).
-    const solution_t& solution() const {
-        if (stale_) {
+    const solution_t& solution() const { return load_solution(false); }
+
+  private:
+    // the solution with fresh macroscopic values of the current state (what
+    // the reference's step start computes before updating the observers)
+    const solution_t& fresh_solution() const { return load_solution(true); }
+
+    const solution_t& load_solution(bool fresh) const {
+        if (stale_ || fresh) {
             const size_t N = domain_.points.size() / dimensions;
             std::vector<float> rho(N), u(N * dimensions);
             std::vector<double> f(N * lattice_size);
-            abi::check(naga_engine_get_macroscopic(live(), rho.data(), u.data(), nullptr, 0),
+            abi::check(naga_engine_get_macroscopic(live(), rho.data(), u.data(), nullptr,
+                                                   fresh ? 1 : 0),
                        "naga::nonlocal_lbm::simulation_engine::solution: ");
             abi::check(naga_engine_get_distributions(live(), f.data(), nullptr),
                        "naga::nonlocal_lbm::simulation_engine::solution: ");
@@ -313,9 +361,16 @@ class simulation_engine {
             for (unsigned a = 0; a < lattice_size; ++a)
                 solution_.lattice_distributions[a].assign(f.begin() + a * N,
                                                           f.begin() + (a + 1) * N);
-            stale_ = false;
+            stale_ = fresh;  // a fresh copy is not what solution() holds
         }
         return solution_;
+    }
+
+  public:
+    // The vtk_observer output of the current state (naga_engine_write_vtp).
+    void write_vtp(const std::string& path, bool fresh = true) const {
+        abi::check(naga_engine_write_vtp(live(), path.c_str(), fresh ? 1 : 0),
+                   "naga::nonlocal_lbm::vtk_observer::update: ");
     }
 
     void register_density_source(const density_source_spec& src) {
@@ -352,6 +407,7 @@ class simulation_engine {
         return handle_.get();
     }
 
+    std::vector<simulation_observer<Lattice>*> observers_;
     stencil_config stencil_;
     problem_parameters_t params_{};
     bool have_params_ = false;
@@ -366,8 +422,12 @@ class simulation_engine {
 // with multiquadric RPIM weights; update() records the value held by the
 // engine (from the start of its last step) and the time of that step.
 template <class Lattice>
-class density_observer {
+class density_observer : public simulation_observer<Lattice> {
   public:
+    using base = simulation_observer<Lattice>;
+    using typename base::problem_parameters_t;
+    using typename base::simulation_domain_t;
+    using typename base::solution_t;
     using value_type = typename lattice_traits<Lattice>::value_type;
     static constexpr unsigned dimensions = lattice_traits<Lattice>::dimensions;
 
@@ -394,6 +454,16 @@ class density_observer {
         time_values_.push_back(value_type((t - dt) * time_multiplier_));
     }
 
+    // as an attached observer: the density of the state at the step's start
+    void update(const value_type& time, const simulation_domain_t&, const problem_parameters_t&,
+                const solution_t&) override {
+        double v = 0.0;
+        abi::check(naga_engine_sample_probes(engine_->native_handle(), set_, 1, &v),
+                   "naga::nonlocal_lbm::density_observer::update: ");
+        density_values_.push_back(value_type(v));
+        time_values_.push_back(value_type(time * time_multiplier_));
+    }
+
     const std::vector<value_type>& density_values() const { return density_values_; }
     const std::vector<value_type>& time_values() const { return time_values_; }
 
@@ -404,9 +474,113 @@ class density_observer {
     std::vector<value_type> density_values_, time_values_;
 };
 
+// vtk_observer<Lattice> (simulation_observers/vtk_observer.cuh:45-431):
+// writes output_directory/result.<frame>.vtp at each update, frame =
+// floor(time_multiplier * t * frame_rate) when frame_rate != 0 (frames
+// already written are skipped), else ceil(t / dt).
+template <class Lattice>
+class vtk_observer : public simulation_observer<Lattice> {
+  public:
+    using base = simulation_observer<Lattice>;
+    using typename base::problem_parameters_t;
+    using typename base::simulation_domain_t;
+    using typename base::solution_t;
+    using typename base::value_type;
+
+    vtk_observer(const simulation_engine<Lattice>& engine, std::string output_directory,
+                 value_type frame_rate = 0, value_type time_multiplier = 1)
+        : engine_(&engine), dir_(std::move(output_directory)), frame_rate_(frame_rate),
+          time_multiplier_(time_multiplier) {}
+
+    void update(const value_type& time, const simulation_domain_t&,
+                const problem_parameters_t& parameters, const solution_t&) override {
+        uint32_t frame;
+        if (frame_rate_ != 0) {
+            frame = uint32_t(time_multiplier_ * time * frame_rate_);
+            if (frame < current_frame_) return;
+        } else {
+            frame = uint32_t(std::ceil(double(time) / double(parameters.time_step) - 1e-9));
+        }
+        current_frame_ = frame + 1;
+        engine_->write_vtp(dir_ + "/result." + std::to_string(frame) + ".vtp", true);
+    }
+
+  private:
+    const simulation_engine<Lattice>* engine_;
+    std::string dir_;
+    value_type frame_rate_, time_multiplier_;
+    uint32_t current_frame_ = 0;
+};
+
 }  // namespace fluids::nonlocal_lbm
 
 namespace segmentation {
+// nd_cubic_segmentation<T, D> (segmentation/nd_cubic_segmentation.cuh:
+// 219-461), binned on the device: points are [i][d].
+template <class T, uint32_t Dimensions>
+class nd_cubic_segmentation {
+  public:
+    nd_cubic_segmentation(const std::vector<T>& points, uint32_t approx_partition_size)
+        : n_(points.size() / Dimensions) {
+        create(points, int32_t(approx_partition_size), 0.0);
+    }
+    nd_cubic_segmentation(const std::vector<T>& points, T partition_width)
+        : n_(points.size() / Dimensions) {
+        create(points, 0, double(partition_width));
+    }
+    std::array<uint32_t, Dimensions> shape() const { return shape_; }
+    size_t partition_count() const { return count_; }
+    size_t point_count() const { return n_; }
+    T partition_width() const { return T(width_); }
+    // partition sizes, index offsets and the indices of the points by cell
+    void partitions(std::vector<uint32_t>& sizes, std::vector<uint32_t>& offsets,
+                    std::vector<uint32_t>& indices) const {
+        sizes.resize(count_);
+        offsets.resize(count_);
+        indices.resize(n_);
+        abi::check(naga_segmentation_partitions(h_.get(), sizes.data(), offsets.data(),
+                                                indices.data()),
+                   "naga::segmentation::nd_cubic_segmentation: ");
+    }
+    naga_segmentation* handle() const { return h_.get(); }
+
+  private:
+    void create(const std::vector<T>& points, int32_t approx, double width) {
+        std::vector<float> p(points.begin(), points.end());
+        naga_segmentation* h = nullptr;
+        abi::check(naga_segmentation_create(p.data(), int64_t(n_), int32_t(Dimensions), approx,
+                                            width, &h),
+                   "naga::segmentation::nd_cubic_segmentation: ");
+        h_.reset(h, [](naga_segmentation* s) { naga_segmentation_destroy(s); });
+        int64_t shape[3];
+        int64_t count = 0;
+        abi::check(naga_segmentation_shape(h, shape, nullptr, &width_, &count),
+                   "naga::segmentation::nd_cubic_segmentation: ");
+        for (uint32_t d = 0; d < Dimensions; ++d) shape_[d] = uint32_t(shape[d]);
+        count_ = size_t(count);
+    }
+    std::shared_ptr<naga_segmentation> h_;
+    size_t n_ = 0, count_ = 0;
+    double width_ = 0.0;
+    std::array<uint32_t, Dimensions> shape_{};
+};
+
+// batched_nearest_neighbors(k, query_points, segmented_ref_points)
+// (segmentation/nearest_neighbors.cuh:454-477): d2 and idx come back
+// [query][k].
+template <class T, uint32_t Dimensions>
+void batched_nearest_neighbors(uint32_t k, const std::vector<T>& query,
+                               const nd_cubic_segmentation<T, Dimensions>& segmented,
+                               std::vector<float>& d2, std::vector<uint32_t>& idx) {
+    std::vector<float> q(query.begin(), query.end());
+    const int64_t nq = int64_t(q.size() / Dimensions);
+    d2.resize(size_t(nq) * k);
+    idx.resize(size_t(nq) * k);
+    abi::check(naga_segmentation_knn(segmented.handle(), q.data(), nq, int32_t(k), d2.data(),
+                                     idx.data()),
+               "naga::segmentation::batched_nearest_neighbors: ");
+}
+
 // Exact k nearest neighbours (k smallest (d^2, index)) of each query among
 // ref; points are [i][d].  d2 and idx come back [query][k].
 template <uint32_t Dimensions>
@@ -435,6 +609,25 @@ inline std::vector<int32_t> batched_lu_solve(int32_t n, int64_t batch,
     abi::check(naga_batched_lu_solve_f64(A.data(), n, batch, B.data(), nrhs, X.data(),
                                          info.data()),
                "naga::linalg::batched_matrix_inverse: ");
+    return info;
+}
+
+// batched_matrix_inverse(A, A_inv, copy_A) (linalg/batched_matrix_inverse.cuh:
+// 325-397): A and A_inv hold `batch` column-major n x n matrices; FP64
+// arithmetic for either T.  copy_A is accepted for the signature (A is never
+// modified here).  Returns the per-item pivot info (the reference leaves it
+// unchecked).
+template <class T>
+std::vector<int32_t> batched_matrix_inverse(int32_t n, int64_t batch, const std::vector<T>& A,
+                                            std::vector<T>& A_inv, bool copy_A = true) {
+    (void)copy_A;
+    if (A.size() != size_t(batch) * n * n)
+        throw std::invalid_argument("naga::linalg::Matrices must be square");
+    std::vector<double> a(A.begin(), A.end()), inv(A.size());
+    std::vector<int32_t> info(size_t(batch), 0);
+    abi::check(naga_batched_matrix_inverse_f64(a.data(), n, batch, inv.data(), info.data()),
+               "naga::linalg::");
+    A_inv.assign(inv.begin(), inv.end());
     return info;
 }
 
@@ -496,6 +689,71 @@ inline void matrix_mult(const sparse_csr_matrix& A, const std::vector<float>& x,
 }  // namespace linalg
 
 namespace interpolation {
+// radial_point_method<T> (interpolation/radial_point_method.cuh:92-280):
+// create_interpolator keeps the weights of each query over its group's
+// stencil on the device; interpolate applies them.  Points are [i][d];
+// interpolating_indices [group][k]; fields [point][component].
+template <class T>
+class radial_point_method {
+  public:
+    static size_t get_scratchpad_size(size_t query_size, uint32_t support_size,
+                                      uint32_t dimensions, uint32_t group_size = 1) {
+        const int64_t v = naga_rpim_scratchpad_size(int64_t(query_size), int32_t(support_size),
+                                                    int32_t(dimensions), int32_t(group_size),
+                                                    int32_t(sizeof(T)));
+        if (v < 0)
+            throw std::invalid_argument(
+                "naga::interpolation::radial_point_method::query_size must be a multiple of "
+                "group_size");
+        return size_t(v);
+    }
+
+    template <uint32_t Dimensions>
+    static radial_point_method create_interpolator(
+        const std::vector<T>& source_points, const std::vector<uint32_t>& interpolating_indices,
+        uint32_t support_size, const std::vector<T>& query_points, T approx_particle_spacing,
+        uint32_t group_size = 1, int rbf = NAGA_RBF_MULTIQUADRIC) {
+        std::vector<float> s(source_points.begin(), source_points.end()),
+            q(query_points.begin(), query_points.end());
+        radial_point_method r;
+        r.n_src_ = int64_t(s.size() / Dimensions);
+        r.n_q_ = int64_t(q.size() / Dimensions);
+        r.k_ = support_size;
+        naga_rpim* h = nullptr;
+        abi::check(naga_rpim_create(s.data(), r.n_src_, interpolating_indices.data(),
+                                    int32_t(support_size), q.data(), r.n_q_, int32_t(Dimensions),
+                                    int32_t(group_size), rbf, double(approx_particle_spacing),
+                                    &h),
+                   "naga::interpolation::radial_point_method::create_interpolator: ");
+        r.h_.reset(h, [](naga_rpim* p) { naga_rpim_destroy(p); });
+        return r;
+    }
+
+    // weights [query][k] (compute_weights)
+    std::vector<float> weights() const {
+        std::vector<float> w(size_t(n_q_) * k_);
+        abi::check(naga_rpim_get_weights(h_.get(), w.data()),
+                   "naga::interpolation::radial_point_method: ");
+        return w;
+    }
+
+    // field [source][components] -> destination [query][components]
+    void interpolate(const std::vector<T>& field, std::vector<T>& destination,
+                     uint32_t components = 1, T centering_offset = T(0)) const {
+        std::vector<float> f(field.begin(), field.end()), d(size_t(n_q_) * components);
+        abi::check(naga_rpim_interpolate(h_.get(), f.data(), int64_t(f.size() / components),
+                                         int32_t(components), d.data(),
+                                         float(centering_offset)),
+                   "naga::interpolation::radial_point_method::interpolate: ");
+        destination.assign(d.begin(), d.end());
+    }
+
+  private:
+    std::shared_ptr<naga_rpim> h_;
+    int64_t n_src_ = 0, n_q_ = 0;
+    uint32_t k_ = 0;
+};
+
 // radial_point_method<T>::compute_weights with group size 1: w[q][k].
 template <uint32_t Dimensions>
 std::vector<double> rpim_weights(const std::vector<float>& source,
@@ -519,12 +777,30 @@ class divergence_operator {
   public:
     static divergence_operator create(const std::vector<float>& domain, uint32_t k = 32,
                                       int rbf = NAGA_RBF_POLYHARMONIC3) {
+        return create(domain, domain, 0, k, rbf);
+    }
+
+    // create(domain, query_points, override_query_size)
+    // (nonlocal_calculus/divergence.cuh:53-166): stencils over the domain
+    // for each query; a non-zero override_query_size sizes the operator for
+    // that many queries, the rows past the query count zero.
+    static divergence_operator create(const std::vector<float>& domain,
+                                      const std::vector<float>& query_points,
+                                      size_t override_query_size, uint32_t k = 32,
+                                      int rbf = NAGA_RBF_POLYHARMONIC3) {
+        const int64_t nd = int64_t(domain.size() / Dimensions);
+        const int64_t nq = int64_t(query_points.size() / Dimensions);
+        const int64_t size = override_query_size == 0 ? nq : int64_t(override_query_size);
+        if (size < nq)
+            throw std::invalid_argument(
+                "naga::nonlocal_calculus::divergence_operator::override_query_size must be at "
+                "least as large as the number of query points.");
         divergence_operator op;
         op.k_ = k;
-        op.n_ = int64_t(domain.size() / Dimensions);
-        op.idx_.resize(size_t(op.n_) * k);
-        std::vector<double> W(size_t(op.n_) * k * Dimensions);
-        abi::check(naga_divergence_weights(domain.data(), op.n_, domain.data(), op.n_,
+        op.n_ = size;
+        op.idx_.assign(size_t(size) * k, 0u);
+        std::vector<double> W(size_t(size) * k * Dimensions, 0.0);
+        abi::check(naga_divergence_weights(domain.data(), nd, query_points.data(), nq,
                                            int32_t(Dimensions), int32_t(k), rbf,
                                            op.idx_.data(), W.data()),
                    "naga::nonlocal_calculus::divergence_operator::create: ");

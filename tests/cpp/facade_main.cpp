@@ -151,11 +151,100 @@ static int operators() {
     return err <= 1e-5 * scale && moved > 0.0 ? 0 : 1;
 }
 
+// reference-named entry points: segmentation and kNN over it, the batched
+// inverse, the RPIM interpolator (linear reproduction), divergence_operator::
+// create(domain, query, override_query_size), attached observers and the VTK
+// output (argv[2]: a directory for the .vtp files)
+static int dropin(const char* dir) {
+    using naga::segmentation::nd_cubic_segmentation;
+    std::vector<float> pts;
+    for (int i = 0; i < 2000; ++i) {
+        pts.push_back(std::fmod(0.618034f * float(i), 1.0f));
+        pts.push_back(std::fmod(0.754878f * float(i) + 0.1f, 1.0f));
+    }
+    nd_cubic_segmentation<float, 2> seg(pts, 32u);
+    std::vector<float> q(pts.begin(), pts.begin() + 200), d2a, d2b;
+    std::vector<uint32_t> ia, ib;
+    naga::segmentation::batched_nearest_neighbors<float, 2>(16, q, seg, d2a, ia);
+    naga::segmentation::batched_nearest_neighbors<2>(16, pts, q, d2b, ib);
+    if (ia != ib || d2a != d2b) {
+        std::puts("FAIL: kNN over the segmentation differs");
+        return 1;
+    }
+    std::vector<uint32_t> sizes, offs, idx;
+    seg.partitions(sizes, offs, idx);
+    size_t total = 0;
+    for (uint32_t s : sizes) total += s;
+    if (total != 2000 || seg.partition_count() != sizes.size()) {
+        std::puts("FAIL: partitions");
+        return 1;
+    }
+    const std::vector<double> A = {2, 1, 0, 1, 3, 1, 0, 1, 4};
+    std::vector<double> Ainv;
+    naga::linalg::batched_matrix_inverse(3, 1, A, Ainv);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int m = 0; m < 3; ++m) s += A[i + 3 * m] * Ainv[m + 3 * j];
+            if (std::fabs(s - (i == j)) > 1e-12) {
+                std::puts("FAIL: inverse");
+                return 1;
+            }
+        }
+    // linear field interpolated exactly (the P-augmentation)
+    auto r = naga::interpolation::radial_point_method<float>::create_interpolator<2>(
+        pts, ia, 16, q, 0.02f);
+    std::vector<float> field(2000), out;
+    for (int i = 0; i < 2000; ++i) field[i] = 0.3f + 2.0f * pts[2 * i] - pts[2 * i + 1];
+    r.interpolate(field, out);
+    for (size_t i = 0; i < out.size(); ++i)
+        if (std::fabs(out[i] - (0.3f + 2.0f * q[2 * i] - q[2 * i + 1])) > 1e-4f) {
+            std::puts("FAIL: rpim linear reproduction");
+            return 1;
+        }
+    auto op = naga::nonlocal_calculus::divergence_operator<float, 2>::create(pts, q, 150, 16);
+    if (op.size() != 150 || op.weights()[size_t(120) * 16 * 2] != 0.0f) {
+        std::puts("FAIL: divergence override size");
+        return 1;
+    }
+    const double h = 0.05;
+    uniform_grid_provider<lattice> provider({0.0, 0.0}, {1.0, 1.0}, h, 0.1, 0.5, 0.25);
+    simulation_engine<lattice> engine;
+    const double dt = 0.4 * h;
+    engine.set_problem_parameters(float((0.6 - 0.5) * dt / 3.0), 1.0f, float(dt),
+                                  float(0.57735026918962576));
+    engine.init_domain(provider);
+    engine.set_initial_pulse({0.5, 0.5}, 1e-3, 0.1);
+    vtk_observer<lattice> vtk(engine, dir);
+    density_observer<lattice> probe(engine, {0.5f, 0.5f});
+    engine.attach_observer(vtk);
+    engine.attach_observer(probe);
+    engine.step_forward(2);
+    engine.detach_observer(vtk);
+    engine.step_forward(1);
+    if (probe.density_values().size() != 3 || std::fabs(probe.density_values()[0] - 1.0f) < 1e-5f) {
+        std::puts("FAIL: attached density observer");
+        return 1;
+    }
+    for (int f = 0; f < 2; ++f) {
+        const std::string path = std::string(dir) + "/result." + std::to_string(f) + ".vtp";
+        FILE* fp = std::fopen(path.c_str(), "rb");
+        if (!fp) {
+            std::printf("FAIL: missing %s\n", path.c_str());
+            return 1;
+        }
+        std::fclose(fp);
+    }
+    std::puts("dropin ok");
+    return 0;
+}
+
 int main(int argc, char** argv) {
     try {
         if (argc > 1 && !std::strcmp(argv[1], "run")) return run(argc > 2 ? std::atoi(argv[2]) : 10);
         if (argc > 1 && !std::strcmp(argv[1], "extras")) return extras();
         if (argc > 1 && !std::strcmp(argv[1], "operators")) return operators();
+        if (argc > 2 && !std::strcmp(argv[1], "dropin")) return dropin(argv[2]);
         return errors();
     } catch (const std::exception& e) {
         std::printf("exception: %s\n", e.what());

@@ -63,3 +63,12 @@ def test_facade_advection_and_csr(facade_exe):
     r = subprocess.run([facade_exe, "operators"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "operators ok" in r.stdout
+
+
+@pytest.mark.gpu
+def test_facade_reference_named_entry_points(facade_exe, tmp_path):
+    r = subprocess.run([facade_exe, "dropin", str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "dropin ok" in r.stdout
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["result.0.vtp", "result.1.vtp"]
