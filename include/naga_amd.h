@@ -299,6 +299,13 @@ int naga_knn(const float* ref, int64_t n_ref, const float* query, int64_t n_q,
  * INT_MAX as in the reference. */
 int naga_batched_matrix_inverse_f64(const double* A, int32_t n, int64_t batch, double* A_inv,
                                     int32_t* info);
+/* The solve form of the same LU (linalg/batched_matrix_inverse.cuh:229-250
+ * inverts and multiplies; this solves directly): A X = B per item in FP64,
+ * A column-major n x n, B column-major n x nrhs; info[b] != 0 flags a zero
+ * pivot. */
+int naga_batched_lu_solve_f64(const double* A, int32_t n, int64_t batch,
+                              const double* B, int32_t nrhs, double* X,
+                              int32_t* info);
 
 /* ---- interpolation::radial_point_method<T>
  *      (interpolation/radial_point_method.cuh:92-280) ---------------------- */

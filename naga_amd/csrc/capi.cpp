@@ -813,6 +813,35 @@ int naga_batched_matrix_inverse_f64(const double* A, int32_t n, int64_t batch, d
     });
 }
 
+int naga_batched_lu_solve_f64(const double* A, int32_t n, int64_t batch, const double* B,
+                              int32_t nrhs, double* X, int32_t* info) {
+    return guarded([&] {
+        require(A && B && X, "null argument");
+        require(n >= 1 && n <= 64, "n must be in [1, 64]");
+        require(nrhs >= 1 && batch >= 0 && batch < int64_t(INT32_MAX), "bad nrhs or batch");
+        if (batch == 0) return;
+        hipStream_t s;
+        NAGA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            DeviceBuffer<double> dA(size_t(batch) * n * n), dB(size_t(batch) * n * nrhs);
+            DeviceBuffer<int> dinfo(batch);
+            dA.upload(A, dA.size(), s);
+            dB.upload(B, dB.size(), s);
+            launch_batched_lu_solve(n, nrhs, batch, dA.get(), dB.get(), dinfo.get(), s);
+            dB.download(X, dB.size(), s);
+            std::vector<int> hi(batch);
+            dinfo.download(hi.data(), batch, s);
+            NAGA_HIP(hipStreamSynchronize(s));
+            for (int64_t b = 0; b < batch; ++b)
+                if (info) info[b] = hi[b];
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+    });
+}
+
 int64_t naga_rpim_scratchpad_size(int64_t query_size, int32_t support_size, int32_t dims,
                                   int32_t group_size, int32_t bytes_per_value) {
     if (query_size < 0 || support_size < 1 || dims < 1 || group_size < 1 ||
