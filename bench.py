@@ -59,32 +59,15 @@ def copy_bandwidth(nbytes: int = 2 << 30, reps: int = 10) -> float:
     """Measured device-to-device copy rate (bytes read + written per second)
     of the runtime's copy over nbytes, the practical HBM ceiling beside the
     8 TB/s nominal peak."""
-    import torch
-    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
-    y = torch.empty_like(x)
-    y.copy_(x)
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    a.record()
-    for _ in range(reps):
-        y.copy_(x)
-    b.record()
-    b.synchronize()
-    rate = 2.0 * nbytes * reps / (a.elapsed_time(b) * 1e-3)
-    del x, y
-    torch.cuda.empty_cache()
-    return rate
+    from naga_amd import devmem
+    return devmem.copy_rate(nbytes, reps)
 
 
 def copy_calibration(nbytes: int) -> int:
     """One device-to-device copy of nbytes (the runtime's copy kernel): the
     known-size reference for scaling HBM byte counters."""
-    import torch
-    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    y.copy_(x)
-    torch.cuda.synchronize()
+    from naga_amd import devmem
+    devmem.copy_once(nbytes)
     print(json.dumps({"copied_bytes": nbytes}))
     return 0
 
@@ -174,12 +157,16 @@ def batched_workload(steps: int, n_scenes: int = 256) -> dict:
     batch.step_forward(steps)
     batched_s = time.perf_counter() - t0
     plain, batched = total * steps / plain_s, total * steps / batched_s
+    # every scene's state after its 2 + steps + 2 + steps steps: finite and
+    # bounded by the initial pulse amplitude (a diverging scene would show)
+    peak = max(float(np.abs(e.macroscopic(fresh=True)["drho"]).max()) for e in engines)
     batch.close()
     return {"scenes": n_scenes, "nodes": total, "steps": steps, "setup_s": setup_s,
+            "max_abs_drho_after": peak,
             "plain_node_updates_per_s": plain, "batched_node_updates_per_s": batched,
             "speedup": batched / plain, "gather_batch": gb,
             "workload": "D2Q9, 256 scenes, N log-uniform 2k-50k, 0-5 obstacles r 0.03-0.1, "
-                        "k 12-24, PML 8h, tau 0.6, dt 0.4h"}
+                        "k 12-24, PML 8h (3h around obstacles), tau 0.6, dt 0.4h"}
 
 
 def served_workload(n_requests: int = 32, rate: float = 50.0) -> dict:

@@ -230,6 +230,8 @@ typedef struct naga_request {
                                   [k_min, k] by local density (naga_scene_config) */
     double refine_width;       /* > 0: 2x node density within this distance of
                                   the obstacle surfaces (naga_obstacle_grid) */
+    double immersed_layer_thickness;  /* absorbing layer around the obstacles */
+    double immersed_absorption;       /* (naga_obstacle_grid; 0: none) */
 } naga_request;
 
 typedef struct naga_server_stats {
@@ -279,13 +281,18 @@ int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h
  *      jitter_frac * h/2 from a second stream; surface nodes at h/2).  The
  *      absorbing layer is the
  *      fluid within layer_thickness of the box faces (exact distance d),
- *      sigma = peak_absorption (1 - d/L)^2.  With points == NULL only
+ *      sigma = peak_absorption (1 - d/L)^2, and, as the provider's immersed
+ *      boundary layers (conforming_point_cloud_provider.cuh:250-280), the
+ *      fluid within immersed_layer_thickness Li of an obstacle surface and
+ *      nearer to it than to the faces, sigma = immersed_absorption
+ *      (1 - d/Li)^2 (0 for either: no such layer).  With points == NULL only
  *      counts[3] = {bulk, layer, boundary} are returned (size query). */
 int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double h,
                        double jitter_frac, uint64_t seed, int32_t n_obstacles,
                        const double* obstacles, double refine_width, double layer_thickness,
-                       double peak_absorption, int64_t capacity, float* points, float* normals,
-                       float* sigma, int64_t* counts);
+                       double peak_absorption, double immersed_layer_thickness,
+                       double immersed_absorption, int64_t capacity, float* points,
+                       float* normals, float* sigma, int64_t* counts);
 
 /* ---- setup primitives ---------------------------------------------------- */
 /* batched_nearest_neighbors (segmentation/nearest_neighbors.cuh:95-477),

@@ -60,6 +60,7 @@ class Request(C.Structure):
         ("tau", C.c_double), ("pulse_center", C.c_double * 3),
         ("pulse_amplitude", C.c_double), ("pulse_width", C.c_double),
         ("k_min", C.c_int32), ("refine_width", C.c_double),
+        ("immersed_layer_thickness", C.c_double), ("immersed_absorption", C.c_double),
     ]
 
 
@@ -137,7 +138,7 @@ def load() -> C.CDLL:
         "naga_batch_step": (C.c_int, [vp, i32]),
         "naga_batch_step_timed": (C.c_int, [vp, i32, vp]),
         "naga_obstacle_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, i32, vp, f64, f64, f64,
-                                         i64, vp, vp, vp, vp]),
+                                         f64, f64, i64, vp, vp, vp, vp]),
         "naga_server_create": (C.c_int, [i64, i32, i32, i32, i32, C.POINTER(vp)]),
         "naga_server_destroy": (C.c_int, [vp]),
         "naga_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(i64)]),
@@ -342,12 +343,18 @@ class DensitySource:
 
 class ObstacleGridProvider:
     """Box minus circles/spheres (naga_obstacle_grid): obstacles is a list of
-    (centre, radius); layer absorption at the box faces."""
+    (centre, radius); layer absorption at the box faces and, with
+    immersed_layer_thickness and immersed_absorption_rate > 0, around the
+    obstacles (the immersed boundary layers of
+    conforming_point_cloud_provider.cuh:250-280)."""
 
     def __init__(self, min_point, max_point, grid_spacing, obstacles=(),
                  absorption_layer_thickness=0.0, peak_absorption_rate=0.0,
-                 jitter=0.0, seed=0x4E414741, refine_width=0.0):
+                 jitter=0.0, seed=0x4E414741, refine_width=0.0,
+                 immersed_layer_thickness=0.0, immersed_absorption_rate=0.0):
         self.refine_width = float(refine_width)
+        self.immersed_layer = float(immersed_layer_thickness)
+        self.immersed_rate = float(immersed_absorption_rate)
         self.lo = _f64(min_point)
         self.hi = _f64(max_point)
         self.D = self.lo.size
@@ -364,7 +371,7 @@ class ObstacleGridProvider:
         ob = np.ascontiguousarray(self.obst.reshape(-1))
         args = (self.D, _ptr(self.lo), _ptr(self.hi), self.h, self.jitter, C.c_uint64(self.seed),
                 len(self.obst), _ptr(ob) if ob.size else None, self.refine_width, self.layer,
-                self.peak)
+                self.peak, self.immersed_layer, self.immersed_rate)
         _check(lib.naga_obstacle_grid(*args, 0, None, None, None, _ptr(counts)),
                "naga::nonlocal_lbm::obstacle_grid_provider::get: ")
         n = int(counts.sum())
@@ -1115,12 +1122,17 @@ _SERVER_PREFIX = "naga::server: "
 def make_request(lattice, h, steps, obstacles=(), k=0, jitter=0.25, seed=0x4E414741,
                  layer_thickness=None, peak_absorption=0.5, tau=0.6, pulse_center=None,
                  pulse_amplitude=1e-3, pulse_width=0.05, k_min=0,
-                 refine_width=0.0) -> Request:
+                 refine_width=0.0, immersed_layer_thickness=None,
+                 immersed_absorption=0.5) -> Request:
     """A solve of the unit box (square or cube) minus obstacles
-    [(centre, radius), ...]; the absorbing layer defaults to 8 spacings.
-    refine_width > 0 doubles the node density within that distance of the
-    obstacles (the time step is then 0.4 of the refined spacing h/2);
-    k_min > 0 gives per-node stencil widths in [k_min, k]."""
+    [(centre, radius), ...].  The absorbing layer at the box faces defaults
+    to 8 spacings, at most 0.1 (wider layers would cover most of a coarse
+    scene); the layer around the obstacles to 3 spacings with peak
+    absorption 0.5 (without it, scattered nodes next to curved surfaces can
+    carry a growing mode over hundreds of steps).  refine_width > 0 doubles
+    the node density within that distance of the obstacles (the time step
+    is then 0.4 of the refined spacing h/2); k_min > 0 gives per-node
+    stencil widths in [k_min, k]."""
     D = 2 if lattice == 9 else 3
     r = Request()
     r.lattice, r.k, r.steps = int(lattice), int(k), int(steps)
@@ -1132,7 +1144,10 @@ def make_request(lattice, h, steps, obstacles=(), k=0, jitter=0.25, seed=0x4E414
             r.obstacles[j * (D + 1) + d] = float(c[d])
         r.obstacles[j * (D + 1) + D] = float(rad)
     r.h, r.jitter, r.seed = float(h), float(jitter), int(seed)
-    r.layer_thickness = 8 * h if layer_thickness is None else float(layer_thickness)
+    r.layer_thickness = min(8 * h, 0.1) if layer_thickness is None else float(layer_thickness)
+    r.immersed_layer_thickness = (3 * h if immersed_layer_thickness is None
+                                  else float(immersed_layer_thickness))
+    r.immersed_absorption = float(immersed_absorption)
     r.peak_absorption, r.tau = float(peak_absorption), float(tau)
     pc = [0.5] * D if pulse_center is None else list(pulse_center)
     for d in range(D):
@@ -1149,7 +1164,9 @@ def request_nodes(req: Request) -> "SimulationNodes":
             for j in range(req.n_obstacles)]
     return ObstacleGridProvider([0] * D, [1] * D, req.h, obst, req.layer_thickness,
                                 req.peak_absorption, jitter=req.jitter, seed=req.seed,
-                                refine_width=req.refine_width).get()
+                                refine_width=req.refine_width,
+                                immersed_layer_thickness=req.immersed_layer_thickness,
+                                immersed_absorption_rate=req.immersed_absorption).get()
 
 
 def solve_alone(req: Request) -> dict:

@@ -555,11 +555,14 @@ int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h
 int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double h,
                        double jitter_frac, uint64_t seed, int32_t n_obstacles,
                        const double* obstacles, double refine_width, double layer_thickness,
-                       double peak_absorption, int64_t capacity, float* points, float* normals,
-                       float* sigma, int64_t* counts) {
+                       double peak_absorption, double immersed_layer_thickness,
+                       double immersed_absorption, int64_t capacity, float* points,
+                       float* normals, float* sigma, int64_t* counts) {
     return guarded([&] {
         const int D = dims;
         require(D == 2 || D == 3, "dims must be 2 or 3");
+        require(immersed_layer_thickness >= 0.0 && immersed_absorption >= 0.0,
+                "immersed layer thickness and absorption must be >= 0");
         require(lo && hi && counts, "null argument");
         require(h > 0, "grid spacing must be positive");
         require(n_obstacles >= 0 && (n_obstacles == 0 || obstacles), "bad obstacle list");
@@ -597,7 +600,44 @@ int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double 
         };
         std::vector<Node> bulk, layer, bnd;
         std::vector<float> sig;
-        const double L = layer_thickness;
+        const double L = layer_thickness, Li = immersed_layer_thickness;
+        // Absorbing layers (conforming_point_cloud_provider.cuh:219-280): a
+        // fluid node within L of the box faces (distance d) gets
+        // sigma = peak_absorption (1 - d/L)^2; one within Li of an obstacle
+        // surface and nearer to it than to the faces gets
+        // immersed_absorption (1 - d/Li)^2 from its nearest obstacle.
+        auto classify = [&](const Node& nd, const double* x) {
+            double dbox = 1e300;
+            for (int d = 0; d < D; ++d) dbox = std::min({dbox, x[d] - lo[d], hi[d] - x[d]});
+            double cur = 1e300, s = 0.0;
+            bool in_layer = false;
+            if (L > 0 && dbox <= L) {
+                cur = dbox;
+                const double r = (L - std::max(dbox, 0.0)) / L;
+                s = peak_absorption * r * r;
+                in_layer = true;
+            }
+            if (Li > 0 && immersed_absorption > 0) {
+                double dob = 1e300;
+                for (int j = 0; j < n_obstacles; ++j) {
+                    const double* o = obstacles + j * (D + 1);
+                    double r2 = 0.0;
+                    for (int d = 0; d < D; ++d) r2 += (x[d] - o[d]) * (x[d] - o[d]);
+                    dob = std::min(dob, std::sqrt(r2) - o[D]);
+                }
+                if (dob <= Li && dob < cur) {
+                    const double r = (Li - std::max(dob, 0.0)) / Li;
+                    s = immersed_absorption * r * r;
+                    in_layer = true;
+                }
+            }
+            if (in_layer) {
+                layer.push_back(nd);
+                sig.push_back(float(s));
+            } else {
+                bulk.push_back(nd);
+            }
+        };
         // grid nodes: the uniform grid's stream and jitter
         uint64_t st = seed;
         for (int64_t g = 0; g < N; ++g) {
@@ -627,15 +667,7 @@ int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double 
                 bnd.push_back(nd);
                 continue;
             }
-            double dmin = 1e300;
-            for (int d = 0; d < D; ++d) dmin = std::min({dmin, x[d] - lo[d], hi[d] - x[d]});
-            if (L > 0 && dmin <= L) {
-                layer.push_back(nd);
-                const double r = (L - std::max(dmin, 0.0)) / L;
-                sig.push_back(float(peak_absorption * r * r));
-            } else {
-                bulk.push_back(nd);
-            }
+            classify(nd, x);
         }
         // refinement: grid cells whose centre lies within refine_width of an
         // obstacle surface (outside it) get the 2^D - 1 half-offset nodes,
@@ -677,15 +709,7 @@ int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double 
                     if (!ok || inside(x, 0.25 * h)) continue;
                     Node nd{qkey(x), order++, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
                     for (int d = 0; d < D; ++d) nd.x[d] = float(x[d]);
-                    double dmin = 1e300;
-                    for (int d = 0; d < D; ++d) dmin = std::min({dmin, x[d] - lo[d], hi[d] - x[d]});
-                    if (L > 0 && dmin <= L) {
-                        layer.push_back(nd);
-                        const double r = (L - std::max(dmin, 0.0)) / L;
-                        sig.push_back(float(peak_absorption * r * r));
-                    } else {
-                        bulk.push_back(nd);
-                    }
+                    classify(nd, x);
                 }
             }
         }

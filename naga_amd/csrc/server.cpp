@@ -29,14 +29,16 @@ void make_nodes(const naga_request& r, Nodes& out) {
         if (st != NAGA_OK) fail(Status(st), naga_last_error());
     };
     check(naga_obstacle_grid(D, lo, hi, r.h, r.jitter, r.seed, r.n_obstacles, r.obstacles,
-                             r.refine_width, r.layer_thickness, r.peak_absorption, 0, nullptr,
+                             r.refine_width, r.layer_thickness, r.peak_absorption,
+                             r.immersed_layer_thickness, r.immersed_absorption, 0, nullptr,
                              nullptr, nullptr, out.counts));
     const int64_t n = out.counts[0] + out.counts[1] + out.counts[2];
     out.pts.resize(size_t(n) * D);
     out.nrm.resize(size_t(std::max<int64_t>(out.counts[2], 1)) * D);
     out.sig.resize(size_t(std::max<int64_t>(out.counts[1], 1)));
     check(naga_obstacle_grid(D, lo, hi, r.h, r.jitter, r.seed, r.n_obstacles, r.obstacles,
-                             r.refine_width, r.layer_thickness, r.peak_absorption, n,
+                             r.refine_width, r.layer_thickness, r.peak_absorption,
+                             r.immersed_layer_thickness, r.immersed_absorption, n,
                              out.pts.data(), out.nrm.data(), out.sig.data(), out.counts));
 }
 

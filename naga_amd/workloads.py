@@ -4,8 +4,8 @@ stream, so a test can draw the same kind of workload at capped sizes.
 
 * batched_scenes: unit squares with 0-5 circular obstacles of radius
   0.03-0.1, node counts log-uniform in [n_lo, n_hi] (2k-50k in the
-  256-scene workload), k in [12, 24], a random pulse position; D2Q9, PML 8h,
-  tau 0.6, dt 0.4h.
+  256-scene workload), k in [12, 24], a random pulse position; D2Q9, PML 8h
+  at the faces and 3h around the obstacles, tau 0.6, dt 0.4h.
 * served_requests: Poisson arrivals, D2Q9/D3Q27 50/50, node counts
   log-uniform in [n_lo, n_hi] (5k-2M), steps uniform in [s_lo, s_hi]
   (100-2000); D3Q27 requests are the unit cube minus a central sphere of
@@ -27,12 +27,15 @@ class BatchedScene:
     seed: int
     pulse: list
     layer_cells: int = 8
+    immersed_cells: int = 3
 
     def provider(self):
         import naga_amd
         return naga_amd.ObstacleGridProvider([0, 0], [1, 1], self.h, self.obstacles,
                                              self.layer_cells * self.h, 0.5, jitter=0.25,
-                                             seed=self.seed)
+                                             seed=self.seed,
+                                             immersed_layer_thickness=self.immersed_cells * self.h,
+                                             immersed_absorption_rate=0.5)
 
 
 def batched_scenes(n_scenes: int = 256, n_lo: float = 2e3, n_hi: float = 5e4,
