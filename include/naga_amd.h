@@ -299,6 +299,20 @@ int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double 
  * exact: the k smallest (d2, index), d2 FP32 = dz^2 + (dy^2 + dx^2). */
 int naga_knn(const float* ref, int64_t n_ref, const float* query, int64_t n_q,
              int32_t dims, int32_t k, float* d2_out, uint32_t* idx_out);
+/* No counterpart in the reference.  naga_knn for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc): the same neighbours, bit for bit, with
+ * every pointer a DEVICE pointer (ref[n_ref*dims], query[n_q*dims],
+ * d2_out/idx_out[n_q*k]) and the cell binning in a static device scratch
+ * array of the library (about 21 MB per GPU, held until the process ends),
+ * so n_ref <= NAGA_KNN_NOALLOC_MAX_REF; larger n_ref is rejected with
+ * NAGA_INVALID_ARGUMENT.  Work is queued on `stream` (a hipStream_t; NULL:
+ * the null stream) and the call returns without waiting.  All calls on one
+ * GPU share its scratch: calls that overlap on one GPU (other streams or
+ * threads) race and must be ordered by the caller. */
+#define NAGA_KNN_NOALLOC_MAX_REF (1 << 20)
+int naga_knn_noalloc(const float* ref, int64_t n_ref, const float* query, int64_t n_q,
+                     int32_t dims, int32_t k, float* d2_out, uint32_t* idx_out, void* stream);
 /* batched_matrix_inverse(A, A_inv) (linalg/batched_matrix_inverse.cuh:
  * 325-397): A_inv[b] = A[b]^-1 for column-major n x n matrices (FP64 LU with
  * partial pivoting against the identity); info[b] = 0 or the first zero pivot

@@ -124,6 +124,7 @@ def load() -> C.CDLL:
         "naga_uniform_grid_size": (i64, [i32, vp, vp, f64]),
         "naga_uniform_grid": (C.c_int, [i32, vp, vp, f64, f64, u64, f64, f64, vp, vp, vp, vp]),
         "naga_knn": (C.c_int, [vp, i64, vp, i64, i32, i32, vp, vp]),
+        "naga_knn_noalloc": (C.c_int, [vp, i64, vp, i64, i32, i32, vp, vp, vp]),
         "naga_batched_lu_solve_f64": (C.c_int, [vp, i32, i64, vp, i32, vp, vp]),
         "naga_rpim_weights": (C.c_int, [vp, i64, vp, i32, vp, i64, i32, i32, f64, vp]),
         "naga_divergence_weights": (C.c_int, [vp, i64, vp, i64, i32, i32, i32, vp, vp]),
@@ -184,7 +185,7 @@ EXPORTED = [
     "naga_rpim_interpolate", "naga_rpim_get_weights", "naga_segmentation_create",
     "naga_segmentation_destroy", "naga_segmentation_shape", "naga_segmentation_partitions",
     "naga_segmentation_knn", "naga_rk4_stage",
-    "naga_uniform_grid_size", "naga_uniform_grid", "naga_knn",
+    "naga_uniform_grid_size", "naga_uniform_grid", "naga_knn", "naga_knn_noalloc",
     "naga_batched_lu_solve_f64", "naga_rpim_weights", "naga_divergence_weights",
     "naga_divergence_apply", "naga_engine_add_density_source",
     "naga_engine_clear_density_sources", "naga_batch_create", "naga_batch_destroy",

@@ -5,6 +5,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/naga_amd.h"
@@ -801,6 +802,54 @@ int naga_knn(const float* ref, int64_t n_ref, const float* query, int64_t n_q, i
             throw;
         }
         (void)hipStreamDestroy(s);
+    });
+}
+
+// ---- naga_knn without allocation ----------------------------------------
+// The cell binning's scratch (grid, bounds, counts, starts, scan sums,
+// sorted indices) for up to kNoAllocMaxRef reference points, one copy per
+// GPU.  Host code never uses the array's name as a device address: the
+// device address comes from hipGetSymbolAddress, once per device.
+namespace {
+constexpr int64_t kNoAllocMaxRef = NAGA_KNN_NOALLOC_MAX_REF;
+constexpr size_t kNoAllocBytes = bin_layout(kNoAllocMaxRef).total;
+}  // namespace
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::array<void*, 64> g_scratch_ptr{};
+
+void* scratch_of_current_device() {
+    int dev = 0;
+    NAGA_HIP(hipGetDevice(&dev));
+    require(dev >= 0 && dev < int(g_scratch_ptr.size()), "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        NAGA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "no device address for the kNN scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+int naga_knn_noalloc(const float* ref, int64_t n_ref, const float* query, int64_t n_q,
+                     int32_t dims, int32_t k, float* d2_out, uint32_t* idx_out, void* stream) {
+    return guarded([&] {
+        require(dims == 2 || dims == 3, "dimension must be 2 or 3");
+        require(k >= 1 && k <= 64, "k must be in [1, 64]");
+        require(n_ref >= k, "fewer reference points than neighbours requested");
+        require(n_ref <= kNoAllocMaxRef,
+                "n_ref exceeds the no-allocation limit of " + std::to_string(kNoAllocMaxRef) +
+                    " reference points (use naga_knn)");
+        require(n_q >= 0, "negative query count");
+        require(ref && (n_q == 0 || (query && d2_out && idx_out)), "null argument");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        void* mem = scratch_of_current_device();
+        const CellBins bins = launch_bin_points(dims, ref, n_ref, 2.0, 0.0, mem, kNoAllocBytes, st);
+        launch_knn(bins, ref, query, n_q, k, d2_out, idx_out, st);
     });
 }
 
