@@ -276,7 +276,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--h", type=float, default=0.01)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
     ap.add_argument("--no-batched", action="store_true",

@@ -1128,9 +1128,11 @@ def make_request(lattice, h, steps, obstacles=(), k=0, jitter=0.25, seed=0x4E414
     """A solve of the unit box (square or cube) minus obstacles
     [(centre, radius), ...].  The absorbing layer at the box faces defaults
     to 8 spacings, at most 0.1 (wider layers would cover most of a coarse
-    scene); the layer around the obstacles to 3 spacings with peak
+    scene); in 2D the layer around the obstacles to 3 spacings with peak
     absorption 0.5 (without it, scattered nodes next to curved surfaces can
-    carry a growing mode over hundreds of steps).  refine_width > 0 doubles
+    carry a growing mode over hundreds of steps; around a refined sphere in
+    3D the same layer itself grows a mode, so 3D requests default to none).
+    refine_width > 0 doubles
     the node density within that distance of the obstacles (the time step
     is then 0.4 of the refined spacing h/2); k_min > 0 gives per-node
     stencil widths in [k_min, k]."""
@@ -1146,7 +1148,7 @@ def make_request(lattice, h, steps, obstacles=(), k=0, jitter=0.25, seed=0x4E414
         r.obstacles[j * (D + 1) + D] = float(rad)
     r.h, r.jitter, r.seed = float(h), float(jitter), int(seed)
     r.layer_thickness = min(8 * h, 0.1) if layer_thickness is None else float(layer_thickness)
-    r.immersed_layer_thickness = (3 * h if immersed_layer_thickness is None
+    r.immersed_layer_thickness = ((3 * h if D == 2 else 0.0) if immersed_layer_thickness is None
                                   else float(immersed_layer_thickness))
     r.immersed_absorption = float(immersed_absorption)
     r.peak_absorption, r.tau = float(peak_absorption), float(tau)

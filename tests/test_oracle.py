@@ -385,6 +385,129 @@ def test_divergence_consistency(oracle64, D, h, k):
         assert np.linalg.norm(got - want) / np.linalg.norm(want) < 0.02
 
 
+# Linear-field constants of the reference's quadrature, in closed form
+# (nonlocal_calculus/detail/quadrature_points.cuh:40-152,
+# detail/divergence.inl:53-227).  The kernel alpha(y - x) = K(r) n is
+# normalised so that the continuum divergence of a linear field is exact
+# (2D: (240/7)/(pi delta^5) k(r), int r^2 k = 7/240 delta^5; 3D:
+# 48/(pi delta^6) k(r), int r^3 k = delta^6/64; k(r) = r(2 delta - 3r) below
+# delta/2, (delta - r)^2 below delta).  RPIM reproduces linear fields, so the
+# discrete divergence of F = x_d e_d is the quadrature of that normalised
+# integral: the 4-point Gauss-Legendre rule (the reference's 6-digit table)
+# on the radial moment times the angular rule.  The theta rule (0, 90, 180,
+# 270 degrees, weight pi/2) integrates cos^2 exactly; the 3D phi rule is the
+# 4-point midpoint rule on [0, pi] (phi = 22.5 + 45 j degrees), which
+# integrates sin^3 (x and y) and sin cos^2 (z) with different errors: the
+# 3D operator is anisotropic.
+GL_X = np.array([0.339981, -0.339981, 0.861136, -0.861136])
+GL_W = np.array([0.652145, 0.652145, 0.347855, 0.347855])
+
+
+def _kernel_profile(r):
+    return np.where(2 * r < 1, r * (2 - 3 * r), np.where(r < 1, (1 - r) ** 2, 0.0))
+
+
+def quadrature_linear_constants(D):
+    """(c_x, c_y[, c_z]): discrete divergence of F = x_d e_d."""
+    from scipy.integrate import quad
+    p = D  # radial moment r^D k(r) (r^(D-1) from the volume element)
+    r = (GL_X + 1) / 2
+    gl = np.sum(GL_W / 2 * r ** p * _kernel_profile(r))
+    exact = (quad(lambda x: x ** p * _kernel_profile(x), 0, 0.5)[0]
+             + quad(lambda x: x ** p * _kernel_profile(x), 0.5, 1)[0])
+    radial = gl / exact
+    th = np.pi / 2 * np.arange(4)
+    theta = (np.pi / 2) * np.sum(np.cos(th) ** 2) / np.pi  # = 1
+    if D == 2:
+        return np.array([radial * theta] * 2)
+    phi = np.pi / 4 * np.arange(4) + np.pi / 8
+    sx = (np.pi / 4) * np.sum(np.sin(phi) ** 3) / (4.0 / 3.0)
+    sz = (np.pi / 4) * np.sum(np.sin(phi) * np.cos(phi) ** 2) / (2.0 / 3.0)
+    return np.array([radial * theta * sx, radial * theta * sx, radial * sz])
+
+
+@pytest.mark.parametrize("D,h,k", [(2, 0.01, 16), (2, 0.05, 16), (3, 0.05, 32), (3, 0.1, 32)])
+def test_divergence_linear_constants(oracle64, D, h, k):
+    """The oracle's divergence of x_d e_d at interior nodes equals the
+    closed-form quadrature constant of axis d at every spacing: 2D
+    0.980808 (radial Gauss-Legendre error alone); 3D 0.972372 along x and
+    y, 1.063616 along z."""
+    o = oracle64
+    nd = o.uniform_grid(D, [0] * D, [1] * D, h, jitter=0.25)
+    pts = nd["points"]
+    d2, idx = o.knn(pts, pts, k)
+    W, _ = o.divergence_weights(pts, pts, idx, d2, 1)
+    interior = np.all((pts > 0.3) & (pts < 0.7), axis=1)
+    x = pts.astype(np.float64)
+    want = quadrature_linear_constants(D)
+    got = []
+    for d in range(D):
+        F = np.zeros_like(x)
+        F[:, d] = x[:, d]
+        got.append(o.divergence_apply(idx, W, F)[interior].mean())
+    print(f"D={D} h={h}: linear-field constants {np.round(got, 7)} (closed form {np.round(want, 7)})")
+    assert np.abs(np.array(got) - want).max() < 1e-8
+    if D == 3:
+        assert abs(want[0] - 0.9723721) < 1e-6 and abs(want[2] - 1.0636160) < 1e-6
+    else:
+        assert abs(want[0] - 0.9808084) < 1e-6
+
+
+def directional_peak_times(step, drho, points, center, radii, h, steps, dt, cone_deg=25.0):
+    """Peak times of the mean density deviation on each sphere |x - c| = R
+    (within 0.6 h), restricted to the cones of half-angle cone_deg around
+    the +-x, +-y and +-z axes, and over the whole sphere.  step() advances
+    one step, drho() returns rho - rho0.  Parabolic interpolation between
+    steps.  Returns {0: t_x, 1: t_y, 2: t_z, "shell": t}."""
+    p = points.astype(np.float64) - np.asarray(center)
+    r = np.linalg.norm(p, axis=1)
+    cos = np.cos(np.radians(cone_deg))
+    masks = {ax: [(np.abs(p[:, ax]) > cos * r) & (np.abs(r - R) < 0.6 * h) for R in radii]
+             for ax in range(3)}
+    masks["shell"] = [np.abs(r - R) < 0.6 * h for R in radii]
+    series = {key: [] for key in masks}
+    for _ in range(steps):
+        step()
+        d = drho()
+        for key, ms in masks.items():
+            series[key].append([d[m].mean() for m in ms])
+    out = {}
+    for key, sr in series.items():
+        sr = np.array(sr)
+        t = []
+        for j in range(len(radii)):
+            i = int(np.argmax(sr[:, j]))
+            assert 0 < i < len(sr) - 1, "no interior peak"
+            a, b, c = sr[i - 1:i + 2, j]
+            t.append((i + 1) * dt + 0.5 * dt * (a - c) / (a - 2 * b + c))
+        out[key] = np.array(t)
+    return out
+
+
+def test_engine_pulse_speed_3d(oracle64):
+    """3D acoustics of the reference method, filter off: a central Gaussian
+    pulse (width 0.06) on the unit cube at h = 0.02 travels along each axis
+    at c_s times that axis's quadrature constant within 1% (0.972 c_s along
+    x and y, 1.064 c_s along z), so the 3D front is not spherical; the
+    whole-shell average lies between the two."""
+    h, width = 0.02, 0.06
+    radii = np.array([0.15, 0.2, 0.25, 0.3])
+    nd = oracle64.uniform_grid(3, [0] * 3, [1] * 3, h, jitter=0.25, layer=0.1, sigma_max=0.5)
+    e = oracle64.engine(nd, Q=27, k=32, h=h, threads=8, filter_strength=0.0)
+    e.set_pulse([0.5, 0.5, 0.5], 1e-3, width)
+    dt = 0.4 * h
+    cs = np.sqrt(CS2)
+    steps = int(1.25 * radii.max() / (0.97 * cs) / dt)
+    t = directional_peak_times(lambda: e.step(1), lambda: e.get()["drho"], nd["points"],
+                               [0.5] * 3, radii, h, steps, dt)
+    c = quadrature_linear_constants(3)
+    speeds = {key: np.polyfit(t[key], radii, 1)[0] / cs for key in t}
+    print("3D directional speed / c_s:", {k_: round(v, 4) for k_, v in speeds.items()})
+    for ax in range(3):
+        assert abs(speeds[ax] / c[ax] - 1) < 0.01, (ax, speeds[ax], c[ax])
+    assert c[0] < speeds["shell"] < c[2]
+
+
 def test_divergence_apply_vs_numpy(oracle64):
     rng = np.random.default_rng(2)
     n, k, D = 300, 16, 3
