@@ -169,6 +169,9 @@ def batched_workload(steps: int, n_scenes: int = 256) -> dict:
                         "k 12-24, PML 8h (3h around obstacles), tau 0.6, dt 0.4h"}
 
 
+SIZE_CLASSES = ((0, 5e4, "<50k"), (5e4, 5e5, "50k-500k"), (5e5, 1e12, ">=500k"))
+
+
 def served_workload(n_requests: int = 32, rate: float = 50.0) -> dict:
     """Poisson arrivals at `rate` requests/s (naga_amd.workloads.served_requests:
     D2Q9/D3Q27 50/50, node counts log-uniform in [5k, 2M], steps uniform in
@@ -177,27 +180,43 @@ def served_workload(n_requests: int = 32, rate: float = 50.0) -> dict:
     24 to 48 by local density; D2Q9 requests the unit square with 0-5 circles
     of radius 0.03-0.1, k = 16).  Served with continuous batching, then the
     same requests solved one at a time with their own launches (setup
-    included in both)."""
+    included in both).  Every served result is checked to stay bounded, and
+    the plain loop's time is broken down by request size."""
     import naga_amd
     from naga_amd import workloads
     trace = workloads.served_requests(n_requests, rate)
     reqs = trace.requests
     srv = naga_amd.SolveServer(hbm_budget_bytes=96 << 30, max_active=64, chunk_steps=50)
     t0 = time.perf_counter()
+    ids = []
     for r, g in zip(reqs, trace.gaps):
         time.sleep(float(g))
-        srv.submit(r)
+        ids.append(srv.submit(r))
     srv.drain()
     st = srv.stats()
+    peaks = [float(np.abs(srv.result(i)["drho"]).max()) for i in ids]
     srv.close()
     print(f"served: {n_requests} requests in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
-    work = 0.0
+    work, classes = 0.0, {name: {"requests": 0, "node_updates": 0.0, "plain_s": 0.0}
+                          for _, _, name in SIZE_CLASSES}
     t1 = time.perf_counter()
     for r in reqs:
+        ta = time.perf_counter()
         out = naga_amd.solve_alone(r)
-        work += out["nodes"] * r.steps
+        dt = time.perf_counter() - ta
+        nu = float(out["nodes"]) * r.steps
+        work += nu
+        for lo, hi, name in SIZE_CLASSES:
+            if lo <= out["nodes"] < hi:
+                c = classes[name]
+                c["requests"] += 1
+                c["node_updates"] += nu
+                c["plain_s"] += dt
     plain_s = time.perf_counter() - t1
+    for c in classes.values():
+        c["work_share"] = c["node_updates"] / work
+        c["plain_node_updates_per_s"] = c["node_updates"] / c["plain_s"] if c["plain_s"] else None
     print(f"served: plain loop {plain_s:.1f} s", file=sys.stderr, flush=True)
     return {"requests": n_requests, "rate_per_s": rate, "p50_ms": st["p50_ms"],
             "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
@@ -205,9 +224,13 @@ def served_workload(n_requests: int = 32, rate: float = 50.0) -> dict:
             "plain_node_updates_per_s": work / plain_s,
             "speedup": st["node_updates_per_s"] / (work / plain_s),
             "trace_s": time.perf_counter() - t0 - plain_s,
+            "results_bounded": int(sum(bool(np.isfinite(p) and p < 1e-3) for p in peaks)),
+            "max_abs_drho_after": max(peaks),
+            "plain_by_size": classes,
             "workload": "Poisson 50 req/s, D2Q9/D3Q27 50/50, N log-uniform 5k-2M, "
                         "steps 100-2000; 3D: cube minus sphere r 0.15, 2x refinement "
-                        "within 0.05, k 24-48; 2D: 0-5 circles, k 16"}
+                        "within 0.05 (h <= 0.05), k 24-48, jitter 0.15h; 2D: 0-5 circles "
+                        "with a 3h absorbing layer, k 16"}
 
 
 def other_configs(threads: int) -> dict:
@@ -282,6 +305,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched and served workloads")
     ap.add_argument("--batched-steps", type=int, default=500)
+    ap.add_argument("--served-requests", type=int, default=32)
     ap.add_argument("--other-configs", action="store_true",
                     help="also time the 2D plain and irregular 3D scenes")
     args = ap.parse_args()
@@ -358,7 +382,7 @@ def main() -> int:
     if not args.no_batched:
         eng.close()
         for name, fn in (("batched", lambda: batched_workload(args.batched_steps)),
-                         ("served", served_workload)):
+                         ("served", lambda: served_workload(args.served_requests))):
             try:
                 out[name] = fn()
             except Exception as exc:  # reported, not fatal to the headline line
@@ -368,7 +392,7 @@ def main() -> int:
         out["other_configs"] = other_configs(threads)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(threads)
-    print(json.dumps(out))
+    print(json.dumps(out, default=float))
     return 0
 
 

@@ -455,7 +455,8 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     rho_.allocate(N);
     u_.allocate(N * D);
     ru_.allocate(N);
-    nparts_ = int(std::max<int64_t>(pml_parts(Q, k, N, n_rq_), 1));
+    nparts_ = int(std::max<int64_t>(pml_parts(Q, k, N, nfluid, n_rq_), 1));
+    part_lo_ = int(std::min<int64_t>(pml_part_lo(Q, k, N, nbulk), nparts_));
     partial_.allocate(size_t(nparts_) * 2 * Q);
     partial_.zero(st);
     means_.allocate(2 * means_stride(Q));
@@ -504,6 +505,7 @@ StepBuffers Engine::buffers() {
     S.partial = partial_.get();
     S.means = means_.get();
     S.nparts = nparts_;
+    S.part_lo = part_lo_;
     S.src = sources_.empty() ? nullptr : src_field_.get();
     S.ulist = ulist_.get();
     S.uoff = uoff_.get();

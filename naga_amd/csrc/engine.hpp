@@ -145,6 +145,7 @@ class Engine {
     DeviceBuffer<double> partial_;
     DeviceBuffer<float> means_;
     int nparts_ = 1;
+    int part_lo_ = 0;
     int64_t n_pml_ = 0;
     int64_t n_ptiles_ = 0;
     int tile_nodes_ = 0;  // nodes per union tile (0: no unions)

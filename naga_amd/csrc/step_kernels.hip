@@ -91,8 +91,8 @@ struct FlatRow {
 // Macroscopic values, density source and FP64 collision of node i, its
 // deviation row r[0..Q) updated in place.
 template <int Q, class Row>
-__device__ __forceinline__ void p1_node(const StepGeom& G, const StepBuffers& S,
-                                        const CollisionParams& cp, int64_t i, Row r) {
+__device__ __forceinline__ float4 p1_node(const StepGeom& G, const StepBuffers& S,
+                                          const CollisionParams& cp, int64_t i, Row r) {
     using L = Lattice<Q>;
     constexpr int D = L::D, RS = L::RS;
     double f[Q];
@@ -122,7 +122,7 @@ __device__ __forceinline__ void p1_node(const StepGeom& G, const StepBuffers& S,
     S.rho[i] = float(rho * double(G.rho0));
 #pragma unroll
     for (int d = 0; d < D; ++d) S.u[d * G.N + i] = bnd ? 0.0f : float(u[d] * double(G.vel_scale));
-    S.ru[i] = make_float4(float(rt), float(u[0]), float(u[1]), float(u[2]));
+    const float4 ru = make_float4(float(rt), float(u[0]), float(u[1]), float(u[2]));
     double df[Q];
     collide<Q>(f, rho, u, cp, df);
     const bool src_node = !bnd && S.bsrc[i];
@@ -132,6 +132,7 @@ __device__ __forceinline__ void p1_node(const StepGeom& G, const StepBuffers& S,
         r.set(a, float(v));
         if (src_node) S.coll[i * RS + a] = float(df[a]);
     }
+    return ru;
 }
 
 template <int Q>
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(kP1Block) void k_p1(StepGeom G, StepBuffers S,
     for (int e = threadIdx.x; e < cnt * RS; e += kP1Block) rows[(e / RS) * LS + e % RS] = src[e];
     __syncthreads();
     const int t = threadIdx.x;
-    if (t < cnt) p1_node<Q>(G, S, cp, node0 + t, FlatRow{rows + t * LS});
+    if (t < cnt) S.ru[node0 + t] = p1_node<Q>(G, S, cp, node0 + t, FlatRow{rows + t * LS});
     __syncthreads();
     float* dst = S.A + node0 * RS;
     for (int e = threadIdx.x; e < cnt * RS; e += kP1Block) dst[e] = rows[(e / RS) * LS + e % RS];
@@ -180,6 +181,24 @@ __device__ __forceinline__ double feq_dev(const float4& ru, int a) {
                                 : u0 * double(L::c(a, 0)) + u1 * double(L::c(a, 1));
     const double uu = L::D == 3 ? u2 * u2 + (u1 * u1 + u0 * u0) : u1 * u1 + u0 * u0;
     return L::w(a) * (rt + rho * (uc * inv_cs2 + (uc * uc) * inv_2cs4 - uu));
+}
+
+// The same in FP32 (the fused collision pass): the terms are deviations of
+// order 1e-3 whose FP32 rounding is that of the FP32 rows they update.
+template <int Q>
+__device__ __forceinline__ float feq_dev32(const float4& ru, int a) {
+    using L = Lattice<Q>;
+    constexpr float cs2 = float(kCsLat * kCsLat);
+    constexpr float inv_cs2 = float(1.0 / (kCsLat * kCsLat));
+    constexpr float inv_2cs4 = float(1.0 / (2.0 * kCsLat * kCsLat * kCsLat * kCsLat));
+    (void)cs2;
+    const float rt = ru.x, rho = 1.0f + rt;
+    const float uc = L::D == 3 ? ru.y * float(L::c(a, 0)) + ru.z * float(L::c(a, 1)) +
+                                     ru.w * float(L::c(a, 2))
+                               : ru.y * float(L::c(a, 0)) + ru.z * float(L::c(a, 1));
+    const float uu = L::D == 3 ? ru.w * ru.w + (ru.z * ru.z + ru.y * ru.y)
+                               : ru.z * ru.z + ru.y * ru.y;
+    return float(L::w(a)) * (rt + rho * (uc * inv_cs2 + (uc * uc) * inv_2cs4 - uu));
 }
 
 template <int Q>
@@ -233,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_pml_local(StepGeom G, StepBuffers S,
         const int which = threadIdx.x / Q, aa = threadIdx.x % Q;
         double t = 0.0;
         for (int sl = 0; sl < NPB; ++sl) t += red[sl][which][aa];
-        S.partial[bc.b * 2 * Q + threadIdx.x] = t;
+        S.partial[int64_t(threadIdx.x) * S.nparts + bc.b] = t;
     }
 }
 
@@ -244,7 +263,8 @@ __global__ __launch_bounds__(256) void k_means(StepGeom G, StepBuffers S, BatchA
     const BlockCtx bc = locate(B, G, S);
     const int col = int(bc.b);  // 0..2Q-1
     double s = 0.0;
-    for (int p = threadIdx.x; p < S.nparts; p += 256) s += S.partial[int64_t(p) * 2 * Q + col];
+    for (int p = S.part_lo + threadIdx.x; p < S.nparts; p += 256)
+        s += S.partial[int64_t(col) * S.nparts + p];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -590,6 +610,12 @@ __global__ __launch_bounds__(kBlock) void k_stage(StepGeom G, StepBuffers S, Bat
 // zero instead of faulting.
 // ---------------------------------------------------------------------------
 constexpr int kV4Lanes = 8;                   // lanes per node
+
+__device__ __forceinline__ double shfl_xor_f64(double v, int m) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __shfl_xor(int(b & 0xffffffffll), m), hi = __shfl_xor(int(b >> 32), m);
+    return __longlong_as_double((long long)(uint32_t(lo)) | ((long long)hi << 32));
+}
 constexpr int kV4Nodes = kBlock / kV4Lanes;  // nodes per 256-thread block
 constexpr int kV4DefaultNodes = 32;          // nodes per block of the stencil kernels
 
@@ -859,6 +885,16 @@ constexpr int kP1Chunk = 64;
 // over a batch group.
 bool p1_uses_dma(int Q, bool unions) { return NAGA_P1_DMA && Q == 27 && unions; }
 
+// The DMA collision carries the PML local terms (k_p1_pml) unless
+// NAGA_P1_PML=0 selects the separate PML pass (read once per process).
+bool p1_fuses_pml(int Q, bool unions) {
+    static const bool on = [] {
+        const char* e = std::getenv("NAGA_P1_PML");
+        return !(e && e[0] == '0');
+    }();
+    return on && p1_uses_dma(Q, unions);
+}
+
 struct XorRow {
     float* buf;  // pieces of the chunk, swizzled as above
     int t;       // node within the chunk
@@ -882,13 +918,148 @@ __global__ __launch_bounds__(kP1Chunk) __attribute__((amdgpu_waves_per_eu(3))) v
     for (int j = 0; j < 8; ++j)
         dma16(gsrc, uint32_t(node0 + (t ^ j)) * 128u + uint32_t(j) * 16u, buf + j * kP1Chunk);
     dma_fence();
-    if (node0 + t < G.N) p1_node<27>(G, S, cp, node0 + t, XorRow{(float*)buf, t});
+    if (node0 + t < G.N) S.ru[node0 + t] = p1_node<27>(G, S, cp, node0 + t, XorRow{(float*)buf, t});
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int p = r * kP1Chunk + t, n = p >> 3, j = p & 7;
         if (node0 + n < G.N)
             st4(S.A, uint32_t(node0 + n) * 128u + uint32_t(j) * 16u, buf[j * kP1Chunk + (n ^ j)]);
+    }
+}
+
+// Collision and PML local terms of one D3Q27 scene in one pass, one wave
+// per block of 64 nodes (as k_p1_dma, whose independent one-wave blocks
+// overlap each other's memory and FP64 work): the wave collides its rows in
+// LDS, then its 64 lanes, eight per node (four distributions each), apply
+// the PML local terms of k_pml_local_v4 in FP32 to the block's nodes whose
+// Q values are maintained, subtracting the layer damping from the collided
+// rows before they are stored.  Blocks that hold layer nodes write their
+// partial layer sums (column-major, part = block).  Saves the PML pass's
+// second read and write of the layer rows, its read of (rho - 1, u) and one
+// launch.
+constexpr int kP1Fused = kP1Chunk;  // nodes per fused block
+#ifndef NAGA_P1_PREFETCH
+#define NAGA_P1_PREFETCH 0
+#endif
+
+__device__ __forceinline__ bool pml_maintained(const StepGeom& G, int64_t i) {
+    return (i >= G.nbulk && i < G.nfluid) || (G.pml_ref && i < G.nlayer && i < G.nfluid);
+}
+
+__global__ __launch_bounds__(kP1Fused) __attribute__((amdgpu_waves_per_eu(3))) void k_p1_pml(
+    StepGeom G, StepBuffers S, CollisionParams cp, BatchArgs B) {
+    __shared__ float4 buf[8 * kP1Chunk];  // collided rows, XorRow layout
+    __shared__ float4 rul[kP1Chunk];      // (rho - 1, u) per node
+    __shared__ float4 prl[kP1Chunk];      // the previous step's (rho - 1, u) per node
+    const int t = threadIdx.x;
+    const BlockCtx bc = locate(B, G, S);
+    if (B.scenes) cp = B.scenes[bc.scene].cp;
+    const uint64_t rows = uint64_t(G.N) * 128u;
+    const __amdgpu_buffer_rsrc_t gsrc = rsrc(S.g, rows);
+    const int64_t node0 = bc.b * kP1Chunk;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        dma16(gsrc, uint32_t(node0 + (t ^ j)) * 128u + uint32_t(j) * 16u, buf + j * kP1Chunk);
+    if (G.nlayer > 0)  // read by the PML terms below, after the collision
+        dma16(rsrc(S.E, uint64_t(G.N) * 16u), uint32_t(node0 + t) * 16u, prl);
+    dma_fence();
+    if (node0 + t < G.N) rul[t] = p1_node<27>(G, S, cp, node0 + t, XorRow{(float*)buf, t});
+    __syncthreads();
+    const int64_t end = node0 + kP1Chunk;
+    const bool any_pml = G.nlayer > 0 && node0 < G.nfluid &&
+                         (end > G.nbulk || (G.pml_ref && node0 < G.nlayer));
+    const bool layer_block = G.nlayer > 0 && node0 < G.nfluid && end > G.nbulk;
+    const int slot = t / kV4Lanes, q = t % kV4Lanes;
+    double s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
+    if (any_pml) {
+        const float hdt = G.dt * 0.5f;
+        const __amdgpu_buffer_rsrc_t rQp = rsrc(S.Qp, rows);
+        constexpr int kRounds = kP1Chunk / 8;
+#if NAGA_P1_PREFETCH
+        // every round's loads first: one memory round trip for the phase
+        float4 qp[kRounds];
+        float sg[kRounds];
+#pragma unroll
+        for (int rd = 0; rd < kRounds; ++rd) {
+            const int64_t i = node0 + rd * 8 + slot;
+            const bool layer = i >= G.nbulk && i < G.nfluid;
+            qp[rd] = ld4(rQp, uint32_t(i) * 128u + uint32_t(q) * 16u);  // past N: zero
+            sg[rd] = layer ? S.sigma[i - G.nbulk] : 0.0f;
+        }
+#pragma unroll
+#else
+#pragma unroll 2
+#endif
+        for (int rd = 0; rd < kRounds; ++rd) {
+            const int n = rd * 8 + slot;
+            const int64_t i = node0 + n;
+            if (i >= G.N || !pml_maintained(G, i)) continue;
+            const float4 prv = prl[n];  // the previous step's (rho - 1, u)
+            const uint32_t own = uint32_t(i) * 128u + uint32_t(q) * 16u;
+            const bool layer = i >= G.nbulk && i < G.nfluid;
+#if NAGA_P1_PREFETCH
+            const float4 p4 = qp[rd];
+            const float sig = sg[rd];
+#else
+            const float4 p4 = ld4(rQp, own);
+            const float sig = layer ? S.sigma[i - G.nbulk] : 0.0f;
+#endif
+            const float4 ru = rul[n];
+            float4& a4 = buf[q * kP1Chunk + (n ^ q)];
+            float4 av = a4;
+            float4 po, qo;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int a = 4 * q + e;
+                if (a >= 27) {  // padding slots stay zero
+                    set(po, e, 0.0f);
+                    set(qo, e, 0.0f);
+                    continue;
+                }
+                const float fte = feq_dev32<27>(ru, a);
+                const float q1 = (fte + feq_dev32<27>(prv, a)) * hdt;
+                const float q2 = (q1 + get(p4, e)) * hdt;
+                set(po, e, q1);
+                set(qo, e, q2);
+                if (layer) {
+                    set(av, e, get(av, e) - sig * (3.0f * fte + 3.0f * sig * q1 + sig * sig * q2));
+                    s1[e] += double(q1);
+                    s2[e] += double(q2);
+                }
+            }
+            // the previous values were staged in LDS before the collision
+            if (q == 0) reinterpret_cast<float4*>(S.E)[i] = ru;
+            st4(S.Qp, own, po);
+            if (G.pml_ref ? i < G.nlayer : layer) st4(S.Q2v, own, qo);  // rows the layer field reads
+            if (layer) a4 = av;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int p = r * kP1Chunk + t, n = p >> 3, j = p & 7;
+        if (node0 + n < G.N)
+            st4(S.A, uint32_t(node0 + n) * 128u + uint32_t(j) * 16u, buf[j * kP1Chunk + (n ^ j)]);
+    }
+    if (!layer_block) return;
+    // sums over the block's eight node slots (lanes q, q + 8, ..., q + 56)
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int o = 8; o < 64; o <<= 1) {
+            s1[e] += shfl_xor_f64(s1[e], o);
+            s2[e] += shfl_xor_f64(s2[e], o);
+        }
+    if (slot == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int a = 4 * q + e;
+            if (a < 27) {
+                S.partial[int64_t(a) * S.nparts + bc.b] = s1[e];
+                S.partial[int64_t(27 + a) * S.nparts + bc.b] = s2[e];
+            }
+        }
     }
 }
 
@@ -1292,7 +1463,7 @@ __global__ __launch_bounds__(kBlock) void k_pml_local_v4(StepGeom G, StepBuffers
         const int which = threadIdx.x / 27, aa = threadIdx.x % 27;
         double t = 0.0;
         for (int sl = 0; sl < kV4Nodes; ++sl) t += red[sl][which][aa];
-        S.partial[bc.b * 2 * 27 + threadIdx.x] = t;
+        S.partial[int64_t(threadIdx.x) * S.nparts + bc.b] = t;
     }
 }
 
@@ -1641,7 +1812,9 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
     }
     switch (pass) {
         case kP1:
-            if (p1_uses_dma(Q, c.unions)) {
+            if (p1_fuses_pml(Q, c.unions)) {
+                k_p1_pml<<<nb, kP1Fused, 0, st>>>(G, S, cp, c.B);
+            } else if (p1_uses_dma(Q, c.unions)) {
                 k_p1_dma<<<nb, kP1Chunk, 0, st>>>(G, S, cp, c.B);
             } else {
                 k_p1<Q><<<nb, kP1Block, 0, st>>>(G, S, cp, c.B);
@@ -1690,12 +1863,17 @@ int ktemplate(int k) { return (k == 48 || k == 32 || k == 16) ? k : 0; }
 
 }  // namespace
 
-int64_t pml_parts(int Q, int k, int64_t N, int64_t n_rq) {
+int64_t pml_parts(int Q, int k, int64_t N, int64_t nfluid, int64_t n_rq) {
+    if (p1_fuses_pml(Q, step_uses_unions(Q, k, N))) return (nfluid + kP1Fused - 1) / kP1Fused;
     const int64_t per = step_uses_unions(Q, k, N)
                             ? int64_t(kV4Nodes) * kPmlRounds
                             : int64_t(kBlock / 64) * (Q == 27 ? Lattice<27>::NPW : Lattice<9>::NPW) *
                                   kPmlRounds;
     return (n_rq + per - 1) / per;
+}
+
+int64_t pml_part_lo(int Q, int k, int64_t N, int64_t nbulk) {
+    return p1_fuses_pml(Q, step_uses_unions(Q, k, N)) ? nbulk / kP1Fused : 0;
 }
 
 void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* nb) {
@@ -1707,8 +1885,11 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
     auto tiles32 = [&](int64_t n) { return (n + kV4Nodes - 1) / kV4Nodes; };
     const bool pml = G.nlayer > 0 && S.n_rq > 0;
     const int64_t nbnd = G.N - G.nfluid;
-    nb[kP1] = p1_uses_dma(Q, un) ? (G.N + kP1Chunk - 1) / kP1Chunk : (G.N + kP1Block - 1) / kP1Block;
-    nb[kPl] = pml ? S.nparts : 0;
+    const bool fused = p1_fuses_pml(Q, un);
+    nb[kP1] = fused ? (G.N + kP1Fused - 1) / kP1Fused
+                    : p1_uses_dma(Q, un) ? (G.N + kP1Chunk - 1) / kP1Chunk
+                                         : (G.N + kP1Block - 1) / kP1Block;
+    nb[kPl] = (pml && !fused) ? S.nparts : 0;
     nb[kPm] = pml ? 2 * Q : 0;
     nb[kPf] = pml ? (un ? tiles32(G.nlayer) : lanes(G.nlayer)) : 0;
     nb[kP2] = (pml && S.n_pml > 0)

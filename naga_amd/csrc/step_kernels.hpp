@@ -73,9 +73,11 @@ struct StepBuffers {
     int64_t n_ptiles;
     float* rho;      // [N] density (reference semantics, FP32)
     float* u;        // [D][N] velocity
-    double* partial; // [nparts][2Q] layer column sums (one row per pl block)
+    double* partial; // [2Q][nparts] layer column sums (one column entry per part:
+                     // a PML-local block, or a fused collision block)
     float* means;    // layer means: Q1 at [a], Q2 at [means_stride(Q) + a]
     int nparts;
+    int part_lo;     // parts [part_lo, nparts) hold the layer sums
     const float* src;  // [N] density source of this step (physical units), or null
     // Tile unions (D3Q27 four-per-lane kernels): the fluid nodes form tiles
     // of 64 or 32; ulist[uoff[t] .. uoff[t+1]) are the distinct stencil neighbours
@@ -138,8 +140,11 @@ void launch_source_field(const int32_t* nodes, const int32_t* off, const int32_t
 // events[t] is recorded before kernel t and events[kStepKernels] after the
 // last (kernels with no work still get their pair of marks).
 constexpr int kStepKernels = 12;
-// Blocks of the PML-local pass (one partial-sum row per block).
-int64_t pml_parts(int Q, int k, int64_t N, int64_t n_rq);
+// Parts of the layer sums: blocks of the PML-local pass, or, when the
+// collision pass carries the PML local terms, its 64-node blocks up to the
+// last layer node; parts below pml_part_lo hold no layer node.
+int64_t pml_parts(int Q, int k, int64_t N, int64_t nfluid, int64_t n_rq);
+int64_t pml_part_lo(int Q, int k, int64_t N, int64_t nbulk);
 template <int Q>
 void launch_step(const StepGeom& geo, const StepBuffers& buf,
                  const CollisionParams& cp, hipStream_t stream,
