@@ -190,16 +190,31 @@ __global__ __launch_bounds__(256) void k_knn(const CellGrid* gp, const float* re
 // On return the lane at position p holds unknown p in rhs[].
 // Returns the lane's position, or -1 for a zero pivot (wave-uniform).
 // ---------------------------------------------------------------------------
-template <int NMAX, int NR, class T>
-__device__ int wave_lu_solve(T (&row)[NMAX], T (&rhs)[NR], int n) {
+// The rows live in LDS (lu[lane * LD + c], LD = lu_stride(NMAX), filled by
+// the caller): the steps run as loops over the system's own size, whose
+// indices are not compile-time constants (a register array indexed at run
+// time would be placed in scratch memory; one unrolled for every size is
+// too large to build).  The odd stride spreads the lanes' rows over banks.
+// Only lanes below NMAX >= n own a row (lu holds NMAX rows); the others
+// read row 0 where a value is needed but never used, and write nothing.
+constexpr int lu_stride(int nmax) { return nmax + 1; }
+template <int NMAX, class T>
+__device__ __forceinline__ T* lu_row(T* lu) {
     const int lane = threadIdx.x & 63;
+    return lu + (lane < NMAX ? lane : 0) * lu_stride(NMAX);
+}
+
+template <int NMAX, int NR, class T>
+__device__ int wave_lu_solve(T* lu, T (&rhs)[NR], int n) {
+    constexpr int LD = lu_stride(NMAX);
+    const int lane = threadIdx.x & 63;
+    T* r = lu_row<NMAX>(lu);
+    __syncthreads();  // the rows are in place
     int pos = lane;
     bool done = lane >= n;
     bool singular = false;
-#pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-        if (j >= n) break;
-        T v = done ? T(-1) : fabs(row[j]);
+    for (int j = 0; j < n; ++j) {
+        T v = done ? T(-1) : fabs(r[j]);
         int bp = done ? 1 << 20 : pos, bl = lane;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
@@ -216,40 +231,38 @@ __device__ int wave_lu_solve(T (&row)[NMAX], T (&rhs)[NR], int n) {
         const int pos_p = __builtin_amdgcn_readlane(pos, p);
         if (pos == j) pos = pos_p;
         if (lane == p) pos = j;
-        const T piv = bcast_d(row[j], p);
+        const T* pr = lu + p * LD;  // the pivot row (every lane reads the same words)
+        const T piv = pr[j];
         T prhs[NR];
 #pragma unroll
         for (int m = 0; m < NR; ++m) prhs[m] = bcast_d(rhs[m], p);
         const bool upd = !done && lane != p;
-        const T l = upd ? row[j] / piv : T(0);
-#pragma unroll
-        for (int c = j + 1; c < NMAX; ++c) {
-            const T pc = bcast_d(row[c], p);
-            if (upd) row[c] -= l * pc;
-        }
+        const T l = upd ? r[j] / piv : T(0);
         if (upd) {
-            row[j] = l;
+            // columns n and above are zero in every row: nothing to update
+            for (int c = j + 1; c < n; ++c) r[c] -= l * pr[c];
+            r[j] = l;
 #pragma unroll
             for (int m = 0; m < NR; ++m) rhs[m] -= l * prhs[m];
         }
         if (lane == p) done = true;
+        __syncthreads();  // this step's rows before the next pivot is read
     }
     if (singular) return -1;
     // back substitution, unknown j lives in the lane at position j
     T s[NR];
 #pragma unroll
     for (int m = 0; m < NR; ++m) s[m] = rhs[m];
-#pragma unroll
-    for (int j = NMAX - 1; j >= 0; --j) {
-        if (j >= n) continue;
+    for (int j = n - 1; j >= 0; --j) {
         const unsigned long long mask = __ballot(pos == j && lane < n);
         const int lj = __builtin_amdgcn_readfirstlane(int(__ffsll((long long)mask) - 1));
-        const T ujj = bcast_d(row[j], lj);
+        const T ujj = lu[lj * LD + j];
+        const T rj = r[j];
 #pragma unroll
         for (int m = 0; m < NR; ++m) {
             const T xj = bcast_d(s[m], lj) / ujj;
             if (pos == j) rhs[m] = xj;
-            if (pos < j && lane < n) s[m] -= row[j] * xj;
+            if (pos < j && lane < n) s[m] -= rj * xj;
         }
     }
     return lane < n ? pos : 64;
@@ -275,9 +288,9 @@ __device__ __forceinline__ double rbf_eval(int rbf, double r2, double spacing) {
 
 // Row r of the augmented RBF matrix [[Phi, P], [P^T, 0]] in raw coordinates
 // (interpolation/detail/radial_point_method.cuh:349-398).
-template <int NMAX>
+template <int NMAX, class T>
 __device__ void g0_row(int D, const double (*xs)[3], int k, int rbf,
-                       double spacing, int r, double (&row)[NMAX]) {
+                       double spacing, int r, T* row) {
     const int n = k + D + 1;
 #pragma unroll
     for (int c = 0; c < NMAX; ++c) {
@@ -290,7 +303,7 @@ __device__ void g0_row(int D, const double (*xs)[3], int k, int rbf,
             else if (c < k)
                 v = (r == k) ? 1.0 : xs[c][r - k - 1];
         }
-        row[c] = v;
+        if (r < NMAX) row[c] = T(v);
     }
 }
 
@@ -370,11 +383,8 @@ __global__ __launch_bounds__(64) void k_divw(const float* src, const float* quer
     for (int d = 0; d < D; ++d) xi[d] = double(query[i * D + d]);
     const double delta = 0.05 * sqrt(double(d2[i * kmax + 1]));
     const double spacing = hbar[i / 16384];
-    double row64[NMAX];
-    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row64);
-    T row[NMAX];
-#pragma unroll
-    for (int c = 0; c < NMAX; ++c) row[c] = T(row64[c]);
+    __shared__ T lu[NMAX * lu_stride(NMAX)];
+    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, lu_row<NMAX>(lu));
     double b64[D], S[D];
     for (int d = 0; d < D; ++d) b64[d] = S[d] = 0.0;
     const int Nq = D == 2 ? 16 : 64;
@@ -391,7 +401,7 @@ __global__ __launch_bounds__(64) void k_divw(const float* src, const float* quer
     }
     T b[D];
     for (int d = 0; d < D; ++d) b[d] = T(b64[d]);
-    const int pos = wave_lu_solve<NMAX, D, T>(row, b, n);
+    const int pos = wave_lu_solve<NMAX, D, T>(lu, b, n);
     if (pos < 0) {
         if (lane == 0) info[i] = 1;
         return;
@@ -423,13 +433,10 @@ __global__ __launch_bounds__(64) void k_rpim(const float* src, const float* quer
     __syncthreads();
     double xq[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < D; ++d) xq[d] = double(query[i * D + d]);
-    double row64[NMAX];
-    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, row64);
-    T row[NMAX];
-#pragma unroll
-    for (int c = 0; c < NMAX; ++c) row[c] = T(row64[c]);
+    __shared__ T lu[NMAX * lu_stride(NMAX)];
+    g0_row<NMAX>(D, xs, k, rbf, spacing, lane, lu_row<NMAX>(lu));
     T b[1] = {T(lane < n ? gx_entry(D, xs, k, rbf, spacing, lane, xq) : 0.0)};
-    const int pos = wave_lu_solve<NMAX, 1, T>(row, b, n);
+    const int pos = wave_lu_solve<NMAX, 1, T>(lu, b, n);
     if (pos < 0) {
         if (lane == 0) info[i] = 1;
         return;
@@ -449,18 +456,19 @@ __global__ __launch_bounds__(64) void k_batched_lu(int n, int nrhs, int64_t batc
     if (b >= batch) return;
     const double* Ab = A + b * int64_t(n) * n;
     double* Bb = B + b * int64_t(n) * nrhs;
-    double row[NMAX];
-#pragma unroll
-    for (int c = 0; c < NMAX; ++c) row[c] = (lane < n && c < n) ? Ab[lane + int64_t(c) * n] : 0.0;
-    // right-hand sides are processed four at a time
+    __shared__ double lu[NMAX * lu_stride(NMAX)];
+    double* r = lu_row<NMAX>(lu);
+    // right-hand sides are processed four at a time, each group against a
+    // fresh copy of the matrix
     for (int m0 = 0; m0 < nrhs; m0 += 4) {
-        double r2[NMAX];
-#pragma unroll
-        for (int c = 0; c < NMAX; ++c) r2[c] = row[c];
+        __syncthreads();  // the previous group's factorisation is done with lu
+#pragma unroll 4
+        for (int c = 0; c < NMAX; ++c)
+            if (lane < NMAX) r[c] = (lane < n && c < n) ? Ab[lane + int64_t(c) * n] : 0.0;
         double rhs[4];
         for (int m = 0; m < 4; ++m)
             rhs[m] = (lane < n && m0 + m < nrhs) ? Bb[lane + int64_t(m0 + m) * n] : 0.0;
-        const int pos = wave_lu_solve<NMAX, 4, double>(r2, rhs, n);
+        const int pos = wave_lu_solve<NMAX, 4, double>(lu, rhs, n);
         if (pos < 0) {
             if (lane == 0) info[b] = 1;
             return;

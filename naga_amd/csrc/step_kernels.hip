@@ -263,6 +263,7 @@ __global__ __launch_bounds__(256) void k_means(StepGeom G, StepBuffers S, BatchA
     const BlockCtx bc = locate(B, G, S);
     const int col = int(bc.b);  // 0..2Q-1
     double s = 0.0;
+#pragma unroll 8
     for (int p = S.part_lo + threadIdx.x; p < S.nparts; p += 256)
         s += S.partial[int64_t(col) * S.nparts + p];
     red[threadIdx.x] = s;
@@ -1556,6 +1557,10 @@ __global__ __launch_bounds__(kBlock) void k_binterp_u(StepGeom G, StepBuffers S,
         union_copy<kBlock>(urows, ux, U, f);
         if (ACCUMULATE) union_copy<kBlock>(urows + kBUnionCap * kV4Lanes, ux, U, cl);
     }
+    const uint32_t own = uint32_t(G.nfluid + m) * 128u + uint32_t(q) * 16u;
+    // the node's own row (the collision increment it accumulates onto),
+    // loaded with the copies in flight (past the rows: zero, never stored)
+    const float4 b = ACCUMULATE ? ld4(f, own) : make_float4(0.f, 0.f, 0.f, 0.f);
     dma_fence();
     if (m >= nbnd) return;
     const float2* rp = reinterpret_cast<const float2*>(recs) + slot * G.kb;
@@ -1591,12 +1596,8 @@ __global__ __launch_bounds__(kBlock) void k_binterp_u(StepGeom G, StepBuffers S,
 #pragma unroll
             for (int e = 0; e < 4; ++e) acc[e] += rec[u].y * get(v[u], e);
     }
-    const uint32_t own = uint32_t(G.nfluid + m) * 128u + lane_off;
     float4 out = make_float4(acc[0], acc[1], acc[2], acc[3]);
-    if (ACCUMULATE) {
-        const float4 b = ld4(f, own);
-        out = make_float4(b.x + out.x, b.y + out.y, b.z + out.z, b.w + out.w);
-    }
+    if (ACCUMULATE) out = make_float4(b.x + out.x, b.y + out.y, b.z + out.z, b.w + out.w);
     st4(field, own, out);
 }
 
