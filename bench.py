@@ -287,7 +287,9 @@ def other_configs(threads: int) -> dict:
     eng.step_forward(300)
     wall = time.perf_counter() - t0
     out["3d_irregular"] = {"nodes": eng.num_nodes, "steps": 300, "setup_s": setup,
-                           "node_updates_per_s": eng.num_nodes * 300 / wall}
+                           "node_updates_per_s": eng.num_nodes * 300 / wall,
+                           "max_abs_drho_after": float(np.abs(
+                               eng.macroscopic(fresh=True)["drho"]).max())}
     eng.close()
     return out
 
@@ -306,8 +308,8 @@ def main() -> int:
                     help="skip the batched and served workloads")
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--served-requests", type=int, default=32)
-    ap.add_argument("--other-configs", action="store_true",
-                    help="also time the 2D plain and irregular 3D scenes")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the 2D plain and irregular 3D scenes")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -388,8 +390,11 @@ def main() -> int:
             except Exception as exc:  # reported, not fatal to the headline line
                 out[name] = {"error": f"{type(exc).__name__}: {exc}"}
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    if args.other_configs:
-        out["other_configs"] = other_configs(threads)
+    if not args.no_other_configs and not args.no_batched:
+        try:
+            out["other_configs"] = other_configs(threads)
+        except Exception as exc:  # reported, not fatal to the headline line
+            out["other_configs"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(threads)
     print(json.dumps(out, default=float))

@@ -947,7 +947,8 @@ void Batch::rebuild() {
         const int Q = e->lattice();
         const bool un = step_uses_unions(Q, e->stencil_k(), e->nodes());
         const bool tiles = e->scene_desc().S.ulist != nullptr;
-        return Q * 1000 + (un ? e->stencil_k() : 0) + (tiles ? 500 : 0);
+        const bool three = q9_three(Q, e->nodes());
+        return Q * 1000 + (un ? e->stencil_k() : 0) + (tiles ? 500 : 0) + (three ? 250 : 0);
     };
     std::vector<int> keys;
     for (Engine* e : scenes_) {
@@ -976,6 +977,7 @@ void Batch::rebuild() {
             descs[s] = e->scene_desc();
             if (e->stencil_k() != kcommon) kcommon = -1;
             P.ucap = std::max(P.ucap, descs[s].S.ucap);
+            P.kmax = std::max(P.kmax, e->stencil_k());
             any_sources_ = any_sources_ || descs[s].S.src != nullptr;
             int64_t nb[kStepKernels];
             step_block_counts(g.Q, descs[s].G, descs[s].S, nb);

@@ -1156,6 +1156,197 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     st4(gout, own, STAGE == 4 ? rk4_post4(pre, r, G.dt) : rk4_out4<STAGE>(g0, r, G.dt, z4, z4, z4));
 }
 
+// ---------------------------------------------------------------------------
+// D2Q9 stencil passes with three distributions per lane: a node's 36-byte
+// row is three 12-byte pieces, so one wavefront gathers 21 neighbour rows
+// per instruction (the lane-over-alpha kernels gather 7), and the records
+// are read by 3 lanes per node instead of 9.  The records of the block's
+// nodes are staged in LDS with one pad float per node row (distinct banks
+// for the 21 nodes of a wavefront).  Same products in the same order as the
+// lane-over-alpha kernels.
+// ---------------------------------------------------------------------------
+constexpr int kQ9Lanes = 3;                          // lanes per node
+constexpr int kQ9Npw = 21;                           // nodes per wavefront
+constexpr int kQ9Npb = kQ9Npw * (kBlock / 64);       // nodes per block (84)
+
+struct Q9Lane {
+    int slot, q;   // node slot in the block, piece (distributions 3q .. 3q + 2)
+    bool active;   // lane 63 of a wavefront serves no node
+    __device__ Q9Lane() {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int sub = lane / kQ9Lanes;
+        active = sub < kQ9Npw;
+        q = active ? lane - sub * kQ9Lanes : 0;
+        slot = wave * kQ9Npw + (active ? sub : 0);
+    }
+};
+
+typedef float f3v __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ f3v ld3(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+    f3v o;
+    o.x = __uint_as_float(v[0]);
+    o.y = __uint_as_float(v[1]);
+    o.z = __uint_as_float(v[2]);
+    return o;
+}
+__device__ __forceinline__ void st3(__amdgpu_buffer_rsrc_t r, uint32_t off, f3v x) {
+    typedef unsigned u3v __attribute__((ext_vector_type(3)));
+    u3v v;
+    v.x = __float_as_uint(x.x);
+    v.y = __float_as_uint(x.y);
+    v.z = __float_as_uint(x.z);
+    __builtin_amdgcn_raw_buffer_store_b96(v, r, off, 0, 0);
+}
+
+// records of nodes [node0, node0 + cnt) (rows of RW floats) into lds rows
+// of RW + 1 floats
+__device__ __forceinline__ void stage_padded(const float* slab, int64_t node0, int cnt, int RW,
+                                             float* lds) {
+    const float* src = slab + node0 * int64_t(RW);
+    const int tot = cnt * RW;
+    for (int t = threadIdx.x; t < tot; t += kBlock) lds[t + t / RW] = src[t];
+}
+
+__device__ __forceinline__ float c9(int a, int d) { return float(Lattice<9>::c(a, d)); }
+
+// sum_s (c_a . W_s) fin(idx_s)_a for the lane's three distributions
+template <int BATCH>
+__device__ __forceinline__ f3v stencil_sum3(const float* rp, int K, __amdgpu_buffer_rsrc_t fin,
+                                            int q) {
+    float cx[3], cy[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        cx[e] = c9(3 * q + e, 0);
+        cy[e] = c9(3 * q + e, 1);
+    }
+    f3v acc = {0.0f, 0.0f, 0.0f};
+    for (int s0 = 0; s0 < K; s0 += BATCH) {
+        float wx[BATCH], wy[BATCH];
+        f3v gv[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            const int s = s0 + u;
+            const bool in = s < K;
+            const float* r = rp + (in ? s : 0) * 3;
+            const uint32_t j = __float_as_uint(r[0]);
+            wx[u] = in ? r[1] : 0.0f;
+            wy[u] = in ? r[2] : 0.0f;
+            gv[u] = ld3(fin, j * 36u + uint32_t(q) * 12u);
+        }
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            acc.x += (cx[0] * wx[u] + cy[0] * wy[u]) * gv[u].x;
+            acc.y += (cx[1] * wx[u] + cy[1] * wy[u]) * gv[u].y;
+            acc.z += (cx[2] * wx[u] + cy[2] * wy[u]) * gv[u].z;
+        }
+    }
+    return acc;
+}
+
+template <int STAGE, int BATCH>
+__global__ __launch_bounds__(kBlock) void k_stage3(StepGeom G, StepBuffers S, BatchArgs B) {
+    extern __shared__ float srec3[];
+    const BlockCtx bc = locate(B, G, S);
+    const int K = G.k, RW = 3 * K;
+    const int64_t node0 = xcd_tile(bc.b, bc.nb) * kQ9Npb;
+    const Q9Lane ln;
+    const int64_t i = node0 + ln.slot;
+    const uint64_t rows = uint64_t(G.N) * 36u;
+    const uint32_t own = uint32_t(i) * 36u + uint32_t(ln.q) * 12u;
+    const __amdgpu_buffer_rsrc_t rg = rsrc(S.g, rows);
+    // own rows first: their latency overlaps the record staging
+    const bool fluid = i < G.nfluid;
+    const f3v g0 = ld3(rg, own);  // past N: zero
+    f3v y1 = {0.f, 0.f, 0.f}, y2 = y1, y3 = y1;
+    if (STAGE == 4 && fluid) {
+        y1 = ld3(rsrc(S.A, rows), own);
+        y2 = ld3(rsrc(S.B, rows), own);
+        y3 = ld3(rsrc(S.acc, rows), own);
+    }
+    const int64_t nf = G.nfluid - node0;
+    stage_padded(S.slab, node0, nf <= 0 ? 0 : (nf < kQ9Npb ? int(nf) : kQ9Npb), RW, srec3);
+    __syncthreads();
+    if (!ln.active || i >= G.N) return;
+    if (STAGE == 1 && !fluid) {
+        st3(rsrc(S.A, rows), own, g0);
+        st3(rsrc(S.B, rows), own, g0);
+        st3(rsrc(S.acc, rows), own, g0);
+        return;
+    }
+    if (!fluid) return;
+    const f3v acc = stencil_sum3<BATCH>(srec3 + ln.slot * (RW + 1), K,
+                                        rsrc(stage_in(S, STAGE), rows), ln.q);
+    f3v out;
+    out.x = rk4_out<STAGE>(g0.x, -acc.x, G.dt, y1.x, y2.x, y3.x);
+    out.y = rk4_out<STAGE>(g0.y, -acc.y, G.dt, y1.y, y2.y, y3.y);
+    out.z = rk4_out<STAGE>(g0.z, -acc.z, G.dt, y1.z, y2.z, y3.z);
+    st3(rsrc(stage_out(S, STAGE), rows), own, out);
+}
+
+// p4 for D2Q9, three per lane: selective filter on fluid nodes, bounce-back
+// on boundary nodes (the arithmetic of k_p4).
+__global__ __launch_bounds__(kBlock) void k_p4_3(StepGeom G, StepBuffers S, BatchArgs B) {
+    extern __shared__ float srec3[];
+    const BlockCtx bc = locate(B, G, S);
+    const int K = G.k, RW = 2 * K;
+    const int64_t node0 = xcd_tile(bc.b, bc.nb) * kQ9Npb;
+    const Q9Lane ln;
+    const int64_t i = node0 + ln.slot;
+    const uint64_t rows = uint64_t(G.N) * 36u;
+    const uint32_t own = uint32_t(i) * 36u + uint32_t(ln.q) * 12u;
+    const __amdgpu_buffer_rsrc_t r1 = rsrc(S.A, rows);
+    const f3v si = ld3(r1, own);
+    const int64_t nf = G.nfluid - node0;
+    stage_padded(S.fslab, node0, nf <= 0 ? 0 : (nf < kQ9Npb ? int(nf) : kQ9Npb), RW, srec3);
+    __syncthreads();
+    if (!ln.active || i >= G.N) return;
+    f3v out;
+    if (i < G.nfluid) {
+        const float* rp = srec3 + ln.slot * (RW + 1);
+        f3v sum = {0.0f, 0.0f, 0.0f};
+        for (int s0 = 0; s0 < K; s0 += 8) {
+            float w[8];
+            f3v gv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int s = s0 + u;
+                const bool in = s < K;
+                const uint32_t j = __float_as_uint(rp[(in ? s : 0) * 2]);
+                w[u] = in ? rp[s * 2 + 1] : 0.0f;
+                gv[u] = ld3(r1, j * 36u + uint32_t(ln.q) * 12u);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                sum.x += w[u] * (gv[u].x - si.x);
+                sum.y += w[u] * (gv[u].y - si.y);
+                sum.z += w[u] * (gv[u].z - si.z);
+            }
+        }
+        out.x = si.x + G.filter * sum.x;
+        out.y = si.y + G.filter * sum.y;
+        out.z = si.z + G.filter * sum.z;
+    } else {
+        const int64_t b = i - G.nfluid;
+        const float n0 = S.normals[b * 2], n1 = S.normals[b * 2 + 1];
+        const float* row = S.A + i * 9;
+        float o3[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const int a = 3 * ln.q + e;
+            float cn = 0.0f;
+            cn += c9(a, 0) * n0;
+            cn += c9(a, 1) * n1;
+            const int src = cn > 0.0f ? Lattice<9>::bb(a) : a;
+            o3[e] = row[src];
+        }
+        out.x = o3[0];
+        out.y = o3[1];
+        out.z = o3[2];
+    }
+    st3(rsrc(S.g, rows), own, out);
+}
+
 // sum_s w_s (g_s - g_i) over the staged filter records rp ({index field, w}),
 // rows fetched by fetch(index field); four slots per lane.
 template <int K, int BATCH, class Fetch>
@@ -1691,6 +1882,16 @@ bool step_uses_unions(int Q, int k, int64_t N) {
     return Q == 27 && on && (k == 48 || k == 32 || k == 16) && N < (int64_t(1) << 25);
 }
 
+// D2Q9 scenes run the three-per-lane stencil kernels (12-byte pieces with
+// 32-bit buffer offsets: fewer than 2^26 nodes) unless NAGA_Q9_THREE=0.
+bool q9_three(int Q, int64_t N) {
+    static const bool on = [] {
+        const char* e = std::getenv("NAGA_Q9_THREE");
+        return !(e && e[0] == '0');
+    }();
+    return on && Q == 9 && N < (int64_t(1) << 26);
+}
+
 // Nodes per block of the D3Q27 stencil kernels: the union tile size with
 // tile unions, else 32, 64 or 128 (records of all of them staged in at
 // most 64 KB of LDS; NAGA_V4_NPB overrides the default 32).
@@ -1733,6 +1934,7 @@ struct PassCfg {
     BatchArgs B;
     int gbatch;    // gather batch of the one-per-lane stage kernels (0: default)
     bool tiles;    // tile unions built (slab index fields are union positions)
+    int kmax;      // largest stencil width of the launch's scenes (LDS of the D2Q9 kernels)
 };
 
 template <int Q, int STAGE, int KT>
@@ -1808,6 +2010,24 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         if (c.ktmpl == 48) launch_v4<48>(pass, nb, G, S, c, st);
         else if (c.ktmpl == 32) launch_v4<32>(pass, nb, G, S, c, st);
         else launch_v4<16>(pass, nb, G, S, c, st);
+        check_launch("step pass");
+        return;
+    }
+    if (Q == 9 && q9_three(Q, G.N > 0 ? G.N : 1) && (pass == kP4 || (pass >= kS1 && pass <= kS4))) {
+        const size_t lds = size_t(kQ9Npb) * ((pass == kP4 ? 2 : 3) * c.kmax + 1) * 4u;
+        const int b = c.gbatch > 0 ? c.gbatch : 8;
+        switch (pass) {
+            case kP4: k_p4_3<<<nb, kBlock, lds, st>>>(G, S, c.B); break;
+#define NAGA_S3(STG)                                                                          \
+    case kS##STG:                                                                             \
+        if (b == 32) k_stage3<STG, 32><<<nb, kBlock, lds, st>>>(G, S, c.B);                   \
+        else if (b == 16) k_stage3<STG, 16><<<nb, kBlock, lds, st>>>(G, S, c.B);              \
+        else k_stage3<STG, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);                            \
+        break;
+            NAGA_S3(1) NAGA_S3(2) NAGA_S3(3) NAGA_S3(4)
+#undef NAGA_S3
+            default: break;
+        }
         check_launch("step pass");
         return;
     }
@@ -1902,6 +2122,11 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
     nb[kP4] = un ? tiles(G.N) + ns : lanes(G.N);
     nb[kS1] = un ? tiles(G.N) + ns : lanes(G.N);
     nb[kS2] = nb[kS3] = nb[kS4] = un ? tiles(G.nfluid) + ns : lanes(G.nfluid);
+    if (q9_three(Q, G.N)) {
+        auto t84 = [](int64_t n) { return (n + kQ9Npb - 1) / kQ9Npb; };
+        nb[kP4] = nb[kS1] = t84(G.N);
+        nb[kS2] = nb[kS3] = nb[kS4] = t84(G.nfluid);
+    }
     nb[kP5] = nb[kP3];
 }
 
@@ -1910,7 +2135,7 @@ void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S, hipStr
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
     const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0,
-                    S.ulist != nullptr};
+                    S.ulist != nullptr, G.k};
     launch_pass<Q>(kS1 + stage - 1, nb[kS1 + stage - 1], G, S, CollisionParams{}, c, st);
 }
 
@@ -1920,7 +2145,7 @@ void launch_step(const StepGeom& G, const StepBuffers& S, const CollisionParams&
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
     const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0,
-                    S.ulist != nullptr};
+                    S.ulist != nullptr, G.k};
     for (int p = 0; p < kStepKernels; ++p) {
         if (ev) hip_check(hipEventRecord(ev[p], st), "event");
         launch_pass<Q>(p, nb[p], G, S, cp, c, st);
@@ -1936,7 +2161,7 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t st) {
         const PassCfg c{plan.k_template, plan.unions, plan.ucap,
                         BatchArgs{plan.dev_scenes, plan.dev_bstart + p * (plan.nscenes + 1),
                                   plan.nscenes},
-                        plan.gather_batch, plan.tiles};
+                        plan.gather_batch, plan.tiles, plan.kmax};
         launch_pass<Q>(p, plan.blocks[p], G, S, CollisionParams{}, c, st);
     }
 }

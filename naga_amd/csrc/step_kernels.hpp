@@ -165,6 +165,7 @@ struct BatchPlan {
     int ucap = 0;                       // max LDS union rows over the scenes
     int gather_batch = 0;               // neighbours gathered per unrolled step (0: default)
     bool tiles = false;                 // tile unions built for every scene
+    int kmax = 0;                       // largest stencil width over the scenes
 };
 // Block counts of each pass for one scene (the same counts launch_step uses).
 void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* blocks);
@@ -175,6 +176,8 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 // the tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
 // positions: D3Q27, k in {16, 32, 48}, fewer than 2^25 nodes.
 bool step_uses_unions(int Q, int k, int64_t N);
+// Whether the D2Q9 stencil passes run three distributions per lane.
+bool q9_three(int Q, int64_t N);
 // Nodes per union tile (= per block of the union kernels) for stencil width
 // k: 64 for k <= 32 (5.4 union rows per node against 7.0 with 32-node
 // tiles, two blocks per CU; 1.40 against 1.44 ms per step on the 1M-node
