@@ -1284,6 +1284,85 @@ __global__ __launch_bounds__(kBlock) void k_stage3(StepGeom G, StepBuffers S, Ba
     st3(rsrc(stage_out(S, STAGE), rows), own, out);
 }
 
+// p2 for D2Q9, three per lane: the PML divergence over the listed nodes
+// (the arithmetic of k_p2), each node's records staged by its three lanes.
+__global__ __launch_bounds__(kBlock) void k_p2_3(StepGeom G, StepBuffers S, BatchArgs B) {
+    extern __shared__ float srec3[];
+    const BlockCtx bc = locate(B, G, S);
+    const int K = G.k, RW = 3 * K;
+    const Q9Lane ln;
+    const int64_t item = xcd_tile(bc.b, bc.nb) * kQ9Npb + ln.slot;
+    const bool valid = ln.active && item < S.n_pml;
+    const int64_t i = valid ? S.pml_list[item] : 0;
+    float* rp = srec3 + ln.slot * (RW + 1);
+    if (valid) {
+        const float* src = S.slab + i * int64_t(RW);
+        for (int t = ln.q; t < RW; t += kQ9Lanes) rp[t] = src[t];
+    }
+    __syncthreads();
+    if (!valid) return;
+    const uint64_t rows = uint64_t(G.N) * 36u;
+    const f3v acc = stencil_sum3<8>(rp, K, rsrc(S.Gf, rows), ln.q);
+    const __amdgpu_buffer_rsrc_t ra = rsrc(S.A, rows);
+    const uint32_t own = uint32_t(i) * 36u + uint32_t(ln.q) * 12u;
+    f3v a = ld3(ra, own);
+    a.x += acc.x;
+    a.y += acc.y;
+    a.z += acc.z;
+    st3(ra, own, a);
+}
+
+// Boundary interpolation for D2Q9, three per lane (the arithmetic of
+// k_binterp): accumulate = true (p3) interpolates s1 minus the sources'
+// collision increments onto the row's own increment, false (p5) overwrites.
+template <bool ACCUMULATE>
+__global__ __launch_bounds__(kBlock) void k_binterp3(StepGeom G, StepBuffers S, BatchArgs B) {
+    const BlockCtx bc = locate(B, G, S);
+    const Q9Lane ln;
+    const int64_t nbnd = G.N - G.nfluid;
+    const int64_t m = xcd_tile(bc.b, bc.nb) * kQ9Npb + ln.slot;
+    if (!ln.active || m >= nbnd) return;
+    float* field = ACCUMULATE ? S.A : S.g;
+    const uint64_t rows = uint64_t(G.N) * 36u;
+    const __amdgpu_buffer_rsrc_t f = rsrc(field, rows), cl = rsrc(S.coll, rows);
+    const float* rec = S.bslab + m * int64_t(G.kb) * 2;
+    const uint32_t lane_off = uint32_t(ln.q) * 12u;
+    const uint32_t own = uint32_t(G.nfluid + m) * 36u + lane_off;
+    const f3v b = ACCUMULATE ? ld3(f, own) : f3v{0.0f, 0.0f, 0.0f};
+    f3v acc = {0.0f, 0.0f, 0.0f};
+    for (int s0 = 0; s0 < G.kb; s0 += 8) {
+        float w[8];
+        f3v v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int s = s0 + u;
+            const bool in = s < G.kb;
+            const uint32_t j = __float_as_uint(rec[2 * (in ? s : 0)]);
+            w[u] = in ? rec[2 * s + 1] : 0.0f;
+            v[u] = ld3(f, j * 36u + lane_off);
+            if (ACCUMULATE) {
+                const f3v c = ld3(cl, j * 36u + lane_off);
+                v[u].x = v[u].x - c.x;
+                v[u].y = v[u].y - c.y;
+                v[u].z = v[u].z - c.z;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            acc.x += w[u] * v[u].x;
+            acc.y += w[u] * v[u].y;
+            acc.z += w[u] * v[u].z;
+        }
+    }
+    f3v out = acc;
+    if (ACCUMULATE) {
+        out.x = b.x + acc.x;
+        out.y = b.y + acc.y;
+        out.z = b.z + acc.z;
+    }
+    st3(f, own, out);
+}
+
 // p4 for D2Q9, three per lane: selective filter on fluid nodes, bounce-back
 // on boundary nodes (the arithmetic of k_p4).
 __global__ __launch_bounds__(kBlock) void k_p4_3(StepGeom G, StepBuffers S, BatchArgs B) {
@@ -2013,10 +2092,15 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         check_launch("step pass");
         return;
     }
-    if (Q == 9 && q9_three(Q, G.N > 0 ? G.N : 1) && (pass == kP4 || (pass >= kS1 && pass <= kS4))) {
+    if (Q == 9 && q9_three(Q, G.N > 0 ? G.N : 1) &&
+        (pass == kP2 || pass == kP3 || pass == kP4 || pass == kP5 ||
+         (pass >= kS1 && pass <= kS4))) {
         const size_t lds = size_t(kQ9Npb) * ((pass == kP4 ? 2 : 3) * c.kmax + 1) * 4u;
         const int b = c.gbatch > 0 ? c.gbatch : 8;
         switch (pass) {
+            case kP2: k_p2_3<<<nb, kBlock, lds, st>>>(G, S, c.B); break;
+            case kP3: k_binterp3<true><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
+            case kP5: k_binterp3<false><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
             case kP4: k_p4_3<<<nb, kBlock, lds, st>>>(G, S, c.B); break;
 #define NAGA_S3(STG)                                                                          \
     case kS##STG:                                                                             \
@@ -2126,6 +2210,8 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
         auto t84 = [](int64_t n) { return (n + kQ9Npb - 1) / kQ9Npb; };
         nb[kP4] = nb[kS1] = t84(G.N);
         nb[kS2] = nb[kS3] = nb[kS4] = t84(G.nfluid);
+        nb[kP2] = (pml && S.n_pml > 0) ? t84(S.n_pml) : 0;
+        nb[kP3] = nb[kP5] = nbnd > 0 ? t84(nbnd) : 0;
     }
     nb[kP5] = nb[kP3];
 }
