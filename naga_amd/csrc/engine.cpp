@@ -236,6 +236,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     fslab_.allocate(nfluid * k * 2);
     launch_make_slab(nfluid, k, 1, idx_.get(), filt_.get(), fslab_.get(), st);
     ucap_ = 0;
+    self_first_ = 0;
     ulist_ = DeviceBuffer<uint32_t>();
     uoff_ = DeviceBuffer<int64_t>();
     usplit_ = DeviceBuffer<int32_t>();
@@ -332,6 +333,20 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         slab_ = std::move(ps);
         fslab_ = std::move(pf);
         ucap_ = std::min(maxu, cap);
+        // whether every fluid node's first stencil entry is itself (the kNN
+        // puts it first unless another node sits at the same point)
+        std::vector<uint32_t> first(size_t(std::max<int64_t>(nfluid, 1)));
+        NAGA_HIP(hipMemcpy2DAsync(first.data(), sizeof(uint32_t), idx_.get(),
+                                  size_t(k) * sizeof(uint32_t), sizeof(uint32_t), size_t(nfluid),
+                                  hipMemcpyDeviceToHost, st));
+        NAGA_HIP(hipStreamSynchronize(st));
+        const char* ol_env = std::getenv("NAGA_OWN_LDS");
+        self_first_ = (ol_env && ol_env[0] == '0') ? 0 : 1;
+        for (int64_t i = 0; i < nfluid && self_first_; ++i)
+            if (first[size_t(i)] != uint32_t(i)) {
+                self_first_ = 0;
+                break;
+            }
     }
 
     marks.mark();  // filter weights, slabs and tile unions done
@@ -521,6 +536,7 @@ StepBuffers Engine::buffers() {
     S.bulist = bulist_.get();
     S.buoff = buoff_.get();
     S.ucap = ucap_;
+    S.self_first = self_first_;
     return S;
 }
 

@@ -168,6 +168,7 @@ class Engine {
     DeviceBuffer<uint32_t> bulist_;
     DeviceBuffer<int64_t> buoff_;
     int ucap_ = 0;
+    int self_first_ = 0;
 
     struct ProbeSet {
         int64_t n = 0;

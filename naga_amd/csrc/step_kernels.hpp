@@ -95,6 +95,9 @@ struct StepBuffers {
     const int32_t* split_tiles;
     int64_t n_split;
     int ucap;
+    int self_first;  // every fluid node's first stencil entry is the node itself
+                     // (its own row is then in its tile's union, record 0);
+                     // NAGA_OWN_LDS=0 clears it
     // The same for the PML divergence, whose unions hold the layer
     // neighbours and one row outside the layer (Gf is zero there): its
     // packed records, lists, offsets and split tiles.
