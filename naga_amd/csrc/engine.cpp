@@ -231,6 +231,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     marks.mark();  // divergence weights done
     filt_.allocate(N * k);
     launch_filter_weights(D, pts_.get(), N, idx_.get(), k, cfg_.h, filt_.get(), st, kvar_.get());
+    if (kvar_.size() == size_t(N)) launch_self_past_k(N, k, kvar_.get(), idx_.get(), st);
     slab_.allocate(nfluid * k * (1 + D));
     launch_make_slab(nfluid, k, D, idx_.get(), W_.get(), slab_.get(), st);
     fslab_.allocate(nfluid * k * 2);
@@ -335,18 +336,9 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         ucap_ = std::min(maxu, cap);
         // whether every fluid node's first stencil entry is itself (the kNN
         // puts it first unless another node sits at the same point)
-        std::vector<uint32_t> first(size_t(std::max<int64_t>(nfluid, 1)));
-        NAGA_HIP(hipMemcpy2DAsync(first.data(), sizeof(uint32_t), idx_.get(),
-                                  size_t(k) * sizeof(uint32_t), sizeof(uint32_t), size_t(nfluid),
-                                  hipMemcpyDeviceToHost, st));
-        NAGA_HIP(hipStreamSynchronize(st));
         const char* ol_env = std::getenv("NAGA_OWN_LDS");
-        self_first_ = (ol_env && ol_env[0] == '0') ? 0 : 1;
-        for (int64_t i = 0; i < nfluid && self_first_; ++i)
-            if (first[size_t(i)] != uint32_t(i)) {
-                self_first_ = 0;
-                break;
-            }
+        self_first_ = (ol_env && ol_env[0] == '0') ? 0
+                                                    : int(stencils_self_first(nfluid, k, idx_.get(), st));
     }
 
     marks.mark();  // filter weights, slabs and tile unions done

@@ -704,6 +704,20 @@ __global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, int t
                 sb ? sb + int64_t(h) * k * (1 + ncb) : nullptr, ncb);
 }
 
+// flag stays 1 if idx[i * k] == i for every i < n, else becomes 0.
+__global__ void k_self_first(int64_t n, int k, const uint32_t* idx, int* flag) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && idx[i * k] != uint32_t(i)) atomicAnd(flag, 0);
+}
+
+// Stencil entries s >= kvar[i] (zero weights) of node i point at i itself.
+__global__ void k_self_past_k(int64_t n, int k, const uint8_t* kvar, uint32_t* idx) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t i = e / k;
+    if (int(e - i * k) >= int(kvar[i])) idx[e] = uint32_t(i);
+}
+
 // out[e] = idx[e] if it is in [lo, hi), else zero_row.
 __global__ void k_layer_only(int64_t n, const uint32_t* idx, uint32_t lo, uint32_t hi,
                              uint32_t zero_row, uint32_t* out) {
@@ -1154,6 +1168,26 @@ void launch_pack_filter_records(int64_t n, int k, const float* src, float* dst,
     k_pack_filter_records<<<(tot + 255) / 256, 256, 0, st>>>(n, k, src,
                                                              reinterpret_cast<uint32_t*>(dst));
     hip_check(hipGetLastError(), "pack filter records");
+}
+
+bool stencils_self_first(int64_t n, int k, const uint32_t* idx, hipStream_t st) {
+    if (n == 0) return true;
+    DeviceBuffer<int> flag(1);
+    const int one = 1;
+    flag.upload(&one, 1, st);
+    k_self_first<<<(n + 255) / 256, 256, 0, st>>>(n, k, idx, flag.get());
+    hip_check(hipGetLastError(), "stencil check");
+    int out = 0;
+    flag.download(&out, 1, st);
+    NAGA_HIP(hipStreamSynchronize(st));
+    return out == 1;
+}
+
+void launch_self_past_k(int64_t n, int k, const uint8_t* kvar, uint32_t* idx, hipStream_t st) {
+    if (n == 0) return;
+    const int64_t tot = n * k;
+    k_self_past_k<<<(tot + 255) / 256, 256, 0, st>>>(n, k, kvar, idx);
+    hip_check(hipGetLastError(), "stencil padding");
 }
 
 void launch_tile_unions(int64_t nfluid, int k, int tile_nodes, const uint32_t* idx, int cap,

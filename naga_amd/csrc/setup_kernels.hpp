@@ -107,6 +107,15 @@ void launch_divergence_weights(int D, const float* src, const float* query,
                                int* info, hipStream_t stream,
                                const uint8_t* kvar = nullptr, int solve_bits = 64);
 
+// Whether idx[i * k] == i for every node i < n (the node is its own first
+// neighbour); synchronises the stream.
+bool stencils_self_first(int64_t n, int k, const uint32_t* idx, hipStream_t stream);
+
+// Per-node stencil widths: the entries s >= kvar[i] of node i (zero weights)
+// are pointed at i itself, so they add no rows to the tile unions and no
+// node to the PML divergence list.
+void launch_self_past_k(int64_t n, int k, const uint8_t* kvar, uint32_t* idx, hipStream_t stream);
+
 // RPIM interpolation weights: w[q*k + s]; the queries of a group of `group`
 // consecutive queries share the stencil idx[(q / group)*k ..]
 // (radial_point_method::compute_weights' group_size).
