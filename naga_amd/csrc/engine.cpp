@@ -237,7 +237,11 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     fslab_.allocate(nfluid * k * 2);
     launch_make_slab(nfluid, k, 1, idx_.get(), filt_.get(), fslab_.get(), st);
     ucap_ = 0;
+    ucap_f_ = 0;
     self_first_ = 0;
+    tile_ng_ = DeviceBuffer<int32_t>();
+    rec_off_ = DeviceBuffer<int64_t>();
+    frec_off_ = DeviceBuffer<int64_t>();
     ulist_ = DeviceBuffer<uint32_t>();
     uoff_ = DeviceBuffer<int64_t>();
     usplit_ = DeviceBuffer<int32_t>();
@@ -261,6 +265,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     tile_nodes_ = 0;
     if (!(un_env && un_env[0] == '0') && step_uses_unions(Q, k, N) && nfluid > 0) {
         // tile unions; slab index fields become union positions
+        std::vector<int32_t> last_counts;  // union rows per tile of the last set built
         const int T = union_tile_nodes(k);
         tile_nodes_ = T;
         const int64_t ntiles = (nfluid + T - 1) / T;
@@ -278,6 +283,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
                          DeviceBuffer<int32_t>& usplit, DeviceBuffer<int32_t>& split_tiles,
                          std::vector<int32_t>& split_of, int64_t& n_split) {
             DeviceBuffer<int32_t> counts(ntiles);
+            last_counts.clear();
             usplit.allocate(ntiles);
             launch_tile_unions(nfluid, k, T, idx_in, split ? cap : 0, counts.get(), usplit.get(),
                                nullptr, nullptr, nullptr, 0, nullptr, 0, st);
@@ -288,6 +294,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
             std::vector<int64_t> off(ntiles + 1, 0);
             int maxu = 0;
             split_of.assign(ntiles, -1);
+            last_counts = hc;
             for (int64_t t = 0; t < ntiles; ++t) {
                 off[t + 1] = off[t] + hc[t];
                 maxu = std::max(maxu, hc[t]);
@@ -334,11 +341,64 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         slab_ = std::move(ps);
         fslab_ = std::move(pf);
         ucap_ = std::min(maxu, cap);
+        // the filter pass stages its union rows with a cap at the 95th
+        // percentile of the tiles' unions (multiple of 8): its records are
+        // smaller, and the fewer rows let more blocks share a CU; the tiles
+        // above it read their rows through the list
+        {
+            std::vector<int32_t> c = last_counts;
+            int p95 = ucap_;
+            if (!c.empty()) {
+                const size_t at = size_t(0.95 * double(c.size() - 1));
+                std::nth_element(c.begin(), c.begin() + at, c.end());
+                p95 = (c[at] + 7) / 8 * 8;
+            }
+            ucap_f_ = std::max(8, std::min(ucap_, p95));
+        }
         // whether every fluid node's first stencil entry is itself (the kNN
         // puts it first unless another node sits at the same point)
         const char* ol_env = std::getenv("NAGA_OWN_LDS");
         self_first_ = (ol_env && ol_env[0] == '0') ? 0
                                                     : int(stencils_self_first(nfluid, k, idx_.get(), st));
+        // per-node stencil widths: each tile keeps the groups of 8 records
+        // its widest node needs (ragged record rows), so the stencil passes
+        // read and sum k_tile records per node instead of k
+        const char* rg_env = std::getenv("NAGA_RAGGED");
+        if (kvar_.size() == size_t(N) && !(rg_env && rg_env[0] == '0')) {
+            std::vector<uint8_t> hk(static_cast<size_t>(N));
+            kvar_.download(hk.data(), size_t(N), st);
+            NAGA_HIP(hipStreamSynchronize(st));
+            std::vector<int32_t> ng(static_cast<size_t>(ntiles));
+            std::vector<int64_t> ro(static_cast<size_t>(ntiles) + 1, 0),
+                fo(static_cast<size_t>(ntiles) + 1, 0);
+            for (int64_t t = 0; t < ntiles; ++t) {
+                const int64_t a = t * T, b = std::min<int64_t>(a + T, nfluid);
+                int m = 1;
+                for (int64_t i = a; i < b; ++i) m = std::max(m, (int(hk[size_t(i)]) + 7) / 8);
+                ng[size_t(t)] = std::min(m, k / 8);
+                ro[size_t(t) + 1] = ro[size_t(t)] + (b - a) * ragged_div_units(ng[size_t(t)]);
+                fo[size_t(t) + 1] = fo[size_t(t)] + (b - a) * ragged_filter_units(ng[size_t(t)]);
+            }
+            tile_ng_ = to_device(ng, st);
+            rec_off_ = to_device(ro, st);
+            frec_off_ = to_device(fo, st);
+            DeviceBuffer<float> rs(size_t(ro[size_t(ntiles)]) * 4),
+                rf(size_t(fo[size_t(ntiles)]) * 4);
+            launch_ragged_records(nfluid, T, packed_div_units(k), 7, tile_ng_.get(),
+                                  rec_off_.get(), slab_.get(), rs.get(), st);
+            launch_ragged_records(nfluid, T, packed_filter_units(k), 3, tile_ng_.get(),
+                                  frec_off_.get(), fslab_.get(), rf.get(), st);
+            if (p2slab_.size() > 0) {
+                DeviceBuffer<float> r2(size_t(ro[size_t(ntiles)]) * 4);
+                launch_ragged_records(nfluid, T, packed_div_units(k), 7, tile_ng_.get(),
+                                      rec_off_.get(), p2slab_.get(), r2.get(), st);
+                NAGA_HIP(hipStreamSynchronize(st));
+                p2slab_ = std::move(r2);
+            }
+            NAGA_HIP(hipStreamSynchronize(st));
+            slab_ = std::move(rs);
+            fslab_ = std::move(rf);
+        }
     }
 
     marks.mark();  // filter weights, slabs and tile unions done
@@ -528,7 +588,11 @@ StepBuffers Engine::buffers() {
     S.bulist = bulist_.get();
     S.buoff = buoff_.get();
     S.ucap = ucap_;
+    S.ucap_f = ucap_f_;
     S.self_first = self_first_;
+    S.tile_ng = tile_ng_.size() ? tile_ng_.get() : nullptr;
+    S.rec_off = rec_off_.size() ? rec_off_.get() : nullptr;
+    S.frec_off = frec_off_.size() ? frec_off_.get() : nullptr;
     return S;
 }
 
@@ -956,7 +1020,9 @@ void Batch::rebuild() {
         const bool un = step_uses_unions(Q, e->stencil_k(), e->nodes());
         const bool tiles = e->scene_desc().S.ulist != nullptr;
         const bool three = q9_three(Q, e->nodes());
-        return Q * 1000 + (un ? e->stencil_k() : 0) + (tiles ? 500 : 0) + (three ? 250 : 0);
+        const bool ragged = e->scene_desc().S.tile_ng != nullptr;
+        return Q * 1000 + (un ? e->stencil_k() : 0) + (tiles ? 500 : 0) + (three ? 250 : 0) +
+               (ragged ? 125 : 0);
     };
     std::vector<int> keys;
     for (Engine* e : scenes_) {
@@ -985,6 +1051,7 @@ void Batch::rebuild() {
             descs[s] = e->scene_desc();
             if (e->stencil_k() != kcommon) kcommon = -1;
             P.ucap = std::max(P.ucap, descs[s].S.ucap);
+            P.ucap_f = std::max(P.ucap_f, descs[s].S.ucap_f);
             P.kmax = std::max(P.kmax, e->stencil_k());
             any_sources_ = any_sources_ || descs[s].S.src != nullptr;
             int64_t nb[kStepKernels];
@@ -1001,6 +1068,7 @@ void Batch::rebuild() {
         P.k_template = (kcommon == 48 || kcommon == 32 || kcommon == 16) ? kcommon : 0;
         P.gather_batch = gather_batch_;
         P.tiles = descs[0].S.ulist != nullptr;
+        P.ragged = descs[0].S.tile_ng != nullptr;
         for (const SceneDesc& d : descs)
             require((d.S.ulist != nullptr) == P.tiles, "scenes with and without tile unions");
         require(!P.unions || P.k_template != 0, "four-per-lane scenes need a common k");

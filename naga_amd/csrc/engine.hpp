@@ -168,7 +168,11 @@ class Engine {
     DeviceBuffer<uint32_t> bulist_;
     DeviceBuffer<int64_t> buoff_;
     int ucap_ = 0;
+    int ucap_f_ = 0;
     int self_first_ = 0;
+    DeviceBuffer<int32_t> tile_ng_;   // ragged record rows: groups of 8 per tile
+    DeviceBuffer<int64_t> rec_off_;   // first unit of each tile's divergence (and PML) records
+    DeviceBuffer<int64_t> frec_off_;  // first unit of each tile's filter records
 
     struct ProbeSet {
         int64_t n = 0;

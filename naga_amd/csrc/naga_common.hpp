@@ -54,6 +54,13 @@ constexpr int packed_div_units(int k) { return 7 * (k / 8) + 1; }
 // of 8 records two units of weights and one unit of 16-bit positions, then
 // one zero unit.
 constexpr int packed_filter_units(int k) { return 3 * (k / 8) + 1; }
+// Record rows of a tile whose nodes use ng groups of 8 records (per-tile
+// stencil widths): the groups' units plus one or two zero units, an odd
+// number of 16-byte units per node (distinct LDS banks for the nodes of a
+// wavefront); equal to the fixed-width rows for ng = k / 8 with k in
+// {16, 32, 48}.
+constexpr int ragged_div_units(int ng) { return 7 * ng + ((ng & 1) ? 2 : 1); }
+constexpr int ragged_filter_units(int ng) { return 3 * ng + ((ng & 1) ? 2 : 1); }
 
 #if defined(__HIPCC__)
 inline void hip_check(hipError_t e, const char* what) {

@@ -704,6 +704,22 @@ __global__ __launch_bounds__(256) void k_tile_union(int64_t nfluid, int k, int t
                 sb ? sb + int64_t(h) * k * (1 + ncb) : nullptr, ncb);
 }
 
+// Ragged record rows: node i of tile t = i / T keeps the first upg *
+// ng[t] units of its full row (KP units) and pads to `ragged(ng[t])` units
+// with zeros; tile t's rows start at unit off[t].
+__global__ void k_ragged_records(int64_t n, int T, int KP, int upg, const int32_t* ng,
+                                 const int64_t* off, const float4* src, float4* dst) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n * KP) return;
+    const int64_t i = e / KP;
+    const int u = int(e - i * KP);
+    const int64_t t = i / T;
+    const int g = ng[t];
+    const int kpt = upg == 7 ? ragged_div_units(g) : ragged_filter_units(g);
+    if (u >= kpt) return;
+    dst[off[t] + (i - t * T) * kpt + u] = u < upg * g ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // flag stays 1 if idx[i * k] == i for every i < n, else becomes 0.
 __global__ void k_self_first(int64_t n, int k, const uint32_t* idx, int* flag) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1168,6 +1184,16 @@ void launch_pack_filter_records(int64_t n, int k, const float* src, float* dst,
     k_pack_filter_records<<<(tot + 255) / 256, 256, 0, st>>>(n, k, src,
                                                              reinterpret_cast<uint32_t*>(dst));
     hip_check(hipGetLastError(), "pack filter records");
+}
+
+void launch_ragged_records(int64_t n, int T, int KP, int upg, const int32_t* ng,
+                           const int64_t* off, const float* src, float* dst, hipStream_t st) {
+    if (n == 0) return;
+    const int64_t tot = n * KP;
+    k_ragged_records<<<(tot + 255) / 256, 256, 0, st>>>(n, T, KP, upg, ng, off,
+                                                        reinterpret_cast<const float4*>(src),
+                                                        reinterpret_cast<float4*>(dst));
+    hip_check(hipGetLastError(), "ragged records");
 }
 
 bool stencils_self_first(int64_t n, int k, const uint32_t* idx, hipStream_t st) {

@@ -107,6 +107,13 @@ void launch_divergence_weights(int D, const float* src, const float* query,
                                int* info, hipStream_t stream,
                                const uint8_t* kvar = nullptr, int solve_bits = 64);
 
+// Ragged copy of packed record rows (KP units per node, upg units per group
+// of 8 records: 7 divergence, 3 filter): node i of tile t = i / T keeps
+// upg * ng[t] units padded to ragged_div_units / ragged_filter_units(ng[t]),
+// tile t's rows from unit off[t] of dst.
+void launch_ragged_records(int64_t n, int T, int KP, int upg, const int32_t* ng,
+                           const int64_t* off, const float* src, float* dst, hipStream_t stream);
+
 // Whether idx[i * k] == i for every node i < n (the node is its own first
 // neighbour); synchronises the stream.
 bool stencils_self_first(int64_t n, int k, const uint32_t* idx, hipStream_t stream);

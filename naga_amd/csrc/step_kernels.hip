@@ -693,7 +693,8 @@ __device__ __forceinline__ float4 rate_fetch(const float4* rp, Fetch fetch, int 
 #define NAGA_PACK_SUB 4
 #endif
 template <int K, class Fetch>
-__device__ __forceinline__ float4 rate_fetch_packed(const float4* rp, Fetch fetch, int q) {
+__device__ __forceinline__ float4 rate_fetch_packed(const float4* rp, Fetch fetch, int q,
+                                                    int ng = K / 8) {
     static_assert(K % 8 == 0, "packed records come in groups of 8");
     constexpr int H = NAGA_PACK_SUB;  // records gathered per sub-batch: 8 or 4
     static_assert(H == 8 || H == 4, "sub-batch of 8 or 4 records");
@@ -701,6 +702,7 @@ __device__ __forceinline__ float4 rate_fetch_packed(const float4* rp, Fetch fetc
           tz[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int g = 0; g < K / 8; ++g) {
+        if (g >= ng) break;  // ragged rows: the tile's ng groups
         const float4 iu = rp[7 * g + 6];
 #pragma unroll
         for (int h = 0; h < 8; h += H) {
@@ -859,6 +861,23 @@ __device__ __forceinline__ UnionWork union_work(const StepGeom& G, const StepBuf
         }
     }
     return w;
+}
+
+// Record rows of a union block's tile: groups of 8 per node, units per node
+// and the tile's first unit (fixed width: K / 8, KP, tile * NPB * KP;
+// ragged: from the per-tile tables).
+struct TileRecs {
+    int ng, kp;
+    int64_t base;
+};
+template <bool RG, int K, int KP, int NPB>
+__device__ __forceinline__ TileRecs tile_recs(const int32_t* tile_ng, const int64_t* off,
+                                              int64_t tile, bool filter) {
+    if (!RG) return {K / 8, KP, tile * NPB * KP};
+    typedef const __attribute__((address_space(4))) int32_t const_i32;
+    typedef const __attribute__((address_space(4))) int64_t const_i64;
+    const int ng = ((const_i32*)tile_ng)[tile];
+    return {ng, filter ? ragged_filter_units(ng) : ragged_div_units(ng), ((const_i64*)off)[tile]};
 }
 
 // Tile entry of block b of nb of a kernel over the tiles: the regular
@@ -1071,7 +1090,7 @@ constexpr int stage_waves() { return NPB >= 64 ? 4 : 3; }
 
 // RK4 stage, D3Q27, four per lane, neighbour rows read from the tile's
 // union staged in LDS (unions larger than ucap rows read through the list).
-template <int STAGE, int K, int BATCH, bool TU, int NPB>
+template <int STAGE, int K, int BATCH, bool TU, int NPB, bool RG = false>
 __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(stage_waves<NPB>()))) void k_stage_u(StepGeom G, StepBuffers S,
                                                             BatchArgs B) {
     constexpr int NT = NPB * kV4Lanes;
@@ -1105,7 +1124,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     const uint32_t* ul = TU ? S.ulist + w.u0 : nullptr;  // TU: tile unions built
     // records: rows padded by one record (distinct banks per node), copied
     // through VGPRs, or with tile unions by LDS DMA from packed rows
-    const int kp = KP;
+    const TileRecs tr = tile_recs<RG, K, KP, NPB>(S.tile_ng, S.rec_off, w.tile, false);
+    const int kp = tr.kp;
     if (w.hi > w.lo) {
         if (!TU) {
             const float4* src = reinterpret_cast<const float4*>(S.slab) + node0 * K;
@@ -1118,8 +1138,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
             __builtin_amdgcn_sched_barrier(0);
             const int n = w.hi - w.lo;
             dma_contiguous<NPB * KP, NT>(
-                rsrc(S.slab + (node0 + w.lo) * KP * 4, uint64_t(n) * KP * 16u),
-                srec + w.lo * KP, n * KP);
+                rsrc(S.slab + (tr.base + int64_t(w.lo) * kp) * 4, uint64_t(n) * kp * 16u),
+                srec + w.lo * kp, n * kp);
             __builtin_amdgcn_sched_barrier(0);
             union_copy<NT>(urows, ux, Uc, gin);
         }
@@ -1136,7 +1156,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     if (TU) dma_fence();
     else __syncthreads();
     if (own_lds) {
-        const uint32_t p0 = __float_as_uint(srec[slot * KP + 6].x) & 0xffffu;
+        const uint32_t p0 = __float_as_uint(srec[slot * kp + 6].x) & 0xffffu;
         y3 = urows[p0 * kV4Lanes + q];
     }
     if (STAGE == 1 && w.first && !fluid && i < G.N) {
@@ -1157,13 +1177,13 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
             [&](uint32_t j) { return ld4(gin, j * 128u + uint32_t(q) * 16u); }, q);
     else if (U <= S.ucap)
         r = rate_fetch_packed<K>(srec + slot * kp,
-                                 [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
+                                 [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q, tr.ng);
     else  // union over the cap: every row read from HBM through the list
         r = rate_fetch_packed<K>(srec + slot * kp,
                                  [&](uint32_t j) {
                                      return ld4(gin, ul[j] * 128u + uint32_t(q) * 16u);
                                  },
-                                 q);
+                                 q, tr.ng);
     st4(gout, own, STAGE == 4 ? rk4_post4(pre, r, G.dt) : rk4_out4<STAGE>(g0, r, G.dt, z4, z4, z4));
 }
 
@@ -1465,13 +1485,14 @@ __device__ __forceinline__ float4 filter_sum(const float2* rp, const float4& si,
 #endif
 template <int K, class Fetch>
 __device__ __forceinline__ float4 filter_sum_packed(const float4* rp, const float4& si,
-                                                    Fetch fetch) {
+                                                    Fetch fetch, int ng = K / 8) {
     static_assert(K % 8 == 0, "packed records come in groups of 8");
     constexpr int H = NAGA_FILTER_SUB;  // records gathered per sub-batch: 8 or 4
     static_assert(H == 8 || H == 4, "sub-batch of 8 or 4 records");
     float sum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int g = 0; g < K / 8; ++g) {
+        if (g >= ng) break;  // ragged rows: the tile's ng groups
         const float4 iu = rp[3 * g + 2];
 #pragma unroll
         for (int h = 0; h < 8; h += H) {  // records h .. h + H - 1, weights in units h/4 ..
@@ -1496,7 +1517,7 @@ __device__ __forceinline__ float4 filter_sum_packed(const float4* rp, const floa
 
 // p4 for D3Q27 with the tile unions: selective filter on fluid nodes,
 // bounce-back on boundary nodes.
-template <int K, int BATCH, bool TU, int NPB>
+template <int K, int BATCH, bool TU, int NPB, bool RG = false>
 __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers S, BatchArgs B) {
     constexpr int NT = NPB * kV4Lanes;
     // record rows with distinct banks per node: K + 1 records through
@@ -1517,18 +1538,20 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
     const float4 si = ld4(s1, own);
     const int U = w.U;
     const uint32_t* ul = TU ? S.ulist + w.u0 : nullptr;  // TU: tile unions built
+    const TileRecs tr = tile_recs<RG, K, KF, NPB>(S.tile_ng, S.frec_off, w.tile, true);
+    const int kf = tr.kp;
     if (w.hi > w.lo) {
         if (!TU) {
             const float2* src = reinterpret_cast<const float2*>(S.fslab) + node0 * K;
             for (int t = threadIdx.x; t < w.hi * K; t += NT) srec[t + t / K] = src[t];
         } else {
-            const int Uc = U < S.ucap ? U : S.ucap;
+            const int Uc = U < S.ucap_f ? U : S.ucap_f;
             const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);
             __builtin_amdgcn_sched_barrier(0);
             const int n = w.hi - w.lo;
             dma_contiguous<NPB * KF, NT>(
-                rsrc(S.fslab + (node0 + w.lo) * KF * 4, uint64_t(n) * KF * 16u),
-                srec4 + w.lo * KF, n * KF);
+                rsrc(S.fslab + (tr.base + int64_t(w.lo) * kf) * 4, uint64_t(n) * kf * 16u),
+                srec4 + w.lo * kf, n * kf);
             __builtin_amdgcn_sched_barrier(0);
             union_copy<NT>(urows, ux, Uc, s1);
         }
@@ -1546,13 +1569,13 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
         if (!ul)
             sum = filter_sum<K, BATCH>(srec + slot * KP, si,
                                        [&](uint32_t j) { return ld4(s1, j * 128u + lane_off); });
-        else if (U <= S.ucap)
-            sum = filter_sum_packed<K>(srec4 + slot * KF, si,
-                                       [&](uint32_t j) { return urows[j * kV4Lanes + q]; });
+        else if (U <= S.ucap_f)
+            sum = filter_sum_packed<K>(srec4 + slot * kf, si,
+                                       [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, tr.ng);
         else  // union over the cap: every row read from HBM through the list
-            sum = filter_sum_packed<K>(srec4 + slot * KF, si, [&](uint32_t j) {
+            sum = filter_sum_packed<K>(srec4 + slot * kf, si, [&](uint32_t j) {
                 return ld4(s1, ul[j] * 128u + lane_off);
-            });
+            }, tr.ng);
 #pragma unroll
         for (int e = 0; e < 4; ++e) set(out, e, get(si, e) + G.filter * get(sum, e));
     } else {
@@ -1604,7 +1627,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_v4(StepGeom G, StepBuffer
 // other nodes of such a tile every neighbour's auxiliary row is zero and
 // the added divergence is an exact zero), the tile's union rows of Gf and
 // its records staged by LDS DMA.
-template <int K, int BATCH, int NPB>
+template <int K, int BATCH, int NPB, bool RG = false>
 __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_t(StepGeom G, StepBuffers S, BatchArgs B) {
     constexpr int KP = packed_div_units(K);  // packed record rows
     constexpr int NT = NPB * kV4Lanes;
@@ -1634,21 +1657,24 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_t(StepGeom G, StepBuffers
     const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);
     __builtin_amdgcn_sched_barrier(0);
     const int n = w.hi - w.lo;
-    dma_contiguous<NPB * KP, NT>(rsrc(S.slab + (node0 + w.lo) * KP * 4, uint64_t(n) * KP * 16u),
-                                 srec + w.lo * KP, n * KP);
+    const TileRecs tr = tile_recs<RG, K, KP, NPB>(S.tile_ng, S.rec_off, w.tile, false);
+    const int kp = tr.kp;
+    dma_contiguous<NPB * KP, NT>(rsrc(S.slab + (tr.base + int64_t(w.lo) * kp) * 4, uint64_t(n) * kp * 16u),
+                                 srec + w.lo * kp, n * kp);
     __builtin_amdgcn_sched_barrier(0);
     union_copy<NT>(urows, ux, Uc, gf);
     const float4 s = ld4(ar, own);
     dma_fence();
     if (slot < w.lo || slot >= w.hi) return;
-    const float4* rp = srec + slot * KP;
+    const float4* rp = srec + slot * kp;
     const uint32_t lane_off = uint32_t(q) * 16u;
     float4 r;
     if (U <= S.ucap)
-        r = rate_fetch_packed<K>(rp, [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q);
+        r = rate_fetch_packed<K>(rp, [&](uint32_t j) { return urows[j * kV4Lanes + q]; }, q,
+                                 tr.ng);
     else  // union over the cap: every row read from HBM through the list
         r = rate_fetch_packed<K>(rp, [&](uint32_t j) { return ld4(gf, ul[j] * 128u + lane_off); },
-                                 q);
+                                 q, tr.ng);
     // the divergence of F = c Gf is the negated rate of Gf
     st4(S.A, own, make_float4(s.x - r.x, s.y - r.y, s.z - r.z, s.w - r.w));
 }
@@ -2021,10 +2047,12 @@ struct PassCfg {
     int ktmpl;     // K template: 32, 16 or 0 (runtime k)
     bool unions;   // D3Q27 four-per-lane kernels with tile unions
     int ucap;      // LDS union rows (dynamic shared memory / 128 B)
+    int ucap_f;    // the same for the filter pass
     BatchArgs B;
     int gbatch;    // gather batch of the one-per-lane stage kernels (0: default)
     bool tiles;    // tile unions built (slab index fields are union positions)
     int kmax;      // largest stencil width of the launch's scenes (LDS of the D2Q9 kernels)
+    bool ragged;   // per-tile record widths (StepBuffers::tile_ng)
 };
 
 template <int Q, int STAGE, int KT>
@@ -2045,7 +2073,18 @@ template <int K, int NPB>
 void launch_v4_npb(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
                    const PassCfg& c, hipStream_t st) {
     constexpr int threads = NPB * kV4Lanes;
-    const size_t lds = size_t(c.ucap) * 128u;
+    const size_t lds = size_t(pass == kP4 ? c.ucap_f : c.ucap) * 128u;
+    if (c.tiles && c.ragged) {
+        switch (pass) {
+            case kP2: k_p2_t<K, 8, NPB, true><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kP4: k_p4_u<K, 8, true, NPB, true><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kS1: k_stage_u<1, K, 8, true, NPB, true><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kS2: k_stage_u<2, K, 8, true, NPB, true><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kS3: k_stage_u<3, K, 8, true, NPB, true><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            case kS4: k_stage_u<4, K, 8, true, NPB, true><<<nb, threads, lds, st>>>(G, S, c.B); return;
+            default: fail(kInvalidArgument, "not a stencil pass");
+        }
+    }
     if (c.tiles) {
         switch (pass) {
             case kP2: k_p2_t<K, 8, NPB><<<nb, threads, lds, st>>>(G, S, c.B); return;
@@ -2231,8 +2270,8 @@ template <int Q>
 void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S, hipStream_t st) {
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
-    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0,
-                    S.ulist != nullptr, G.k};
+    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, S.ucap_f, BatchArgs{}, 0,
+                    S.ulist != nullptr, G.k, S.tile_ng != nullptr};
     launch_pass<Q>(kS1 + stage - 1, nb[kS1 + stage - 1], G, S, CollisionParams{}, c, st);
 }
 
@@ -2241,8 +2280,8 @@ void launch_step(const StepGeom& G, const StepBuffers& S, const CollisionParams&
                  hipStream_t st, hipEvent_t* ev) {
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
-    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, BatchArgs{}, 0,
-                    S.ulist != nullptr, G.k};
+    const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, S.ucap_f, BatchArgs{}, 0,
+                    S.ulist != nullptr, G.k, S.tile_ng != nullptr};
     for (int p = 0; p < kStepKernels; ++p) {
         if (ev) hip_check(hipEventRecord(ev[p], st), "event");
         launch_pass<Q>(p, nb[p], G, S, cp, c, st);
@@ -2255,10 +2294,10 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t st) {
     const StepGeom G{};
     const StepBuffers S{};
     for (int p = 0; p < kStepKernels; ++p) {
-        const PassCfg c{plan.k_template, plan.unions, plan.ucap,
+        const PassCfg c{plan.k_template, plan.unions, plan.ucap, plan.ucap_f,
                         BatchArgs{plan.dev_scenes, plan.dev_bstart + p * (plan.nscenes + 1),
                                   plan.nscenes},
-                        plan.gather_batch, plan.tiles, plan.kmax};
+                        plan.gather_batch, plan.tiles, plan.kmax, plan.ragged};
         launch_pass<Q>(p, plan.blocks[p], G, S, CollisionParams{}, c, st);
     }
 }

@@ -95,6 +95,16 @@ struct StepBuffers {
     const int32_t* split_tiles;
     int64_t n_split;
     int ucap;
+    int ucap_f;      // LDS union rows of the filter pass (at most ucap: its smaller
+                     // records leave room for more blocks per CU)
+    // Per-node stencil widths with tile unions: tile t's nodes keep
+    // tile_ng[t] groups of 8 records (ragged rows of ragged_div_units /
+    // ragged_filter_units(tile_ng[t]) units), its divergence and PML records
+    // from unit rec_off[t] of slab / p2slab, its filter records from unit
+    // frec_off[t] of fslab.  Null: every tile keeps k records per node.
+    const int32_t* tile_ng;
+    const int64_t* rec_off;
+    const int64_t* frec_off;
     int self_first;  // every fluid node's first stencil entry is the node itself
                      // (its own row is then in its tile's union, record 0);
                      // NAGA_OWN_LDS=0 clears it
@@ -166,9 +176,11 @@ struct BatchPlan {
     int k_template = 0;                 // K template of the gather kernels (0: dynamic)
     bool unions = false;                // D3Q27 four-per-lane kernels
     int ucap = 0;                       // max LDS union rows over the scenes
+    int ucap_f = 0;                     // the same for the filter pass
     int gather_batch = 0;               // neighbours gathered per unrolled step (0: default)
     bool tiles = false;                 // tile unions built for every scene
     int kmax = 0;                       // largest stencil width over the scenes
+    bool ragged = false;                // per-tile record widths in every scene
 };
 // Block counts of each pass for one scene (the same counts launch_step uses).
 void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* blocks);

@@ -279,6 +279,31 @@ def test_variable_stencils_parity(oracle64, Q, kmax, kmin):
     assert err_rho < TOL and err_u < TOL
 
 
+def test_ragged_records_bit_identical(monkeypatch):
+    """Per-node stencil widths with tile unions: each tile keeps only the
+    groups of 8 records its widest node needs (the records past a node's
+    width have zero weights), so the state matches full-width records bit
+    for bit; NAGA_RAGGED=0 keeps full-width rows."""
+    import naga_amd
+    h = 0.05
+    nodes = naga_amd.ObstacleGridProvider([0] * 3, [1] * 3, h, [([0.5] * 3, 0.2)], 3 * h, 0.5,
+                                          jitter=0.25, refine_width=0.1).get()
+    runs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("NAGA_RAGGED", flag)
+        eng = naga_amd.SimulationEngine(lattice=27, k=48, k_boundary=32,
+                                        rbf=naga_amd.RBF_POLYHARMONIC3,
+                                        rbf_boundary=naga_amd.RBF_POLYHARMONIC3, k_min=24)
+        p = naga_amd.problem_parameters(27, 0.6, 0.2 * h)
+        eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+        eng.init_domain(nodes)
+        eng.set_initial_pulse([0.3, 0.5, 0.5], 1e-3, 0.1)
+        eng.step_forward(15)
+        runs.append(eng.distributions()["f"])
+        eng.close()
+    assert np.array_equal(runs[0], runs[1])
+
+
 def test_tile_unions_bit_identical(oracle64, monkeypatch):
     """The gather kernels read neighbour rows from the tile unions staged in
     LDS by default and straight from HBM with NAGA_UNIONS=0: the same values
