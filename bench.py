@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 # The RK4 stage kernels of the 1M-node workload (tile unions, 64-node tiles);
 # the dominant one, stage 4, is the roofline line.
-STAGE_KERNELS = {s: f"k_stage_u<{s}, 32, 8, true, 64>" for s in (1, 2, 3, 4)}
+STAGE_KERNELS = {s: f"k_stage_u<{s}, 32, 8, true, 64, false>" for s in (1, 2, 3, 4)}
 ROOFLINE_STAGE = 4
 
 
@@ -233,6 +233,43 @@ def served_workload(n_requests: int = 32, rate: float = 50.0) -> dict:
                         "with a 3h absorbing layer, k 16"}
 
 
+def profile_workload(name: str, steps: int) -> dict:
+    """One workload alone for a profiler run: the 256-scene batched D2Q9
+    workload (autotuned, then `steps` batched steps) or the irregular 3D
+    scene (`steps` steps); returns its time per step."""
+    import naga_amd
+    from naga_amd import workloads
+    if name == "batched":
+        engines = [workloads.batched_engine(s)[0] for s in workloads.batched_scenes(256)]
+        batch = naga_amd.SceneBatch(engines)
+        batch.autotune(3)
+        batch.step_forward(5)
+        t0 = time.perf_counter()
+        batch.step_forward(steps)
+        wall = time.perf_counter() - t0
+        nodes = sum(e.num_nodes for e in engines)
+        batch.close()
+    else:
+        h = 0.008
+        nd = naga_amd.ObstacleGridProvider([0, 0, 0], [1, 1, 1], h, [([0.5, 0.5, 0.5], 0.15)],
+                                           8 * h, 0.5, jitter=0.25, refine_width=0.05).get()
+        eng = naga_amd.SimulationEngine(lattice=27, k=48, k_boundary=32,
+                                        rbf=naga_amd.RBF_POLYHARMONIC3,
+                                        rbf_boundary=naga_amd.RBF_POLYHARMONIC3, k_min=24)
+        p = naga_amd.problem_parameters(27, 0.6, 0.4 * (h / 2))
+        eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+        eng.init_domain(nd)
+        eng.set_initial_pulse([0.3, 0.5, 0.5], 1e-3, 0.05)
+        eng.step_forward(5)
+        t0 = time.perf_counter()
+        eng.step_forward(steps)
+        wall = time.perf_counter() - t0
+        nodes = eng.num_nodes
+        eng.close()
+    return {"workload": name, "nodes": nodes, "steps": steps, "ms_per_step": wall * 1e3 / steps,
+            "node_updates_per_s": nodes * steps / wall}
+
+
 def other_configs(threads: int) -> dict:
     """Throughput of the 2D plain scene (D2Q9, 10,201 nodes, k = 16, PML
     10h, 1000 steps) and the irregular 3D scene (cube minus a refined
@@ -310,9 +347,14 @@ def main() -> int:
     ap.add_argument("--served-requests", type=int, default=32)
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the 2D plain and irregular 3D scenes")
+    ap.add_argument("--profile-workload", choices=("batched", "irregular"), default=None,
+                    help="only run that workload for --steps steps (for a profiler)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.profile_workload:
+        print(json.dumps(profile_workload(args.profile_workload, args.steps)))
+        return 0
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; multi-GPU sharding is "
               "designed (halo exchange per stencil pass) but not built", file=sys.stderr)
