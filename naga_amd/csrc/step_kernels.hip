@@ -865,15 +865,18 @@ __device__ __forceinline__ UnionWork union_work(const StepGeom& G, const StepBuf
 
 // Record rows of a union block's tile: groups of 8 per node, units per node
 // and the tile's first unit (fixed width: K / 8, KP, tile * NPB * KP;
-// ragged: from the per-tile tables).
+// ragged: from the per-tile tables, which have one entry per tile of fluid
+// nodes; the tiles past them, of boundary nodes only, hold no records and
+// read nothing).
 struct TileRecs {
     int ng, kp;
     int64_t base;
 };
 template <bool RG, int K, int KP, int NPB>
 __device__ __forceinline__ TileRecs tile_recs(const int32_t* tile_ng, const int64_t* off,
-                                              int64_t tile, bool filter) {
+                                              int64_t tile, bool fluid_tile, bool filter) {
     if (!RG) return {K / 8, KP, tile * NPB * KP};
+    if (!fluid_tile) return {0, 0, 0};
     typedef const __attribute__((address_space(4))) int32_t const_i32;
     typedef const __attribute__((address_space(4))) int64_t const_i64;
     const int ng = ((const_i32*)tile_ng)[tile];
@@ -1124,7 +1127,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
     const uint32_t* ul = TU ? S.ulist + w.u0 : nullptr;  // TU: tile unions built
     // records: rows padded by one record (distinct banks per node), copied
     // through VGPRs, or with tile unions by LDS DMA from packed rows
-    const TileRecs tr = tile_recs<RG, K, KP, NPB>(S.tile_ng, S.rec_off, w.tile, false);
+    const TileRecs tr =
+        tile_recs<RG, K, KP, NPB>(S.tile_ng, S.rec_off, w.tile, w.hi > w.lo, false);
     const int kp = tr.kp;
     if (w.hi > w.lo) {
         if (!TU) {
@@ -1538,7 +1542,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p4_u(StepGeom G, StepBuffers
     const float4 si = ld4(s1, own);
     const int U = w.U;
     const uint32_t* ul = TU ? S.ulist + w.u0 : nullptr;  // TU: tile unions built
-    const TileRecs tr = tile_recs<RG, K, KF, NPB>(S.tile_ng, S.frec_off, w.tile, true);
+    const TileRecs tr =
+        tile_recs<RG, K, KF, NPB>(S.tile_ng, S.frec_off, w.tile, w.hi > w.lo, true);
     const int kf = tr.kp;
     if (w.hi > w.lo) {
         if (!TU) {
@@ -1657,7 +1662,8 @@ __global__ __launch_bounds__(NPB * kV4Lanes) void k_p2_t(StepGeom G, StepBuffers
     const UnionIdx<NT> ux = union_indices<NT>(ul, Uc);
     __builtin_amdgcn_sched_barrier(0);
     const int n = w.hi - w.lo;
-    const TileRecs tr = tile_recs<RG, K, KP, NPB>(S.tile_ng, S.rec_off, w.tile, false);
+    const TileRecs tr =
+        tile_recs<RG, K, KP, NPB>(S.tile_ng, S.rec_off, w.tile, w.hi > w.lo, false);
     const int kp = tr.kp;
     dma_contiguous<NPB * KP, NT>(rsrc(S.slab + (tr.base + int64_t(w.lo) * kp) * 4, uint64_t(n) * kp * 16u),
                                  srec + w.lo * kp, n * kp);
