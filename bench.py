@@ -225,12 +225,15 @@ def served_workload(n_requests: int = 32, rate: float = 50.0) -> dict:
             "speedup": st["node_updates_per_s"] / (work / plain_s),
             "trace_s": time.perf_counter() - t0 - plain_s,
             "results_bounded": int(sum(bool(np.isfinite(p) and p < 1e-3) for p in peaks)),
-            "max_abs_drho_after": max(peaks),
+            "max_abs_drho_after": max((p for p in peaks if np.isfinite(p)), default=None),
+            "unbounded": [{"lattice": r.lattice, "h": r.h, "steps": r.steps,
+                           "max_abs_drho_after": p}
+                          for r, p in zip(reqs, peaks) if not (np.isfinite(p) and p < 1e-3)],
             "plain_by_size": classes,
             "workload": "Poisson 50 req/s, D2Q9/D3Q27 50/50, N log-uniform 5k-2M, "
                         "steps 100-2000; 3D: cube minus sphere r 0.15, 2x refinement "
-                        "within 0.05 (h <= 0.05), k 24-48, jitter 0.15h; 2D: 0-5 circles "
-                        "with a 3h absorbing layer, k 16"}
+                        "within 0.05 and jitter 0.1h (h <= 0.05; else unjittered), "
+                        "k 24-48; 2D: 0-5 circles with a 3h absorbing layer, k 16"}
 
 
 def profile_workload(name: str, steps: int) -> dict:

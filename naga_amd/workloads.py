@@ -9,8 +9,9 @@ stream, so a test can draw the same kind of workload at capped sizes.
 * served_requests: Poisson arrivals, D2Q9/D3Q27 50/50, node counts
   log-uniform in [n_lo, n_hi] (5k-2M), steps uniform in [s_lo, s_hi]
   (100-2000); D3Q27 requests are the unit cube minus a central sphere of
-  radius 0.15 refined 2x within 0.05 of it (for h <= 0.05), k 24-48 by
-  local density, jitter 0.15h; D2Q9 requests the unit square with 0-5
+  radius 0.15 refined 2x within 0.05 of it (for h <= 0.05, jitter 0.1h;
+  coarser requests unrefined and unjittered), k 24-48 by local density;
+  D2Q9 requests the unit square with 0-5
   circles, k 16, jitter 0.25h, an absorbing layer of 3h around the circles.
 """
 from __future__ import annotations
@@ -95,11 +96,17 @@ def served_requests(n_requests: int = 32, rate: float = 50.0, n_lo: float = 5e3,
             kw = dict(k=16)
         else:
             obst = [([0.5, 0.5, 0.5], 0.15)]
-            # jitter 0.15h, and the refinement shell only where the sphere
-            # spans at least 3 spacings: at 0.25h some requests of
-            # 1000-2000 steps, and a refined sphere of 2.4 spacings at any
-            # jitter, grow a mode next to the boundary
-            kw = dict(k=48, k_min=24, refine_width=0.05 if h <= 0.05 else 0.0, jitter=0.15)
+            # The refinement shell only where the sphere spans at least 3
+            # spacings (h <= 0.05), with jitter 0.1h; coarser requests on the
+            # unjittered grid.  Over the 200-request trace (up to 2000 steps)
+            # these choices keep every request bounded, while a mode grows
+            # next to the sphere with jitter 0.25h (some refined requests),
+            # 0.15h (one refined request of 1223 steps, and two coarse ones
+            # of about 1500 steps whose sphere spans 2.9 spacings) and on a
+            # refined sphere of 2.4 spacings at any jitter.
+            fine = h <= 0.05
+            kw = dict(k=48, k_min=24, refine_width=0.05 if fine else 0.0,
+                      jitter=0.1 if fine else 0.0)
         reqs.append(naga_amd.make_request(
             lat, h, int(rng.integers(s_lo, s_hi + 1)), obst, seed=int(rng.integers(1, 1 << 62)),
             pulse_center=pc, pulse_width=0.05, **kw))
