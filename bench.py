@@ -152,6 +152,7 @@ def batched_workload(steps: int, n_scenes: int = 256) -> dict:
     plain_s = time.perf_counter() - t0
     batch = naga_amd.SceneBatch(engines)
     gb = batch.autotune(3)
+    tune = batch.tuning()
     batch.step_forward(2)
     t0 = time.perf_counter()
     batch.step_forward(steps)
@@ -164,7 +165,8 @@ def batched_workload(steps: int, n_scenes: int = 256) -> dict:
     return {"scenes": n_scenes, "nodes": total, "steps": steps, "setup_s": setup_s,
             "max_abs_drho_after": peak,
             "plain_node_updates_per_s": plain, "batched_node_updates_per_s": batched,
-            "speedup": batched / plain, "gather_batch": gb,
+            "speedup": batched / plain, "gather_batch": gb, "q9_lanes": tune["q9_lanes"],
+            "graph": tune["graph"],
             "workload": "D2Q9, 256 scenes, N log-uniform 2k-50k, 0-5 obstacles r 0.03-0.1, "
                         "k 12-24, PML 8h (3h around obstacles), tau 0.6, dt 0.4h"}
 

@@ -259,6 +259,20 @@ int naga_batch_autotune(naga_batch* b, int32_t trial_steps, int32_t* gather_batc
     });
 }
 
+int naga_batch_set_q9_lanes(naga_batch* b, int32_t lanes) {
+    return guarded([&] { batch_of(b).set_q9_lanes(lanes); });
+}
+
+int naga_batch_tuning(naga_batch* b, int32_t* gather_batch, int32_t* q9_lanes, int32_t* graph) {
+    return guarded([&] {
+        int gb = 0, lanes = 0, gr = 0;
+        batch_of(b).tuning(&gb, &lanes, &gr);
+        if (gather_batch) *gather_batch = gb;
+        if (q9_lanes) *q9_lanes = lanes;
+        if (graph) *graph = gr;
+    });
+}
+
 int naga_engine_add_probes(naga_engine* e, const float* points, int64_t n, int32_t rbf,
                            int32_t* probe_set) {
     return guarded([&] {

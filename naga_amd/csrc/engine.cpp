@@ -935,7 +935,9 @@ void Engine::profile_step(double* ms) {
 // ---------------------------------------------------------------------------
 // Batch
 // ---------------------------------------------------------------------------
-Batch::Batch() { NAGA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking)); }
+Batch::Batch() : q9_three_(q9_three(9, 1)) {
+    NAGA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
 
 Batch::~Batch() {
     for (Engine* e : scenes_) e->batch = nullptr;
@@ -977,6 +979,18 @@ void Batch::set_graph(bool on) {
     dirty_ = true;
 }
 
+void Batch::set_q9_lanes(int lanes) {
+    require(lanes == 3 || lanes == 9, "D2Q9 lanes per node must be 3 or 9");
+    q9_three_ = lanes == 3;
+    dirty_ = true;
+}
+
+void Batch::tuning(int* gather_batch, int* q9_lanes, int* graph) const {
+    if (gather_batch) *gather_batch = gather_batch_;
+    if (q9_lanes) *q9_lanes = q9_three_ ? 3 : 9;
+    if (graph) *graph = use_graph_ ? 1 : 0;
+}
+
 int Batch::autotune(int trial_steps) {
     require(trial_steps > 0, "trial_steps must be positive");
     if (scenes_.empty()) return 0;
@@ -985,22 +999,30 @@ int Batch::autotune(int trial_steps) {
     for (Engine* e : scenes_) saved.push_back(e->snapshot());
     double best = 1e300;
     int best_gb = 8;
-    bool best_graph = true;
-    for (int gb : {8, 16, 32})
-        for (bool graph : {true, false}) {
-            gather_batch_ = gb;
-            use_graph_ = graph;
-            dirty_ = true;
-            step(1);  // builds the plan (and captures the graph)
-            const double ms = step_timed(trial_steps);
-            if (ms < best) {
-                best = ms;
-                best_gb = gb;
-                best_graph = graph;
+    bool best_graph = true, best_three = q9_three_;
+    bool any_q9 = false;
+    for (Engine* e : scenes_) any_q9 = any_q9 || e->lattice() == 9;
+    std::vector<bool> threes{q9_three_};
+    if (any_q9) threes = {true, false};
+    for (bool three : threes)
+        for (int gb : {8, 16, 32})
+            for (bool graph : {true, false}) {
+                gather_batch_ = gb;
+                use_graph_ = graph;
+                q9_three_ = three;
+                dirty_ = true;
+                step(1);  // builds the plan (and captures the graph)
+                const double ms = step_timed(trial_steps);
+                if (ms < best) {
+                    best = ms;
+                    best_gb = gb;
+                    best_graph = graph;
+                    best_three = three;
+                }
             }
-        }
     gather_batch_ = best_gb;
     use_graph_ = best_graph;
+    q9_three_ = best_three;
     dirty_ = true;
     for (size_t i = 0; i < scenes_.size(); ++i) scenes_[i]->restore(saved[i]);
     return best_gb;
@@ -1008,18 +1030,21 @@ int Batch::autotune(int trial_steps) {
 
 // Scenes are grouped by lattice and kernel variant (the four-per-lane D3Q27
 // kernels by stencil width; D2Q9 scenes share the runtime-k kernels unless
-// all have the same specialised width).
+// all have the same specialised width, and the lanes per node).
 void Batch::rebuild() {
     drop_graph();
     groups_.clear();
     any_sources_ = false;
     // groups: lattice, four-per-lane stencil width, and whether the tile
     // unions (and their padded slab layout) were built
-    auto key_of = [](Engine* e) {
+    auto three_of = [this](Engine* e) {
+        return e->lattice() == 9 && q9_three_ && q9_three_fits(e->nodes());
+    };
+    auto key_of = [&](Engine* e) {
         const int Q = e->lattice();
         const bool un = step_uses_unions(Q, e->stencil_k(), e->nodes());
         const bool tiles = e->scene_desc().S.ulist != nullptr;
-        const bool three = q9_three(Q, e->nodes());
+        const bool three = three_of(e);
         const bool ragged = e->scene_desc().S.tile_ng != nullptr;
         return Q * 1000 + (un ? e->stencil_k() : 0) + (tiles ? 500 : 0) + (three ? 250 : 0) +
                (ragged ? 125 : 0);
@@ -1033,6 +1058,7 @@ void Batch::rebuild() {
             keys.push_back(key);
             groups_.push_back(std::make_unique<Group>());
             groups_.back()->Q = e->lattice();
+            groups_.back()->three = three_of(e);
         }
         groups_[g]->members.push_back(e);
     }
@@ -1045,6 +1071,7 @@ void Batch::rebuild() {
         P = BatchPlan{};
         P.Q = g.Q;
         P.nscenes = n;
+        P.three = g.three;
         int kcommon = g.members[0]->stencil_k();
         for (int s = 0; s < n; ++s) {
             Engine* e = g.members[s];
@@ -1055,7 +1082,7 @@ void Batch::rebuild() {
             P.kmax = std::max(P.kmax, e->stencil_k());
             any_sources_ = any_sources_ || descs[s].S.src != nullptr;
             int64_t nb[kStepKernels];
-            step_block_counts(g.Q, descs[s].G, descs[s].S, nb);
+            step_block_counts(g.Q, descs[s].G, descs[s].S, nb, g.three ? 1 : 0);
             for (int p = 0; p < kStepKernels; ++p) {
                 int32_t* row = bstart.data() + size_t(p) * (n + 1);
                 const int64_t next = int64_t(row[s]) + nb[p];

@@ -206,11 +206,18 @@ class Batch {
     void step(int nsteps);
     double step_timed(int nsteps);
     void set_graph(bool on);
-    // Times trial_steps steps for each gather batch (8, 16, 32) with and
-    // without graph replay, keeps the fastest and restores every scene's
-    // state (the trials do not advance the scenes).  Returns the chosen
-    // gather batch, or 0 if the batch is empty.
+    // Times trial_steps steps for each gather batch (8, 16, 32), with and
+    // without graph replay, and, when the batch holds D2Q9 scenes, with one
+    // and three distributions per lane (9 or 3 lanes per node, 28 or 84
+    // nodes per 256-thread block); keeps the fastest and restores every
+    // scene's state (the trials do not advance the scenes).  Returns the
+    // chosen gather batch, or 0 if the batch is empty.
     int autotune(int trial_steps);
+    // D2Q9 lanes per node: 3 (three distributions per lane) or 9.
+    void set_q9_lanes(int lanes);
+    // The current choices: gather batch (0: the kernels' default), D2Q9
+    // lanes per node, graph replay.
+    void tuning(int* gather_batch, int* q9_lanes, int* graph) const;
     int size() const { return int(scenes_.size()); }
     hipStream_t stream() const { return stream_; }
     // A member's scene description changed (density sources added or
@@ -221,6 +228,7 @@ class Batch {
   private:
     struct Group {
         int Q = 9;
+        bool three = false;  // D2Q9 with three distributions per lane
         std::vector<Engine*> members;
         BatchPlan plan;
         DeviceBuffer<SceneDesc> table;
@@ -237,6 +245,7 @@ class Batch {
     bool use_graph_ = true;
     bool any_sources_ = false;
     int gather_batch_ = 0;
+    bool q9_three_ = true;  // D2Q9 scenes with three distributions per lane
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };

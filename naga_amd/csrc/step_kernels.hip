@@ -2006,12 +2006,13 @@ bool step_uses_unions(int Q, int k, int64_t N) {
 
 // D2Q9 scenes run the three-per-lane stencil kernels (12-byte pieces with
 // 32-bit buffer offsets: fewer than 2^26 nodes) unless NAGA_Q9_THREE=0.
+bool q9_three_fits(int64_t N) { return N < (int64_t(1) << 26); }
 bool q9_three(int Q, int64_t N) {
     static const bool on = [] {
         const char* e = std::getenv("NAGA_Q9_THREE");
         return !(e && e[0] == '0');
     }();
-    return on && Q == 9 && N < (int64_t(1) << 26);
+    return on && Q == 9 && q9_three_fits(N);
 }
 
 // Nodes per block of the D3Q27 stencil kernels: the union tile size with
@@ -2059,6 +2060,7 @@ struct PassCfg {
     bool tiles;    // tile unions built (slab index fields are union positions)
     int kmax;      // largest stencil width of the launch's scenes (LDS of the D2Q9 kernels)
     bool ragged;   // per-tile record widths (StepBuffers::tile_ng)
+    bool three;    // D2Q9 stencil passes with three distributions per lane
 };
 
 template <int Q, int STAGE, int KT>
@@ -2148,7 +2150,7 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
         check_launch("step pass");
         return;
     }
-    if (Q == 9 && q9_three(Q, G.N > 0 ? G.N : 1) &&
+    if (Q == 9 && c.three &&
         (pass == kP2 || pass == kP3 || pass == kP4 || pass == kP5 ||
          (pass >= kS1 && pass <= kS4))) {
         const size_t lds = size_t(kQ9Npb) * ((pass == kP4 ? 2 : 3) * c.kmax + 1) * 4u;
@@ -2237,7 +2239,7 @@ int64_t pml_part_lo(int Q, int k, int64_t N, int64_t nbulk) {
     return p1_fuses_pml(Q, step_uses_unions(Q, k, N)) ? nbulk / kP1Fused : 0;
 }
 
-void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* nb) {
+void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* nb, int three) {
     const bool un = step_uses_unions(Q, G.k, G.N);
     const int npb = kBlock / 64 * (Q == 27 ? Lattice<27>::NPW : Lattice<9>::NPW);
     auto lanes = [&](int64_t n) { return (n + npb - 1) / npb; };
@@ -2262,7 +2264,7 @@ void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* 
     nb[kP4] = un ? tiles(G.N) + ns : lanes(G.N);
     nb[kS1] = un ? tiles(G.N) + ns : lanes(G.N);
     nb[kS2] = nb[kS3] = nb[kS4] = un ? tiles(G.nfluid) + ns : lanes(G.nfluid);
-    if (q9_three(Q, G.N)) {
+    if (three < 0 ? q9_three(Q, G.N) : (Q == 9 && three > 0)) {
         auto t84 = [](int64_t n) { return (n + kQ9Npb - 1) / kQ9Npb; };
         nb[kP4] = nb[kS1] = t84(G.N);
         nb[kS2] = nb[kS3] = nb[kS4] = t84(G.nfluid);
@@ -2277,7 +2279,7 @@ void launch_rk4_stage(int stage, const StepGeom& G, const StepBuffers& S, hipStr
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
     const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, S.ucap_f, BatchArgs{}, 0,
-                    S.ulist != nullptr, G.k, S.tile_ng != nullptr};
+                    S.ulist != nullptr, G.k, S.tile_ng != nullptr, q9_three(Q, G.N)};
     launch_pass<Q>(kS1 + stage - 1, nb[kS1 + stage - 1], G, S, CollisionParams{}, c, st);
 }
 
@@ -2287,7 +2289,7 @@ void launch_step(const StepGeom& G, const StepBuffers& S, const CollisionParams&
     int64_t nb[kStepKernels];
     step_block_counts(Q, G, S, nb);
     const PassCfg c{ktemplate(G.k), step_uses_unions(Q, G.k, G.N), S.ucap, S.ucap_f, BatchArgs{}, 0,
-                    S.ulist != nullptr, G.k, S.tile_ng != nullptr};
+                    S.ulist != nullptr, G.k, S.tile_ng != nullptr, q9_three(Q, G.N)};
     for (int p = 0; p < kStepKernels; ++p) {
         if (ev) hip_check(hipEventRecord(ev[p], st), "event");
         launch_pass<Q>(p, nb[p], G, S, cp, c, st);
@@ -2303,7 +2305,7 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t st) {
         const PassCfg c{plan.k_template, plan.unions, plan.ucap, plan.ucap_f,
                         BatchArgs{plan.dev_scenes, plan.dev_bstart + p * (plan.nscenes + 1),
                                   plan.nscenes},
-                        plan.gather_batch, plan.tiles, plan.kmax, plan.ragged};
+                        plan.gather_batch, plan.tiles, plan.kmax, plan.ragged, plan.three};
         launch_pass<Q>(p, plan.blocks[p], G, S, CollisionParams{}, c, st);
     }
 }

@@ -181,9 +181,13 @@ struct BatchPlan {
     bool tiles = false;                 // tile unions built for every scene
     int kmax = 0;                       // largest stencil width over the scenes
     bool ragged = false;                // per-tile record widths in every scene
+    bool three = false;                 // D2Q9: three distributions per lane (else one)
 };
-// Block counts of each pass for one scene (the same counts launch_step uses).
-void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* blocks);
+// Block counts of each pass for one scene (the same counts launch_step
+// uses).  three: D2Q9 lanes per node, 1 for three distributions per lane,
+// 0 for one, -1 for the default (q9_three).
+void step_block_counts(int Q, const StepGeom& G, const StepBuffers& S, int64_t* blocks,
+                       int three = -1);
 template <int Q>
 void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 
@@ -191,8 +195,11 @@ void launch_step_batch(const BatchPlan& plan, hipStream_t stream);
 // the tile unions (StepBuffers::ulist/uoff/ucap) and slabs holding union
 // positions: D3Q27, k in {16, 32, 48}, fewer than 2^25 nodes.
 bool step_uses_unions(int Q, int k, int64_t N);
-// Whether the D2Q9 stencil passes run three distributions per lane.
+// Whether the D2Q9 stencil passes run three distributions per lane by
+// default (NAGA_Q9_THREE=0: one), and whether they can (fewer than 2^26
+// nodes: 32-bit buffer offsets of 36-byte rows).
 bool q9_three(int Q, int64_t N);
+bool q9_three_fits(int64_t N);
 // Nodes per union tile (= per block of the union kernels) for stencil width
 // k: 64 for k <= 32 (5.4 union rows per node against 7.0 with 32-node
 // tiles, two blocks per CU; 1.40 against 1.44 ms per step on the 1M-node

@@ -100,12 +100,34 @@ def test_autotune_leaves_scenes_unchanged(oracle64):
     batch = naga_amd.SceneBatch(mixed)
     gb = batch.autotune(3)
     assert gb in (8, 16, 32)
+    t = batch.tuning()
+    assert t["gather_batch"] == gb and t["q9_lanes"] in (3, 9)
     batch.step_forward(12)
     for e in alone:
         e.step_forward(12)
     for a, b in zip(alone, mixed):
         assert a.time() == pytest.approx(b.time(), abs=1e-12)
         assert rel_l2(b.macroscopic(fresh=True)["drho"], a.macroscopic(fresh=True)["drho"]) < 1e-6
+
+
+def test_q9_lanes_per_node(oracle64):
+    """D2Q9 stencil passes with three distributions per lane (3 lanes per
+    node, 84 nodes per block) and with one (9 lanes per node, 28 per block),
+    the autotuner's two choices, advance every scene alike."""
+    specs = [s for s in _specs() if s[0] == 2]
+    runs = {}
+    for lanes in (3, 9):
+        scenes = [_scene(oracle64, *s) for s in specs]
+        batch = naga_amd.SceneBatch(scenes)
+        batch.set_q9_lanes(lanes)
+        assert batch.tuning()["q9_lanes"] == lanes
+        batch.step_forward(20)
+        runs[lanes] = [e.macroscopic(fresh=True) for e in scenes]
+    with pytest.raises(ValueError):
+        batch.set_q9_lanes(4)
+    for a, b in zip(runs[3], runs[9]):
+        assert rel_l2(a["drho"], b["drho"]) < 1e-6
+        assert rel_l2(a["u"], b["u"]) < 1e-6
 
 
 def test_batch_mixes_union_and_direct_scenes(oracle64, monkeypatch):
