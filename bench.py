@@ -166,7 +166,7 @@ def batched_workload(steps: int, n_scenes: int = 256) -> dict:
             "max_abs_drho_after": peak,
             "plain_node_updates_per_s": plain, "batched_node_updates_per_s": batched,
             "speedup": batched / plain, "gather_batch": gb, "q9_lanes": tune["q9_lanes"],
-            "graph": tune["graph"],
+            "q9_unions": tune["q9_unions"], "graph": tune["graph"],
             "workload": "D2Q9, 256 scenes, N log-uniform 2k-50k, 0-5 obstacles r 0.03-0.1, "
                         "k 12-24, PML 8h (3h around obstacles), tau 0.6, dt 0.4h"}
 

@@ -192,16 +192,21 @@ int naga_batch_add(naga_batch* b, naga_engine* e);     /* e needs a domain */
 int naga_batch_remove(naga_batch* b, naga_engine* e);
 int naga_batch_set_graph(naga_batch* b, int32_t on);  /* default on */
 /* Autotuner: times trial_steps steps of the batch for each gather batch
- * (neighbours gathered per unrolled step: 8, 16, 32), with and without graph
- * replay and, when the batch holds D2Q9 scenes, with 3 and 9 lanes per D2Q9
- * node (three or one distributions per lane: 84 or 28 nodes per 256-thread
- * block); keeps the fastest, and restores every scene's state afterwards. */
+ * (neighbours gathered per unrolled step: 8, 16, 32), with and without
+ * graph replay and, when the batch holds D2Q9 scenes, for each D2Q9 kernel
+ * form (3 lanes per node on 84-node tiles reading the tile unions staged in
+ * LDS, the same gathering directly, 9 lanes per node on 28-node tiles);
+ * keeps the fastest, and restores every scene's state afterwards. */
 int naga_batch_autotune(naga_batch* b, int32_t trial_steps, int32_t* gather_batch);
-/* D2Q9 lanes per node of the batch's stencil passes: 3 (default) or 9. */
-int naga_batch_set_q9_lanes(naga_batch* b, int32_t lanes);
+/* D2Q9 kernel form of the batch's stencil passes: lanes per node 3
+ * (default) or 9; unions 1 (default; lanes 3 only): scenes with tile
+ * unions read them from LDS. */
+int naga_batch_set_q9_kernels(naga_batch* b, int32_t lanes, int32_t unions);
 /* The batch's current choices: gather batch (0: the kernels' default), D2Q9
- * lanes per node, graph replay (1 or 0); null outputs are skipped. */
-int naga_batch_tuning(naga_batch* b, int32_t* gather_batch, int32_t* q9_lanes, int32_t* graph);
+ * lanes per node and unions, graph replay (1 or 0); null outputs are
+ * skipped. */
+int naga_batch_tuning(naga_batch* b, int32_t* gather_batch, int32_t* q9_lanes,
+                      int32_t* q9_unions, int32_t* graph);
 /* nsteps steps of every scene (each engine's time advances by its dt) */
 int naga_batch_step(naga_batch* b, int32_t nsteps);
 int naga_batch_step_timed(naga_batch* b, int32_t nsteps, double* ms);

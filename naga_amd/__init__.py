@@ -149,8 +149,9 @@ def load() -> C.CDLL:
         "naga_server_drain": (C.c_int, [vp]),
         "naga_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
         "naga_batch_autotune": (C.c_int, [vp, i32, C.POINTER(i32)]),
-        "naga_batch_set_q9_lanes": (C.c_int, [vp, i32]),
-        "naga_batch_tuning": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+        "naga_batch_set_q9_kernels": (C.c_int, [vp, i32, i32]),
+        "naga_batch_tuning": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                        C.POINTER(i32)]),
         "naga_engine_state_size": (C.c_int, [vp, C.POINTER(i64)]),
         "naga_engine_save_state": (C.c_int, [vp, vp, i64]),
         "naga_engine_load_state": (C.c_int, [vp, vp, i64]),
@@ -195,7 +196,7 @@ EXPORTED = [
     "naga_batch_step_timed", "naga_obstacle_grid", "naga_server_create",
     "naga_server_destroy", "naga_server_submit", "naga_server_status", "naga_server_result",
     "naga_server_release", "naga_server_drain", "naga_server_get_stats",
-    "naga_batch_autotune", "naga_batch_set_q9_lanes", "naga_batch_tuning",
+    "naga_batch_autotune", "naga_batch_set_q9_kernels", "naga_batch_tuning",
     "naga_engine_state_size", "naga_engine_save_state",
     "naga_engine_load_state", "naga_engine_add_probes", "naga_engine_sample_probes",
     "naga_engine_get_stencil_sizes", "naga_advection_create", "naga_advection_from_weights",
@@ -1105,16 +1106,19 @@ class SceneBatch:
         _check(load().naga_batch_autotune(self._h, int(trial_steps), C.byref(gb)), _BATCH_PREFIX)
         return gb.value
 
-    def set_q9_lanes(self, lanes: int):
-        """D2Q9 lanes per node of the stencil passes: 3 or 9."""
-        _check(load().naga_batch_set_q9_lanes(self._h, int(lanes)), _BATCH_PREFIX)
+    def set_q9_kernels(self, lanes: int, unions: bool = True):
+        """D2Q9 kernel form of the stencil passes: 3 or 9 lanes per node;
+        with 3, whether scenes with tile unions read them from LDS."""
+        _check(load().naga_batch_set_q9_kernels(self._h, int(lanes), int(bool(unions))),
+               _BATCH_PREFIX)
 
     def tuning(self) -> dict:
-        """The current gather batch, D2Q9 lanes per node and graph replay."""
-        gb, lanes, graph = C.c_int32(), C.c_int32(), C.c_int32()
-        _check(load().naga_batch_tuning(self._h, C.byref(gb), C.byref(lanes), C.byref(graph)),
-               _BATCH_PREFIX)
-        return {"gather_batch": gb.value, "q9_lanes": lanes.value, "graph": bool(graph.value)}
+        """The current gather batch, D2Q9 kernel form and graph replay."""
+        gb, lanes, un, graph = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _check(load().naga_batch_tuning(self._h, C.byref(gb), C.byref(lanes), C.byref(un),
+                                        C.byref(graph)), _BATCH_PREFIX)
+        return {"gather_batch": gb.value, "q9_lanes": lanes.value, "q9_unions": bool(un.value),
+                "graph": bool(graph.value)}
 
     def __len__(self):
         return len(self._engines)

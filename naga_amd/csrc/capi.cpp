@@ -259,16 +259,18 @@ int naga_batch_autotune(naga_batch* b, int32_t trial_steps, int32_t* gather_batc
     });
 }
 
-int naga_batch_set_q9_lanes(naga_batch* b, int32_t lanes) {
-    return guarded([&] { batch_of(b).set_q9_lanes(lanes); });
+int naga_batch_set_q9_kernels(naga_batch* b, int32_t lanes, int32_t unions) {
+    return guarded([&] { batch_of(b).set_q9_kernels(lanes, unions != 0); });
 }
 
-int naga_batch_tuning(naga_batch* b, int32_t* gather_batch, int32_t* q9_lanes, int32_t* graph) {
+int naga_batch_tuning(naga_batch* b, int32_t* gather_batch, int32_t* q9_lanes,
+                      int32_t* q9_unions, int32_t* graph) {
     return guarded([&] {
-        int gb = 0, lanes = 0, gr = 0;
-        batch_of(b).tuning(&gb, &lanes, &gr);
+        int gb = 0, lanes = 0, un = 0, gr = 0;
+        batch_of(b).tuning(&gb, &lanes, &un, &gr);
         if (gather_batch) *gather_batch = gb;
         if (q9_lanes) *q9_lanes = lanes;
+        if (q9_unions) *q9_unions = un;
         if (graph) *graph = gr;
     });
 }

@@ -126,6 +126,12 @@ void Engine::require_domain() const {
     require(ready_, "init_domain must be called first");
 }
 
+// D2Q9 tile unions: built when NAGA_Q9_UNIONS=1 (off by default).
+static bool q9_unions_wanted(int64_t /*N*/) {
+    const char* e = std::getenv("NAGA_Q9_UNIONS");
+    return e && e[0] == '1';
+}
+
 void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_t nbnd,
                          const float* normals, const float* sigma) {
     require(batch == nullptr, "remove the engine from its batch before init_domain");
@@ -253,6 +259,8 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
     p2usplit_ = DeviceBuffer<int32_t>();
     p2split_tiles_ = DeviceBuffer<int32_t>();
     p2slab_ = DeviceBuffer<float>();
+    uslab_ = DeviceBuffer<float>();
+    ufslab_ = DeviceBuffer<float>();
     p2split_of_.clear();
     p2n_split_ = 0;
     bulist_ = DeviceBuffer<uint32_t>();
@@ -399,6 +407,41 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
             slab_ = std::move(rs);
             fslab_ = std::move(rf);
         }
+    } else if (q9_unions_wanted(N) && !(un_env && un_env[0] == '0') && q9_three(Q, N) &&
+               nfluid > 0 && k * kQ9TileNodes <= 4096) {
+        // D2Q9 tile unions over the three-per-lane kernels' 84-node tiles
+        // (no split tiles: a tile whose union is over the cap reads its rows
+        // through the list).  Copies of the stage and filter records get
+        // union positions in their index fields (uslab, ufslab); the
+        // records with node indices stay for the PML divergence (a node
+        // list) and for batches that step D2Q9 scenes without unions.
+        const int T = kQ9TileNodes;
+        const int64_t ntiles = (nfluid + T - 1) / T;
+        uslab_.allocate(slab_.size());
+        ufslab_.allocate(fslab_.size());
+        NAGA_HIP(hipMemcpyAsync(uslab_.get(), slab_.get(), slab_.size() * sizeof(float),
+                                hipMemcpyDeviceToDevice, st));
+        NAGA_HIP(hipMemcpyAsync(ufslab_.get(), fslab_.get(), fslab_.size() * sizeof(float),
+                                hipMemcpyDeviceToDevice, st));
+        DeviceBuffer<int32_t> counts(ntiles);
+        launch_tile_unions(nfluid, k, T, idx_.get(), 0, counts.get(), nullptr, nullptr, nullptr,
+                           nullptr, 0, nullptr, 0, st);
+        std::vector<int32_t> hc(static_cast<size_t>(ntiles));
+        counts.download(hc.data(), size_t(ntiles), st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        std::vector<int64_t> off(static_cast<size_t>(ntiles) + 1, 0);
+        int maxu = 0;
+        for (int64_t t = 0; t < ntiles; ++t) {
+            off[size_t(t) + 1] = off[size_t(t)] + hc[size_t(t)];
+            maxu = std::max(maxu, hc[size_t(t)]);
+        }
+        uoff_ = to_device(off, st);
+        ulist_.allocate(std::max<int64_t>(off[size_t(ntiles)], 1));
+        launch_tile_unions(nfluid, k, T, idx_.get(), 0, nullptr, nullptr, uoff_.get(),
+                           ulist_.get(), uslab_.get(), D, ufslab_.get(), 1, st);
+        NAGA_HIP(hipStreamSynchronize(st));
+        ucap_ = std::min(maxu, kQ9UnionRowsMax);
+        ucap_f_ = ucap_;
     }
 
     marks.mark();  // filter weights, slabs and tile unions done
@@ -470,7 +513,7 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
         pml_list_ = to_device(list, st);
         // with tile unions the PML divergence runs over whole tiles: every
         // tile holding a listed node (the others add an exact zero)
-        if (ulist_.get()) {
+        if (ulist_.get() && tile_nodes_ > 0) {
             std::vector<int32_t> tl;
             const int T = tile_nodes_;
             for (int64_t t0 = 0; t0 < nfluid; t0 += T) {
@@ -580,6 +623,8 @@ StepBuffers Engine::buffers() {
     S.split_tiles = split_tiles_.get();
     S.n_split = n_split_;
     S.p2slab = p2slab_.get();
+    S.uslab = uslab_.get();
+    S.ufslab = ufslab_.get();
     S.p2ulist = p2ulist_.get();
     S.p2uoff = p2uoff_.get();
     S.p2usplit = p2usplit_.get();
@@ -979,15 +1024,18 @@ void Batch::set_graph(bool on) {
     dirty_ = true;
 }
 
-void Batch::set_q9_lanes(int lanes) {
+void Batch::set_q9_kernels(int lanes, bool unions) {
     require(lanes == 3 || lanes == 9, "D2Q9 lanes per node must be 3 or 9");
+    require(lanes == 3 || !unions, "D2Q9 tile unions need 3 lanes per node");
     q9_three_ = lanes == 3;
+    q9_unions_ = unions;
     dirty_ = true;
 }
 
-void Batch::tuning(int* gather_batch, int* q9_lanes, int* graph) const {
+void Batch::tuning(int* gather_batch, int* q9_lanes, int* q9_unions, int* graph) const {
     if (gather_batch) *gather_batch = gather_batch_;
     if (q9_lanes) *q9_lanes = q9_three_ ? 3 : 9;
+    if (q9_unions) *q9_unions = q9_unions_ ? 1 : 0;
     if (graph) *graph = use_graph_ ? 1 : 0;
 }
 
@@ -999,17 +1047,26 @@ int Batch::autotune(int trial_steps) {
     for (Engine* e : scenes_) saved.push_back(e->snapshot());
     double best = 1e300;
     int best_gb = 8;
-    bool best_graph = true, best_three = q9_three_;
-    bool any_q9 = false;
-    for (Engine* e : scenes_) any_q9 = any_q9 || e->lattice() == 9;
-    std::vector<bool> threes{q9_three_};
-    if (any_q9) threes = {true, false};
-    for (bool three : threes)
+    bool best_graph = true;
+    std::pair<bool, bool> best_q9{q9_three_, q9_unions_};
+    bool any_q9 = false, any_q9_unions = false;
+    for (Engine* e : scenes_)
+        if (e->lattice() == 9) {
+            any_q9 = true;
+            any_q9_unions = any_q9_unions || e->scene_desc().S.ulist != nullptr;
+        }
+    // D2Q9 forms: (three per lane, unions); the union form where some
+    // scene has its tile unions built
+    std::vector<std::pair<bool, bool>> q9_forms{best_q9};
+    if (any_q9) q9_forms = {{true, false}, {false, false}};
+    if (any_q9_unions) q9_forms.push_back({true, true});
+    for (const auto& form : q9_forms)
         for (int gb : {8, 16, 32})
             for (bool graph : {true, false}) {
                 gather_batch_ = gb;
                 use_graph_ = graph;
-                q9_three_ = three;
+                q9_three_ = form.first;
+                q9_unions_ = form.second;
                 dirty_ = true;
                 step(1);  // builds the plan (and captures the graph)
                 const double ms = step_timed(trial_steps);
@@ -1017,12 +1074,13 @@ int Batch::autotune(int trial_steps) {
                     best = ms;
                     best_gb = gb;
                     best_graph = graph;
-                    best_three = three;
+                    best_q9 = form;
                 }
             }
     gather_batch_ = best_gb;
     use_graph_ = best_graph;
-    q9_three_ = best_three;
+    q9_three_ = best_q9.first;
+    q9_unions_ = best_q9.second;
     dirty_ = true;
     for (size_t i = 0; i < scenes_.size(); ++i) scenes_[i]->restore(saved[i]);
     return best_gb;
@@ -1040,10 +1098,17 @@ void Batch::rebuild() {
     auto three_of = [this](Engine* e) {
         return e->lattice() == 9 && q9_three_ && q9_three_fits(e->nodes());
     };
+    // whether the scene's passes read its tile unions: D3Q27 scenes with
+    // unions always (their records hold union positions); D2Q9 scenes
+    // with unions when the batch runs them three per lane with unions
+    auto tiles_of = [&](Engine* e) {
+        const bool built = e->scene_desc().S.ulist != nullptr;
+        return e->lattice() == 27 ? built : built && three_of(e) && q9_unions_;
+    };
     auto key_of = [&](Engine* e) {
         const int Q = e->lattice();
         const bool un = step_uses_unions(Q, e->stencil_k(), e->nodes());
-        const bool tiles = e->scene_desc().S.ulist != nullptr;
+        const bool tiles = tiles_of(e);
         const bool three = three_of(e);
         const bool ragged = e->scene_desc().S.tile_ng != nullptr;
         return Q * 1000 + (un ? e->stencil_k() : 0) + (tiles ? 500 : 0) + (three ? 250 : 0) +
@@ -1059,6 +1124,7 @@ void Batch::rebuild() {
             groups_.push_back(std::make_unique<Group>());
             groups_.back()->Q = e->lattice();
             groups_.back()->three = three_of(e);
+            groups_.back()->tiles = tiles_of(e);
         }
         groups_[g]->members.push_back(e);
     }
@@ -1094,10 +1160,11 @@ void Batch::rebuild() {
         P.unions = step_uses_unions(g.Q, g.members[0]->stencil_k(), g.members[0]->nodes());
         P.k_template = (kcommon == 48 || kcommon == 32 || kcommon == 16) ? kcommon : 0;
         P.gather_batch = gather_batch_;
-        P.tiles = descs[0].S.ulist != nullptr;
+        P.tiles = g.tiles;
         P.ragged = descs[0].S.tile_ng != nullptr;
         for (const SceneDesc& d : descs)
-            require((d.S.ulist != nullptr) == P.tiles, "scenes with and without tile unions");
+            require(g.Q == 9 ? (!P.tiles || d.S.ulist != nullptr) : (d.S.ulist != nullptr) == P.tiles,
+                    "scenes with and without tile unions");
         require(!P.unions || P.k_template != 0, "four-per-lane scenes need a common k");
         g.table = to_device(descs, stream_);
         g.bstart = to_device(bstart, stream_);

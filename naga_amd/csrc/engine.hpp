@@ -167,6 +167,7 @@ class Engine {
     // boundary-interpolation tiles: their distinct source nodes
     DeviceBuffer<uint32_t> bulist_;
     DeviceBuffer<int64_t> buoff_;
+    DeviceBuffer<float> uslab_, ufslab_;  // D2Q9 records with union positions
     int ucap_ = 0;
     int ucap_f_ = 0;
     int self_first_ = 0;
@@ -207,17 +208,20 @@ class Batch {
     double step_timed(int nsteps);
     void set_graph(bool on);
     // Times trial_steps steps for each gather batch (8, 16, 32), with and
-    // without graph replay, and, when the batch holds D2Q9 scenes, with one
-    // and three distributions per lane (9 or 3 lanes per node, 28 or 84
-    // nodes per 256-thread block); keeps the fastest and restores every
-    // scene's state (the trials do not advance the scenes).  Returns the
-    // chosen gather batch, or 0 if the batch is empty.
+    // without graph replay, and, when the batch holds D2Q9 scenes, for each
+    // D2Q9 kernel form: three distributions per lane (3 lanes per node,
+    // 84-node tiles) reading the tile unions staged in LDS, the same
+    // gathering directly, or one distribution per lane (9 lanes per node,
+    // 28-node tiles); keeps the fastest and restores every scene's state
+    // (the trials do not advance the scenes).  Returns the chosen gather
+    // batch, or 0 if the batch is empty.
     int autotune(int trial_steps);
-    // D2Q9 lanes per node: 3 (three distributions per lane) or 9.
-    void set_q9_lanes(int lanes);
+    // D2Q9 kernel form: lanes per node 3 or 9; unions (lanes 3 only): read
+    // the tile unions in LDS where the scene has them.
+    void set_q9_kernels(int lanes, bool unions);
     // The current choices: gather batch (0: the kernels' default), D2Q9
-    // lanes per node, graph replay.
-    void tuning(int* gather_batch, int* q9_lanes, int* graph) const;
+    // lanes per node and unions, graph replay.
+    void tuning(int* gather_batch, int* q9_lanes, int* q9_unions, int* graph) const;
     int size() const { return int(scenes_.size()); }
     hipStream_t stream() const { return stream_; }
     // A member's scene description changed (density sources added or
@@ -229,6 +233,7 @@ class Batch {
     struct Group {
         int Q = 9;
         bool three = false;  // D2Q9 with three distributions per lane
+        bool tiles = false;  // the members' tile unions are read (D3Q27: always with them)
         std::vector<Engine*> members;
         BatchPlan plan;
         DeviceBuffer<SceneDesc> table;
@@ -245,7 +250,8 @@ class Batch {
     bool use_graph_ = true;
     bool any_sources_ = false;
     int gather_batch_ = 0;
-    bool q9_three_ = true;  // D2Q9 scenes with three distributions per lane
+    bool q9_three_ = true;   // D2Q9 scenes with three distributions per lane
+    bool q9_unions_ = true;  // ... reading their tile unions in LDS
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };

@@ -1203,6 +1203,7 @@ __global__ __launch_bounds__(NPB * kV4Lanes) __attribute__((amdgpu_waves_per_eu(
 constexpr int kQ9Lanes = 3;                          // lanes per node
 constexpr int kQ9Npw = 21;                           // nodes per wavefront
 constexpr int kQ9Npb = kQ9Npw * (kBlock / 64);       // nodes per block (84)
+static_assert(kQ9Npb == kQ9TileNodes, "D2Q9 union tiles are the three-per-lane blocks");
 
 struct Q9Lane {
     int slot, q;   // node slot in the block, piece (distributions 3q .. 3q + 2)
@@ -1245,10 +1246,11 @@ __device__ __forceinline__ void stage_padded(const float* slab, int64_t node0, i
 
 __device__ __forceinline__ float c9(int a, int d) { return float(Lattice<9>::c(a, d)); }
 
-// sum_s (c_a . W_s) fin(idx_s)_a for the lane's three distributions
-template <int BATCH>
-__device__ __forceinline__ f3v stencil_sum3(const float* rp, int K, __amdgpu_buffer_rsrc_t fin,
-                                            int q) {
+// sum_s (c_a . W_s) fin(idx_s)_a for the lane's three distributions, the
+// neighbour pieces fetched by fetch(index field) (a node index, or a
+// position in the tile's union)
+template <int BATCH, class Fetch>
+__device__ __forceinline__ f3v stencil_sum3(const float* rp, int K, int q, Fetch fetch) {
     float cx[3], cy[3];
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -1267,7 +1269,7 @@ __device__ __forceinline__ f3v stencil_sum3(const float* rp, int K, __amdgpu_buf
             const uint32_t j = __float_as_uint(r[0]);
             wx[u] = in ? r[1] : 0.0f;
             wy[u] = in ? r[2] : 0.0f;
-            gv[u] = ld3(fin, j * 36u + uint32_t(q) * 12u);
+            gv[u] = fetch(j);
         }
 #pragma unroll
         for (int u = 0; u < BATCH; ++u) {
@@ -1279,17 +1281,64 @@ __device__ __forceinline__ f3v stencil_sum3(const float* rp, int K, __amdgpu_buf
     return acc;
 }
 
-template <int STAGE, int BATCH>
+// D2Q9 tile unions: the distinct stencil neighbours of the 84 fluid nodes
+// of a block's tile, their rows copied to LDS after the block's records
+// (urows[9 r + 3 q + e] = distribution 3 q + e of row ul[r]) so every
+// gather of the stencil sums reads LDS.  The block's threads copy the
+// 3 Uc 12-byte pieces: every index load first (past Uc they read zero
+// through the buffer resource), then every row load, then the stores.
+constexpr int kQ9CopyIt = (3 * kQ9UnionRowsMax + kBlock - 1) / kBlock;
+struct Q9UnionIdx {
+    uint32_t j[kQ9CopyIt];
+};
+__device__ __forceinline__ Q9UnionIdx q9_union_indices(const uint32_t* ul, int Uc) {
+    const __amdgpu_buffer_rsrc_t ur = rsrc(ul, uint64_t(Uc) * 4u);
+    const int nit = (3 * Uc + kBlock - 1) / kBlock;
+    Q9UnionIdx x;
+#pragma unroll
+    for (int it = 0; it < kQ9CopyIt; ++it) {
+        const uint32_t t = threadIdx.x + it * kBlock;
+        x.j[it] = it < nit ? __builtin_amdgcn_raw_buffer_load_b32(ur, (t / 3u) * 4u, 0, 0) : 0u;
+    }
+    return x;
+}
+__device__ __forceinline__ void q9_union_rows(float* urows, const Q9UnionIdx& x, int Uc,
+                                              __amdgpu_buffer_rsrc_t src) {
+    const int nit = (3 * Uc + kBlock - 1) / kBlock;
+    f3v v[kQ9CopyIt];
+#pragma unroll
+    for (int it = 0; it < kQ9CopyIt; ++it) {
+        const uint32_t t = threadIdx.x + it * kBlock;
+        if (it < nit) v[it] = ld3(src, x.j[it] * 36u + (t % 3u) * 12u);
+    }
+#pragma unroll
+    for (int it = 0; it < kQ9CopyIt; ++it) {
+        const int t = int(threadIdx.x) + it * kBlock;
+        if (it < nit && t < 3 * Uc) {
+            urows[3 * t] = v[it].x;
+            urows[3 * t + 1] = v[it].y;
+            urows[3 * t + 2] = v[it].z;
+        }
+    }
+}
+__device__ __forceinline__ f3v q9_lds_row(const float* urows, uint32_t p, int q) {
+    const float* r = urows + p * 9u + uint32_t(q) * 3u;
+    return f3v{r[0], r[1], r[2]};
+}
+
+template <int STAGE, int BATCH, bool TU = false>
 __global__ __launch_bounds__(kBlock) void k_stage3(StepGeom G, StepBuffers S, BatchArgs B) {
     extern __shared__ float srec3[];
     const BlockCtx bc = locate(B, G, S);
     const int K = G.k, RW = 3 * K;
-    const int64_t node0 = xcd_tile(bc.b, bc.nb) * kQ9Npb;
+    const int64_t tile = xcd_tile(bc.b, bc.nb);
+    const int64_t node0 = tile * kQ9Npb;
     const Q9Lane ln;
     const int64_t i = node0 + ln.slot;
     const uint64_t rows = uint64_t(G.N) * 36u;
     const uint32_t own = uint32_t(i) * 36u + uint32_t(ln.q) * 12u;
     const __amdgpu_buffer_rsrc_t rg = rsrc(S.g, rows);
+    const __amdgpu_buffer_rsrc_t gin = rsrc(stage_in(S, STAGE), rows);
     // own rows first: their latency overlaps the record staging
     const bool fluid = i < G.nfluid;
     const f3v g0 = ld3(rg, own);  // past N: zero
@@ -1300,7 +1349,17 @@ __global__ __launch_bounds__(kBlock) void k_stage3(StepGeom G, StepBuffers S, Ba
         y3 = ld3(rsrc(S.acc, rows), own);
     }
     const int64_t nf = G.nfluid - node0;
-    stage_padded(S.slab, node0, nf <= 0 ? 0 : (nf < kQ9Npb ? int(nf) : kQ9Npb), RW, srec3);
+    const int cnt = nf <= 0 ? 0 : (nf < kQ9Npb ? int(nf) : kQ9Npb);
+    // with tile unions: the tile's union range (tiles of fluid nodes only)
+    int64_t u0 = 0;
+    int U = 0;
+    if (TU && cnt > 0) union_range(S.uoff, tile, u0, U);
+    const int Uc = U < S.ucap ? U : S.ucap;
+    float* urows = srec3 + kQ9Npb * (RW + 1);
+    Q9UnionIdx ux;
+    if (TU) ux = q9_union_indices(S.ulist + u0, Uc);
+    stage_padded(TU ? S.uslab : S.slab, node0, cnt, RW, srec3);
+    if (TU) q9_union_rows(urows, ux, Uc, gin);
     __syncthreads();
     if (!ln.active || i >= G.N) return;
     if (STAGE == 1 && !fluid) {
@@ -1310,8 +1369,20 @@ __global__ __launch_bounds__(kBlock) void k_stage3(StepGeom G, StepBuffers S, Ba
         return;
     }
     if (!fluid) return;
-    const f3v acc = stencil_sum3<BATCH>(srec3 + ln.slot * (RW + 1), K,
-                                        rsrc(stage_in(S, STAGE), rows), ln.q);
+    const float* rp = srec3 + ln.slot * (RW + 1);
+    const uint32_t lane_off = uint32_t(ln.q) * 12u;
+    f3v acc;
+    if (!TU) {
+        acc = stencil_sum3<BATCH>(rp, K, ln.q,
+                                  [&](uint32_t j) { return ld3(gin, j * 36u + lane_off); });
+    } else if (U <= S.ucap) {
+        acc = stencil_sum3<BATCH>(rp, K, ln.q,
+                                  [&](uint32_t p) { return q9_lds_row(urows, p, ln.q); });
+    } else {  // union over the cap: rows read from HBM through the list
+        const uint32_t* ul = S.ulist + u0;
+        acc = stencil_sum3<BATCH>(rp, K, ln.q,
+                                  [&](uint32_t p) { return ld3(gin, ul[p] * 36u + lane_off); });
+    }
     f3v out;
     out.x = rk4_out<STAGE>(g0.x, -acc.x, G.dt, y1.x, y2.x, y3.x);
     out.y = rk4_out<STAGE>(g0.y, -acc.y, G.dt, y1.y, y2.y, y3.y);
@@ -1337,7 +1408,10 @@ __global__ __launch_bounds__(kBlock) void k_p2_3(StepGeom G, StepBuffers S, Batc
     __syncthreads();
     if (!valid) return;
     const uint64_t rows = uint64_t(G.N) * 36u;
-    const f3v acc = stencil_sum3<8>(rp, K, rsrc(S.Gf, rows), ln.q);
+    const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
+    const uint32_t lane_off = uint32_t(ln.q) * 12u;
+    const f3v acc =
+        stencil_sum3<8>(rp, K, ln.q, [&](uint32_t j) { return ld3(gf, j * 36u + lane_off); });
     const __amdgpu_buffer_rsrc_t ra = rsrc(S.A, rows);
     const uint32_t own = uint32_t(i) * 36u + uint32_t(ln.q) * 12u;
     f3v a = ld3(ra, own);
@@ -1400,11 +1474,13 @@ __global__ __launch_bounds__(kBlock) void k_binterp3(StepGeom G, StepBuffers S, 
 
 // p4 for D2Q9, three per lane: selective filter on fluid nodes, bounce-back
 // on boundary nodes (the arithmetic of k_p4).
+template <bool TU = false>
 __global__ __launch_bounds__(kBlock) void k_p4_3(StepGeom G, StepBuffers S, BatchArgs B) {
     extern __shared__ float srec3[];
     const BlockCtx bc = locate(B, G, S);
     const int K = G.k, RW = 2 * K;
-    const int64_t node0 = xcd_tile(bc.b, bc.nb) * kQ9Npb;
+    const int64_t tile = xcd_tile(bc.b, bc.nb);
+    const int64_t node0 = tile * kQ9Npb;
     const Q9Lane ln;
     const int64_t i = node0 + ln.slot;
     const uint64_t rows = uint64_t(G.N) * 36u;
@@ -1412,12 +1488,24 @@ __global__ __launch_bounds__(kBlock) void k_p4_3(StepGeom G, StepBuffers S, Batc
     const __amdgpu_buffer_rsrc_t r1 = rsrc(S.A, rows);
     const f3v si = ld3(r1, own);
     const int64_t nf = G.nfluid - node0;
-    stage_padded(S.fslab, node0, nf <= 0 ? 0 : (nf < kQ9Npb ? int(nf) : kQ9Npb), RW, srec3);
+    const int cnt = nf <= 0 ? 0 : (nf < kQ9Npb ? int(nf) : kQ9Npb);
+    int64_t u0 = 0;
+    int U = 0;
+    if (TU && cnt > 0) union_range(S.uoff, tile, u0, U);
+    const int Uc = U < S.ucap ? U : S.ucap;
+    float* urows = srec3 + kQ9Npb * (RW + 1);
+    Q9UnionIdx ux;
+    if (TU) ux = q9_union_indices(S.ulist + u0, Uc);
+    stage_padded(TU ? S.ufslab : S.fslab, node0, cnt, RW, srec3);
+    if (TU) q9_union_rows(urows, ux, Uc, r1);
     __syncthreads();
     if (!ln.active || i >= G.N) return;
     f3v out;
     if (i < G.nfluid) {
         const float* rp = srec3 + ln.slot * (RW + 1);
+        const uint32_t lane_off = uint32_t(ln.q) * 12u;
+        const uint32_t* ul = S.ulist + u0;
+        const bool lds = TU && U <= S.ucap;
         f3v sum = {0.0f, 0.0f, 0.0f};
         for (int s0 = 0; s0 < K; s0 += 8) {
             float w[8];
@@ -1428,7 +1516,9 @@ __global__ __launch_bounds__(kBlock) void k_p4_3(StepGeom G, StepBuffers S, Batc
                 const bool in = s < K;
                 const uint32_t j = __float_as_uint(rp[(in ? s : 0) * 2]);
                 w[u] = in ? rp[s * 2 + 1] : 0.0f;
-                gv[u] = ld3(r1, j * 36u + uint32_t(ln.q) * 12u);
+                if (!TU) gv[u] = ld3(r1, j * 36u + lane_off);
+                else if (lds) gv[u] = q9_lds_row(urows, j, ln.q);
+                else gv[u] = ld3(r1, ul[j] * 36u + lane_off);  // over the cap: through the list
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -2153,18 +2243,30 @@ void launch_pass(int pass, int64_t nb, const StepGeom& G, const StepBuffers& S,
     if (Q == 9 && c.three &&
         (pass == kP2 || pass == kP3 || pass == kP4 || pass == kP5 ||
          (pass >= kS1 && pass <= kS4))) {
-        const size_t lds = size_t(kQ9Npb) * ((pass == kP4 ? 2 : 3) * c.kmax + 1) * 4u;
+        // records of the block's nodes (one pad float per node row), then
+        // with tile unions the union rows (36 bytes each)
+        const size_t recs = size_t(kQ9Npb) * ((pass == kP4 ? 2 : 3) * c.kmax + 1) * 4u;
+        const size_t lds = recs + (c.tiles ? size_t(c.ucap) * 36u : 0u);
         const int b = c.gbatch > 0 ? c.gbatch : 8;
         switch (pass) {
-            case kP2: k_p2_3<<<nb, kBlock, lds, st>>>(G, S, c.B); break;
+            case kP2: k_p2_3<<<nb, kBlock, recs, st>>>(G, S, c.B); break;
             case kP3: k_binterp3<true><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
             case kP5: k_binterp3<false><<<nb, kBlock, 0, st>>>(G, S, c.B); break;
-            case kP4: k_p4_3<<<nb, kBlock, lds, st>>>(G, S, c.B); break;
+            case kP4:
+                if (c.tiles) k_p4_3<true><<<nb, kBlock, lds, st>>>(G, S, c.B);
+                else k_p4_3<false><<<nb, kBlock, lds, st>>>(G, S, c.B);
+                break;
 #define NAGA_S3(STG)                                                                          \
     case kS##STG:                                                                             \
-        if (b == 32) k_stage3<STG, 32><<<nb, kBlock, lds, st>>>(G, S, c.B);                   \
-        else if (b == 16) k_stage3<STG, 16><<<nb, kBlock, lds, st>>>(G, S, c.B);              \
-        else k_stage3<STG, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);                            \
+        if (c.tiles) {                                                                        \
+            if (b == 32) k_stage3<STG, 32, true><<<nb, kBlock, lds, st>>>(G, S, c.B);         \
+            else if (b == 16) k_stage3<STG, 16, true><<<nb, kBlock, lds, st>>>(G, S, c.B);    \
+            else k_stage3<STG, 8, true><<<nb, kBlock, lds, st>>>(G, S, c.B);                  \
+        } else {                                                                              \
+            if (b == 32) k_stage3<STG, 32><<<nb, kBlock, lds, st>>>(G, S, c.B);               \
+            else if (b == 16) k_stage3<STG, 16><<<nb, kBlock, lds, st>>>(G, S, c.B);          \
+            else k_stage3<STG, 8><<<nb, kBlock, lds, st>>>(G, S, c.B);                        \
+        }                                                                                     \
         break;
             NAGA_S3(1) NAGA_S3(2) NAGA_S3(3) NAGA_S3(4)
 #undef NAGA_S3

@@ -62,6 +62,10 @@ struct StepBuffers {
     float* Q2v;      // [N][RS] current Q2 value
     const float* slab;    // [nfluid][k][1+D] stencil records
     const float* fslab;   // [nfluid][k][2] filter records {idx, weight}
+    // D2Q9 tile unions: the same records with the index fields holding
+    // positions in the node's tile union (null without unions)
+    const float* uslab;
+    const float* ufslab;
     const float* bslab;   // [nbnd][kb][2] boundary interpolation records
     const float* sigma;   // [nlayer] absorption
     const float* normals; // [nbnd][D] inward normals
@@ -216,6 +220,11 @@ constexpr int means_stride(int Q) { return Q == 27 ? 32 : Q; }
 constexpr int kBUnionCap = 256;   // LDS rows per boundary-interpolation tile
 constexpr int kBTileNodes = 32;   // boundary nodes per boundary-interpolation tile
 constexpr int kUnionRowsMax = 512;  // LDS union rows per block, upper bound
+// D2Q9 tile unions: fluid nodes per tile (the block of the three-per-lane
+// kernels: 21 nodes per wavefront, four wavefronts) and LDS union rows per
+// tile, upper bound (36-byte rows).
+constexpr int kQ9TileNodes = 84;
+constexpr int kQ9UnionRowsMax = 512;
 
 // Per-kernel entry points (used by the profiler and the standalone C-ABI).
 template <int Q>

@@ -101,7 +101,8 @@ def test_autotune_leaves_scenes_unchanged(oracle64):
     gb = batch.autotune(3)
     assert gb in (8, 16, 32)
     t = batch.tuning()
-    assert t["gather_batch"] == gb and t["q9_lanes"] in (3, 9)
+    assert t["gather_batch"] == gb and (t["q9_lanes"], t["q9_unions"]) in ((3, True), (3, False),
+                                                                            (9, False))
     batch.step_forward(12)
     for e in alone:
         e.step_forward(12)
@@ -110,22 +111,28 @@ def test_autotune_leaves_scenes_unchanged(oracle64):
         assert rel_l2(b.macroscopic(fresh=True)["drho"], a.macroscopic(fresh=True)["drho"]) < 1e-6
 
 
-def test_q9_lanes_per_node(oracle64):
-    """D2Q9 stencil passes with three distributions per lane (3 lanes per
-    node, 84 nodes per block) and with one (9 lanes per node, 28 per block),
-    the autotuner's two choices, advance every scene alike."""
+@pytest.mark.parametrize("form", [(3, False), (9, False)])
+def test_q9_kernel_forms(oracle64, monkeypatch, form):
+    """The autotuner's D2Q9 kernel forms advance every scene alike: three
+    distributions per lane reading the tile unions staged in LDS (scenes
+    set up with NAGA_Q9_UNIONS=1), the same gathering directly, one
+    distribution per lane."""
+    monkeypatch.setenv("NAGA_Q9_UNIONS", "1")
     specs = [s for s in _specs() if s[0] == 2]
     runs = {}
-    for lanes in (3, 9):
+    for lanes, unions in ((3, True), form):
         scenes = [_scene(oracle64, *s) for s in specs]
         batch = naga_amd.SceneBatch(scenes)
-        batch.set_q9_lanes(lanes)
-        assert batch.tuning()["q9_lanes"] == lanes
+        batch.set_q9_kernels(lanes, unions)
+        t = batch.tuning()
+        assert (t["q9_lanes"], t["q9_unions"]) == (lanes, unions)
         batch.step_forward(20)
-        runs[lanes] = [e.macroscopic(fresh=True) for e in scenes]
+        runs[(lanes, unions)] = [e.macroscopic(fresh=True) for e in scenes]
     with pytest.raises(ValueError):
-        batch.set_q9_lanes(4)
-    for a, b in zip(runs[3], runs[9]):
+        batch.set_q9_kernels(4)
+    with pytest.raises(ValueError):
+        batch.set_q9_kernels(9, True)
+    for a, b in zip(runs[(3, True)], runs[form]):
         assert rel_l2(a["drho"], b["drho"]) < 1e-6
         assert rel_l2(a["u"], b["u"]) < 1e-6
 
