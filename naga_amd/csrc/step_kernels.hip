@@ -1369,6 +1369,15 @@ __global__ __launch_bounds__(kBlock) void k_stage3(StepGeom G, StepBuffers S, Ba
         return;
     }
     if (!fluid) return;
+    // stage 4: the part without the rate, formed before the gather (three
+    // live values instead of twelve; the arithmetic of rk4_out<4>)
+    f3v pre = g0;
+    if (STAGE == 4) {
+        pre.x = rk4_pre(g0.x, y1.x, y2.x, y3.x);
+        pre.y = rk4_pre(g0.y, y1.y, y2.y, y3.y);
+        pre.z = rk4_pre(g0.z, y1.z, y2.z, y3.z);
+        asm volatile("" : "+v"(pre.x), "+v"(pre.y), "+v"(pre.z));
+    }
     const float* rp = srec3 + ln.slot * (RW + 1);
     const uint32_t lane_off = uint32_t(ln.q) * 12u;
     f3v acc;
@@ -1384,9 +1393,16 @@ __global__ __launch_bounds__(kBlock) void k_stage3(StepGeom G, StepBuffers S, Ba
                                   [&](uint32_t p) { return ld3(gin, ul[p] * 36u + lane_off); });
     }
     f3v out;
-    out.x = rk4_out<STAGE>(g0.x, -acc.x, G.dt, y1.x, y2.x, y3.x);
-    out.y = rk4_out<STAGE>(g0.y, -acc.y, G.dt, y1.y, y2.y, y3.y);
-    out.z = rk4_out<STAGE>(g0.z, -acc.z, G.dt, y1.z, y2.z, y3.z);
+    if (STAGE == 4) {
+        const float c = G.dt * (1.0f / 6.0f);
+        out.x = pre.x + c * -acc.x;
+        out.y = pre.y + c * -acc.y;
+        out.z = pre.z + c * -acc.z;
+    } else {
+        out.x = rk4_out<STAGE>(g0.x, -acc.x, G.dt, y1.x, y2.x, y3.x);
+        out.y = rk4_out<STAGE>(g0.y, -acc.y, G.dt, y1.y, y2.y, y3.y);
+        out.z = rk4_out<STAGE>(g0.z, -acc.z, G.dt, y1.z, y2.z, y3.z);
+    }
     st3(rsrc(stage_out(S, STAGE), rows), own, out);
 }
 
