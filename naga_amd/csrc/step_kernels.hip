@@ -1236,12 +1236,27 @@ __device__ __forceinline__ void st3(__amdgpu_buffer_rsrc_t r, uint32_t off, f3v 
 }
 
 // records of nodes [node0, node0 + cnt) (rows of RW floats) into lds rows
-// of RW + 1 floats
+// of RW + 1 floats by LDS DMA: the block's waves copy 64-float chunks of
+// the padded layout, lane l of chunk c filling LDS float 64 c + l with its
+// source float (a pad slot takes the tile's first float; lanes past the
+// last row copy nothing).  Wait with dma_fence.
 __device__ __forceinline__ void stage_padded(const float* slab, int64_t node0, int cnt, int RW,
                                              float* lds) {
-    const float* src = slab + node0 * int64_t(RW);
-    const int tot = cnt * RW;
-    for (int t = threadIdx.x; t < tot; t += kBlock) lds[t + t / RW] = src[t];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int RP = RW + 1, tot = cnt * RP;
+    const __amdgpu_buffer_rsrc_t src = rsrc(slab + node0 * int64_t(RW), uint64_t(cnt) * RW * 4u);
+    const int nchunks = (tot + 63) / 64;
+    // t / RP by a float reciprocal: t < 2^15 and RP <= 193 keep (t + 1/2) / RP
+    // at least 1/(2 RP) from an integer, far above the rounding error
+    const float rinv = 1.0f / float(RP);
+    for (int c = wave; c < nchunks; c += kBlock / 64) {
+        const int t = c * 64 + lane;
+        const int node = int((float(t) + 0.5f) * rinv), off = t - node * RP;
+        const uint32_t e = off < RW ? uint32_t(node * RW + off) : 0u;
+        if (t < tot)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(src, (lds_void_t*)(lds + c * 64), 4, e * 4u,
+                                                     0, 0, kStreamPolicy);
+    }
 }
 
 __device__ __forceinline__ float c9(int a, int d) { return float(Lattice<9>::c(a, d)); }
@@ -1360,7 +1375,7 @@ __global__ __launch_bounds__(kBlock) void k_stage3(StepGeom G, StepBuffers S, Ba
     if (TU) ux = q9_union_indices(S.ulist + u0, Uc);
     stage_padded(TU ? S.uslab : S.slab, node0, cnt, RW, srec3);
     if (TU) q9_union_rows(urows, ux, Uc, gin);
-    __syncthreads();
+    dma_fence();
     if (!ln.active || i >= G.N) return;
     if (STAGE == 1 && !fluid) {
         st3(rsrc(S.A, rows), own, g0);
@@ -1514,7 +1529,7 @@ __global__ __launch_bounds__(kBlock) void k_p4_3(StepGeom G, StepBuffers S, Batc
     if (TU) ux = q9_union_indices(S.ulist + u0, Uc);
     stage_padded(TU ? S.ufslab : S.fslab, node0, cnt, RW, srec3);
     if (TU) q9_union_rows(urows, ux, Uc, r1);
-    __syncthreads();
+    dma_fence();
     if (!ln.active || i >= G.N) return;
     f3v out;
     if (i < G.nfluid) {
