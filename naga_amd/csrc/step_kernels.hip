@@ -1428,15 +1428,39 @@ __global__ __launch_bounds__(kBlock) void k_p2_3(StepGeom G, StepBuffers S, Batc
     const BlockCtx bc = locate(B, G, S);
     const int K = G.k, RW = 3 * K;
     const Q9Lane ln;
-    const int64_t item = xcd_tile(bc.b, bc.nb) * kQ9Npb + ln.slot;
+    const int64_t item0 = xcd_tile(bc.b, bc.nb) * kQ9Npb, item = item0 + ln.slot;
     const bool valid = ln.active && item < S.n_pml;
     const int64_t i = valid ? S.pml_list[item] : 0;
     float* rp = srec3 + ln.slot * (RW + 1);
-    if (valid) {
-        const float* src = S.slab + i * int64_t(RW);
-        for (int t = ln.q; t < RW; t += kQ9Lanes) rp[t] = src[t];
+    const int64_t left = S.n_pml - item0;
+    const int cnt = left < kQ9Npb ? int(left) : kQ9Npb;
+    if (uint64_t(G.nfluid) * uint64_t(RW) * 4u < (uint64_t(1) << 32)) {
+        // the listed nodes' records by LDS DMA (32-bit offsets into the slab)
+        __shared__ uint32_t rows_of[kQ9Npb];
+        if (int(threadIdx.x) < cnt) rows_of[threadIdx.x] = uint32_t(S.pml_list[item0 + threadIdx.x]);
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int RP = RW + 1, tot = cnt * RP;
+        const __amdgpu_buffer_rsrc_t src = rsrc(S.slab, uint64_t(G.nfluid) * RW * 4u);
+        const float rinv = 1.0f / float(RP);  // as in stage_padded
+        for (int c = wave; c < (tot + 63) / 64; c += kBlock / 64) {
+            const int t = c * 64 + lane;
+            const int node = int((float(t) + 0.5f) * rinv), off = t - node * RP;
+            const uint32_t e =
+                (t < tot && off < RW) ? rows_of[node < cnt ? node : 0] * uint32_t(RW) + uint32_t(off)
+                                      : 0u;
+            if (t < tot)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(src, (lds_void_t*)(srec3 + c * 64), 4,
+                                                         e * 4u, 0, 0, kStreamPolicy);
+        }
+        dma_fence();
+    } else {
+        if (valid) {
+            const float* src = S.slab + i * int64_t(RW);
+            for (int t = ln.q; t < RW; t += kQ9Lanes) rp[t] = src[t];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (!valid) return;
     const uint64_t rows = uint64_t(G.N) * 36u;
     const __amdgpu_buffer_rsrc_t gf = rsrc(S.Gf, rows);
