@@ -362,6 +362,8 @@ void Engine::init_domain(const float* pts, int64_t nbulk, int64_t nlayer, int64_
                 p95 = (c[at] + 7) / 8 * 8;
             }
             ucap_f_ = std::max(8, std::min(ucap_, p95));
+            if (const char* fe = std::getenv("NAGA_UCAP_F"))
+                ucap_f_ = std::max(8, std::min(ucap_, std::atoi(fe)));
         }
         // whether every fluid node's first stencil entry is itself (the kNN
         // puts it first unless another node sits at the same point)
