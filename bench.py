@@ -240,10 +240,27 @@ def served_workload(n_requests: int = 32, rate: float = 50.0) -> dict:
 
 def profile_workload(name: str, steps: int) -> dict:
     """One workload alone for a profiler run: the 256-scene batched D2Q9
-    workload (autotuned, then `steps` batched steps) or the irregular 3D
-    scene (`steps` steps); returns its time per step."""
+    workload (autotuned, then `steps` batched steps), the irregular 3D
+    scene (`steps` steps), or the set-up of the 1M-node D3Q27 scene (done
+    twice, the second timed by phase); returns its time per step or its
+    set-up phases."""
     import naga_amd
     from naga_amd import workloads
+    if name == "setup":
+        h = 0.01
+        nodes = naga_amd.UniformGridProvider([0, 0, 0], [1, 1, 1], h, 8 * h, 0.5,
+                                             jitter=0.25).get()
+        p = naga_amd.problem_parameters(27, 0.6, 0.4 * h)
+        out = {}
+        for rep in range(2):
+            eng = naga_amd.SimulationEngine(lattice=27, k=32, rbf=naga_amd.RBF_POLYHARMONIC3)
+            eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+            t0 = time.perf_counter()
+            eng.init_domain(nodes)
+            out = {"workload": "setup", "nodes": eng.num_nodes,
+                   "setup_s": time.perf_counter() - t0, "phases_ms": eng.setup_phases()}
+            eng.close()
+        return out
     if name == "batched":
         engines = [workloads.batched_engine(s)[0] for s in workloads.batched_scenes(256)]
         batch = naga_amd.SceneBatch(engines)
@@ -352,7 +369,7 @@ def main() -> int:
     ap.add_argument("--served-requests", type=int, default=32)
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the 2D plain and irregular 3D scenes")
-    ap.add_argument("--profile-workload", choices=("batched", "irregular"), default=None,
+    ap.add_argument("--profile-workload", choices=("batched", "irregular", "setup"), default=None,
                     help="only run that workload for --steps steps (for a profiler)")
     args = ap.parse_args()
     if args.copy_calibration:
