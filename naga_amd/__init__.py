@@ -1106,9 +1106,12 @@ class SceneBatch:
         _check(load().naga_batch_autotune(self._h, int(trial_steps), C.byref(gb)), _BATCH_PREFIX)
         return gb.value
 
-    def set_q9_kernels(self, lanes: int, unions: bool = True):
+    def set_q9_kernels(self, lanes: int, unions: bool | None = None):
         """D2Q9 kernel form of the stencil passes: 3 or 9 lanes per node;
-        with 3, whether scenes with tile unions read them from LDS."""
+        with 3, whether scenes with tile unions read them from LDS (default:
+        yes for 3 lanes, no for 9)."""
+        if unions is None:
+            unions = int(lanes) == 3
         _check(load().naga_batch_set_q9_kernels(self._h, int(lanes), int(bool(unions))),
                _BATCH_PREFIX)
 
