@@ -28,8 +28,8 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def test_config2_full_size_parity(oracle64):
-    """BASELINE config 2 at its size: unit cube h = 0.01 (101^3 nodes),
+def test_1m_node_cube_full_size_parity(oracle64):
+    """The D3Q27 benchmark cube at its size: h = 0.01 (101^3 nodes),
     jitter 0.25h, k = 32, r^3 RBF, PML 8h sigma_max 0.5, tau 0.6, dt 0.4h,
     pulse 1e-3 / 0.05 at (0.3, 0.5, 0.5)."""
     h = 0.01
@@ -37,7 +37,7 @@ def test_config2_full_size_parity(oracle64):
                                   sigma_max=0.5)
     t0 = time.perf_counter()
     ref = oracle64.engine(nodes, Q=27, k=32, h=h, tau=0.6, threads=16)
-    _say(f"config 2: oracle setup {time.perf_counter() - t0:.1f} s, {ref.N} nodes")
+    _say(f"1M-node cube: oracle setup {time.perf_counter() - t0:.1f} s, {ref.N} nodes")
     sim = naga_amd.SimulationNodes(nodes["points"], nodes["normals"], nodes["sigma"],
                                    nodes["n_bulk"], nodes["n_layer"], nodes["n_boundary"], h)
     eng = naga_amd.SimulationEngine(lattice=27, k=32, rbf=naga_amd.RBF_POLYHARMONIC3)
@@ -49,21 +49,21 @@ def test_config2_full_size_parity(oracle64):
     assert np.array_equal(g["bidx"], o["bidx"])
     assert np.abs(g["W"] - o["W"]).max() <= 1e-6 * np.abs(o["W"]).max()
     assert np.abs(g["bw"] - o["bw"]).max() <= 1e-6
-    _say("config 2: kNN bit-exact, weights within 1e-6")
+    _say("1M-node cube: kNN bit-exact, weights within 1e-6")
     eng.set_initial_pulse([0.3, 0.5, 0.5], 1e-3, 0.05)
     ref.set_pulse([0.3, 0.5, 0.5], 1e-3, 0.05)
     eng.step_forward(20)
     t0 = time.perf_counter()
     ref.step(20)
-    _say(f"config 2: oracle 20 steps {time.perf_counter() - t0:.1f} s")
+    _say(f"1M-node cube: oracle 20 steps {time.perf_counter() - t0:.1f} s")
     a, b = eng.macroscopic(fresh=True), ref.get()
     e_rho, e_u = rel_l2(a["drho"], b["drho"]), rel_l2(a["u"], b["u"])
-    _say(f"config 2: 20 steps rel-L2 drho {e_rho:.2e} u {e_u:.2e}")
+    _say(f"1M-node cube: 20 steps rel-L2 drho {e_rho:.2e} u {e_u:.2e}")
     assert e_rho <= BAR and e_u <= BAR
     eng.close()
 
 
-def test_config4_batched_scenes_vs_oracle(oracle64):
+def test_batched_scenes_vs_oracle(oracle64):
     scenes = workloads.batched_scenes(32, n_lo=2e3, n_hi=8e3)
     engines, nodes = [], []
     for s in scenes:
@@ -74,7 +74,7 @@ def test_config4_batched_scenes_vs_oracle(oracle64):
     gb = batch.autotune(3)
     steps = 60
     batch.step_forward(steps)
-    _say(f"config 4: {len(scenes)} scenes, "
+    _say(f"batched scenes: {len(scenes)} scenes, "
          f"{sum(e.num_nodes for e in engines)} nodes, gather batch {gb}")
     worst = 0.0
     for s, e, nd in zip(scenes, engines, nodes):
@@ -85,7 +85,7 @@ def test_config4_batched_scenes_vs_oracle(oracle64):
         err = max(rel_l2(a["drho"], b["drho"]), rel_l2(a["u"], b["u"]))
         worst = max(worst, err)
         assert err <= BAR, (s.h, s.k, len(s.obstacles), err)
-    _say(f"config 4: worst rel-L2 {worst:.2e}")
+    _say(f"batched scenes: worst rel-L2 {worst:.2e}")
     batch.close()
 
 
@@ -103,7 +103,7 @@ def _oracle_solve(oracle, req):
     return ref.get()
 
 
-def test_config5_served_trace_vs_oracle(oracle64):
+def test_served_trace_vs_oracle(oracle64):
     trace = workloads.served_requests(12, n_lo=4e3, n_hi=1.5e4, s_lo=30, s_hi=90)
     srv = naga_amd.SolveServer(hbm_budget_bytes=16 << 30, max_active=64, chunk_steps=25)
     ids = []
@@ -113,7 +113,7 @@ def test_config5_served_trace_vs_oracle(oracle64):
     srv.drain()
     st = srv.stats()
     assert st["completed"] == len(ids) and st["failed"] == 0
-    _say(f"config 5: {len(ids)} requests served, p50 {st['p50_ms']:.0f} ms "
+    _say(f"served trace: {len(ids)} requests served, p50 {st['p50_ms']:.0f} ms "
          f"p99 {st['p99_ms']:.0f} ms")
     worst = 0.0
     for r, i in zip(trace.requests, ids):
@@ -122,5 +122,5 @@ def test_config5_served_trace_vs_oracle(oracle64):
         err = max(rel_l2(got["drho"], ref["drho"]), rel_l2(got["u"], ref["u"]))
         worst = max(worst, err)
         assert err <= BAR, (r.lattice, r.h, r.steps, err)
-    _say(f"config 5: worst rel-L2 {worst:.2e}")
+    _say(f"served trace: worst rel-L2 {worst:.2e}")
     srv.close()

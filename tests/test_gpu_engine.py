@@ -16,7 +16,7 @@ TOL = 1e-5
 
 @pytest.fixture(scope="module")
 def cfg1(oracle64):
-    """Config 1: D2Q9, unit square, h = 0.01, jitter 0.25h, k = 16, r^3 RBF,
+    """The 2D benchmark scene: D2Q9, unit square, h = 0.01, jitter 0.25h, k = 16, r^3 RBF,
     PML 10h with sigma_max 0.5, pulse 1e-3 / 0.05 at the centre."""
     h = 0.01
     nodes = grid_nodes(oracle64, 2, h, layer=10 * h)
