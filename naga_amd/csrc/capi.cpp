@@ -99,6 +99,49 @@ uint64_t morton(int D, const int64_t* g) {
     return key;
 }
 
+// Hilbert-curve key of grid point g (J. Skilling, "Programming the Hilbert
+// curve", AIP Conf. Proc. 707, 2004: axes to transposed Hilbert index, then
+// the transposed bits interleaved most significant first).
+uint64_t hilbert(int D, const int64_t* g) {
+    const int bits = D == 2 ? 32 : 21;
+    uint32_t X[3] = {uint32_t(g[0]), uint32_t(g[1]), D == 3 ? uint32_t(g[2]) : 0u};
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {  // inverse undo
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < D; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < D; ++i) X[i] ^= X[i - 1];  // Gray encode
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[D - 1] & Q) t ^= Q - 1;
+    for (int i = 0; i < D; ++i) X[i] ^= t;
+    uint64_t key = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < D; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+    return key;
+}
+
+// Sort key of the node providers within each node class: Hilbert order
+// (consecutive nodes are grid neighbours, so a tile of consecutive nodes is
+// compact and its stencils' halo is shared with the tiles next to it in
+// the launch), or Morton order with NAGA_NODE_ORDER=morton (read once per
+// process; the oracle orders by the Hilbert key).
+uint64_t node_key(int D, const int64_t* g) {
+    static const bool use_morton = [] {
+        const char* e = std::getenv("NAGA_NODE_ORDER");
+        return e && std::string(e) == "morton";
+    }();
+    return use_morton ? morton(D, g) : hilbert(D, g);
+}
+
 // ranges/uniform_grid.cuh:55-65, size per dimension ceil(extent/h) + 1 in FP32
 int64_t grid_dims(int D, const double* lo, const double* hi, double h, int64_t* dims) {
     int64_t n = 1;
@@ -524,7 +567,7 @@ int naga_uniform_grid(int32_t dims, const double* lo, const double* hi, double h
                 gi[d] = rem % gd[d];
                 rem /= gd[d];
             }
-            return morton(D, gi);
+            return node_key(D, gi);
         };
         std::vector<std::pair<uint64_t, int64_t>> kb, kl, kbd;
         for (int64_t t = 0; t < ni; ++t) {
@@ -607,7 +650,7 @@ int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double 
                 const double t = std::round((x[d] - lo[d]) / sp[d]);
                 gi[d] = int64_t(std::min(std::max(t, 0.0), double(gd[d] - 1)));
             }
-            return morton(D, gi);
+            return node_key(D, gi);
         };
         struct Node {
             uint64_t key;
@@ -672,7 +715,7 @@ int naga_obstacle_grid(int32_t dims, const double* lo, const double* hi, double 
                 if (!onface) x[d] += (2.0 * u01 - 1.0) * jitter_frac * h;
             }
             if (inside(x, 0.5 * h)) continue;
-            Node nd{morton(D, gi), g, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+            Node nd{node_key(D, gi), g, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
             for (int d = 0; d < D; ++d) nd.x[d] = float(x[d]);
             if (onface) {
                 float nv[3] = {0.f, 0.f, 0.f}, n2 = 0.f;

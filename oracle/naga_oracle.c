@@ -419,16 +419,38 @@ static uint64_t splitmix64(uint64_t* s) {
     return z ^ (z >> 31);
 }
 
-static uint64_t spread_bits(uint64_t v, int D) {
-    uint64_t r = 0;
-    for (int b = 0; b < (D == 2 ? 32 : 21); ++b)
-        r |= ((v >> b) & 1ull) << (b * D);
-    return r;
-}
-
-static uint64_t morton_key(int D, const int64_t* g) {
+/* Order of the nodes within each class: the Hilbert curve through the
+ * grid points (J. Skilling, "Programming the Hilbert curve", AIP Conf.
+ * Proc. 707, 2004): the axes are turned into the transposed Hilbert index
+ * by the inverse undo and a Gray code, whose bits are then read most
+ * significant first, one bit of each axis per level. */
+static uint64_t hilbert_key(int D, const int64_t* g) {
+    const int bits = D == 2 ? 32 : 21;
+    uint32_t x[3];
+    x[0] = (uint32_t)g[0];
+    x[1] = (uint32_t)g[1];
+    x[2] = D == 3 ? (uint32_t)g[2] : 0u;
+    const uint32_t top = 1u << (bits - 1);
+    for (uint32_t q = top; q > 1; q >>= 1) {
+        const uint32_t p = q - 1;
+        for (int i = 0; i < D; ++i) {
+            if (x[i] & q) {
+                x[0] ^= p;
+            } else {
+                const uint32_t t = (x[0] ^ x[i]) & p;
+                x[0] ^= t;
+                x[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < D; ++i) x[i] ^= x[i - 1];
+    uint32_t flip = 0;
+    for (uint32_t q = top; q > 1; q >>= 1)
+        if (x[D - 1] & q) flip ^= q - 1;
+    for (int i = 0; i < D; ++i) x[i] ^= flip;
     uint64_t key = 0;
-    for (int d = 0; d < D; ++d) key |= spread_bits((uint64_t)g[d], D) << d;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < D; ++i) key = (key << 1) | ((x[i] >> b) & 1u);
     return key;
 }
 
@@ -520,7 +542,7 @@ int NO_FN(uniform_grid)(int D, const double* lo, const double* hi, double h,
             gi[d] = rem % dims[d];
             rem /= dims[d];
         }
-        keyed_t e = {morton_key(D, gi), t};
+        keyed_t e = {hilbert_key(D, gi), t};
         if (layer_L > 0 && d2[t] <= L2)
             kl[nlay++] = e;
         else
@@ -532,7 +554,7 @@ int NO_FN(uniform_grid)(int D, const double* lo, const double* hi, double h,
             gi[d] = rem % dims[d];
             rem /= dims[d];
         }
-        keyed_t e = {morton_key(D, gi), t};
+        keyed_t e = {hilbert_key(D, gi), t};
         kbd[t] = e;
     }
     qsort(kb, nbulk, sizeof(keyed_t), cmp_keyed);

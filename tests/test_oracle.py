@@ -208,12 +208,47 @@ def test_splitmix64_published_vector():
                                             0x06C45D188009454F]
 
 
-def morton(g, D):
+def hilbert(g, D):
+    """Hilbert key of grid point g (Skilling's transposed-index algorithm)."""
+    bits = 32 if D == 2 else 21
+    x = [int(v) for v in g] + [0] * (3 - D)
+    top = 1 << (bits - 1)
+    q = top
+    while q > 1:
+        p = q - 1
+        for i in range(D):
+            if x[i] & q:
+                x[0] ^= p
+            else:
+                t = (x[0] ^ x[i]) & p
+                x[0] ^= t
+                x[i] ^= t
+        q >>= 1
+    for i in range(1, D):
+        x[i] ^= x[i - 1]
+    flip, q = 0, top
+    while q > 1:
+        if x[D - 1] & q:
+            flip ^= q - 1
+        q >>= 1
     key = 0
-    for b in range(32 if D == 2 else 21):
-        for d in range(D):
-            key |= ((int(g[d]) >> b) & 1) << (b * D + d)
+    for b in range(bits - 1, -1, -1):
+        for i in range(D):
+            key = (key << 1) | (((x[i] ^ flip) >> b) & 1)
     return key
+
+
+@pytest.mark.parametrize("D,n", [(2, 16), (3, 8)])
+def test_hilbert_key_is_a_curve(D, n):
+    """What fixes the curve: on a 2^m grid the keys are distinct and
+    consecutive keys are grid neighbours (L1 distance 1), with the walk
+    starting at the origin."""
+    pts = [tuple(int(v) for v in np.unravel_index(t, (n,) * D)) for t in range(n ** D)]
+    order = sorted(pts, key=lambda g: hilbert(g, D))
+    assert len({hilbert(g, D) for g in pts}) == n ** D
+    assert order[0] == (0,) * D
+    steps = [sum(abs(a - b) for a, b in zip(p, q)) for p, q in zip(order, order[1:])]
+    assert all(s == 1 for s in steps)
 
 
 @pytest.mark.parametrize("D,h,L", [(2, 0.02, 0.1), (3, 0.1, 0.25)])
@@ -253,9 +288,9 @@ def test_uniform_grid(oracle64, D, h, L):
     assert np.allclose(np.linalg.norm(nrm, axis=1), 1, atol=1e-6)
     side = np.where(gi[bnd] == 0, 1.0, np.where(gi[bnd] == n1 - 1, -1.0, 0.0))
     assert np.allclose(nrm, side / np.linalg.norm(side, axis=1, keepdims=True), atol=1e-7)
-    # Morton order within each class
+    # Hilbert order within each class
     for sl in (slice(0, nb), slice(nb, nb + nl), bnd):
-        keys = [morton(g, D) for g in gi[sl]]
+        keys = [hilbert(g, D) for g in gi[sl]]
         assert keys == sorted(keys)
 
 
