@@ -101,12 +101,31 @@ uint64_t morton(int D, const int64_t* g) {
 
 // Hilbert-curve key of grid point g (J. Skilling, "Programming the Hilbert
 // curve", AIP Conf. Proc. 707, 2004: axes to transposed Hilbert index, then
-// the transposed bits interleaved most significant first).
-uint64_t hilbert(int D, const int64_t* g) {
-    const int bits = D == 2 ? 32 : 21;
+// the transposed bits interleaved most significant first) on the curve of
+// 2^21 (3D) or 2^32 (2D) points per side.
+template <int D>
+uint64_t hilbert_t(const int64_t* g) {
+    constexpr int bits = D == 2 ? 32 : 21;
     uint32_t X[3] = {uint32_t(g[0]), uint32_t(g[1]), D == 3 ? uint32_t(g[2]) : 0u};
-    const uint32_t M = 1u << (bits - 1);
-    for (uint32_t Q = M; Q > 1; Q >>= 1) {  // inverse undo
+    // the levels above the coordinates' highest set bit only permute the
+    // axes (each swaps X[0] with X[1], then with X[2]): applied at once
+    const uint32_t mx = X[0] | X[1] | X[2];
+    const int b = mx ? 32 - __builtin_clz(mx) : 1;
+    const int lv = bits - b;
+    if (D == 2) {
+        if (lv & 1) std::swap(X[0], X[1]);
+    } else if (lv % 3 == 1) {
+        const uint32_t x0 = X[0];
+        X[0] = X[2];
+        X[2] = X[1];
+        X[1] = x0;
+    } else if (lv % 3 == 2) {
+        const uint32_t x0 = X[0];
+        X[0] = X[1];
+        X[1] = X[2];
+        X[2] = x0;
+    }
+    for (uint32_t Q = 1u << (b - 1); Q > 1; Q >>= 1) {  // inverse undo
         const uint32_t P = Q - 1;
         for (int i = 0; i < D; ++i) {
             if (X[i] & Q) {
@@ -120,14 +139,15 @@ uint64_t hilbert(int D, const int64_t* g) {
     }
     for (int i = 1; i < D; ++i) X[i] ^= X[i - 1];  // Gray encode
     uint32_t t = 0;
-    for (uint32_t Q = M; Q > 1; Q >>= 1)
+    for (uint32_t Q = 1u << (b - 1); Q > 1; Q >>= 1)
         if (X[D - 1] & Q) t ^= Q - 1;
     for (int i = 0; i < D; ++i) X[i] ^= t;
     uint64_t key = 0;
-    for (int b = bits - 1; b >= 0; --b)
-        for (int i = 0; i < D; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+    for (int l = b - 1; l >= 0; --l)
+        for (int i = 0; i < D; ++i) key = (key << 1) | ((X[i] >> l) & 1u);
     return key;
 }
+uint64_t hilbert(int D, const int64_t* g) { return D == 2 ? hilbert_t<2>(g) : hilbert_t<3>(g); }
 
 // Sort key of the node providers within each node class: Hilbert order
 // (consecutive nodes are grid neighbours, so a tile of consecutive nodes is
