@@ -62,7 +62,7 @@ double NO_FN(rk4_scalar)(double y0, double lambda, double dt, int nsteps);
  * Uniform grid on [lo, hi]^D (node_providers/uniform_grid_provider.cuh:65-161,
  * ranges/uniform_grid.cuh:55-97,238-294) with seeded SplitMix64 jitter of the
  * interior nodes, PML layer by 1-NN distance to the boundary nodes and the
- * build's deterministic [bulk | layer | boundary] order (Morton key of the
+ * build's deterministic [bulk | layer | boundary] order (Hilbert key of the
  * grid index within each class).
  * counts[0..2] = bulk, layer, boundary.  Output arrays sized by the caller
  * after naga grid size query (grid_size). */
