@@ -304,6 +304,41 @@ def test_ragged_records_bit_identical(monkeypatch):
     assert np.array_equal(runs[0], runs[1])
 
 
+def test_ragged_scenes_batched_like_alone():
+    """Two scenes with per-node stencil widths (ragged record rows) and
+    many boundary tiles past their last fluid tile, stepped in one batch,
+    advance exactly as each does alone: the stage-1 and filter passes run
+    over those boundary-only tiles without touching the per-tile record
+    tables (one entry per fluid tile)."""
+    import naga_amd
+
+    def scene(h):
+        nodes = naga_amd.ObstacleGridProvider([0] * 3, [1] * 3, h, [([0.5] * 3, 0.25)], 3 * h,
+                                              0.5, jitter=0.25, refine_width=0.1).get()
+        eng = naga_amd.SimulationEngine(lattice=27, k=48, k_boundary=32,
+                                        rbf=naga_amd.RBF_POLYHARMONIC3,
+                                        rbf_boundary=naga_amd.RBF_POLYHARMONIC3, k_min=24)
+        p = naga_amd.problem_parameters(27, 0.6, 0.2 * h)
+        eng.set_problem_parameters(p["nu"], 1.0, p["dt"], p["c"])
+        eng.init_domain(nodes)
+        assert nodes.num_boundary_points > 48 * 32  # boundary-only tiles exist
+        eng.set_initial_pulse([0.3, 0.5, 0.5], 1e-3, 0.1)
+        return eng
+
+    hs = (0.04, 0.05)
+    alone = [scene(h) for h in hs]
+    for e in alone:
+        e.step_forward(12)
+    together = [scene(h) for h in hs]
+    batch = naga_amd.SceneBatch(together)
+    batch.step_forward(12)
+    for a, b in zip(alone, together):
+        assert np.array_equal(a.macroscopic(fresh=True)["drho"], b.macroscopic(fresh=True)["drho"])
+    batch.close()
+    for e in alone + together:
+        e.close()
+
+
 def test_tile_unions_bit_identical(oracle64, monkeypatch):
     """The gather kernels read neighbour rows from the tile unions staged in
     LDS by default and straight from HBM with NAGA_UNIONS=0: the same values
